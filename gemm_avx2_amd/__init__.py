@@ -1,0 +1,241 @@
+"""MI355X (gfx950) SGEMM/DGEMM: Python binding of DGEMM.so / SGEMM.so.
+
+The product is the pair of C-ABI shared libraries in ``lib/`` (built by
+``python -m gemm_avx2_amd.build``); this module only loads them with ctypes so
+that tests, the benchmark and Python callers can reach the same entry points a
+Fortran or C program links against.  It mirrors the reference's single entry
+point, ``dgemm_``/``sgemm_`` of wjc404/GEMM_AVX2 src/gemm_driver.c:118, with
+the same argument order and meaning:
+
+    dgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+
+``a``, ``b`` and ``c`` may be numpy arrays (host memory: staged to the GPU and
+back), torch tensors on the GPU (used in place), or raw integer addresses.
+Like the reference, nothing is returned and C is updated in place; unlike it,
+invalid arguments and GPU errors are reported through :func:`last_error`
+(``dgemm`` raises :class:`GemmError` when ``check=True``).
+
+There is no CPU fallback: when the libraries are missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+__all__ = [
+    "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
+    "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "sgemm_bf16x3_dev",
+    "set_dgemm_mode", "set_sgemm_mode", "last_error", "fill",
+    "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
+]
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+MODE_NATIVE, MODE_OZAKI, MODE_BF16X3 = 0, 1, 2
+OK, ERR_ARG, ERR_NODEVICE, ERR_ALLOC, ERR_LAUNCH, ERR_UNSUPPORTED = range(6)
+
+
+class GemmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_libs: dict = {}
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+def _declare(lib, dtype):
+    fl = ctypes.c_double if dtype == "d" else ctypes.c_float
+    flp = _c_double_p if dtype == "d" else _c_float_p
+    p = dtype
+    f = getattr(lib, f"{p}gemm_")
+    f.restype = None
+    f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p, flp,
+                  ctypes.c_void_p, _c_int_p, ctypes.c_void_p, _c_int_p, flp, ctypes.c_void_p,
+                  _c_int_p]
+    f = getattr(lib, f"{p}gemm_x_dev")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, fl,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, fl,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    f = getattr(lib, f"{p}gemm_x_last_error")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.POINTER(ctypes.c_char_p)]
+    if dtype == "d":
+        lib.dgemm_x_set_mode.restype = ctypes.c_int
+        lib.dgemm_x_set_mode.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.dgemm_x_ozaki_workspace.restype = ctypes.c_size_t
+        lib.dgemm_x_ozaki_workspace.argtypes = [ctypes.c_int] * 4
+        lib.dgemm_x_ozaki_dev.restype = ctypes.c_int
+        lib.dgemm_x_ozaki_dev.argtypes = [
+            ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+            ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+            ctypes.c_size_t, ctypes.c_void_p]
+    else:
+        lib.sgemm_x_set_mode.restype = ctypes.c_int
+        lib.sgemm_x_set_mode.argtypes = [ctypes.c_int]
+        lib.sgemm_x_bf16x3_dev.restype = ctypes.c_int
+        lib.sgemm_x_bf16x3_dev.argtypes = [
+            ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+            ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+
+
+def load(name: str):
+    """Loads lib/<name> ('DGEMM.so', 'SGEMM.so', 'libgemmx_gen.so') once."""
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise GemmError(ERR_UNSUPPORTED,
+                        f"{path} is not built; run `python -m gemm_avx2_amd.build`")
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    if name == "DGEMM.so":
+        _declare(lib, "d")
+    elif name == "SGEMM.so":
+        _declare(lib, "s")
+    elif name == "libgemmx_gen.so":
+        for fn, t in (("gemmx_fill_f64", ctypes.c_void_p), ("gemmx_fill_f32", ctypes.c_void_p)):
+            f = getattr(lib, fn)
+            f.restype = ctypes.c_int
+            f.argtypes = [t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                          ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+    _libs[name] = lib
+    return lib
+
+
+def _addr(x) -> int:
+    """Address of a numpy array, torch tensor, ctypes object or raw int."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    if hasattr(x, "ctypes"):
+        return int(x.ctypes.data)
+    return ctypes.addressof(x)
+
+
+def _stream(s) -> int:
+    if s is None:
+        return 0
+    if hasattr(s, "cuda_stream"):
+        return int(s.cuda_stream)
+    return int(s)
+
+
+def last_error(dtype: str = "d"):
+    """(code, message) of the calling thread's last call into DGEMM/SGEMM."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    msg = ctypes.c_char_p()
+    code = getattr(lib, f"{dtype}gemm_x_last_error")(ctypes.byref(msg))
+    return code, (msg.value or b"").decode()
+
+
+def _check(dtype, code=None):
+    c, msg = last_error(dtype)
+    if code is not None and code != OK:
+        raise GemmError(code, msg)
+    if c != OK:
+        raise GemmError(c, msg)
+
+
+def _fortran(dtype, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check):
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    fl = ctypes.c_double if dtype == "d" else ctypes.c_float
+    ta = transa.encode() if isinstance(transa, str) else bytes(transa)
+    tb = transb.encode() if isinstance(transb, str) else bytes(transb)
+    I = ctypes.c_int
+    getattr(lib, f"{dtype}gemm_")(ta, tb, ctypes.byref(I(m)), ctypes.byref(I(n)),
+                                   ctypes.byref(I(k)), ctypes.byref(fl(alpha)), _addr(a),
+                                   ctypes.byref(I(lda)), _addr(b), ctypes.byref(I(ldb)),
+                                   ctypes.byref(fl(beta)), _addr(c), ctypes.byref(I(ldc)))
+    if check:
+        _check(dtype)
+
+
+def dgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    """The drop-in dgemm_ (src/gemm_driver.c:118): C = alpha*op(A)op(B) + beta*C."""
+    _fortran("d", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def sgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    """The drop-in sgemm_ (src/gemm_driver.c:118 without -DDOUBLE)."""
+    _fortran("s", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def _ch(t):
+    return (t.encode() if isinstance(t, str) else bytes(t))[:1]
+
+
+def dgemm_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    """Device-pointer DGEMM enqueued on `stream` (no host synchronisation)."""
+    lib = load("DGEMM.so")
+    rc = lib.dgemm_x_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda, _addr(b),
+                         ldb, beta, _addr(c), ldc, _stream(stream))
+    if rc:
+        _check("d", rc)
+
+
+def sgemm_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    lib = load("SGEMM.so")
+    rc = lib.sgemm_x_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda, _addr(b),
+                         ldb, beta, _addr(c), ldc, _stream(stream))
+    if rc:
+        _check("s", rc)
+
+
+def dgemm_ozaki_workspace(m, n, k, slices) -> int:
+    return int(load("DGEMM.so").dgemm_x_ozaki_workspace(m, n, k, slices))
+
+
+def dgemm_ozaki_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices,
+                    work=None, work_bytes=0, stream=None):
+    """Emulated DGEMM (Ozaki-I, INT8 slices) on device pointers."""
+    lib = load("DGEMM.so")
+    rc = lib.dgemm_x_ozaki_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda,
+                               _addr(b), ldb, beta, _addr(c), ldc, slices, _addr(work),
+                               work_bytes, _stream(stream))
+    if rc:
+        _check("d", rc)
+
+
+def sgemm_bf16x3_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                     stream=None):
+    """Emulated SGEMM (split-BF16x3) on device pointers."""
+    lib = load("SGEMM.so")
+    rc = lib.sgemm_x_bf16x3_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda,
+                                _addr(b), ldb, beta, _addr(c), ldc, _stream(stream))
+    if rc:
+        _check("s", rc)
+
+
+def set_dgemm_mode(mode: int, slices: int = 8):
+    rc = load("DGEMM.so").dgemm_x_set_mode(mode, slices)
+    if rc:
+        _check("d", rc)
+
+
+def set_sgemm_mode(mode: int):
+    rc = load("SGEMM.so").sgemm_x_set_mode(mode)
+    if rc:
+        _check("s", rc)
+
+
+def fill(t, rows, cols, ld, seed, mid, dist=0, param=0.0, stream=None):
+    """Fills a device matrix (torch tensor or address) from the counter-based
+    generator: element (i, j) draws counter i + j*rows of stream (seed, mid);
+    rows..ld-1 are NaN.  dist 0 U[-1,1), 1 (u-.5)exp(param*g), 2 exp(param*g)."""
+    lib = load("libgemmx_gen.so")
+    dt = getattr(t, "dtype", None)
+    is32 = dt is not None and "float32" in str(dt)
+    fn = lib.gemmx_fill_f32 if is32 else lib.gemmx_fill_f64
+    rc = fn(_addr(t), rows, cols, ld, seed, mid, dist, param, _stream(stream))
+    if rc:
+        raise GemmError(rc, "gemmx_fill failed")

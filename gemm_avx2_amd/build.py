@@ -1,0 +1,116 @@
+"""Builds the gfx950 libraries of this package in-tree.
+
+Outputs (gemm_avx2_amd/lib/):
+  DGEMM.so         dgemm_ and the dgemm_x_* extensions
+  SGEMM.so         sgemm_ and the sgemm_x_* extensions
+  libgemmx_gen.so  device-side input generator used by bench.py and the tests
+
+Each HIP source is compiled once with hipcc for gfx950; the host C layer is
+compiled with gcc; the two BLAS libraries are linked from the same objects and
+differ only in their export list (the reference builds one source tree twice,
+-DDOUBLE selecting dgemm_, wjc404/GEMM_AVX2 Makefile:14-18).
+
+    python -m gemm_avx2_amd.build [-j N] [--verbose]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+BUILD = os.path.join(HERE, "_build")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["gemm_native.hip", "gemm_ozaki.hip", "gemm_bf16x3.hip"]
+C_SOURCES = ["gemmx_api.c", "gemmx_tune.c"]
+GEN_SOURCES = ["gemmx_gen.hip"]
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-fvisibility=hidden", "-ffp-contract=fast",
+    # Packed FP32 arithmetic is kept out of every kernel.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+CC_FLAGS = [
+    "-O2", "-fPIC", "-std=gnu11", "-fvisibility=hidden", "-Wall", "-Wextra",
+    "-Wno-unused-parameter", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
+]
+EXPORTS = {
+    "DGEMM.so": ["dgemm_", "dgemm_x_*"],
+    "SGEMM.so": ["sgemm_", "sgemm_x_*"],
+}
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ... ({r.returncode})")
+    return r
+
+
+def _hipcc():
+    p = os.path.join(ROCM, "bin", "hipcc")
+    return p if os.path.exists(p) else shutil.which("hipcc")
+
+
+def build(jobs: int = 8, verbose: bool = False) -> dict:
+    """Compiles every source and links the libraries; returns their paths."""
+    hipcc = _hipcc()
+    if not hipcc:
+        raise RuntimeError("hipcc not found (ROCm is required to build)")
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+
+    def obj(src):
+        return os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+
+    steps = []
+    for src in HIP_SOURCES + GEN_SOURCES:
+        steps.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj(src)])
+    for src in C_SOURCES:
+        steps.append(["gcc", *CC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj(src)])
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(lambda c: _run(c, verbose), steps))
+
+    outputs = {}
+    objs = [obj(s) for s in HIP_SOURCES + C_SOURCES]
+    for lib, syms in EXPORTS.items():
+        vs = os.path.join(BUILD, lib + ".map")
+        with open(vs, "w") as f:
+            f.write("{\n  global:\n" + "".join(f"    {s};\n" for s in syms) + "  local: *;\n};\n")
+        out = os.path.join(LIB, lib)
+        tmp = out + ".tmp"
+        _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+              f"-Wl,--version-script={vs}", "-Wl,-Bsymbolic", "-Wl,-z,defs",
+              f"-L{ROCM}/lib", "-lamdhip64", "-lpthread", "-ldl"], verbose)
+        os.replace(tmp, out)
+        outputs[lib] = out
+    gen = os.path.join(LIB, "libgemmx_gen.so")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", gen + ".tmp",
+          *[obj(s) for s in GEN_SOURCES], f"-L{ROCM}/lib", "-lamdhip64"], verbose)
+    os.replace(gen + ".tmp", gen)
+    outputs["libgemmx_gen.so"] = gen
+    return outputs
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("-j", "--jobs", type=int, default=8)
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args(argv)
+    for name, path in build(args.jobs, args.verbose).items():
+        print(f"{name}: {path}")
+
+
+if __name__ == "__main__":
+    main()

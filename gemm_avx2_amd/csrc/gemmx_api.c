@@ -1,0 +1,428 @@
+/*
+ * gemmx_api.c -- host C layer of DGEMM.so / SGEMM.so.
+ *
+ * Replaces the reference's API/driver layer, GEMM_DRIVER (wjc404/GEMM_AVX2
+ * src/gemm_driver.c:118-197): argument decoding (:121-127), the beta rule
+ * (:133, C_MULT_BETA :106-116), the alpha gate (:134) and the per-call scratch
+ * (:135-142).  The OpenMP team, the N-split and the spin barrier
+ * (:23-33, :128-160) have no host counterpart here: the tile kernels schedule
+ * themselves over the GPU's compute units.
+ *
+ * Additions of this layer: pointer classification (host operands are staged
+ * to device memory and C is copied back; device operands are used in place),
+ * argument checking with error reporting, process-wide arithmetic mode
+ * (native / emulated), and the device-pointer extension entry points.
+ */
+#define _GNU_SOURCE
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/gemm_mi355x.h"
+#include "gemmx_internal.h"
+#include "gemmx_tune.h"
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------ */
+/* Errors: per-thread last error, printed once per error code per process.  */
+/* ------------------------------------------------------------------------ */
+static __thread int t_err;
+static __thread char t_msg[320];
+static pthread_mutex_t g_report_lock = PTHREAD_MUTEX_INITIALIZER;
+static unsigned g_reported;
+
+static int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(t_msg, sizeof t_msg, fmt, ap);
+  va_end(ap);
+  t_err = code;
+  const char *q = getenv("GEMMX_QUIET");
+  if (!(q && *q && *q != '0')) {
+    pthread_mutex_lock(&g_report_lock);
+    if (!(g_reported & (1u << code))) {
+      g_reported |= 1u << code;
+      fprintf(stderr, "gemm_mi355x: %s\n", t_msg);
+    }
+    pthread_mutex_unlock(&g_report_lock);
+  }
+  return code;
+}
+
+static void clear_error(void) {
+  t_err = GEMMX_OK;
+  t_msg[0] = 0;
+}
+
+static int hip_fail(hipError_t e, const char *what) {
+  return set_error(e == hipErrorOutOfMemory ? GEMMX_ERR_ALLOC : GEMMX_ERR_LAUNCH,
+                   "%s failed: %s", what, hipGetErrorString(e));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Process state: lazy HIP init, per-device scratch and staging stream.     */
+/* ------------------------------------------------------------------------ */
+#define GEMMX_MAX_DEV 64
+typedef struct {
+  pthread_mutex_t lock; /* held for the whole of a call that uses scratch */
+  void *scratch;
+  size_t scratch_bytes;
+} devstate_t;
+
+static pthread_once_t g_once = PTHREAD_ONCE_INIT;
+static int g_ndev;
+static hipError_t g_init_err;
+static devstate_t g_dev[GEMMX_MAX_DEV];
+static int g_dmode = GEMMX_MODE_NATIVE, g_dslices = 8;
+static int g_smode = GEMMX_MODE_NATIVE;
+
+static void init_once(void) {
+  for (int i = 0; i < GEMMX_MAX_DEV; i++) pthread_mutex_init(&g_dev[i].lock, NULL);
+  const char *dm = getenv("GEMMX_DGEMM_MODE");
+  if (dm && strncmp(dm, "ozaki", 5) == 0) {
+    g_dmode = GEMMX_MODE_OZAKI;
+    if (dm[5] == ':') {
+      int s = atoi(dm + 6);
+      if (s >= 3 && s <= 8) g_dslices = s;
+    }
+  }
+  const char *sm = getenv("GEMMX_SGEMM_MODE");
+  if (sm && strcmp(sm, "bf16x3") == 0) g_smode = GEMMX_MODE_BF16X3;
+  g_init_err = hipGetDeviceCount(&g_ndev);
+  if (g_init_err != hipSuccess) {
+    g_ndev = 0;
+    (void)hipGetLastError();
+  }
+  gemmx_tune_init();
+}
+
+static int current_device(int *dev) {
+  pthread_once(&g_once, init_once);
+  if (g_ndev <= 0)
+    return set_error(GEMMX_ERR_NODEVICE, "no HIP device available (%s)",
+                     hipGetErrorString(g_init_err));
+  hipError_t e = hipGetDevice(dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (*dev < 0 || *dev >= GEMMX_MAX_DEV)
+    return set_error(GEMMX_ERR_UNSUPPORTED, "device ordinal %d out of range", *dev);
+  return GEMMX_OK;
+}
+
+/* Grows the device's scratch to `bytes`; caller holds g_dev[dev].lock and
+ * has no work in flight on the old scratch. */
+static int get_scratch(int dev, size_t bytes, void **out) {
+  devstate_t *d = &g_dev[dev];
+  if (bytes > d->scratch_bytes) {
+    const char *fail = getenv("GEMMX_FAIL_ALLOC"); /* failure-path testing */
+    if (fail && *fail && *fail != '0')
+      return set_error(GEMMX_ERR_ALLOC, "device scratch of %zu bytes: allocation failure injected", bytes);
+    if (d->scratch) {
+      hipError_t e = hipFree(d->scratch);
+      d->scratch = NULL;
+      d->scratch_bytes = 0;
+      if (e != hipSuccess) return hip_fail(e, "hipFree(scratch)");
+    }
+    size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+    hipError_t e = hipMalloc(&d->scratch, want);
+    if (e != hipSuccess) {
+      d->scratch = NULL;
+      (void)hipGetLastError();
+      return set_error(GEMMX_ERR_ALLOC, "hipMalloc of %zu bytes of device scratch failed: %s",
+                       want, hipGetErrorString(e));
+    }
+    d->scratch_bytes = want;
+  }
+  *out = d->scratch;
+  return GEMMX_OK;
+}
+
+static int is_device_ptr(const void *p) {
+  if (!p) return 0;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+         attr.type == hipMemoryTypeUnified;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Arguments.                                                               */
+/* ------------------------------------------------------------------------ */
+static int notrans(char c) { return c == 'N' || c == 'n'; }
+
+typedef struct {
+  int es; /* element size: 8 or 4 */
+  char ta, tb;
+  int m, n, k, lda, ldb, ldc;
+  double alpha, beta;
+  const void *a, *b;
+  void *c;
+} call_t;
+
+static int validate(const call_t *x) {
+  const int ra = notrans(x->ta) ? x->m : x->k;
+  const int rb = notrans(x->tb) ? x->k : x->n;
+  if (x->m < 0 || x->n < 0 || x->k < 0)
+    return set_error(GEMMX_ERR_ARG, "negative dimension (m=%d n=%d k=%d)", x->m, x->n, x->k);
+  if (x->m == 0 || x->n == 0) return GEMMX_OK;
+  if (x->ldc < (x->m > 1 ? x->m : 1))
+    return set_error(GEMMX_ERR_ARG, "ldc=%d < max(1,m=%d)", x->ldc, x->m);
+  if (!x->c) return set_error(GEMMX_ERR_ARG, "C is NULL");
+  if (x->alpha != 0.0 && x->k > 0) {
+    if (x->lda < (ra > 1 ? ra : 1))
+      return set_error(GEMMX_ERR_ARG, "lda=%d < max(1,%d)", x->lda, ra);
+    if (x->ldb < (rb > 1 ? rb : 1))
+      return set_error(GEMMX_ERR_ARG, "ldb=%d < max(1,%d)", x->ldb, rb);
+    if (!x->a || !x->b) return set_error(GEMMX_ERR_ARG, "A or B is NULL");
+  }
+  return GEMMX_OK;
+}
+
+static size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+/* Launches the arithmetic for device-resident operands (already checked). */
+static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
+                         void *work, hipStream_t s) {
+  hipError_t e;
+  if (p->alpha == 0.0 || p->k == 0) {
+    if (p->beta == 1.0) return GEMMX_OK;
+    e = es == 8 ? gemmx_scale_f64((double *)p->c, p->ldc, p->m, p->n, p->beta, s)
+                : gemmx_scale_f32((float *)p->c, p->ldc, p->m, p->n, (float)p->beta, s);
+    return e == hipSuccess ? GEMMX_OK : hip_fail(e, "scale kernel launch");
+  }
+  if (es == 8 && mode == GEMMX_MODE_OZAKI) {
+    e = gemmx_ozaki_f64(p, slices, work, s);
+    return e == hipSuccess ? GEMMX_OK : hip_fail(e, "ozaki kernel launch");
+  }
+  if (es == 4 && mode == GEMMX_MODE_BF16X3) {
+    e = gemmx_bf16x3_f32(p, s);
+    return e == hipSuccess ? GEMMX_OK : hip_fail(e, "bf16x3 kernel launch");
+  }
+  const int cfg = gemmx_tune_pick(es, p->ta, p->tb, p->m, p->n, p->k);
+  e = es == 8 ? gemmx_native_f64(p, cfg, s) : gemmx_native_f32(p, cfg, s);
+  return e == hipSuccess ? GEMMX_OK : hip_fail(e, "native kernel launch");
+}
+
+static size_t ozaki_ws(const call_t *x, int mode, int slices) {
+  if (x->es != 8 || mode != GEMMX_MODE_OZAKI || x->alpha == 0.0 || x->k == 0) return 0;
+  return gemmx_ozaki_workspace(x->m, x->n, x->k, slices);
+}
+
+/* The Fortran-ABI path: synchronous, host or device operands. */
+static void fortran_call(call_t *x, int mode, int slices) {
+  clear_error();
+  if (validate(x) != GEMMX_OK) return;
+  if (x->m == 0 || x->n == 0) return;
+  int dev;
+  if (current_device(&dev) != GEMMX_OK) return;
+
+  const int need_ab = x->alpha != 0.0 && x->k > 0;
+  const int ra = notrans(x->ta) ? x->m : x->k, ca = notrans(x->ta) ? x->k : x->m;
+  const int rb = notrans(x->tb) ? x->k : x->n, cb = notrans(x->tb) ? x->n : x->k;
+  const int host_a = need_ab && !is_device_ptr(x->a);
+  const int host_b = need_ab && !is_device_ptr(x->b);
+  const int host_c = !is_device_ptr(x->c);
+  const size_t es = (size_t)x->es;
+  /* Staged operands get a leading dimension rounded to 64 bytes so the tile
+   * kernels can use 16-byte loads. */
+  const size_t al = 64 / es;
+  const size_t lda2 = host_a ? round_up((size_t)ra, al) : (size_t)x->lda;
+  const size_t ldb2 = host_b ? round_up((size_t)rb, al) : (size_t)x->ldb;
+  const size_t ldc2 = host_c ? round_up((size_t)x->m, al) : (size_t)x->ldc;
+  const size_t bytes_a = host_a ? round_up(lda2 * ca * es, 256) : 0;
+  const size_t bytes_b = host_b ? round_up(ldb2 * cb * es, 256) : 0;
+  const size_t bytes_c = host_c ? round_up(ldc2 * x->n * es, 256) : 0;
+  const size_t bytes_w = round_up(ozaki_ws(x, mode, slices), 256);
+  const size_t total = bytes_a + bytes_b + bytes_c + bytes_w;
+
+  gemmx_problem p = {.ta = !notrans(x->ta), .tb = !notrans(x->tb), .m = x->m,
+                     .n = x->n, .k = x->k, .lda = (int64_t)lda2,
+                     .ldb = (int64_t)ldb2, .ldc = (int64_t)ldc2,
+                     .alpha = x->alpha, .beta = x->beta, .a = x->a, .b = x->b,
+                     .c = x->c};
+  /* The legacy default stream orders this call after work the caller left
+   * on blocking streams; the call waits for it before returning. */
+  hipStream_t s = hipStreamLegacy;
+  devstate_t *d = &g_dev[dev];
+  pthread_mutex_lock(&d->lock);
+  char *base = NULL;
+  if (total > 0 && get_scratch(dev, total, (void **)&base) != GEMMX_OK) goto out;
+  char *pa = base, *pb = pa + bytes_a, *pc = pb + bytes_b, *pw = pc + bytes_c;
+  hipError_t e;
+  if (host_a) {
+    e = hipMemcpy2DAsync(pa, lda2 * es, x->a, (size_t)x->lda * es, (size_t)ra * es,
+                         (size_t)ca, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { hip_fail(e, "copy of A to the device"); goto out; }
+    p.a = pa;
+  }
+  if (host_b) {
+    e = hipMemcpy2DAsync(pb, ldb2 * es, x->b, (size_t)x->ldb * es, (size_t)rb * es,
+                         (size_t)cb, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { hip_fail(e, "copy of B to the device"); goto out; }
+    p.b = pb;
+  }
+  if (host_c) {
+    /* C is always read: beta == 0 still multiplies (NaN propagates). */
+    e = hipMemcpy2DAsync(pc, ldc2 * es, x->c, (size_t)x->ldc * es, (size_t)x->m * es,
+                         (size_t)x->n, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { hip_fail(e, "copy of C to the device"); goto out; }
+    p.c = pc;
+  }
+  if (launch_device(&p, x->es, mode, slices, bytes_w ? pw : NULL, s) != GEMMX_OK) {
+    (void)hipStreamSynchronize(s);
+    goto out;
+  }
+  if (host_c) {
+    e = hipMemcpy2DAsync(x->c, (size_t)x->ldc * es, pc, ldc2 * es, (size_t)x->m * es,
+                         (size_t)x->n, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) { hip_fail(e, "copy of C to the host"); (void)hipStreamSynchronize(s); goto out; }
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) hip_fail(e, "GEMM execution");
+out:
+  pthread_mutex_unlock(&d->lock);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in entry points (src/gemm_driver.c:118).                            */
+/* ------------------------------------------------------------------------ */
+EXPORT void dgemm_(const char *transa, const char *transb, const int *m,
+                   const int *n, const int *k, const double *alpha,
+                   const double *a, const int *lda, const double *b,
+                   const int *ldb, const double *beta, double *c,
+                   const int *ldc) {
+  if (!transa || !transb || !m || !n || !k || !alpha || !lda || !ldb || !beta || !ldc) {
+    set_error(GEMMX_ERR_ARG, "dgemm_: NULL scalar argument");
+    return;
+  }
+  call_t x = {8, *transa, *transb, *m, *n, *k, *lda, *ldb, *ldc, *alpha, *beta, a, b, c};
+  pthread_once(&g_once, init_once);
+  fortran_call(&x, g_dmode, g_dslices);
+}
+
+EXPORT void sgemm_(const char *transa, const char *transb, const int *m,
+                   const int *n, const int *k, const float *alpha,
+                   const float *a, const int *lda, const float *b,
+                   const int *ldb, const float *beta, float *c,
+                   const int *ldc) {
+  if (!transa || !transb || !m || !n || !k || !alpha || !lda || !ldb || !beta || !ldc) {
+    set_error(GEMMX_ERR_ARG, "sgemm_: NULL scalar argument");
+    return;
+  }
+  call_t x = {4, *transa, *transb, *m, *n, *k, *lda, *ldb, *ldc, *alpha, *beta, a, b, c};
+  pthread_once(&g_once, init_once);
+  fortran_call(&x, g_smode, 0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Extensions.                                                              */
+/* ------------------------------------------------------------------------ */
+EXPORT int dgemm_x_set_mode(int mode, int slices) {
+  pthread_once(&g_once, init_once);
+  if (mode == GEMMX_MODE_NATIVE) {
+    g_dmode = mode;
+    return GEMMX_OK;
+  }
+  if (mode == GEMMX_MODE_OZAKI && slices >= 3 && slices <= 8) {
+    g_dmode = mode;
+    g_dslices = slices;
+    return GEMMX_OK;
+  }
+  return set_error(GEMMX_ERR_ARG, "dgemm_x_set_mode: mode %d slices %d", mode, slices);
+}
+
+EXPORT int sgemm_x_set_mode(int mode) {
+  pthread_once(&g_once, init_once);
+  if (mode == GEMMX_MODE_NATIVE || mode == GEMMX_MODE_BF16X3) {
+    g_smode = mode;
+    return GEMMX_OK;
+  }
+  return set_error(GEMMX_ERR_ARG, "sgemm_x_set_mode: mode %d", mode);
+}
+
+EXPORT int dgemm_x_last_error(const char **msg) {
+  if (msg) *msg = t_msg;
+  return t_err;
+}
+EXPORT int sgemm_x_last_error(const char **msg) {
+  if (msg) *msg = t_msg;
+  return t_err;
+}
+
+static int dev_call(call_t *x, int mode, int slices, void *work, size_t work_bytes,
+                    void *stream) {
+  clear_error();
+  if (validate(x) != GEMMX_OK) return t_err;
+  if (x->m == 0 || x->n == 0) return GEMMX_OK;
+  int dev;
+  if (current_device(&dev) != GEMMX_OK) return t_err;
+  gemmx_problem p = {.ta = !notrans(x->ta), .tb = !notrans(x->tb), .m = x->m,
+                     .n = x->n, .k = x->k, .lda = x->lda, .ldb = x->ldb,
+                     .ldc = x->ldc, .alpha = x->alpha, .beta = x->beta,
+                     .a = x->a, .b = x->b, .c = x->c};
+  hipStream_t s = (hipStream_t)stream;
+  const size_t need = ozaki_ws(x, mode, slices);
+  if (need == 0 || work) {
+    if (need > 0 && work_bytes < need)
+      return set_error(GEMMX_ERR_ARG, "workspace of %zu bytes < %zu needed", work_bytes, need);
+    return launch_device(&p, x->es, mode, slices, work, s);
+  }
+  /* No caller workspace: borrow the device scratch for the duration. */
+  devstate_t *d = &g_dev[dev];
+  pthread_mutex_lock(&d->lock);
+  void *w = NULL;
+  int rc = get_scratch(dev, need, &w);
+  if (rc == GEMMX_OK) {
+    rc = launch_device(&p, x->es, mode, slices, w, s);
+    hipError_t e = hipStreamSynchronize(s);
+    if (rc == GEMMX_OK && e != hipSuccess) rc = hip_fail(e, "GEMM execution");
+  }
+  pthread_mutex_unlock(&d->lock);
+  return rc;
+}
+
+EXPORT int dgemm_x_dev(char transa, char transb, int m, int n, int k,
+                       double alpha, const double *a, int lda, const double *b,
+                       int ldb, double beta, double *c, int ldc, void *stream) {
+  call_t x = {8, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return dev_call(&x, GEMMX_MODE_NATIVE, 0, NULL, 0, stream);
+}
+
+EXPORT int sgemm_x_dev(char transa, char transb, int m, int n, int k,
+                       float alpha, const float *a, int lda, const float *b,
+                       int ldb, float beta, float *c, int ldc, void *stream) {
+  call_t x = {4, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return dev_call(&x, GEMMX_MODE_NATIVE, 0, NULL, 0, stream);
+}
+
+EXPORT size_t dgemm_x_ozaki_workspace(int m, int n, int k, int slices) {
+  if (m <= 0 || n <= 0 || k <= 0 || slices < 3 || slices > 8) return 0;
+  return gemmx_ozaki_workspace(m, n, k, slices);
+}
+
+EXPORT int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
+                             double alpha, const double *a, int lda,
+                             const double *b, int ldb, double beta, double *c,
+                             int ldc, int slices, void *work, size_t work_bytes,
+                             void *stream) {
+  if (slices < 3 || slices > 8) {
+    clear_error();
+    return set_error(GEMMX_ERR_ARG, "slices=%d outside 3..8", slices);
+  }
+  call_t x = {8, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return dev_call(&x, GEMMX_MODE_OZAKI, slices, work, work_bytes, stream);
+}
+
+EXPORT int sgemm_x_bf16x3_dev(char transa, char transb, int m, int n, int k,
+                              float alpha, const float *a, int lda,
+                              const float *b, int ldb, float beta, float *c,
+                              int ldc, void *stream) {
+  call_t x = {4, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return dev_call(&x, GEMMX_MODE_BF16X3, 0, NULL, 0, stream);
+}

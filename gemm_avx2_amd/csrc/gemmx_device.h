@@ -1,0 +1,130 @@
+/*
+ * gemmx_device.h -- device-side building blocks shared by the gfx950 tile
+ * kernels: MFMA wrappers, the global->LDS tile copy with the transpose folded
+ * into the choice of LDS layout, and the XCD-aware tile order.
+ *
+ * Replaces the reference's packing routines (wjc404/GEMM_AVX2
+ * src/gemm_copy.c:4-143, src/gemm_driver.c:34-79): instead of one thread
+ * packing an A panel into a shared buffer while the others spin
+ * (src/gemm_driver.c:23-33), every workgroup stages its own A and B tiles in
+ * LDS, copying contiguous global runs straight into LDS lines.
+ */
+#ifndef GEMMX_DEVICE_H
+#define GEMMX_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gemmx {
+
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef float f16_t __attribute__((ext_vector_type(16)));
+typedef int i4_t __attribute__((ext_vector_type(4)));
+typedef int i16_t __attribute__((ext_vector_type(16)));
+
+/* 16x16x4 MFMA for the native types.  Operand layout (CDNA3/CDNA4):
+ *   A (16x4):  lane l holds A[l%16][l/16]
+ *   B (4x16):  lane l holds B[l/16][l%16]
+ *   D (16x16): lane l, element r holds D[4*(l/16)+r][l%16]            */
+template <typename T> struct Mfma16;
+template <> struct Mfma16<double> {
+  typedef d4_t acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma16<float> {
+  typedef f4_t acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T> struct Vec16;
+template <> struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  enum { N = 2 };
+};
+template <> struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  enum { N = 4 };
+};
+template <> struct Vec16<int> {
+  typedef int type __attribute__((ext_vector_type(4)));
+  enum { N = 4 };
+};
+
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+/* Copies an NL x LEN tile -- NL lines of LEN contiguous elements, line
+ * stride `ld` in global memory -- into LDS lines of stride LS, in 16-byte
+ * chunks.  Elements outside (nl_valid, len_valid) read as zero and are never
+ * dereferenced.  The register stage `r` lets the next tile's global loads be
+ * in flight while the current tile is multiplied. */
+template <typename T, int NL, int LEN, int NT>
+struct TileCopy {
+  typedef typename Vec16<T>::type vec_t;
+  static constexpr int VE = Vec16<T>::N;
+  static constexpr int CPL = LEN / VE;
+  static constexpr int NCH = NL * CPL;
+  static constexpr int PER = NCH / NT;
+  static_assert(LEN % VE == 0, "tile line must be a whole number of 16-byte chunks");
+  static_assert(NCH % NT == 0, "tile chunks must divide evenly over the threads");
+  vec_t r[PER];
+
+  __device__ __forceinline__ void load(const T *__restrict__ g, int64_t ld,
+                                       int nl_valid, int len_valid, bool vec_ok,
+                                       int tid) {
+    if (vec_ok && nl_valid >= NL && len_valid >= LEN) {
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const int ch = tid + c * NT, line = ch / CPL, off = (ch % CPL) * VE;
+        r[c] = *reinterpret_cast<const vec_t *>(g + line * ld + off);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < PER; ++c) {
+        const int ch = tid + c * NT, line = ch / CPL, off = (ch % CPL) * VE;
+        vec_t v;
+#pragma unroll
+        for (int e = 0; e < VE; ++e)
+          v[e] = (line < nl_valid && off + e < len_valid) ? g[line * ld + off + e] : T(0);
+        r[c] = v;
+      }
+    }
+  }
+
+  template <int LS>
+  __device__ __forceinline__ void store(T *s, int tid) const {
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const int ch = tid + c * NT, line = ch / CPL, off = (ch % CPL) * VE;
+      *reinterpret_cast<vec_t *>(s + line * LS + off) = r[c];
+    }
+  }
+};
+
+/* Workgroups are dispatched round-robin over the 8 XCDs of MI355X, each with
+ * its own L2.  Renumber them so that consecutive tile ids run on one XCD,
+ * then walk tiles in groups of GROUP_M tile-rows so the workgroups that an
+ * XCD runs together share A rows and B columns in its L2. */
+__device__ __forceinline__ void tile_coords(int pid, int num, int tiles_m,
+                                            int tiles_n, int *tm, int *tn) {
+  constexpr int XCDS = 8, GROUP_M = 8;
+  const int xcd = pid % XCDS, local = pid / XCDS;
+  const int per = num / XCDS, rem = num % XCDS;
+  pid = (xcd < rem) ? xcd * (per + 1) + local
+                    : rem * (per + 1) + (xcd - rem) * per + local;
+  const int in_group = GROUP_M * tiles_n;
+  const int g = pid / in_group, first = g * GROUP_M;
+  const int gsize = (tiles_m - first < GROUP_M) ? tiles_m - first : GROUP_M;
+  const int r = pid % in_group;
+  *tm = first + r % gsize;
+  *tn = r / gsize;
+}
+
+} // namespace gemmx
+#endif

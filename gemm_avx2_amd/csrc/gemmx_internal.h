@@ -1,0 +1,69 @@
+/*
+ * gemmx_internal.h -- internal C ABI between the host C layer (gemmx_api.c)
+ * and the gfx950 HIP kernels (*.hip).  Not installed; the public surface is
+ * include/gemm_mi355x.h.
+ *
+ * All matrices are column-major with the Fortran BLAS conventions of the
+ * reference entry point (wjc404/GEMM_AVX2 src/gemm_driver.c:118-127):
+ * op(A) is m x k, op(B) is k x n, C is m x n; lda/ldb/ldc are in elements.
+ */
+#ifndef GEMMX_INTERNAL_H
+#define GEMMX_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifndef __HIP_PLATFORM_AMD__
+#define __HIP_PLATFORM_AMD__ 1
+#endif
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One GEMM on device-resident operands.  Pointers are device addresses. */
+typedef struct gemmx_problem {
+  int ta, tb;      /* 1 = transpose (any first char other than 'N'/'n') */
+  int m, n, k;
+  int64_t lda, ldb, ldc;
+  double alpha, beta; /* FP32 problems hold the float values widened */
+  const void *a;
+  const void *b;
+  void *c;
+} gemmx_problem;
+
+/* Kernel configurations of the native MFMA tile kernels.  The tuning table
+ * (gemmx_tune.c) picks one per (dtype, trans, shape bucket). */
+enum {
+  GEMMX_NCFG_128x128 = 0, /* 128x128x16 tile, 4 waves of 64x64 */
+  GEMMX_NCFG_64x64 = 1,   /* 64x64x16 tile, 4 waves of 32x32 */
+  GEMMX_NCFG_256x128 = 2, /* 256x128x16 tile, 8 waves of 64x64 */
+  GEMMX_NCFG_COUNT = 3
+};
+
+/* Native tile kernels (gemm_native.hip).  alpha != 0 and k > 0 here; the
+ * host layer handles the alpha == 0 / k == 0 / empty cases. */
+hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, hipStream_t s);
+hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, hipStream_t s);
+/* C[0:m,0:n] *= beta (beta != 1), the reference's C_MULT_BETA
+ * (src/gemm_driver.c:106-116). */
+hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta,
+                           hipStream_t s);
+hipError_t gemmx_scale_f32(float *c, int64_t ldc, int m, int n, float beta,
+                           hipStream_t s);
+
+/* Emulated DGEMM, Ozaki-I INT8 slices (gemm_ozaki.hip).  `work` is device
+ * scratch of gemmx_ozaki_workspace() bytes. */
+size_t gemmx_ozaki_workspace(int m, int n, int k, int slices);
+int gemmx_ozaki_kchunk(int slices);
+hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int slices, void *work,
+                           hipStream_t s);
+
+/* Emulated SGEMM, split-BF16x3 (gemm_bf16x3.hip). */
+hipError_t gemmx_bf16x3_f32(const gemmx_problem *p, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,95 @@
+/*
+ * gemmx_tune.c -- run-time tuning table (see gemmx_tune.h).
+ *
+ * The reference fixes its blocking at compile time (dgemm_tune.h:6-10,
+ * sgemm_tune.h:7-11, derived in src/gemm_set_parameters.h:54-61).  This layer
+ * looks the tile configuration up per call, so one build serves every shape
+ * and the autotuner can change the choice without a rebuild.
+ */
+#define _GNU_SOURCE
+#include "gemmx_tune.h"
+
+#include <dlfcn.h>
+#include <libgen.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gemmx_internal.h"
+
+#define NB 12 /* buckets 4..15 */
+static signed char g_table[2][4][NB][NB][NB]; /* [f64/f32][trans][m][n][k] */
+static int g_entries;
+static char g_path[1024];
+
+int gemmx_tune_bucket(int dim) {
+  int b = 0;
+  while (dim > 1 && b < 31) {
+    dim >>= 1;
+    b++;
+  }
+  if (b < 4) b = 4;
+  if (b > 15) b = 15;
+  return b;
+}
+
+static int trans_index(const char *t) {
+  if (strcmp(t, "NN") == 0) return 0;
+  if (strcmp(t, "NT") == 0) return 1;
+  if (strcmp(t, "TN") == 0) return 2;
+  if (strcmp(t, "TT") == 0) return 3;
+  return -1;
+}
+
+static void load_table(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    char *hash = strchr(line, '#');
+    if (hash) *hash = 0;
+    char dt[8], tr[8];
+    int mb, nb, kb, cfg;
+    if (sscanf(line, "%7s %7s %d %d %d %d", dt, tr, &mb, &nb, &kb, &cfg) != 6) continue;
+    int d = strcmp(dt, "f64") == 0 ? 0 : strcmp(dt, "f32") == 0 ? 1 : -1;
+    int t = trans_index(tr);
+    if (d < 0 || t < 0 || mb < 4 || mb > 15 || nb < 4 || nb > 15 || kb < 4 || kb > 15) continue;
+    if (cfg < 0 || cfg >= GEMMX_NCFG_COUNT) continue;
+    g_table[d][t][mb - 4][nb - 4][kb - 4] = (signed char)(cfg + 1);
+    g_entries++;
+  }
+  fclose(f);
+  snprintf(g_path, sizeof g_path, "%s", path);
+}
+
+void gemmx_tune_init(void) {
+  memset(g_table, 0, sizeof g_table);
+  g_entries = 0;
+  g_path[0] = 0;
+  const char *env = getenv("GEMMX_TUNE_FILE");
+  if (env && *env) {
+    load_table(env);
+    return;
+  }
+  Dl_info info;
+  if (dladdr((void *)&gemmx_tune_init, &info) && info.dli_fname) {
+    char buf[1024];
+    snprintf(buf, sizeof buf, "%s", info.dli_fname);
+    char path[1100];
+    snprintf(path, sizeof path, "%s/gemm_mi355x_tune.txt", dirname(buf));
+    load_table(path);
+  }
+}
+
+int gemmx_tune_entries(void) { return g_entries; }
+const char *gemmx_tune_path(void) { return g_path; }
+
+int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {
+  const int d = es == 8 ? 0 : 1, t = (ta ? 2 : 0) + (tb ? 1 : 0);
+  const int c = g_table[d][t][gemmx_tune_bucket(m) - 4][gemmx_tune_bucket(n) - 4]
+                       [gemmx_tune_bucket(k) - 4];
+  if (c > 0) return c - 1;
+  /* Heuristic: keep at least ~2 workgroups per CU of the 256 on MI355X. */
+  const long tiles128 = (long)((m + 127) / 128) * ((n + 127) / 128);
+  return tiles128 >= 512 ? GEMMX_NCFG_128x128 : GEMMX_NCFG_64x64;
+}

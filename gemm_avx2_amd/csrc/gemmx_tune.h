@@ -1,0 +1,34 @@
+/*
+ * gemmx_tune.h -- run-time tuning table.
+ *
+ * Replaces the reference's compile-time tuning headers dgemm_tune.h /
+ * sgemm_tune.h and their clamping in src/gemm_set_parameters.h:10-61 (edited
+ * by hand and rebuilt, README.md:13-15).  Here the block shape is a run-time
+ * choice among the compiled tile configurations (GEMMX_NCFG_*), read from a
+ * text table written by the autotuner (gemm_avx2_amd/autotune.py).
+ *
+ * Table file: GEMMX_TUNE_FILE, else gemm_mi355x_tune.txt beside the loaded
+ * library.  One entry per line, '#' starts a comment:
+ *     <dtype:f64|f32> <trans:NN|NT|TN|TT> <mbucket> <nbucket> <kbucket> <cfg>
+ * where a bucket is floor(log2(dim)) clamped to 4..15 and cfg is a
+ * configuration id.  Missing entries fall back to a size heuristic.
+ */
+#ifndef GEMMX_TUNE_H
+#define GEMMX_TUNE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void gemmx_tune_init(void);
+int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k);
+int gemmx_tune_bucket(int dim);
+/* Number of entries loaded from the table file (0 when none). */
+int gemmx_tune_entries(void);
+/* Path of the table that was loaded, or "" when none. */
+const char *gemmx_tune_path(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+/*
+ * gemm_mi355x.h -- public C ABI of the MI355X (gfx950) SGEMM/DGEMM library.
+ *
+ * Shipped as two shared objects, like the reference (wjc404/GEMM_AVX2
+ * Makefile:12-18, README.md:10):
+ *   DGEMM.so  exports dgemm_  and the dgemm_x_* extensions
+ *   SGEMM.so  exports sgemm_  and the sgemm_x_* extensions
+ * Link against or LD_PRELOAD either one.  All matrices are column-major.
+ *
+ * The two Fortran-ABI symbols are drop-ins for the reference's; everything
+ * named *_x_* is an extension with no reference counterpart.
+ */
+#ifndef GEMM_MI355X_H
+#define GEMM_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Drop-in entry points.
+ *
+ * Replace GEMM_DRIVER = dgemm_/sgemm_ of wjc404/GEMM_AVX2
+ * src/gemm_driver.c:118 (built twice, -DDOUBLE selecting dgemm_, :8-14).
+ * Same Fortran calling convention (every argument by pointer, LP64 int, no
+ * hidden string lengths) and the same semantics:
+ *   - only *transa / *transb are read; 'N'/'n' = no transpose, any other
+ *     character = transpose (src/gemm_driver.c:17-18);
+ *   - if *beta != 1, C[0:m,0:n] is scaled by beta first, also for beta == 0,
+ *     so NaN/Inf in C propagate (:106-116, :133);
+ *   - if *alpha != 0, C += alpha * op(A) * op(B) (:134);
+ *   - rows m..ldc-1 of C are never written;
+ *   - C is complete when the call returns (synchronous).
+ * A, B and C may be host memory (staged to the GPU and back) or device
+ * memory of the current HIP device (used in place), independently.
+ * Additive behaviour: invalid arguments (negative sizes, ld too small, NULL
+ * operands that would be read) and GPU errors leave C untouched and are
+ * reported through *_x_last_error() and, once per process, on stderr
+ * (silenced by GEMMX_QUIET=1).  The reference does not check arguments.
+ * ------------------------------------------------------------------------ */
+void dgemm_(const char *transa, const char *transb, const int *m, const int *n,
+            const int *k, const double *alpha, const double *a, const int *lda,
+            const double *b, const int *ldb, const double *beta, double *c,
+            const int *ldc);
+void sgemm_(const char *transa, const char *transb, const int *m, const int *n,
+            const int *k, const float *alpha, const float *a, const int *lda,
+            const float *b, const int *ldb, const float *beta, float *c,
+            const int *ldc);
+
+/* ------------------------------------------------------------------------
+ * Extensions (no reference counterpart).
+ * ------------------------------------------------------------------------ */
+
+/* Error codes returned by the extension entry points. */
+#define GEMMX_OK 0
+#define GEMMX_ERR_ARG 1      /* invalid argument */
+#define GEMMX_ERR_NODEVICE 2 /* no HIP device */
+#define GEMMX_ERR_ALLOC 3    /* device or pinned allocation failed */
+#define GEMMX_ERR_LAUNCH 4   /* kernel launch or HIP runtime error */
+#define GEMMX_ERR_UNSUPPORTED 5
+
+/* Arithmetic used by dgemm_ / sgemm_ (process-wide, default native).
+ * Also set by the environment at first use:
+ *   GEMMX_DGEMM_MODE = native | ozaki[:slices]     (slices 3..8, default 8)
+ *   GEMMX_SGEMM_MODE = native | bf16x3                                     */
+#define GEMMX_MODE_NATIVE 0
+#define GEMMX_MODE_OZAKI 1  /* FP64 by INT8 slices on INT8 MFMA */
+#define GEMMX_MODE_BF16X3 2 /* FP32 by split-BF16x3 on BF16 MFMA */
+int dgemm_x_set_mode(int mode, int slices);
+int sgemm_x_set_mode(int mode);
+
+/* Last error of the calling thread: returns its code and, when msg is not
+ * NULL, a static description.  Cleared by the next successful call. */
+int dgemm_x_last_error(const char **msg);
+int sgemm_x_last_error(const char **msg);
+
+/* Device-resident form: A, B, C are device pointers on the current device;
+ * the work is enqueued on `stream` (a hipStream_t, NULL = legacy default
+ * stream) and the call returns without waiting.  Same semantics as dgemm_.
+ * Returns a GEMMX_* code. */
+int dgemm_x_dev(char transa, char transb, int m, int n, int k, double alpha,
+                const double *a, int lda, const double *b, int ldb, double beta,
+                double *c, int ldc, void *stream);
+int sgemm_x_dev(char transa, char transb, int m, int n, int k, float alpha,
+                const float *a, int lda, const float *b, int ldb, float beta,
+                float *c, int ldc, void *stream);
+
+/* Emulated DGEMM on device pointers: Ozaki-I with `slices` signed 7-bit INT8
+ * slices per operand (3..8), exact INT32 accumulation on INT8 MFMA, FP64 only
+ * for the row/column exponent scaling and the recombination.  `work` is
+ * device scratch of at least dgemm_x_ozaki_workspace() bytes, or NULL to use
+ * the library's own (then the call synchronises `stream`). */
+size_t dgemm_x_ozaki_workspace(int m, int n, int k, int slices);
+int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
+                      double alpha, const double *a, int lda, const double *b,
+                      int ldb, double beta, double *c, int ldc, int slices,
+                      void *work, size_t work_bytes, void *stream);
+
+/* Emulated SGEMM on device pointers: split-BF16x3 (a = hi + lo in BF16,
+ * C ~ hi*hi + hi*lo + lo*hi) on BF16 MFMA with FP32 accumulation. */
+int sgemm_x_bf16x3_dev(char transa, char transb, int m, int n, int k,
+                       float alpha, const float *a, int lda, const float *b,
+                       int ldb, float beta, float *c, int ldc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

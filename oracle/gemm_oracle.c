@@ -1,0 +1,483 @@
+/*
+ * gemm_oracle.c -- CPU oracle for the SGEMM/DGEMM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and bench.py's
+ * cpu_baseline leg as the checker; never linked into DGEMM.so / SGEMM.so.
+ * Built with -ffp-contract=off so the error-free transformations below are
+ * not fused by the compiler.  Every function names the reference lines it
+ * restates; the emulated forms have no reference counterpart and restate the
+ * algorithm of this library's emulated kernels.
+ */
+#include "gemm_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------------- */
+/* Counter-based generator: SplitMix64.  The stream key is the SplitMix64   */
+/* finaliser of (seed, id); element idx is the idx-th SplitMix64 output of  */
+/* that key, so any entry can be drawn without drawing the others.          */
+/* ---------------------------------------------------------------------- */
+#define SM64_GAMMA 0x9E3779B97F4A7C15ULL
+
+static uint64_t sm64_fin(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t oracle_counter_u64(uint64_t seed, uint64_t id, uint64_t idx) {
+  uint64_t key = sm64_fin(seed * SM64_GAMMA + (id + 1) * 0xD1B54A32D192ED03ULL);
+  return sm64_fin(key + (idx + 1) * SM64_GAMMA);
+}
+
+static double u01_f64(uint64_t x) { return (double)(x >> 11) * 0x1p-53; }
+
+double oracle_uniform_f64(uint64_t seed, uint64_t id, uint64_t idx) {
+  return u01_f64(oracle_counter_u64(seed, id, idx)) * 2.0 - 1.0;
+}
+
+float oracle_uniform_f32(uint64_t seed, uint64_t id, uint64_t idx) {
+  float u = (float)(oracle_counter_u64(seed, id, idx) >> 40) * 0x1p-24f;
+  return u * 2.0f - 1.0f;
+}
+
+static double gauss(uint64_t seed, uint64_t id, uint64_t i1, uint64_t i2) {
+  double u1 = 1.0 - u01_f64(oracle_counter_u64(seed, id, i1)); /* (0,1] */
+  double u2 = u01_f64(oracle_counter_u64(seed, id, i2));
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+double oracle_emu_f64(uint64_t seed, uint64_t id, uint64_t idx, double phi) {
+  double u = u01_f64(oracle_counter_u64(seed, id, 3 * idx));
+  double g = gauss(seed, id, 3 * idx + 1, 3 * idx + 2);
+  return (u - 0.5) * exp(phi * g);
+}
+
+float oracle_lognormal_f32(uint64_t seed, uint64_t id, uint64_t idx, double sigma) {
+  return (float)exp(sigma * gauss(seed, id, 2 * idx, 2 * idx + 1));
+}
+
+static double gen_f64(uint64_t seed, uint64_t id, uint64_t idx, int dist, double param) {
+  if (dist == 1) return oracle_emu_f64(seed, id, idx, param);
+  if (dist == 2) return (double)oracle_lognormal_f32(seed, id, idx, param);
+  return oracle_uniform_f64(seed, id, idx);
+}
+
+static float gen_f32(uint64_t seed, uint64_t id, uint64_t idx, int dist, double param) {
+  if (dist == 1) return (float)oracle_emu_f64(seed, id, idx, param);
+  if (dist == 2) return oracle_lognormal_f32(seed, id, idx, param);
+  return oracle_uniform_f32(seed, id, idx);
+}
+
+void oracle_fill_f64(double *p, int rows, int cols, int ld, uint64_t seed,
+                     uint64_t id, int dist, double param) {
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < cols; j++) {
+    double *col = p + (int64_t)j * ld;
+    for (int i = 0; i < rows; i++)
+      col[i] = gen_f64(seed, id, (uint64_t)i + (uint64_t)j * rows, dist, param);
+    for (int i = rows; i < ld; i++) col[i] = NAN;
+  }
+}
+
+void oracle_fill_f32(float *p, int rows, int cols, int ld, uint64_t seed,
+                     uint64_t id, int dist, double param) {
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < cols; j++) {
+    float *col = p + (int64_t)j * ld;
+    for (int i = 0; i < rows; i++)
+      col[i] = gen_f32(seed, id, (uint64_t)i + (uint64_t)j * rows, dist, param);
+    for (int i = rows; i < ld; i++) col[i] = NAN;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Error-free transformations (Knuth TwoSum, FMA TwoProd) and Dot2          */
+/* (Ogita, Rump, Oishi 2005): the result is as accurate as if computed in   */
+/* twice the working precision and then rounded.                            */
+/* ---------------------------------------------------------------------- */
+typedef struct { double hi, lo; } dd_t;
+
+static inline void dot2_step(dd_t *s, double a, double b) {
+  double p = a * b;
+  double pe = fma(a, b, -p);
+  double t = s->hi + p;
+  double bb = t - s->hi;
+  double se = (s->hi - (t - bb)) + (p - bb);
+  s->hi = t;
+  s->lo += se + pe;
+}
+
+static inline double dd_round(dd_t s) { return s.hi + s.lo; }
+
+/* op(A)[i,p] and op(B)[p,j] for the reference's transpose convention:
+ * only the first character matters, 'N'/'n' is no-transpose, anything else
+ * transposes (src/gemm_driver.c:17-18). */
+static inline int is_notrans(char c) { return c == 'N' || c == 'n'; }
+
+/* ---------------------------------------------------------------------- */
+/* Reference semantics with Dot2 accumulation.                              */
+/* Restates GEMM_DRIVER, src/gemm_driver.c:118-197:                         */
+/*   :133  if beta != 1, C[0:m,0:n] *= beta (C_MULT_BETA :106-116)          */
+/*   :134  if alpha != 0, C += A * (alpha*B) with alpha folded into the      */
+/*         packed B block (src/gemm_copy.c:65 outb=in*ALPHA)                 */
+/* ---------------------------------------------------------------------- */
+void oracle_dgemm(const char *transa, const char *transb, const int *pm,
+                  const int *pn, const int *pk, const double *palpha,
+                  const double *a, const int *plda, const double *b,
+                  const int *pldb, const double *pbeta, double *c,
+                  const int *pldc) {
+  const int m = *pm, n = *pn, k = *pk, lda = *plda, ldb = *pldb, ldc = *pldc;
+  const double alpha = *palpha, beta = *pbeta;
+  const int na = is_notrans(*transa), nb = is_notrans(*transb);
+  if (m <= 0 || n <= 0) return;
+#pragma omp parallel
+  {
+    double *bcol = (double *)malloc(sizeof(double) * (k > 0 ? k : 1));
+    dd_t *acc = (dd_t *)malloc(sizeof(dd_t) * m);
+#pragma omp for schedule(dynamic, 1)
+    for (int j = 0; j < n; j++) {
+      double *cj = c + (int64_t)j * ldc;
+      for (int i = 0; i < m; i++) {
+        double c1 = (beta != 1.0) ? beta * cj[i] : cj[i];
+        acc[i].hi = c1;
+        acc[i].lo = 0.0;
+      }
+      if (alpha != 0.0) {
+        for (int p = 0; p < k; p++) {
+          double bv = nb ? b[p + (int64_t)j * ldb] : b[j + (int64_t)p * ldb];
+          bcol[p] = bv * alpha;
+        }
+        if (na) {
+          for (int p = 0; p < k; p++) {
+            const double *ap = a + (int64_t)p * lda;
+            double bp = bcol[p];
+            for (int i = 0; i < m; i++) dot2_step(&acc[i], ap[i], bp);
+          }
+        } else {
+          for (int i = 0; i < m; i++) {
+            const double *ai = a + (int64_t)i * lda;
+            for (int p = 0; p < k; p++) dot2_step(&acc[i], ai[p], bcol[p]);
+          }
+        }
+      }
+      for (int i = 0; i < m; i++) cj[i] = dd_round(acc[i]);
+    }
+    free(bcol);
+    free(acc);
+  }
+}
+
+void oracle_sgemm(const char *transa, const char *transb, const int *pm,
+                  const int *pn, const int *pk, const float *palpha,
+                  const float *a, const int *plda, const float *b,
+                  const int *pldb, const float *pbeta, float *c,
+                  const int *pldc) {
+  const int m = *pm, n = *pn, k = *pk, lda = *plda, ldb = *pldb, ldc = *pldc;
+  const float alpha = *palpha, beta = *pbeta;
+  const int na = is_notrans(*transa), nb = is_notrans(*transb);
+  if (m <= 0 || n <= 0) return;
+#pragma omp parallel
+  {
+    float *bcol = (float *)malloc(sizeof(float) * (k > 0 ? k : 1));
+    dd_t *acc = (dd_t *)malloc(sizeof(dd_t) * m);
+#pragma omp for schedule(dynamic, 1)
+    for (int j = 0; j < n; j++) {
+      float *cj = c + (int64_t)j * ldc;
+      for (int i = 0; i < m; i++) {
+        float c1 = (beta != 1.0f) ? beta * cj[i] : cj[i];
+        acc[i].hi = (double)c1;
+        acc[i].lo = 0.0;
+      }
+      if (alpha != 0.0f) {
+        for (int p = 0; p < k; p++) {
+          float bv = nb ? b[p + (int64_t)j * ldb] : b[j + (int64_t)p * ldb];
+          bcol[p] = bv * alpha; /* one FP32 rounding, as the packed B block */
+        }
+        if (na) {
+          for (int p = 0; p < k; p++) {
+            const float *ap = a + (int64_t)p * lda;
+            double bp = (double)bcol[p];
+            for (int i = 0; i < m; i++) dot2_step(&acc[i], (double)ap[i], bp);
+          }
+        } else {
+          for (int i = 0; i < m; i++) {
+            const float *ai = a + (int64_t)i * lda;
+            for (int p = 0; p < k; p++)
+              dot2_step(&acc[i], (double)ai[p], (double)bcol[p]);
+          }
+        }
+      }
+      for (int i = 0; i < m; i++) cj[i] = (float)dd_round(acc[i]);
+    }
+    free(bcol);
+    free(acc);
+  }
+}
+
+void oracle_dgemm_longdouble(char transa, char transb, int m, int n, int k,
+                             double alpha, const double *a, int lda,
+                             const double *b, int ldb, double beta,
+                             double *c, int ldc) {
+  const int na = is_notrans(transa), nb = is_notrans(transb);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < m; i++) {
+      long double s = 0.0L;
+      for (int p = 0; p < k; p++) {
+        long double av = na ? a[i + (int64_t)p * lda] : a[p + (int64_t)i * lda];
+        long double bv = nb ? b[p + (int64_t)j * ldb] : b[j + (int64_t)p * ldb];
+        s += av * bv;
+      }
+      long double cv = c[i + (int64_t)j * ldc];
+      long double r = (beta != 1.0 ? (long double)beta * cv : cv);
+      if (alpha != 0.0) r += (long double)alpha * s;
+      c[i + (int64_t)j * ldc] = (double)r;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Sampled entries for generated inputs (A id 0, B id 1, C id 2).  Stored   */
+/* shapes: A is m x k (transa N) or k x m (T); B is k x n (N) or n x k (T); */
+/* the counter index of a stored element (r, s) is r + s*rows.              */
+/* ---------------------------------------------------------------------- */
+void oracle_sampled_f64(char transa, char transb, int m, int n, int k,
+                        double alpha, double beta, uint64_t seed, int dist,
+                        double param, int count, const int *is, const int *js,
+                        double *out, double *absprod) {
+  const int na = is_notrans(transa), nb = is_notrans(transb);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < count; t++) {
+    const uint64_t i = (uint64_t)is[t], j = (uint64_t)js[t];
+    double cv = gen_f64(seed, 2, i + j * (uint64_t)m, dist, param);
+    dd_t s = {(beta != 1.0) ? beta * cv : cv, 0.0};
+    double ab = 0.0;
+    if (alpha != 0.0) {
+      for (uint64_t p = 0; p < (uint64_t)k; p++) {
+        double av = gen_f64(seed, 0, na ? i + p * (uint64_t)m : p + i * (uint64_t)k, dist, param);
+        double bv = gen_f64(seed, 1, nb ? p + j * (uint64_t)k : j + p * (uint64_t)n, dist, param);
+        dot2_step(&s, av, bv * alpha);
+        ab += fabs(av * bv);
+      }
+    }
+    out[t] = dd_round(s);
+    if (absprod) absprod[t] = ab;
+  }
+}
+
+void oracle_sampled_f32(char transa, char transb, int m, int n, int k,
+                        float alpha, float beta, uint64_t seed, int dist,
+                        double param, int count, const int *is, const int *js,
+                        double *out, double *absprod) {
+  const int na = is_notrans(transa), nb = is_notrans(transb);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < count; t++) {
+    const uint64_t i = (uint64_t)is[t], j = (uint64_t)js[t];
+    float cv = gen_f32(seed, 2, i + j * (uint64_t)m, dist, param);
+    float c1 = (beta != 1.0f) ? beta * cv : cv;
+    dd_t s = {(double)c1, 0.0};
+    double ab = 0.0;
+    if (alpha != 0.0f) {
+      for (uint64_t p = 0; p < (uint64_t)k; p++) {
+        float av = gen_f32(seed, 0, na ? i + p * (uint64_t)m : p + i * (uint64_t)k, dist, param);
+        float bv = gen_f32(seed, 1, nb ? p + j * (uint64_t)k : j + p * (uint64_t)n, dist, param);
+        dot2_step(&s, (double)av, (double)(bv * alpha));
+        ab += fabs((double)av * (double)bv);
+      }
+    }
+    out[t] = dd_round(s);
+    if (absprod) absprod[t] = ab;
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Ozaki-I INT8 slicing.  No reference counterpart: this restates the       */
+/* emulated DGEMM kernel of this library.                                   */
+/*   op(A)[i,:] = 2^ea_i * x_i,  op(B)[:,j] = 2^fb_j * y_j,  |x|,|y| < 1     */
+/*   x = sum_{t=1..s} d_t 2^-7t (+ dropped residual), digits truncated       */
+/*   S_w = sum_{t+u=w} sum_p d_t(i,p) e_u(p,j)  for w = 2..s+1 (exact int32) */
+/*   acc = 0; for w = s+1 .. 2: acc = acc*2^-7 + S_w   (FP64, one rounding   */
+/*   per step; acc*2^-7 is exact)                                           */
+/*   C = fma(alpha, ldexp(acc, ea_i + fb_j - 14), c1),                       */
+/*   c1 = beta != 1 ? beta*C : C (same rule as the native path).             */
+/* K is processed in chunks of oracle_ozaki_kchunk(s) so that every S_w     */
+/* fits int32; chunk q > 0 uses c1 = C (the previous chunk's result).       */
+/* ---------------------------------------------------------------------- */
+int oracle_ozaki_exponent(const double *x, int count, int stride) {
+  double amax = 0.0;
+  for (int t = 0; t < count; t++) {
+    double v = fabs(x[(int64_t)t * stride]);
+    if (!(v <= DBL_MAX)) return OZAKI_NONFINITE; /* Inf or NaN */
+    if (v > amax) amax = v;
+  }
+  if (amax == 0.0) return 0;
+  int e;
+  frexp(amax, &e); /* amax = f*2^e, f in [0.5,1) */
+  return e;
+}
+
+void oracle_ozaki_digits(double x, int e, int slices, int8_t *d) {
+  double r = (e == OZAKI_NONFINITE) ? 0.0 : ldexp(x, -e);
+  for (int t = 0; t < slices; t++) {
+    double y = r * 128.0;
+    double q = trunc(y);
+    d[t] = (int8_t)q;
+    r = y - q;
+  }
+}
+
+int oracle_ozaki_kchunk(int slices) {
+  int64_t lim = (int64_t)2147483647 / ((int64_t)slices * 127 * 127);
+  return (int)(lim - lim % 128);
+}
+
+void oracle_ozaki_dgemm(char transa, char transb, int m, int n, int k,
+                        double alpha, const double *a, int lda,
+                        const double *b, int ldb, double beta, double *c,
+                        int ldc, int s) {
+  const int na = is_notrans(transa), nb = is_notrans(transb);
+  if (m <= 0 || n <= 0) return;
+  if (alpha == 0.0 || k == 0) {
+    if (beta != 1.0)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) c[i + (int64_t)j * ldc] *= beta;
+    return;
+  }
+  int *ea = (int *)malloc(sizeof(int) * m);
+  int *fb = (int *)malloc(sizeof(int) * n);
+  /* digits laid out [t][i][p] and [u][j][p] */
+  int8_t *da = (int8_t *)malloc((size_t)s * m * k);
+  int8_t *db = (int8_t *)malloc((size_t)s * n * k);
+  for (int i = 0; i < m; i++)
+    ea[i] = na ? oracle_ozaki_exponent(a + i, k, lda)
+               : oracle_ozaki_exponent(a + (int64_t)i * lda, k, 1);
+  for (int j = 0; j < n; j++)
+    fb[j] = nb ? oracle_ozaki_exponent(b + (int64_t)j * ldb, k, 1)
+               : oracle_ozaki_exponent(b + j, k, ldb);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < m; i++) {
+    int8_t dig[16];
+    for (int p = 0; p < k; p++) {
+      double v = na ? a[i + (int64_t)p * lda] : a[p + (int64_t)i * lda];
+      oracle_ozaki_digits(v, ea[i], s, dig);
+      for (int t = 0; t < s; t++) da[((size_t)t * m + i) * k + p] = dig[t];
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++) {
+    int8_t dig[16];
+    for (int p = 0; p < k; p++) {
+      double v = nb ? b[p + (int64_t)j * ldb] : b[j + (int64_t)p * ldb];
+      oracle_ozaki_digits(v, fb[j], s, dig);
+      for (int u = 0; u < s; u++) db[((size_t)u * n + j) * k + p] = dig[u];
+    }
+  }
+  const int kc = oracle_ozaki_kchunk(s);
+  for (int k0 = 0; k0 < k; k0 += kc) {
+    const int k1 = (k0 + kc < k) ? k0 + kc : k;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < n; j++) {
+      for (int i = 0; i < m; i++) {
+        double acc = 0.0;
+        for (int w = s + 1; w >= 2; w--) {
+          int32_t sw = 0;
+          for (int t = 1; t <= s; t++) {
+            int u = w - t;
+            if (u < 1 || u > s) continue;
+            const int8_t *pa = da + ((size_t)(t - 1) * m + i) * k;
+            const int8_t *pb = db + ((size_t)(u - 1) * n + j) * k;
+            int32_t part = 0;
+            for (int p = k0; p < k1; p++) part += (int32_t)pa[p] * (int32_t)pb[p];
+            sw += part;
+          }
+          acc = fma(acc, 0x1p-7, (double)sw);
+        }
+        double *cij = c + i + (int64_t)j * ldc;
+        double c1 = (k0 == 0 && beta != 1.0) ? beta * *cij : *cij;
+        double scaled;
+        if (ea[i] == OZAKI_NONFINITE || fb[j] == OZAKI_NONFINITE)
+          scaled = NAN;
+        else
+          scaled = ldexp(acc, ea[i] + fb[j] - 14);
+        *cij = fma(alpha, scaled, c1);
+      }
+    }
+  }
+  free(ea);
+  free(fb);
+  free(da);
+  free(db);
+}
+
+/* ---------------------------------------------------------------------- */
+/* split-BF16x3.  No reference counterpart: restates the emulated SGEMM     */
+/* kernel.  x = hi + lo + r with hi = bf16_rne(x), lo = bf16_rne(x - hi)    */
+/* (x - hi is exact in FP32); the product keeps hi*hi + hi*lo + lo*hi.      */
+/* ---------------------------------------------------------------------- */
+uint16_t oracle_bf16_rne(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) /* Inf/NaN: keep, quiet NaN */
+    return (uint16_t)((u >> 16) | ((u & 0x007fffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+float oracle_bf16_to_f32(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+
+void oracle_bf16_split(float x, uint16_t *hi, uint16_t *lo) {
+  uint16_t h = oracle_bf16_rne(x);
+  float r = x - oracle_bf16_to_f32(h);
+  *hi = h;
+  *lo = oracle_bf16_rne(r);
+}
+
+void oracle_bf16x3_sgemm(char transa, char transb, int m, int n, int k,
+                         float alpha, const float *a, int lda, const float *b,
+                         int ldb, float beta, float *c, int ldc) {
+  const int na = is_notrans(transa), nb = is_notrans(transb);
+  if (m <= 0 || n <= 0) return;
+#pragma omp parallel
+  {
+    float *bh = (float *)malloc(sizeof(float) * (k > 0 ? k : 1));
+    float *bl = (float *)malloc(sizeof(float) * (k > 0 ? k : 1));
+#pragma omp for schedule(dynamic, 1)
+    for (int j = 0; j < n; j++) {
+      for (int p = 0; p < k; p++) {
+        float bv = nb ? b[p + (int64_t)j * ldb] : b[j + (int64_t)p * ldb];
+        uint16_t h, l;
+        oracle_bf16_split(bv, &h, &l);
+        bh[p] = oracle_bf16_to_f32(h);
+        bl[p] = oracle_bf16_to_f32(l);
+      }
+      for (int i = 0; i < m; i++) {
+        float *cij = c + i + (int64_t)j * ldc;
+        float c1 = (beta != 1.0f) ? beta * *cij : *cij;
+        if (alpha == 0.0f) {
+          *cij = c1;
+          continue;
+        }
+        dd_t s = {0.0, 0.0};
+        for (int p = 0; p < k; p++) {
+          float av = na ? a[i + (int64_t)p * lda] : a[p + (int64_t)i * lda];
+          uint16_t h, l;
+          oracle_bf16_split(av, &h, &l);
+          double ah = oracle_bf16_to_f32(h), al = oracle_bf16_to_f32(l);
+          dot2_step(&s, ah, bh[p]);
+          dot2_step(&s, ah, bl[p]);
+          dot2_step(&s, al, bh[p]);
+        }
+        *cij = (float)((double)alpha * dd_round(s) + (double)c1);
+      }
+    }
+    free(bh);
+    free(bl);
+  }
+}

@@ -1,0 +1,122 @@
+/*
+ * gemm_oracle.h -- CPU oracle for the SGEMM/DGEMM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library (DGEMM.so /
+ * SGEMM.so) links, loads or calls this code.  Only tests/, the smoke check and
+ * the cpu_baseline leg of bench.py use it, and only as the checker.
+ *
+ * What it restates:
+ *   - oracle_dgemm / oracle_sgemm: the semantics of the reference's Fortran-ABI
+ *     entry point GEMM_DRIVER (wjc404/GEMM_AVX2 src/gemm_driver.c:118-197):
+ *       * only the first character of transa/transb is read, 'N'/'n' is
+ *         no-transpose and any other character is transpose (:17-18);
+ *       * beta != 1 scales C[0:m,0:n] in place first, including beta == 0,
+ *         so NaN/Inf already in C propagate (:106-116, :133);
+ *       * alpha != 0 gates the product (:134); alpha is folded into B with one
+ *         rounding before the product (src/gemm_copy.c:65, :88, :117, :136);
+ *       * rows m..ldc-1 of C are never written (masked stores,
+ *         src/kernel/gemm_kernel_unroll6.S:145-155,192-229).
+ *     The products are accumulated with Ogita-Rump-Oishi Dot2 (double-double),
+ *     so the oracle is *more* accurate than the reference; parity is by
+ *     tolerance, not bits (the reference sums in a cyclic blocked order,
+ *     src/kernel/gemm_kernel_unroll6.S:70-125).
+ *   - the counter-based input generator (SplitMix64 on (seed, id, index)).
+ *   - the emulated forms this build adds (no reference counterpart):
+ *     Ozaki-I INT8 slicing of FP64 and the split-BF16x3 form of FP32.  These
+ *     restate the algorithm the GPU kernels implement, bit for bit where the
+ *     GPU arithmetic is exact (integer slice products, FP64 recombination).
+ *
+ * Pinning (see tests/test_oracle.py): golden outputs of the reference built
+ * from source (tests/golden/), the closed-form 12x12x12 known-answer test of
+ * OpenBLAS-like_implementation/dgemm_kernel_4x4_haswell.c:472-492, brute force
+ * in long double on tiny shapes, and invariants (ones, alpha=0, k=0,
+ * transpose identities, linearity).
+ */
+#ifndef GEMM_ORACLE_H
+#define GEMM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- counter-based generator ------------------------------------------ */
+uint64_t oracle_counter_u64(uint64_t seed, uint64_t id, uint64_t idx);
+double oracle_uniform_f64(uint64_t seed, uint64_t id, uint64_t idx);   /* U[-1,1) */
+float oracle_uniform_f32(uint64_t seed, uint64_t id, uint64_t idx);    /* U[-1,1) */
+/* (u-0.5)*exp(phi*g), u~U[0,1), g~N(0,1) by Box-Muller */
+double oracle_emu_f64(uint64_t seed, uint64_t id, uint64_t idx, double phi);
+/* exp(sigma*g), g~N(0,1) */
+float oracle_lognormal_f32(uint64_t seed, uint64_t id, uint64_t idx, double sigma);
+
+/* Fill a rows x cols column-major matrix with leading dimension ld.  Element
+ * (i,j) takes counter index i + j*rows; rows..ld-1 of every column are NaN.
+ * dist: 0 uniform[-1,1), 1 emu(phi), 2 lognormal(sigma=param). */
+void oracle_fill_f64(double *p, int rows, int cols, int ld, uint64_t seed,
+                     uint64_t id, int dist, double param);
+void oracle_fill_f32(float *p, int rows, int cols, int ld, uint64_t seed,
+                     uint64_t id, int dist, double param);
+
+/* ---- reference semantics, Dot2 accumulation --------------------------- */
+void oracle_dgemm(const char *transa, const char *transb, const int *m,
+                  const int *n, const int *k, const double *alpha,
+                  const double *a, const int *lda, const double *b,
+                  const int *ldb, const double *beta, double *c,
+                  const int *ldc);
+void oracle_sgemm(const char *transa, const char *transb, const int *m,
+                  const int *n, const int *k, const float *alpha,
+                  const float *a, const int *lda, const float *b,
+                  const int *ldb, const float *beta, float *c, const int *ldc);
+
+/* Brute force in long double, no alpha pre-rounding, for tiny shapes. */
+void oracle_dgemm_longdouble(char transa, char transb, int m, int n, int k,
+                             double alpha, const double *a, int lda,
+                             const double *b, int ldb, double beta,
+                             double *c, int ldc);
+
+/* Sampled entries of C for inputs drawn from the generator (A id 0, B id 1,
+ * C id 2, all uniform[-1,1) unless dist/param say otherwise), without
+ * materialising the matrices.  out[t] = expected C[is[t], js[t]] after the
+ * call, computed with Dot2.  Also returns sum|a||b| per entry in absprod
+ * (may be NULL) for componentwise error bounds. */
+void oracle_sampled_f64(char transa, char transb, int m, int n, int k,
+                        double alpha, double beta, uint64_t seed, int dist,
+                        double param, int count, const int *is, const int *js,
+                        double *out, double *absprod);
+void oracle_sampled_f32(char transa, char transb, int m, int n, int k,
+                        float alpha, float beta, uint64_t seed, int dist,
+                        double param, int count, const int *is, const int *js,
+                        double *out, double *absprod);
+
+/* ---- Ozaki-I INT8 slicing (emulated DGEMM restatement) ---------------- */
+/* Exponent of a row/column: the e with max|x| < 2^e, max|x| >= 2^(e-1); 0 for
+ * an all-zero vector; OZAKI_NONFINITE when any entry is Inf/NaN. */
+#define OZAKI_NONFINITE 0x7fffffff
+int oracle_ozaki_exponent(const double *x, int count, int stride);
+/* Signed 7-bit digits of x*2^-e (|.| < 1), truncated toward zero:
+ * x*2^-e = sum_t d[t]*2^(-7(t+1)) + r, |d[t]| <= 127. */
+void oracle_ozaki_digits(double x, int e, int slices, int8_t *d);
+/* Largest K chunk (a multiple of 128) whose weight sums stay exact in int32
+ * for this slice count. */
+int oracle_ozaki_kchunk(int slices);
+/* The emulated product, same semantics as oracle_dgemm. */
+void oracle_ozaki_dgemm(char transa, char transb, int m, int n, int k,
+                        double alpha, const double *a, int lda,
+                        const double *b, int ldb, double beta, double *c,
+                        int ldc, int slices);
+
+/* ---- split-BF16x3 (emulated SGEMM restatement) ------------------------ */
+uint16_t oracle_bf16_rne(float x);
+float oracle_bf16_to_f32(uint16_t h);
+void oracle_bf16_split(float x, uint16_t *hi, uint16_t *lo);
+/* C = alpha*(Ah*Bh + Ah*Bl + Al*Bh) + beta*C with exact (Dot2) accumulation
+ * of the three bf16 products; same beta/alpha semantics as oracle_sgemm
+ * except that alpha is applied after the product, as the GPU form does. */
+void oracle_bf16x3_sgemm(char transa, char transb, int m, int n, int k,
+                         float alpha, const float *a, int lda, const float *b,
+                         int ldb, float beta, float *c, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,365 @@
+"""Pins the CPU oracle before anything is checked against it.
+
+The oracle (oracle/gemm_oracle.c) restates the reference's dgemm_/sgemm_
+semantics (wjc404/GEMM_AVX2 src/gemm_driver.c:118-197) with Dot2
+accumulation.  It is checked here against
+  * the golden vectors written by the reference itself (tests/golden/),
+  * the closed-form 12x12x12 known-answer test of
+    OpenBLAS-like_implementation/dgemm_kernel_4x4_haswell.c:472-492,
+  * brute force in long double on tiny shapes,
+  * invariants that a dropped term, a wrong sign, a transposed operand or a
+    wrong beta rule would break,
+and, when oracle/_ref/ is built, against the reference on extra shapes.
+The emulated restatements (Ozaki-I, split-BF16x3) are pinned by exact
+integer cases, by a library BF16 rounding routine and by their error against
+Dot2 shrinking as the slice count grows.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from gemm_util import (U32, U64, abs_bound, case_inputs, load_fixtures, max_comp, op,
+                       rel_fro, stored_shape)
+
+FIX = load_fixtures()
+
+
+def _tol(dtype, k):
+    u = U64 if dtype == "f64" else U32
+    return 64 * max(k, 1) * u
+
+
+# --------------------------------------------------------------- fixtures
+@pytest.mark.parametrize("case", [c for c in FIX if c["group"] == "small"],
+                         ids=lambda c: f"{c['dtype']}-{c['transa']}{c['transb']}-"
+                                       f"{c['m']}x{c['n']}x{c['k']}-a{c['alpha']}b{c['beta']}")
+def test_oracle_matches_reference_small(case):
+    A, B, C = case_inputs(case, oracle)
+    C0 = C.copy(order="F")
+    m, n, k = case["m"], case["n"], case["k"]
+    oracle.gemm(case["transa"], case["transb"], m, n, k, case["alpha"], A, case["lda"], B,
+                case["ldb"], case["beta"], C, case["ldc"])
+    ref = case["values"].reshape((m, n), order="F")
+    assert case["pad_touched"] == 0
+    assert np.all(np.isnan(C[m:, :n])), "oracle wrote into the ld padding"
+    if case["alpha"] == 0.0 or k == 0:
+        np.testing.assert_array_equal(C[:m, :n].astype(np.float64), ref)
+        return
+    bound = abs_bound(A, B, C0, case["transa"], case["transb"], m, n, k, case["alpha"],
+                      case["beta"])
+    assert max_comp(C[:m, :n], ref, bound) <= _tol(case["dtype"], k)
+
+
+@pytest.mark.parametrize("case", [c for c in FIX if c["group"] == "medium"],
+                         ids=lambda c: f"{c['dtype']}-{c['transa']}{c['transb']}-"
+                                       f"{c['m']}x{c['n']}x{c['k']}")
+def test_oracle_matches_reference_medium(case):
+    A, B, C = case_inputs(case, oracle)
+    C0 = C.copy(order="F")
+    m, n, k = case["m"], case["n"], case["k"]
+    oracle.gemm(case["transa"], case["transb"], m, n, k, case["alpha"], A, case["lda"], B,
+                case["ldb"], case["beta"], C, case["ldc"])
+    got = C[case["is"], case["js"]].astype(np.float64)
+    bound = abs_bound(A, B, C0, case["transa"], case["transb"], m, n, k, case["alpha"],
+                      case["beta"])[case["is"], case["js"]]
+    assert max_comp(got, case["values"], bound) <= _tol(case["dtype"], k)
+    full = C[:m, :n].astype(np.float64)
+    assert abs(np.linalg.norm(full) - case["fro"]) <= 1e-6 * case["fro"]
+
+
+@pytest.mark.parametrize("case", [c for c in FIX if c["group"] not in ("small", "medium")],
+                         ids=lambda c: f"{c['group']}-{c['dtype']}-{c['transa']}{c['transb']}"
+                                       f"-p{c['param']}")
+def test_oracle_sampled_matches_reference_large(case):
+    dt = np.float64 if case["dtype"] == "f64" else np.float32
+    exp, ab = oracle.sampled(dt, case["transa"], case["transb"], case["m"], case["n"],
+                             case["k"], case["alpha"], case["beta"], case["seed"], case["is"],
+                             case["js"], case["dist"], case["param"])
+    bound = abs(case["alpha"]) * ab + abs(case["beta"]) + 1e-300
+    err = np.max(np.abs(exp - case["values"]) / bound)
+    assert err <= _tol(case["dtype"], case["k"])
+
+
+def test_fixture_inventory():
+    groups = {c["group"] for c in FIX}
+    assert {"small", "medium", "nn1024", "sq8192", "ragged8192", "emu8192"} <= groups
+    sq = [c for c in FIX if c["group"] == "sq8192"]
+    assert {(c["dtype"], c["transa"] + c["transb"]) for c in sq} == {
+        (d, t) for d in ("f64", "f32") for t in ("NN", "NT", "TN", "TT")}
+    assert all(c["lda"] == c["ldb"] == c["ldc"] == 8200 for c in sq)
+    assert all(c["pad_touched"] == 0 for c in FIX)
+
+
+# --------------------------------------------------------------- closed forms
+def test_known_answer_12x12x12():
+    """OpenBLAS-like_implementation/dgemm_kernel_4x4_haswell.c:472-492:
+    A[idx]=5+idx%2, B[idx]=20+(idx/ldb)%8, C=0, alpha=beta=1, so
+    C[i,j] = 12*(5+i%2)*(20+j%8)."""
+    n = 12
+    idx = np.arange(n * n)
+    A = np.asfortranarray((5.0 + idx % 2).reshape((n, n), order="F"))
+    B = np.asfortranarray((20.0 + (idx // n) % 8).reshape((n, n), order="F"))
+    C = np.zeros((n, n), order="F")
+    oracle.gemm("N", "N", n, n, n, 1.0, A, n, B, n, 1.0, C, n)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    np.testing.assert_array_equal(C, 12.0 * (5 + i % 2) * (20 + j % 8))
+    Cs = np.zeros((n, n), dtype=np.float32, order="F")
+    oracle.gemm("N", "N", n, n, n, 1.0, A.astype(np.float32), n, B.astype(np.float32), n, 1.0,
+                Cs, n)
+    np.testing.assert_array_equal(Cs, (12.0 * (5 + i % 2) * (20 + j % 8)).astype(np.float32))
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (2.0, -0.5), (-0.25, 1.0), (0.5, 3.0)])
+def test_dot2_matches_long_double(ta, tb, alpha, beta):
+    rng = np.random.default_rng(7)
+    for m, n, k in [(1, 1, 1), (3, 5, 7), (9, 4, 33), (17, 13, 64)]:
+        ra, ca = stored_shape(ta, m, k)
+        rb, cb = stored_shape(tb, k, n)
+        A = np.asfortranarray(rng.uniform(-1, 1, (ra + 2, ca)))
+        B = np.asfortranarray(rng.uniform(-1, 1, (rb + 1, cb)))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + 3, n)))
+        C1, C2 = C.copy(order="F"), C.copy(order="F")
+        oracle.gemm(ta, tb, m, n, k, alpha, A, ra + 2, B, rb + 1, beta, C1, m + 3)
+        oracle.longdouble_gemm(ta, tb, m, n, k, alpha, A, ra + 2, B, rb + 1, beta, C2, m + 3)
+        bound = abs_bound(A, B, C, ta, tb, m, n, k, alpha, beta)
+        assert max_comp(C1[:m], C2[:m], bound) <= 4 * U64
+        np.testing.assert_array_equal(C1[m:], C[m:])  # padding untouched
+
+
+def test_ones_give_k():
+    for k in (1, 7, 288, 1000):
+        A = np.ones((5, k), order="F")
+        B = np.ones((k, 6), order="F")
+        C = np.full((5, 6), np.nan, order="F")
+        oracle.gemm("N", "N", 5, 6, k, 1.0, A, 5, B, k, 0.0, C, 5)
+        # beta = 0 multiplies NaN: the reference's rule keeps the NaN
+        assert np.all(np.isnan(C))
+        C = np.zeros((5, 6), order="F")
+        oracle.gemm("N", "N", 5, 6, k, 1.0, A, 5, B, k, 0.0, C, 5)
+        np.testing.assert_array_equal(C, float(k))
+
+
+def test_beta_rule_alpha_zero_k_zero():
+    rng = np.random.default_rng(3)
+    A = np.asfortranarray(rng.uniform(-1, 1, (4, 4)))
+    A[0, 0] = np.nan  # never read when alpha == 0
+    B = np.asfortranarray(rng.uniform(-1, 1, (4, 4)))
+    C = np.asfortranarray(rng.uniform(-1, 1, (4, 4)))
+    for beta in (0.0, 1.0, -2.5):
+        C1 = C.copy(order="F")
+        oracle.gemm("N", "N", 4, 4, 4, 0.0, A, 4, B, 4, beta, C1, 4)
+        np.testing.assert_array_equal(C1, C * beta if beta != 1.0 else C)
+        C2 = C.copy(order="F")
+        oracle.gemm("N", "N", 4, 4, 0, 1.0, A, 4, B, 4, beta, C2, 4)
+        np.testing.assert_array_equal(C2, C * beta if beta != 1.0 else C)
+    C3 = C.copy(order="F")
+    oracle.gemm("N", "N", 0, 4, 4, 1.0, A, 4, B, 4, 0.0, C3, 4)
+    np.testing.assert_array_equal(C3, C)
+
+
+def test_transpose_chars_and_identity():
+    """'N'/'n' are no-transpose, anything else transposes; op(A) stored
+    transposed gives bit-identical results; (AB)^T = B^T A^T."""
+    rng = np.random.default_rng(11)
+    m, n, k = 13, 9, 21
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    base = np.zeros((m, n), order="F")
+    oracle.gemm("N", "N", m, n, k, 1.5, A, m, B, k, 0.0, base, m)
+    At, Bt = np.asfortranarray(A.T), np.asfortranarray(B.T)
+    for ta, tb, a, lda, b, ldb in [("n", "n", A, m, B, k), ("T", "N", At, k, B, k),
+                                   ("C", "n", At, k, B, k), ("N", "t", A, m, Bt, n),
+                                   ("x", "c", At, k, Bt, n)]:
+        C = np.zeros((m, n), order="F")
+        oracle.gemm(ta, tb, m, n, k, 1.5, a, lda, b, ldb, 0.0, C, m)
+        np.testing.assert_array_equal(C, base)
+    # C^T = op(B)^T op(A)^T: swap roles, compare within rounding
+    CT = np.zeros((n, m), order="F")
+    oracle.gemm("T", "T", n, m, k, 1.5, B, k, A, m, 0.0, CT, n)
+    np.testing.assert_allclose(CT.T, base, rtol=0, atol=4 * U64 * np.abs(base).max())
+
+
+def test_linearity_in_alpha():
+    rng = np.random.default_rng(5)
+    A = np.asfortranarray(rng.uniform(-1, 1, (8, 30)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (30, 7)))
+    C1 = np.zeros((8, 7), order="F")
+    C2 = np.zeros((8, 7), order="F")
+    oracle.gemm("N", "N", 8, 7, 30, 0.75, A, 8, B, 30, 0.0, C1, 8)
+    oracle.gemm("N", "N", 8, 7, 30, 1.5, A, 8, B, 30, 0.0, C2, 8)
+    np.testing.assert_array_equal(2.0 * C1, C2)  # scaling by 2 is exact
+
+
+def test_reference_agrees_on_extra_shapes():
+    """Extra shapes straight against the reference when oracle/_ref is built."""
+    if oracle.ref("d") is None:
+        pytest.skip("reference not built (oracle/_ref)")
+    rng = np.random.default_rng(13)
+    for dt, dts in ((np.float64, "f64"), (np.float32, "f32")):
+        for (m, n, k) in [(1, 1, 289), (8, 96, 288), (100, 3, 5), (70, 193, 337)]:
+            for ta in "NT":
+                for tb in "NT":
+                    ra, ca = stored_shape(ta, m, k)
+                    rb, cb = stored_shape(tb, k, n)
+                    A = np.asfortranarray(rng.uniform(-1, 1, (ra, ca)).astype(dt))
+                    B = np.asfortranarray(rng.uniform(-1, 1, (rb, cb)).astype(dt))
+                    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(dt))
+                    C1, C2 = C.copy(order="F"), C.copy(order="F")
+                    oracle.gemm(ta, tb, m, n, k, 1.5, A, ra, B, rb, -0.5, C1, m)
+                    assert oracle.ref_gemm(ta, tb, m, n, k, 1.5, A, ra, B, rb, -0.5, C2, m)
+                    bound = abs_bound(A, B, C, ta, tb, m, n, k, 1.5, -0.5)
+                    assert max_comp(C1, C2, bound) <= _tol(dts, k)
+
+
+# --------------------------------------------------------------- generator
+def test_generator_values():
+    L = oracle.lib()
+    xs = np.array([L.oracle_uniform_f64(0, 0, i) for i in range(20000)])
+    assert xs.min() >= -1.0 and xs.max() < 1.0
+    assert abs(xs.mean()) < 0.02 and abs(xs.var() - 1 / 3) < 0.01
+    # exactly (x>>11) * 2^-53 * 2 - 1: every value is a multiple of 2^-52
+    assert np.all(np.ldexp(xs + 1.0, 52) == np.floor(np.ldexp(xs + 1.0, 52)))
+    fs = np.array([L.oracle_uniform_f32(0, 0, i) for i in range(20000)], dtype=np.float64)
+    assert np.all(np.ldexp(fs + 1.0, 23) == np.floor(np.ldexp(fs + 1.0, 23)))
+    # the FP32 draw uses the top 24 bits of the same counter as the FP64 draw
+    for i in range(100):
+        x = L.oracle_counter_u64(5, 1, i)
+        assert L.oracle_uniform_f64(5, 1, i) == (x >> 11) * 2.0 ** -53 * 2 - 1
+        assert L.oracle_uniform_f32(5, 1, i) == np.float32((x >> 40) * 2.0 ** -24 * 2 - 1)
+    # streams differ by seed and by matrix id
+    assert L.oracle_counter_u64(0, 0, 0) != L.oracle_counter_u64(0, 1, 0)
+    assert L.oracle_counter_u64(0, 0, 0) != L.oracle_counter_u64(1, 0, 0)
+    buf = oracle.fill(3, 2, 5, 9, 0)
+    assert np.all(np.isnan(buf[3:])) and not np.any(np.isnan(buf[:3]))
+    assert buf[1, 1] == L.oracle_uniform_f64(9, 0, 1 + 1 * 3)
+
+
+# --------------------------------------------------------------- Ozaki-I
+def test_ozaki_exponent_and_digits():
+    L = oracle.lib()
+    x = np.array([0.75, -3.0, 1e-3], dtype=np.float64)
+    e = L.oracle_ozaki_exponent(x.ctypes.data, 3, 1)
+    assert e == 2  # max 3.0 in [2^1, 2^2)
+    z = np.zeros(4)
+    assert L.oracle_ozaki_exponent(z.ctypes.data, 4, 1) == 0
+    z[2] = np.inf
+    assert L.oracle_ozaki_exponent(z.ctypes.data, 4, 1) == 0x7FFFFFFF
+    rng = np.random.default_rng(2)
+    d = (ctypes.c_int8 * 8)()
+    for v in rng.uniform(-1, 1, 200) * 2.0 ** rng.integers(-40, 40, 200):
+        ev = math.frexp(abs(v))[1] + int(rng.integers(0, 5))
+        for s in (3, 5, 8):
+            L.oracle_ozaki_digits(float(v), ev, s, d)
+            digs = np.array(d[:s], dtype=np.int64)
+            assert np.all(np.abs(digs) <= 127)
+            assert np.all(digs * np.sign(v) >= 0)  # truncation keeps the sign
+            rec = sum(float(digs[t]) * 2.0 ** (-7 * (t + 1)) for t in range(s))
+            assert abs(math.ldexp(v, -ev) - rec) < 2.0 ** (-7 * s)
+
+
+@pytest.mark.parametrize("s", [3, 4, 5, 6, 7, 8])
+def test_ozaki_exact_on_small_integers(s):
+    """Entries that fit in the first digit make every slice product exact, so
+    the emulated GEMM equals the exact integer GEMM for every slice count."""
+    rng = np.random.default_rng(s)
+    for ta in "NT":
+        for tb in "NT":
+            m, n, k = 7, 5, 37
+            ra, ca = stored_shape(ta, m, k)
+            rb, cb = stored_shape(tb, k, n)
+            A = np.asfortranarray(rng.integers(-3, 4, (ra, ca)).astype(np.float64))
+            B = np.asfortranarray(rng.integers(-3, 4, (rb, cb)).astype(np.float64))
+            C = np.asfortranarray(rng.integers(-3, 4, (m, n)).astype(np.float64))
+            exact = 2.0 * (op(A, ta, m, k) @ op(B, tb, k, n)) - 1.0 * C
+            oracle.ozaki_gemm(ta, tb, m, n, k, 2.0, A, ra, B, rb, -1.0, C, m, s)
+            np.testing.assert_array_equal(C, exact)
+
+
+def test_ozaki_error_shrinks_with_slices():
+    rng = np.random.default_rng(4)
+    m, n, k = 40, 30, 300
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    R = np.zeros((m, n), order="F")
+    oracle.gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m)
+    errs = []
+    for s in range(3, 9):
+        C = np.zeros((m, n), order="F")
+        oracle.ozaki_gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m, s)
+        e = rel_fro(C, R)
+        errs.append(e)
+        # truncation error: k terms, each off by < 2*2^-7s relative to the maxima
+        assert e <= 4 * k * 2.0 ** (-7 * s) * 1.0 / (np.linalg.norm(R) / np.sqrt(m * n)) + 4 * U64
+    assert all(b < a for a, b in zip(errs, errs[1:-1]))
+    assert errs[-1] < 1e-15
+
+
+def test_ozaki_nonfinite_rows_and_kchunk():
+    L = oracle.lib()
+    assert L.oracle_ozaki_kchunk(8) == 16640  # 8*16640*127^2 < 2^31 <= 8*16768*127^2
+    assert 8 * 16640 * 127 ** 2 < 2 ** 31 <= 8 * (16640 + 128) * 127 ** 2
+    A = np.ones((3, 4), order="F")
+    A[1, 2] = np.inf
+    B = np.ones((4, 2), order="F")
+    C = np.zeros((3, 2), order="F")
+    oracle.ozaki_gemm("N", "N", 3, 2, 4, 1.0, A, 3, B, 4, 0.0, C, 3, 4)
+    assert np.all(np.isnan(C[1])) and np.all(C[[0, 2]] == 4.0)
+    # K split across chunks still reaches FP64-level error
+    k = 16640 + 300
+    rng = np.random.default_rng(8)
+    A = np.asfortranarray(rng.uniform(-1, 1, (2, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, 3)))
+    R = np.zeros((2, 3), order="F")
+    oracle.gemm("N", "N", 2, 3, k, 1.0, A, 2, B, k, 0.0, R, 2)
+    C = np.zeros((2, 3), order="F")
+    oracle.ozaki_gemm("N", "N", 2, 3, k, 1.0, A, 2, B, k, 0.0, C, 2, 8)
+    assert rel_fro(C, R) < 1e-14
+
+
+# --------------------------------------------------------------- BF16x3
+def test_bf16_rounding_matches_torch():
+    import torch
+    rng = np.random.default_rng(9)
+    x = np.concatenate([
+        rng.standard_normal(5000).astype(np.float32) * np.float32(2.0) ** rng.integers(-30, 30, 5000).astype(np.float32),
+        np.array([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -1.0 - 2 ** -8, 65504.0, 3e38, 1e-40,
+                  0.0, -0.0, np.inf, -np.inf], dtype=np.float32)])
+    t = torch.from_numpy(x).to(torch.bfloat16).view(torch.int16).numpy().astype(np.uint16)
+    L = oracle.lib()
+    mine = np.array([L.oracle_bf16_rne(float(v)) for v in x], dtype=np.uint16)
+    np.testing.assert_array_equal(mine, t)
+
+
+def test_bf16_split_and_bf16x3_gemm():
+    L = oracle.lib()
+    rng = np.random.default_rng(10)
+    hi, lo = ctypes.c_uint16(), ctypes.c_uint16()
+
+    def f(h):
+        return np.frombuffer(np.array([h << 16], dtype=np.uint32).tobytes(), dtype=np.float32)[0]
+    for v in rng.uniform(-1, 1, 500).astype(np.float32):
+        L.oracle_bf16_split(float(v), ctypes.byref(hi), ctypes.byref(lo))
+        r = float(v) - float(f(hi.value)) - float(f(lo.value))
+        assert abs(r) <= 2.0 ** -17 * abs(float(v)) + 1e-45
+    # inputs already in BF16: lo = 0 and the product is exact (Dot2 == bf16x3)
+    m, n, k = 9, 11, 40
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(np.float32))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)).astype(np.float32))
+    Ab = np.vectorize(lambda v: f(L.oracle_bf16_rne(float(v))))(A).astype(np.float32)
+    Bb = np.vectorize(lambda v: f(L.oracle_bf16_rne(float(v))))(B).astype(np.float32)
+    Ab, Bb = np.asfortranarray(Ab), np.asfortranarray(Bb)
+    C1 = np.zeros((m, n), dtype=np.float32, order="F")
+    C2 = np.zeros((m, n), dtype=np.float32, order="F")
+    oracle.bf16x3_gemm("N", "N", m, n, k, 1.0, Ab, m, Bb, k, 0.0, C1, m)
+    oracle.gemm("N", "N", m, n, k, 1.0, Ab, m, Bb, k, 0.0, C2, m)
+    np.testing.assert_array_equal(C1, C2)
+    # general FP32 inputs: error ~2^-16 relative, well above FP32 rounding
+    C3 = np.zeros((m, n), dtype=np.float32, order="F")
+    oracle.bf16x3_gemm("T", "N", m, n, k, 1.0, np.asfortranarray(A.T), k, B, k, 0.0, C3, m)
+    R = op(A, "N", m, k).astype(np.float64) @ B.astype(np.float64)
+    bound = np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)
+    assert max_comp(C3, R, bound) <= 3 * 2.0 ** -16
