@@ -246,9 +246,9 @@ static void fortran_call(call_t *x, int mode, int slices) {
                      .ldb = (int64_t)ldb2, .ldc = (int64_t)ldc2,
                      .alpha = x->alpha, .beta = x->beta, .a = x->a, .b = x->b,
                      .c = x->c};
-  /* The legacy default stream orders this call after work the caller left
+  /* The null (default) stream orders this call after work the caller left
    * on blocking streams; the call waits for it before returning. */
-  hipStream_t s = hipStreamLegacy;
+  hipStream_t s = NULL;
   devstate_t *d = &g_dev[dev];
   pthread_mutex_lock(&d->lock);
   char *base = NULL;

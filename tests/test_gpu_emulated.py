@@ -1,0 +1,106 @@
+"""Parity of the emulated forms on the GPU.
+
+Ozaki-I (dgemm_x_ozaki_dev): every INT8 slice product is exact and every
+FP64 step is the same operation as in the oracle's restatement, so the GPU
+result must be bit-identical to oracle.ozaki_gemm for every transpose pair,
+slice count and ragged shape.
+
+split-BF16x3 (sgemm_x_bf16x3_dev): the split is exact integer work, the three
+products accumulate in FP32 inside the MFMA in an order the oracle cannot
+follow, so the GPU is compared with the oracle's exactly-accumulated BF16x3
+within FP32 accumulation error (64*k*2^-24 componentwise), and with the FP64
+product within the split error (3*2^-16 componentwise).
+"""
+import numpy as np
+import pytest
+
+import gemm_avx2_amd as gx
+import oracle
+from gemm_util import U32, abs_bound, from_device, max_comp, rel_fro, stored_shape, to_device
+
+pytestmark = pytest.mark.gpu
+TRANS = [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")]
+
+
+def _ozaki_gpu(torch, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s):
+    dA, dB, dC = to_device(A, torch), to_device(B, torch), to_device(C, torch)
+    gx.dgemm_ozaki_dev(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, s)
+    torch.cuda.synchronize()
+    return from_device(dC, C.shape)
+
+
+@pytest.mark.parametrize("s", [3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_ozaki_bit_exact(cuda, ta, tb, s):
+    import torch
+    rng = np.random.default_rng(100 + s)
+    for (m, n, k, pad) in [(1, 1, 1, 0), (17, 9, 33, 1), (130, 129, 200, 2), (64, 256, 513, 0)]:
+        ra, ca = stored_shape(ta, m, k)
+        rb, cb = stored_shape(tb, k, n)
+        A = np.asfortranarray(rng.uniform(-1, 1, (ra + pad, ca)) * np.exp(2 * rng.standard_normal((ra + pad, ca))))
+        B = np.asfortranarray(rng.uniform(-1, 1, (rb + pad, cb)))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + pad, n)))
+        C[m:] = 555.0
+        R = C.copy(order="F")
+        oracle.ozaki_gemm(ta, tb, m, n, k, 1.5, A, ra + pad, B, rb + pad, -0.5, R, m + pad, s)
+        G = _ozaki_gpu(torch, ta, tb, m, n, k, 1.5, A, ra + pad, B, rb + pad, -0.5, C, m + pad, s)
+        np.testing.assert_array_equal(G, R)
+
+
+def test_ozaki_nonfinite_and_semantics(cuda):
+    import torch
+    m, n, k = 5, 4, 6
+    A = np.ones((m, k), order="F")
+    A[2, 3] = np.inf
+    B = np.ones((k, n), order="F")
+    C = np.zeros((m, n), order="F")
+    G = _ozaki_gpu(torch, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m, 6)
+    assert np.all(np.isnan(G[2])) and np.all(G[[0, 1, 3, 4]] == 6.0)
+    # alpha = 0: C = beta*C without touching A, B
+    C = np.full((m, n), 3.0, order="F")
+    G = _ozaki_gpu(torch, "N", "N", m, n, k, 0.0, A, m, B, k, 2.0, C, m, 6)
+    assert np.all(G == 6.0)
+
+
+def test_ozaki_accuracy_vs_dot2(cuda):
+    """Error against the Dot2 product shrinks with the slice count and at 8
+    slices reaches FP64-level error for well-scaled inputs."""
+    import torch
+    rng = np.random.default_rng(5)
+    m, n, k = 96, 80, 700
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    R = np.zeros((m, n), order="F")
+    oracle.gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m)
+    errs = []
+    for s in range(3, 9):
+        G = _ozaki_gpu(torch, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0, np.zeros((m, n), order="F"), m, s)
+        errs.append(rel_fro(G, R))
+    assert all(b < a for a, b in zip(errs, errs[1:-1])), errs
+    assert errs[-1] < 1e-15, errs
+
+
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_bf16x3(cuda, ta, tb):
+    import torch
+    rng = np.random.default_rng(31)
+    for (m, n, k, pad) in [(1, 1, 1, 0), (19, 33, 45, 3), (130, 129, 257, 1), (128, 256, 512, 0)]:
+        ra, ca = stored_shape(ta, m, k)
+        rb, cb = stored_shape(tb, k, n)
+        A = np.asfortranarray(rng.uniform(-1, 1, (ra + pad, ca)).astype(np.float32))
+        B = np.asfortranarray(rng.uniform(-1, 1, (rb + pad, cb)).astype(np.float32))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + pad, n)).astype(np.float32))
+        C[m:] = 555.0
+        R3 = C.copy(order="F")
+        oracle.bf16x3_gemm(ta, tb, m, n, k, 1.5, A, ra + pad, B, rb + pad, -0.5, R3, m + pad)
+        R64 = C.astype(np.float64, order="F")
+        oracle.gemm(ta, tb, m, n, k, 1.5, A.astype(np.float64, order="F"), ra + pad,
+                    B.astype(np.float64, order="F"), rb + pad, -0.5, R64, m + pad)
+        dA, dB, dC = to_device(A, torch), to_device(B, torch), to_device(C, torch)
+        gx.sgemm_bf16x3_dev(ta, tb, m, n, k, 1.5, dA, ra + pad, dB, rb + pad, -0.5, dC, m + pad)
+        torch.cuda.synchronize()
+        G = from_device(dC, C.shape)
+        bound = abs_bound(A, B, C, ta, tb, m, n, k, 1.5, -0.5)
+        assert max_comp(G[:m], R3[:m], bound) <= 64 * k * U32, (m, n, k)
+        assert max_comp(G[:m], R64[:m], bound) <= 3 * 2.0 ** -16 + 64 * k * U32, (m, n, k)
+        assert np.all(G[m:] == 555.0)
