@@ -120,12 +120,14 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const int col = n0 + wn * WTN + j * 16 + li;
-      const int row = m0 + wm * WTM + i * 16 + kg * 4;
+      const int row0 = m0 + wm * WTM + i * 16;
       if (col < N) {
-        T *cp = C + row + col * ldc;
+        T *cp = C + row0 + col * ldc;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (row + r < M) cp[r] = fma_t(alpha, acc[i][j][r], beta * cp[r]);
+        for (int r = 0; r < 4; ++r) {
+          const int dr = Mfma16<T>::row(kg, r);
+          if (row0 + dr < M) cp[dr] = fma_t(alpha, acc[i][j][r], beta * cp[dr]);
+        }
       }
     }
 }

@@ -24,22 +24,27 @@ typedef float f16_t __attribute__((ext_vector_type(16)));
 typedef int i4_t __attribute__((ext_vector_type(4)));
 typedef int i16_t __attribute__((ext_vector_type(16)));
 
-/* 16x16x4 MFMA for the native types.  Operand layout (CDNA3/CDNA4):
+/* 16x16x4 MFMA for the native types.  Operand layout on gfx950 (checked on
+ * the device with one-hot operands):
  *   A (16x4):  lane l holds A[l%16][l/16]
  *   B (4x16):  lane l holds B[l/16][l%16]
- *   D (16x16): lane l, element r holds D[4*(l/16)+r][l%16]            */
+ *   D (16x16): lane l, element r holds D[row(l/16, r)][l%16] with
+ *              FP32: row = 4*(l/16) + r    (four consecutive rows)
+ *              FP64: row = (l/16) + 4*r    (rows interleaved by 4)         */
 template <typename T> struct Mfma16;
 template <> struct Mfma16<double> {
   typedef d4_t acc_t;
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int group, int r) { return group + 4 * r; }
 };
 template <> struct Mfma16<float> {
   typedef f4_t acc_t;
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int group, int r) { return 4 * group + r; }
 };
 
 template <typename T> struct Vec16;
