@@ -247,9 +247,11 @@ def main():
             _, per = time_launches(torch, f, 3, 1)
             sg["native_" + ta + tb] = round(flops / float(np.mean(per)) / 1e9, 2)
 
+        wsb = torch.empty(gx.sgemm_bf16x3_workspace(M, N, K), dtype=torch.uint8, device=dev)
+
         def fb():
             gx.sgemm_bf16x3_dev("N", "N", M, N, K, ALPHA, sA, LD, sB, LD, BETA, sC, LD,
-                                stream=stream)
+                                wsb, wsb.numel(), stream=stream)
         _, per = time_launches(torch, fb, 3, 1)
         sg["bf16x3_NN"] = round(flops / float(np.mean(per)) / 1e9, 2)
         result["sgemm_tflops"] = sg

@@ -24,7 +24,7 @@ import os
 
 __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
-    "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "sgemm_bf16x3_dev",
+    "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "last_error", "fill",
     "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
 ]
@@ -79,11 +79,14 @@ def _declare(lib, dtype):
     else:
         lib.sgemm_x_set_mode.restype = ctypes.c_int
         lib.sgemm_x_set_mode.argtypes = [ctypes.c_int]
+        lib.sgemm_x_bf16x3_workspace.restype = ctypes.c_size_t
+        lib.sgemm_x_bf16x3_workspace.argtypes = [ctypes.c_int] * 3
         lib.sgemm_x_bf16x3_dev.restype = ctypes.c_int
         lib.sgemm_x_bf16x3_dev.argtypes = [
             ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
-            ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+            ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+            ctypes.c_void_p]
 
 
 def load(name: str):
@@ -206,12 +209,17 @@ def dgemm_ozaki_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc
         _check("d", rc)
 
 
+def sgemm_bf16x3_workspace(m, n, k) -> int:
+    return int(load("SGEMM.so").sgemm_x_bf16x3_workspace(m, n, k))
+
+
 def sgemm_bf16x3_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
-                     stream=None):
+                     work=None, work_bytes=0, stream=None):
     """Emulated SGEMM (split-BF16x3) on device pointers."""
     lib = load("SGEMM.so")
     rc = lib.sgemm_x_bf16x3_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda,
-                                _addr(b), ldb, beta, _addr(c), ldc, _stream(stream))
+                                _addr(b), ldb, beta, _addr(c), ldc, _addr(work), work_bytes,
+                                _stream(stream))
     if rc:
         _check("s", rc)
 

@@ -201,7 +201,7 @@ static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "ozaki kernel launch");
   }
   if (es == 4 && mode == GEMMX_MODE_BF16X3) {
-    e = gemmx_bf16x3_f32(p, s);
+    e = gemmx_bf16x3_f32(p, work, s);
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "bf16x3 kernel launch");
   }
   const int cfg = gemmx_tune_pick(es, p->ta, p->tb, p->m, p->n, p->k);
@@ -209,9 +209,14 @@ static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
   return e == hipSuccess ? GEMMX_OK : hip_fail(e, "native kernel launch");
 }
 
-static size_t ozaki_ws(const call_t *x, int mode, int slices) {
-  if (x->es != 8 || mode != GEMMX_MODE_OZAKI || x->alpha == 0.0 || x->k == 0) return 0;
-  return gemmx_ozaki_workspace(x->m, x->n, x->k, slices);
+/* Device scratch of the emulated forms (0 for the native path). */
+static size_t emu_ws(const call_t *x, int mode, int slices) {
+  if (x->alpha == 0.0 || x->k == 0 || x->m == 0 || x->n == 0) return 0;
+  if (x->es == 8 && mode == GEMMX_MODE_OZAKI)
+    return gemmx_ozaki_workspace(x->m, x->n, x->k, slices);
+  if (x->es == 4 && mode == GEMMX_MODE_BF16X3)
+    return gemmx_bf16x3_workspace(x->m, x->n, x->k);
+  return 0;
 }
 
 /* The Fortran-ABI path: synchronous, host or device operands. */
@@ -238,7 +243,7 @@ static void fortran_call(call_t *x, int mode, int slices) {
   const size_t bytes_a = host_a ? round_up(lda2 * ca * es, 256) : 0;
   const size_t bytes_b = host_b ? round_up(ldb2 * cb * es, 256) : 0;
   const size_t bytes_c = host_c ? round_up(ldc2 * x->n * es, 256) : 0;
-  const size_t bytes_w = round_up(ozaki_ws(x, mode, slices), 256);
+  const size_t bytes_w = round_up(emu_ws(x, mode, slices), 256);
   const size_t total = bytes_a + bytes_b + bytes_c + bytes_w;
 
   gemmx_problem p = {.ta = !notrans(x->ta), .tb = !notrans(x->tb), .m = x->m,
@@ -367,7 +372,7 @@ static int dev_call(call_t *x, int mode, int slices, void *work, size_t work_byt
                      .ldc = x->ldc, .alpha = x->alpha, .beta = x->beta,
                      .a = x->a, .b = x->b, .c = x->c};
   hipStream_t s = (hipStream_t)stream;
-  const size_t need = ozaki_ws(x, mode, slices);
+  const size_t need = emu_ws(x, mode, slices);
   if (need == 0 || work) {
     if (need > 0 && work_bytes < need)
       return set_error(GEMMX_ERR_ARG, "workspace of %zu bytes < %zu needed", work_bytes, need);
@@ -419,10 +424,16 @@ EXPORT int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
   return dev_call(&x, GEMMX_MODE_OZAKI, slices, work, work_bytes, stream);
 }
 
+EXPORT size_t sgemm_x_bf16x3_workspace(int m, int n, int k) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  return gemmx_bf16x3_workspace(m, n, k);
+}
+
 EXPORT int sgemm_x_bf16x3_dev(char transa, char transb, int m, int n, int k,
                               float alpha, const float *a, int lda,
                               const float *b, int ldb, float beta, float *c,
-                              int ldc, void *stream) {
+                              int ldc, void *work, size_t work_bytes,
+                              void *stream) {
   call_t x = {4, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
-  return dev_call(&x, GEMMX_MODE_BF16X3, 0, NULL, 0, stream);
+  return dev_call(&x, GEMMX_MODE_BF16X3, 0, work, work_bytes, stream);
 }

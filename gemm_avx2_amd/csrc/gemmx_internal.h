@@ -53,15 +53,17 @@ hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta,
 hipError_t gemmx_scale_f32(float *c, int64_t ldc, int m, int n, float beta,
                            hipStream_t s);
 
-/* Emulated DGEMM, Ozaki-I INT8 slices (gemm_ozaki.hip).  `work` is device
+/* Emulated DGEMM, Ozaki-I INT8 slices (gemm_slices.hip).  `work` is device
  * scratch of gemmx_ozaki_workspace() bytes. */
 size_t gemmx_ozaki_workspace(int m, int n, int k, int slices);
 int gemmx_ozaki_kchunk(int slices);
 hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int slices, void *work,
                            hipStream_t s);
 
-/* Emulated SGEMM, split-BF16x3 (gemm_bf16x3.hip). */
-hipError_t gemmx_bf16x3_f32(const gemmx_problem *p, hipStream_t s);
+/* Emulated SGEMM, split-BF16x3 (gemm_slices.hip).  `work` is device scratch
+ * of gemmx_bf16x3_workspace() bytes. */
+size_t gemmx_bf16x3_workspace(int m, int n, int k);
+hipError_t gemmx_bf16x3_f32(const gemmx_problem *p, void *work, hipStream_t s);
 
 #ifdef __cplusplus
 }

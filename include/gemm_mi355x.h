@@ -99,10 +99,14 @@ int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
                       void *work, size_t work_bytes, void *stream);
 
 /* Emulated SGEMM on device pointers: split-BF16x3 (a = hi + lo in BF16,
- * C ~ hi*hi + hi*lo + lo*hi) on BF16 MFMA with FP32 accumulation. */
+ * C ~ hi*hi + hi*lo + lo*hi) on BF16 MFMA with FP32 accumulation.  `work` is
+ * device scratch of at least sgemm_x_bf16x3_workspace() bytes, or NULL to
+ * use the library's own (then the call synchronises `stream`). */
+size_t sgemm_x_bf16x3_workspace(int m, int n, int k);
 int sgemm_x_bf16x3_dev(char transa, char transb, int m, int n, int k,
                        float alpha, const float *a, int lda, const float *b,
-                       int ldb, float beta, float *c, int ldc, void *stream);
+                       int ldb, float beta, float *c, int ldc, void *work,
+                       size_t work_bytes, void *stream);
 
 #ifdef __cplusplus
 }

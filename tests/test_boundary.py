@@ -98,5 +98,9 @@ def test_mode_switches():
     assert gx.load("SGEMM.so").sgemm_x_set_mode(gx.MODE_OZAKI) == gx.ERR_ARG
     assert gx.load("SGEMM.so").sgemm_x_set_mode(gx.MODE_NATIVE) == gx.OK
     ws = gx.dgemm_ozaki_workspace(8192, 8192, 8192, 8)
-    # exponents + 8 slices of A and B, K padded to 128
-    assert ws >= 8 * 2 * 8192 * 8192 and ws < 8 * 2 * 8192 * 8192 + (1 << 20)
+    # exponents + 8 INT8 slices of A and B (K padded to 128) + 8 INT32 weight sums
+    need = 8 * 2 * 8192 * 8192 + 8 * 4 * 8192 * 8192
+    assert need <= ws < need + (1 << 20)
+    wb = gx.sgemm_bf16x3_workspace(1000, 700, 333)
+    # hi and lo BF16 slices of A and B, K padded to 64 elements
+    assert wb >= 2 * 2 * (1000 + 700) * 384
