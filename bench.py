@@ -189,8 +189,25 @@ def main():
         gx.dgemm_ozaki_dev("N", "N", M, N, K, ALPHA, dA, LD, dB, LD, BETA, dC, LD, s,
                            ws, ws.numel(), stream=stream)
 
-    if args.profile_only in ("native", "ozaki"):
-        fn = native if args.profile_only == "native" else ozaki
+    if args.profile_only:
+        if args.profile_only in ("native", "ozaki"):
+            fn = native if args.profile_only == "native" else ozaki
+        else:
+            del ws, dA, dB, dC, C0
+            sA, sB, sC = alloc(torch.float32), alloc(torch.float32), alloc(torch.float32)
+            for t, r, c, i in ((sA, M, K, 0), (sB, K, N, 1), (sC, M, N, 2)):
+                gx.fill(t, r, c, LD, 1, i, stream=stream)
+            if args.profile_only == "sgemm":
+                def fn():
+                    gx.sgemm_dev("N", "N", M, N, K, ALPHA, sA, LD, sB, LD, BETA, sC, LD,
+                                 stream=stream)
+            else:
+                wsb = torch.empty(gx.sgemm_bf16x3_workspace(M, N, K), dtype=torch.uint8,
+                                  device=dev)
+
+                def fn():
+                    gx.sgemm_bf16x3_dev("N", "N", M, N, K, ALPHA, sA, LD, sB, LD, BETA, sC,
+                                        LD, wsb, wsb.numel(), stream=stream)
         wall, per = time_launches(torch, fn, args.steps, args.warmup)
         print(json.dumps({"profile_only": args.profile_only, "ms": per}))
         return

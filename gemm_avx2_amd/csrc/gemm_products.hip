@@ -1,0 +1,201 @@
+/*
+ * gemm_products.hip -- slice-pair products on the INT8 and BF16 matrix cores
+ * for the emulated forms (no reference counterpart; see gemm_emulated.hip).
+ *
+ * One workgroup computes one BM x BN tile of one group: it walks the group's
+ * slice pairs and K stages, staging K-contiguous slice tiles through LDS
+ * (double-buffered, the next stage's global loads in flight during the
+ * current stage's MFMAs), and multiplies them with
+ *   INT8: v_mfma_i32_16x16x64_i8   -> exact INT32 sums
+ *   BF16: v_mfma_f32_16x16x32_bf16 -> FP32 sums
+ * Both consume 64 K-bytes per lane-group step: lane l holds row l%16 and
+ * bytes [16*(l/16), 16*(l/16)+16) of each 64-byte K chunk, for A and B
+ * alike, so the hardware's order of those bytes cancels out.  D: lane l,
+ * element r is row 4*(l/16)+r, column l%16.
+ * Grid: groups x tiles, heaviest group first (the host orders the groups).
+ */
+#include <stdlib.h>
+
+#include "gemmx_device.h"
+#include "gemmx_slices.h"
+
+namespace gemmx {
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+template <bool BF16> struct SliceMfma;
+template <> struct SliceMfma<false> {
+  typedef i4_t acc_t;
+  static __device__ __forceinline__ acc_t mma(i4_t a, i4_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct SliceMfma<true> {
+  typedef f4_t acc_t;
+  static __device__ __forceinline__ acc_t mma(i4_t a, i4_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                   __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+  }
+};
+
+template <bool BF16, int BM, int BN, int WM, int WN, int BKB>
+__global__ void __launch_bounds__(64 * WM * WN)
+slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
+                      int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
+                      SliceGroups G, int tiles_m, int tiles_n, int *__restrict__ S,
+                      float *__restrict__ C, int64_t ldc, float alpha, float beta) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  constexpr int LEN = BKB / 4; /* ints per LDS line: one K stage of BKB bytes */
+  constexpr int LS = LEN + 4;  /* +16 bytes against bank conflicts */
+  constexpr int KSTEPS = BKB / 64;
+  typedef SliceMfma<BF16> MM;
+  typedef typename MM::acc_t acc_t;
+  extern __shared__ __align__(16) int smem[];
+  /* LDS: A stage 0 | A stage 1 | B stage 0 | B stage 1 */
+  auto sA = [&](int st) { return smem + st * (BM * LS); };
+  auto sB = [&](int st) { return smem + 2 * BM * LS + st * (BN * LS); };
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int li = lane & 15, kg = lane >> 4;
+  const int ntiles = tiles_m * tiles_n;
+  const int g = blockIdx.x / ntiles;
+  int tm, tn;
+  tile_coords(blockIdx.x - g * ntiles, ntiles, tiles_m, tiles_n, &tm, &tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int64_t ld_i = kpad_bytes / 4;
+  const int pb = G.gbeg[g], pc = G.gcnt[g];
+
+  TileCopy<int, BM, LEN, NT> ca;
+  TileCopy<int, BN, LEN, NT> cb;
+  const int kt_begin = kb_begin / BKB;
+  const int nkt = (kb_end - kb_begin) / BKB;
+  const int iters = pc * nkt;
+  auto load_iter = [&](int q) {
+    const int pi = pb + q / nkt, kt = kt_begin + q % nkt;
+    const int t = G.pair_t[pi], u = G.pair_u[pi];
+    const int *ga = reinterpret_cast<const int *>(As + ((int64_t)t * M + m0) * kpad_bytes) + kt * LEN;
+    const int *gb = reinterpret_cast<const int *>(Bs + ((int64_t)u * N + n0) * kpad_bytes) + kt * LEN;
+    ca.load(ga, ld_i, M - m0, LEN, true, tid);
+    cb.load(gb, ld_i, N - n0, LEN, true, tid);
+  };
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  load_iter(0);
+  ca.template store<LS>(sA(0), tid);
+  cb.template store<LS>(sB(0), tid);
+  __syncthreads();
+
+  for (int q = 0; q < iters; ++q) {
+    const int cur = q & 1;
+    if (q + 1 < iters) load_iter(q + 1);
+    const int *as = sA(cur);
+    const int *bs = sB(cur);
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+      const int koff = ks * 16 + kg * 4;
+      i4_t af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        af[i] = *reinterpret_cast<const i4_t *>(as + (wm * WTM + i * 16 + li) * LS + koff);
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bf[j] = *reinterpret_cast<const i4_t *>(bs + (wn * WTN + j * 16 + li) * LS + koff);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MM::mma(af[i], bf[j], acc[i][j]);
+    }
+    if (q + 1 < iters) {
+      ca.template store<LS>(sA(cur ^ 1), tid);
+      cb.template store<LS>(sB(cur ^ 1), tid);
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * WTN + j * 16 + li;
+      const int row = m0 + wm * WTM + i * 16 + kg * 4;
+      if (col >= N) continue;
+      if (BF16) {
+        float *cp = C + row + col * ldc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row + r < M) cp[r] = __builtin_fmaf(alpha, (float)acc[i][j][r], beta * cp[r]);
+      } else {
+        int *sp = S + (int64_t)G.gout[g] * M * N + row + (int64_t)col * M;
+        if (row + 3 < M && (M & 3) == 0) {
+          *reinterpret_cast<i4_t *>(sp) = *reinterpret_cast<i4_t *>(&acc[i][j]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (row + r < M) sp[r] = (int)acc[i][j][r];
+        }
+      }
+    }
+}
+
+template <bool BF16, int BM, int BN, int WM, int WN, int BKB>
+hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
+                      int kb0, int kb1, const SliceGroups &G, int *S, float *C,
+                      int64_t ldc, float alpha, float beta, hipStream_t st) {
+  auto kern = slice_products_kernel<BF16, BM, BN, WM, WN, BKB>;
+  const size_t lds = (size_t)2 * (BM + BN) * (BKB / 4 + 4) * 4;
+  hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+  hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n * G.ngroups), dim3(64 * WM * WN), lds,
+                     st, as, bs, m, n, kpad, kb0, kb1, G, tiles_m, tiles_n, S, C, ldc,
+                     alpha, beta);
+  return hipGetLastError();
+}
+
+/* Tile configuration: 0 = 256x256 with 8 waves of 128x64 and 128-byte K
+ * stages, 1 = 256x256 with 4 waves of 128x128 and 64-byte K stages.
+ * GEMMX_SLICE_CFG overrides the default. */
+int slice_cfg() {
+  static int cfg = -2;
+  if (cfg == -2) {
+    const char *e = getenv("GEMMX_SLICE_CFG");
+    cfg = (e && *e) ? atoi(e) : 1;
+    if (cfg < 0 || cfg > 1) cfg = 1;
+  }
+  return cfg;
+}
+
+template <bool BF16>
+hipError_t products(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
+                    int kb0, int kb1, const SliceGroups &G, int *S, float *C,
+                    int64_t ldc, float alpha, float beta, hipStream_t st) {
+  if (slice_cfg() == 0)
+    return launch_cfg<BF16, 256, 256, 2, 4, 128>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+                                                 ldc, alpha, beta, st);
+  return launch_cfg<BF16, 256, 256, 2, 2, 64>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc,
+                                              alpha, beta, st);
+}
+
+} // namespace
+
+hipError_t slice_products(bool bf16, const int8_t *as, const int8_t *bs, int m, int n,
+                          int64_t kpad_bytes, int kb0, int kb1, const SliceGroups &g,
+                          int *S, float *C, int64_t ldc, float alpha, float beta,
+                          hipStream_t st) {
+  return bf16 ? products<true>(as, bs, m, n, kpad_bytes, kb0, kb1, g, S, C, ldc, alpha,
+                               beta, st)
+              : products<false>(as, bs, m, n, kpad_bytes, kb0, kb1, g, S, C, ldc, alpha,
+                                beta, st);
+}
+
+} // namespace gemmx

@@ -1,0 +1,298 @@
+/*
+ * gemm_split.hip -- the split pass of the emulated forms: every row of op(A)
+ * and column of op(B) is cut into low-precision slices stored K-contiguous
+ * (see gemmx_slices.h), with any transpose folded into this pass.
+ *
+ * Ozaki-I (FP64 -> INT8): per vector the exponent e with max|x| < 2^e, then s
+ * signed 7-bit digits of x*2^-e by truncation toward zero,
+ * x*2^-e = sum_t d_t 2^-7t + r, |d_t| <= 127; every step is exact in FP64.
+ * BF16x3 (FP32 -> 2 x BF16): hi = bf16_rne(x), lo = bf16_rne(x - hi), the
+ * difference being exact in FP32.
+ * No reference counterpart (the reference packs A and B for its AVX2
+ * kernels, wjc404/GEMM_AVX2 src/gemm_copy.c:4-143, without changing their
+ * precision).
+ */
+#include "gemmx_device.h"
+#include "gemmx_slices.h"
+
+namespace gemmx {
+namespace {
+
+__device__ __forceinline__ int exponent_of_absbits(unsigned long long b) {
+  if (b >= 0x7ff0000000000000ull) return OZ_NONFINITE;
+  if (b == 0) return 0;
+  int e;
+  (void)frexp(__longlong_as_double((long long)b), &e);
+  return e;
+}
+
+/* Exponent per vector with elements contiguous (x[v*vld + p]): a wave per
+ * vector. */
+__global__ void __launch_bounds__(256)
+exp_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len,
+                  int *__restrict__ out) {
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (v >= nvec) return;
+  const double *p = x + v * vld;
+  unsigned long long mx = 0;
+  for (int i = lane; i < len; i += 64) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(p[i]) & 0x7fffffffffffffffull;
+    mx = b > mx ? b : mx;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mx, off, 64);
+    mx = o > mx ? o : mx;
+  }
+  if (lane == 0) out[v] = exponent_of_absbits(mx);
+}
+
+/* Max |x| bits per vector with vectors adjacent (x[v + p*eld]).  Block
+ * (64 vectors) x (256 K entries): each thread scans 64 entries of one vector
+ * (a wave reads 64 adjacent vectors per entry), the block reduces over its 4
+ * K quarters and folds the result into maxbits[v] with an atomic max.
+ * maxbits must start at zero. */
+__global__ void __launch_bounds__(256)
+absmax_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec, int len,
+                      unsigned long long *__restrict__ maxbits) {
+  __shared__ unsigned long long part[4][64];
+  const int vl = threadIdx.x & 63, kq = threadIdx.x >> 6;
+  const int v = blockIdx.x * 64 + vl;
+  const int p0 = blockIdx.y * 256 + kq * 64;
+  unsigned long long mx = 0;
+  if (v < nvec) {
+    const int p1 = p0 + 64 < len ? p0 + 64 : len;
+    for (int p = p0; p < p1; ++p) {
+      const unsigned long long b = (unsigned long long)__double_as_longlong(x[v + p * eld]) & 0x7fffffffffffffffull;
+      mx = b > mx ? b : mx;
+    }
+  }
+  part[kq][vl] = mx;
+  __syncthreads();
+  if (kq == 0 && v < nvec) {
+#pragma unroll
+    for (int q = 1; q < 4; ++q) mx = part[q][vl] > mx ? part[q][vl] : mx;
+    atomicMax(maxbits + v, mx);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+zero_u64_kernel(unsigned long long *__restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(256)
+exp_from_bits_kernel(const unsigned long long *__restrict__ maxbits, int nvec,
+                     int *__restrict__ out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nvec) out[v] = exponent_of_absbits(maxbits[v]);
+}
+
+/* 16 consecutive K entries of one vector per thread -> 16 bytes per slice.
+ * v_fast: consecutive threads take consecutive vectors (strided input). */
+template <int S>
+__global__ void __launch_bounds__(256)
+ozaki_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld,
+                   int nvec, int len, int64_t kpad, const int *__restrict__ ex,
+                   int v_fast, int8_t *__restrict__ out) {
+  const int64_t nblk = kpad / 16;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (tid >= nblk * nvec) return;
+  int64_t v, blk;
+  if (v_fast) { v = tid % nvec; blk = tid / nvec; }
+  else        { v = tid / nblk; blk = tid % nblk; }
+  const int e = ex[v];
+  i4_t dig[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t) dig[t] = i4_t{0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int64_t p = blk * 16 + q;
+    const double val = (p < len) ? x[v * vld + p * eld] : 0.0;
+    double r = (e == OZ_NONFINITE) ? 0.0 : ldexp(val, -e);
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      const double y = r * 128.0;
+      const double d = trunc(y);
+      r = y - d;
+      const unsigned byte = (unsigned)((int)d) & 0xffu;
+      dig[t][q >> 2] |= (int)(byte << (8 * (q & 3)));
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < S; ++t)
+    *reinterpret_cast<i4_t *>(out + ((int64_t)t * nvec + v) * kpad + blk * 16) = dig[t];
+}
+
+/* Same digits for vectors adjacent in memory (x[v + p*eld]), through an LDS
+ * transpose so that both the reads (64 adjacent vectors per entry) and the
+ * writes (128 contiguous bytes of one vector per slice) coalesce.  Block:
+ * 32 vectors x 128 K entries.  LDS row of a vector keeps entry p at
+ * (p % 16) * 8 + p / 16, so the 8 threads of one vector read 8 adjacent
+ * words for each of their 16 entries. */
+template <int S>
+__global__ void __launch_bounds__(256)
+ozaki_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
+                           int len, int64_t kpad, const int *__restrict__ ex,
+                           int8_t *__restrict__ out) {
+  constexpr int ROW = 129; /* 128 entries + 1 against bank conflicts */
+  __shared__ double tile[32 * ROW];
+  const int v0 = blockIdx.x * 32;
+  const int64_t p0 = (int64_t)blockIdx.y * 128;
+  {
+    const int vl = threadIdx.x & 31, pr = threadIdx.x >> 5;
+    const int v = v0 + vl;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int p = pr + 8 * j;
+      const double val = (v < nvec && p0 + p < len) ? x[v + (p0 + p) * eld] : 0.0;
+      tile[vl * ROW + (p % 16) * 8 + p / 16] = val;
+    }
+  }
+  __syncthreads();
+  const int vl = threadIdx.x >> 3, q = threadIdx.x & 7;
+  const int v = v0 + vl;
+  if (v >= nvec) return;
+  const int e = ex[v];
+  i4_t dig[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t) dig[t] = i4_t{0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const double val = tile[vl * ROW + i * 8 + q];
+    double r = (e == OZ_NONFINITE) ? 0.0 : ldexp(val, -e);
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      const double y = r * 128.0;
+      const double d = trunc(y);
+      r = y - d;
+      const unsigned byte = (unsigned)((int)d) & 0xffu;
+      dig[t][i >> 2] |= (int)(byte << (8 * (i & 3)));
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < S; ++t)
+    *reinterpret_cast<i4_t *>(out + ((int64_t)t * nvec + v) * kpad + p0 + 16 * q) = dig[t];
+}
+
+__device__ __forceinline__ unsigned bf16_rne_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0x007fffffu) ? 0x40u : 0u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+/* 8 consecutive K entries of one vector per thread -> 16 bytes of hi and of
+ * lo.  out is [2][nvec][kpad] bf16 (hi slice first). */
+__global__ void __launch_bounds__(256)
+bf16_split_kernel(const float *__restrict__ x, int64_t vld, int64_t eld,
+                  int nvec, int len, int64_t kpad, int v_fast,
+                  unsigned short *__restrict__ out) {
+  const int64_t nblk = kpad / 8;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (tid >= nblk * nvec) return;
+  int64_t v, blk;
+  if (v_fast) { v = tid % nvec; blk = tid / nvec; }
+  else        { v = tid / nblk; blk = tid % nblk; }
+  i4_t hi, lo;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    unsigned h2[2], l2[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int64_t p = blk * 8 + 2 * q + e;
+      const float val = (p < len) ? x[v * vld + p * eld] : 0.0f;
+      h2[e] = bf16_rne_bits(val);
+      l2[e] = bf16_rne_bits(val - __uint_as_float(h2[e] << 16));
+    }
+    hi[q] = (int)(h2[0] | (h2[1] << 16));
+    lo[q] = (int)(l2[0] | (l2[1] << 16));
+  }
+  *reinterpret_cast<i4_t *>(out + v * kpad + blk * 8) = hi;
+  *reinterpret_cast<i4_t *>(out + ((int64_t)nvec + v) * kpad + blk * 8) = lo;
+}
+
+template <int S>
+hipError_t launch_ozaki_split(const double *x, int64_t vld, int64_t eld, int nvec,
+                              int len, int64_t kpad, const int *ex, int v_fast,
+                              int8_t *out, hipStream_t st) {
+  const int64_t threads = (kpad / 16) * nvec;
+  hipLaunchKernelGGL((ozaki_split_kernel<S>), dim3((unsigned)((threads + 255) / 256)),
+                     dim3(256), 0, st, x, vld, eld, nvec, len, kpad, ex, v_fast, out);
+  return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_split_strided(const double *x, int64_t eld, int nvec, int len,
+                                int64_t kpad, const int *ex, int8_t *out, hipStream_t st) {
+  hipLaunchKernelGGL((ozaki_split_strided_kernel<S>), dim3((nvec + 31) / 32, (unsigned)(kpad / 128)),
+                     dim3(256), 0, st, x, eld, nvec, len, kpad, ex, out);
+  return hipGetLastError();
+}
+
+hipError_t ozaki_split_strided(int s, const double *x, int64_t eld, int nvec, int len,
+                               int64_t kpad, const int *ex, int8_t *out, hipStream_t st) {
+  switch (s) {
+  case 3: return launch_split_strided<3>(x, eld, nvec, len, kpad, ex, out, st);
+  case 4: return launch_split_strided<4>(x, eld, nvec, len, kpad, ex, out, st);
+  case 5: return launch_split_strided<5>(x, eld, nvec, len, kpad, ex, out, st);
+  case 6: return launch_split_strided<6>(x, eld, nvec, len, kpad, ex, out, st);
+  case 7: return launch_split_strided<7>(x, eld, nvec, len, kpad, ex, out, st);
+  default: return launch_split_strided<8>(x, eld, nvec, len, kpad, ex, out, st);
+  }
+}
+
+hipError_t ozaki_split(int s, const double *x, int64_t vld, int64_t eld, int nvec,
+                       int len, int64_t kpad, const int *ex, int v_fast, int8_t *out,
+                       hipStream_t st) {
+  switch (s) {
+  case 3: return launch_ozaki_split<3>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
+  case 4: return launch_ozaki_split<4>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
+  case 5: return launch_ozaki_split<5>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
+  case 6: return launch_ozaki_split<6>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
+  case 7: return launch_ozaki_split<7>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
+  default: return launch_ozaki_split<8>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
+  }
+}
+
+/* Exponents and slices of op(A) rows (vectors = rows) or op(B) columns.
+ * contiguous: vector v at x + v*ld (elements adjacent); else at x + v with
+ * element stride ld. */
+
+} // namespace
+
+hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
+                         int k, int64_t kpad, int s, int *ex, unsigned long long *mbits,
+                         int8_t *slices, hipStream_t st) {
+  if (contiguous) {
+    hipLaunchKernelGGL(exp_contig_kernel, dim3((nvec + 3) / 4), dim3(256), 0, st, x,
+                       ld, nvec, k, ex);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return ozaki_split(s, x, ld, 1, nvec, k, kpad, ex, 0, slices, st);
+  }
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, mbits,
+                     nvec);
+  hipLaunchKernelGGL(absmax_strided_kernel, dim3((nvec + 63) / 64, (k + 255) / 256),
+                     dim3(256), 0, st, x, ld, nvec, k, mbits);
+  hipLaunchKernelGGL(exp_from_bits_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
+                     mbits, nvec, ex);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return ozaki_split_strided(s, x, ld, nvec, k, kpad, ex, slices, st);
+}
+
+hipError_t bf16_prepare(const float *x, int64_t ld, bool contiguous, int nvec,
+                        int k, int64_t kpad, unsigned short *out, hipStream_t st) {
+  const int64_t threads = (kpad / 8) * nvec;
+  const dim3 grid((unsigned)((threads + 255) / 256));
+  if (contiguous)
+    hipLaunchKernelGGL(bf16_split_kernel, grid, dim3(256), 0, st, x, ld, (int64_t)1,
+                       nvec, k, kpad, 0, out);
+  else
+    hipLaunchKernelGGL(bf16_split_kernel, grid, dim3(256), 0, st, x, (int64_t)1, ld,
+                       nvec, k, kpad, 1, out);
+  return hipGetLastError();
+}
+
+} // namespace gemmx

@@ -1,0 +1,58 @@
+/*
+ * gemmx_slices.h -- shared pieces of the emulated forms (split, products,
+ * recombination), which live in gemm_split.hip, gemm_products.hip and
+ * gemm_emulated.hip.  Internal to the library.
+ *
+ * Slices are stored K-contiguous: slice t of a matrix with `nvec` vectors
+ * (rows of op(A) or columns of op(B)) is nvec lines of kpad bytes, line v at
+ * base + (t*nvec + v)*kpad.  kpad is K padded with zeros to a multiple of
+ * SL_BK bytes.
+ */
+#ifndef GEMMX_SLICES_H
+#define GEMMX_SLICES_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace gemmx {
+
+constexpr int SL_BK = 128;               /* K padding granule in bytes */
+constexpr int OZ_NONFINITE = 0x7fffffff; /* exponent of a row/column with Inf/NaN */
+constexpr int MAX_PAIRS = 36;
+constexpr int MAX_GROUPS = 24;
+
+inline size_t rup(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+/* Slice-pair work of the products kernel.  Group g multiplies the pairs
+ * pair_t[b..b+c) x pair_u[b..b+c) (b = gbeg[g], c = gcnt[g]) into one
+ * accumulator and writes it to output slot gout[g]. */
+struct SliceGroups {
+  int ngroups;
+  signed char pair_t[MAX_PAIRS], pair_u[MAX_PAIRS];
+  signed char gbeg[MAX_GROUPS], gcnt[MAX_GROUPS], gout[MAX_GROUPS];
+};
+
+/* Exponents (per row of op(A) / column of op(B)) and s INT8 slices of an FP64
+ * operand.  contiguous: vector v at x + v*ld, entries adjacent; otherwise
+ * vector v at x + v with entry stride ld.  mbits: nvec words of scratch. */
+hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec, int k,
+                         int64_t kpad, int s, int *ex, unsigned long long *mbits,
+                         int8_t *slices, hipStream_t st);
+
+/* hi and lo BF16 slices ([2][nvec][kpad] bf16, kpad in elements) of an FP32
+ * operand, same vector convention. */
+hipError_t bf16_prepare(const float *x, int64_t ld, bool contiguous, int nvec, int k,
+                        int64_t kpad, unsigned short *out, hipStream_t st);
+
+/* Products of slice pairs over the K byte range [kb0, kb1) (multiples of
+ * SL_BK).  INT8 (bf16 = false): group g's INT32 sums go to
+ * S + gout[g]*m*n (column-major, ld m).  BF16 (bf16 = true): a single group;
+ * C = fma(alpha, acc, beta*C) with leading dimension ldc. */
+hipError_t slice_products(bool bf16, const int8_t *as, const int8_t *bs, int m, int n,
+                          int64_t kpad_bytes, int kb0, int kb1, const SliceGroups &g,
+                          int *S, float *C, int64_t ldc, float alpha, float beta,
+                          hipStream_t st);
+
+} // namespace gemmx
+#endif
