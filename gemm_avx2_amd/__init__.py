@@ -29,7 +29,8 @@ __all__ = [
     "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
 ]
 
-LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIB_DIR = os.environ.get("GEMMX_LIB_DIR") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "lib")
 
 MODE_NATIVE, MODE_OZAKI, MODE_BF16X3 = 0, 1, 2
 OK, ERR_ARG, ERR_NODEVICE, ERR_ALLOC, ERR_LAUNCH, ERR_UNSUPPORTED = range(6)

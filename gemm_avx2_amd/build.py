@@ -63,8 +63,10 @@ def _hipcc():
     return p if os.path.exists(p) else shutil.which("hipcc")
 
 
-def build(jobs: int = 8, verbose: bool = False) -> dict:
+def build(jobs: int = 8, verbose: bool = False, lib_dir: str = LIB, build_dir: str = BUILD,
+          extra_hip_flags=()) -> dict:
     """Compiles every source and links the libraries; returns their paths."""
+    LIB, BUILD = lib_dir, build_dir
     hipcc = _hipcc()
     if not hipcc:
         raise RuntimeError("hipcc not found (ROCm is required to build)")
@@ -76,7 +78,8 @@ def build(jobs: int = 8, verbose: bool = False) -> dict:
 
     steps = []
     for src in HIP_SOURCES + GEN_SOURCES:
-        steps.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj(src)])
+        steps.append([hipcc, *HIPCC_FLAGS, *extra_hip_flags, "-c", os.path.join(CSRC, src),
+                      "-o", obj(src)])
     for src in C_SOURCES:
         steps.append(["gcc", *CC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj(src)])
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:

@@ -66,17 +66,14 @@ OzLayout oz_layout(int m, int n, int k, int s) {
 SliceGroups ozaki_groups(int s) {
   SliceGroups G;
   G.ngroups = 0;
-  int np = 0;
   for (int w = s + 1; w >= 2; --w) {
     const int tlo = w - s > 1 ? w - s : 1, thi = w - 1 < s ? w - 1 : s;
-    G.gbeg[G.ngroups] = (signed char)np;
-    for (int t = tlo; t <= thi; ++t, ++np) {
-      G.pair_t[np] = (signed char)(t - 1);
-      G.pair_u[np] = (signed char)(w - t - 1);
+    int t[MAX_GROUP_PAIRS], u[MAX_GROUP_PAIRS], c = 0;
+    for (int ts = tlo; ts <= thi; ++ts, ++c) {
+      t[c] = ts - 1;
+      u[c] = w - ts - 1;
     }
-    G.gcnt[G.ngroups] = (signed char)(thi - tlo + 1);
-    G.gout[G.ngroups] = (signed char)(w - 2);
-    G.ngroups++;
+    G.add(w - 2, c, t, u);
   }
   return G;
 }
@@ -147,15 +144,9 @@ extern "C" hipError_t gemmx_bf16x3_f32(const gemmx_problem *p, void *work, hipSt
   if (e != hipSuccess) return e;
   /* slice 0 = hi, 1 = lo; one group: (lo,hi), (hi,lo), (hi,hi) */
   SliceGroups G;
-  G.ngroups = 1;
-  G.gbeg[0] = 0;
-  G.gcnt[0] = 3;
-  G.gout[0] = 0;
-  const signed char tt[3] = {1, 0, 0}, uu[3] = {0, 1, 0};
-  for (int i = 0; i < 3; ++i) {
-    G.pair_t[i] = tt[i];
-    G.pair_u[i] = uu[i];
-  }
+  G.ngroups = 0;
+  const int tt[3] = {1, 0, 0}, uu[3] = {0, 1, 0};
+  G.add(0, 3, tt, uu);
   const int64_t kpad_bytes = kpad * 2;
   return slice_products(true, (const int8_t *)as, (const int8_t *)bs, p->m, p->n, kpad_bytes,
                         0, (int)kpad_bytes, G, NULL, (float *)p->c, p->ldc, (float)p->alpha,

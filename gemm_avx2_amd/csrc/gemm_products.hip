@@ -39,7 +39,39 @@ template <> struct SliceMfma<true> {
   }
 };
 
-template <bool BF16, int BM, int BN, int WM, int WN, int BKB>
+/* Global->LDS copy of an NL x LEN-int slice tile (NL lines, line stride ld
+ * ints), 16 bytes per chunk.  Slices are zero-padded in K, so only lines can
+ * fall outside the matrix; those are clamped to the last valid line (their
+ * products land in rows/columns that are never stored).  No branches: every
+ * chunk is one 16-byte load whatever the edge, so the loads of a stage stay
+ * in flight together. */
+template <int NL, int LEN, int NT>
+struct SliceTileCopy {
+  static constexpr int CPL = LEN / 4;
+  static constexpr int PER = NL * CPL / NT;
+  static_assert((NL * CPL) % NT == 0, "chunks must divide evenly over the threads");
+  i4_t r[PER];
+
+  __device__ __forceinline__ void load(const int *__restrict__ g, int64_t ld, int nl_valid,
+                                       int tid) {
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const int ch = tid + c * NT, line = ch / CPL, off = (ch % CPL) * 4;
+      const int ln = line < nl_valid ? line : nl_valid - 1;
+      r[c] = *reinterpret_cast<const i4_t *>(g + ln * ld + off);
+    }
+  }
+  template <int LS>
+  __device__ __forceinline__ void store(int *s, int tid) const {
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const int ch = tid + c * NT, line = ch / CPL, off = (ch % CPL) * 4;
+      *reinterpret_cast<i4_t *>(s + line * LS + off) = r[c];
+    }
+  }
+};
+
+template <bool BF16, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
 __global__ void __launch_bounds__(64 * WM * WN)
 slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
                       int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
@@ -67,20 +99,25 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
   tile_coords(blockIdx.x - g * ntiles, ntiles, tiles_m, tiles_n, &tm, &tn);
   const int m0 = tm * BM, n0 = tn * BN;
   const int64_t ld_i = kpad_bytes / 4;
-  const int pb = G.gbeg[g], pc = G.gcnt[g];
+  /* Scalar (uniform) group data; the pair of an iteration is unpacked with
+   * shifts, so the loop issues no memory access besides the tile loads. */
+  const int pc = G.gcnt[g];
+  const unsigned tpack = G.tpack[g], upack = G.upack[g];
 
-  TileCopy<int, BM, LEN, NT> ca;
-  TileCopy<int, BN, LEN, NT> cb;
+  SliceTileCopy<BM, LEN, NT> ca;
+  SliceTileCopy<BN, LEN, NT> cb;
   const int kt_begin = kb_begin / BKB;
   const int nkt = (kb_end - kb_begin) / BKB;
   const int iters = pc * nkt;
+  const int8_t *a_tile = As + (int64_t)m0 * kpad_bytes;
+  const int8_t *b_tile = Bs + (int64_t)n0 * kpad_bytes;
   auto load_iter = [&](int q) {
-    const int pi = pb + q / nkt, kt = kt_begin + q % nkt;
-    const int t = G.pair_t[pi], u = G.pair_u[pi];
-    const int *ga = reinterpret_cast<const int *>(As + ((int64_t)t * M + m0) * kpad_bytes) + kt * LEN;
-    const int *gb = reinterpret_cast<const int *>(Bs + ((int64_t)u * N + n0) * kpad_bytes) + kt * LEN;
-    ca.load(ga, ld_i, M - m0, LEN, true, tid);
-    cb.load(gb, ld_i, N - n0, LEN, true, tid);
+    const int pl = q / nkt, kt = kt_begin + q - pl * nkt;
+    const int t = (tpack >> (4 * pl)) & 15, u = (upack >> (4 * pl)) & 15;
+    const int *ga = reinterpret_cast<const int *>(a_tile + (int64_t)t * M * kpad_bytes) + kt * LEN;
+    const int *gb = reinterpret_cast<const int *>(b_tile + (int64_t)u * N * kpad_bytes) + kt * LEN;
+    ca.load(ga, ld_i, M - m0, tid);
+    cb.load(gb, ld_i, N - n0, tid);
   };
 
   acc_t acc[FM][FN];
@@ -96,7 +133,9 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
 
   for (int q = 0; q < iters; ++q) {
     const int cur = q & 1;
-    if (q + 1 < iters) load_iter(q + 1);
+    /* Branch-free body: the last iteration reloads its own stage into the
+     * other buffer, which nobody reads any more. */
+    load_iter(q + 1 < iters ? q + 1 : q);
     const int *as = sA(cur);
     const int *bs = sB(cur);
 #pragma unroll
@@ -114,9 +153,32 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = MM::mma(af[i], bf[j], acc[i][j]);
     }
-    if (q + 1 < iters) {
-      ca.template store<LS>(sA(cur ^ 1), tid);
-      cb.template store<LS>(sB(cur ^ 1), tid);
+    ca.template store<LS>(sA(cur ^ 1), tid);
+    cb.template store<LS>(sB(cur ^ 1), tid);
+    if (PIPE) {
+      /* Issue order: the stage's global loads, the first K step's fragment
+       * reads, then every later K step's fragment reads spread over the
+       * previous step's MFMAs, and the LDS writes of the next stage spread
+       * over the last step's MFMAs. */
+      constexpr int NR = FM + FN, NMF = FM * FN;
+      constexpr int NW = SliceTileCopy<BM, LEN, NT>::PER + SliceTileCopy<BN, LEN, NT>::PER;
+      __builtin_amdgcn_sched_group_barrier(0x020, NW, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
+#pragma unroll
+      for (int ks = 1; ks < KSTEPS; ++ks) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          __builtin_amdgcn_sched_group_barrier(0x008, NMF / NR, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NR) * NR, 0);
+      }
+#pragma unroll
+      for (int w = 0; w < NW; ++w) {
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF / NW, 0);
+        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NW) * NW, 0);
     }
     __syncthreads();
   }
@@ -146,11 +208,11 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
     }
 }
 
-template <bool BF16, int BM, int BN, int WM, int WN, int BKB>
+template <bool BF16, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
 hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
                       int kb0, int kb1, const SliceGroups &G, int *S, float *C,
                       int64_t ldc, float alpha, float beta, hipStream_t st) {
-  auto kern = slice_products_kernel<BF16, BM, BN, WM, WN, BKB>;
+  auto kern = slice_products_kernel<BF16, BM, BN, WM, WN, BKB, PIPE>;
   const size_t lds = (size_t)2 * (BM + BN) * (BKB / 4 + 4) * 4;
   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -163,14 +225,15 @@ hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
 }
 
 /* Tile configuration: 0 = 256x256 with 8 waves of 128x64 and 128-byte K
- * stages, 1 = 256x256 with 4 waves of 128x128 and 64-byte K stages.
- * GEMMX_SLICE_CFG overrides the default. */
+ * stages, MFMAs interleaved with the fragment reads and LDS writes;
+ * 1 = 256x256 with 4 waves of 128x128 and 64-byte K stages; 2 = as 0 with
+ * the compiler's own order.  GEMMX_SLICE_CFG overrides the default. */
 int slice_cfg() {
   static int cfg = -2;
   if (cfg == -2) {
     const char *e = getenv("GEMMX_SLICE_CFG");
-    cfg = (e && *e) ? atoi(e) : 1;
-    if (cfg < 0 || cfg > 1) cfg = 1;
+    cfg = (e && *e) ? atoi(e) : 0;
+    if (cfg < 0 || cfg > 2) cfg = 0;
   }
   return cfg;
 }
@@ -179,11 +242,17 @@ template <bool BF16>
 hipError_t products(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
                     int kb0, int kb1, const SliceGroups &G, int *S, float *C,
                     int64_t ldc, float alpha, float beta, hipStream_t st) {
-  if (slice_cfg() == 0)
-    return launch_cfg<BF16, 256, 256, 2, 4, 128>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
-                                                 ldc, alpha, beta, st);
-  return launch_cfg<BF16, 256, 256, 2, 2, 64>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc,
-                                              alpha, beta, st);
+  switch (slice_cfg()) {
+  case 1:
+    return launch_cfg<BF16, 256, 256, 2, 2, 64, false>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+                                                       ldc, alpha, beta, st);
+  case 2:
+    return launch_cfg<BF16, 256, 256, 2, 4, 128, false>(as, bs, m, n, kpad, kb0, kb1, G, S,
+                                                        C, ldc, alpha, beta, st);
+  default:
+    return launch_cfg<BF16, 256, 256, 2, 4, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+                                                       ldc, alpha, beta, st);
+  }
 }
 
 } // namespace

@@ -21,16 +21,32 @@ constexpr int SL_BK = 128;               /* K padding granule in bytes */
 constexpr int OZ_NONFINITE = 0x7fffffff; /* exponent of a row/column with Inf/NaN */
 constexpr int MAX_PAIRS = 36;
 constexpr int MAX_GROUPS = 24;
+constexpr int MAX_GROUP_PAIRS = 8; /* pairs in one group (an Ozaki weight: <= slices) */
 
 inline size_t rup(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-/* Slice-pair work of the products kernel.  Group g multiplies the pairs
- * pair_t[b..b+c) x pair_u[b..b+c) (b = gbeg[g], c = gcnt[g]) into one
- * accumulator and writes it to output slot gout[g]. */
+/* Slice-pair work of the products kernel.  Group g multiplies gcnt[g] pairs
+ * of slices (t_c, u_c), c < gcnt[g], into one accumulator and writes it to
+ * output slot gout[g]; t_c = (tpack[g] >> 4c) & 15, u_c likewise from
+ * upack[g].  All fields are 32-bit so that the kernel reads them with scalar
+ * loads and unpacks the pair of each iteration with shifts. */
 struct SliceGroups {
   int ngroups;
-  signed char pair_t[MAX_PAIRS], pair_u[MAX_PAIRS];
-  signed char gbeg[MAX_GROUPS], gcnt[MAX_GROUPS], gout[MAX_GROUPS];
+  int gcnt[MAX_GROUPS], gout[MAX_GROUPS];
+  unsigned tpack[MAX_GROUPS], upack[MAX_GROUPS];
+
+  void add(int out, int count, const int *t, const int *u) {
+    unsigned tp = 0, up = 0;
+    for (int c = 0; c < count; ++c) {
+      tp |= (unsigned)t[c] << (4 * c);
+      up |= (unsigned)u[c] << (4 * c);
+    }
+    gcnt[ngroups] = count;
+    gout[ngroups] = out;
+    tpack[ngroups] = tp;
+    upack[ngroups] = up;
+    ngroups++;
+  }
 };
 
 /* Exponents (per row of op(A) / column of op(B)) and s INT8 slices of an FP64
