@@ -25,7 +25,7 @@ import os
 __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
-    "set_dgemm_mode", "set_sgemm_mode", "last_error", "fill",
+    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "last_error", "fill",
     "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
 ]
 
@@ -66,6 +66,12 @@ def _declare(lib, dtype):
     f = getattr(lib, f"{p}gemm_x_last_error")
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.POINTER(ctypes.c_char_p)]
+    f = getattr(lib, f"{p}gemm_x_set_tile_config")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int]
+    f = getattr(lib, f"{p}gemm_x_load_tuning")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char_p]
     if dtype == "d":
         lib.dgemm_x_set_mode.restype = ctypes.c_int
         lib.dgemm_x_set_mode.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -223,6 +229,25 @@ def sgemm_bf16x3_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ld
                                 _stream(stream))
     if rc:
         _check("s", rc)
+
+
+TILE_CONFIGS = 7
+
+
+def set_tile_config(dtype: str, cfg: int):
+    """Forces native tile configuration `cfg` for every call ('d' or 's');
+    -1 returns to the tuning table."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    rc = getattr(lib, f"{dtype}gemm_x_set_tile_config")(cfg)
+    if rc:
+        _check(dtype, rc)
+
+
+def load_tuning(path, dtype: str = "d") -> int:
+    """Loads a tuning table into the library of `dtype`; returns entries read."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    return int(getattr(lib, f"{dtype}gemm_x_load_tuning")(
+        None if path is None else str(path).encode()))
 
 
 def set_dgemm_mode(mode: int, slices: int = 8):

@@ -23,8 +23,18 @@
  *   op(B), transb N: LDS [n][k]     transb T: LDS [k][n]
  * In MFMA step s of a BK slice, lane group g = lane/16 supplies k = g*BK/4+s,
  * identically for A and B, so the four k-groups of each MFMA together cover
- * the slice.
+ * the slice; where LDS is K-contiguous, consecutive steps are read with one
+ * 16-byte LDS read.
+ *
+ * Workgroups whose tile lies inside C and whose operands allow 16-byte loads
+ * run a branch-free loop over the full K slices (the K remainder, if any, is
+ * one peeled predicated slice); edge tiles run a predicated loop.  In the
+ * branch-free loop the issue order is set explicitly: the next slice's global
+ * loads first, each MFMA step's fragment reads spread over the previous
+ * step's MFMAs, and the next slice's LDS writes spread over the last step.
  */
+#include <type_traits>
+
 #include "gemmx_device.h"
 #include "gemmx_internal.h"
 
@@ -32,20 +42,31 @@ using namespace gemmx;
 
 namespace {
 
-template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
-__global__ void __launch_bounds__(64 * WM * WN)
+/* FAST: grid over the interior tiles (tiles_m x tiles_n of them, all inside
+ * C, operands 16-byte loadable).  !FAST: grid over the edge tiles -- the
+ * partial tile-row tm = tiles_m (when M % BM != 0, tiles_n + (N % BN != 0)
+ * tiles) then the partial tile-column tn = tiles_n for tm < tiles_m -- or,
+ * with edge_all, over every tile of a tiles_m x tiles_n grid. */
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW,
+          int KVMAX, bool FAST>
+__global__ void __launch_bounds__(64 * WM * WN, MINW)
 gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
                    T *__restrict__ C, int M, int N, int K, int64_t lda,
                    int64_t ldb, int64_t ldc, T alpha, T beta, int tiles_m,
-                   int tiles_n, int vecA, int vecB) {
+                   int tiles_n, int vecA, int vecB, int edge_all) {
   constexpr int NT = 64 * WM * WN;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 16, FN = WTN / 16;
-  constexpr int KS = BK / 4;
-  constexpr int VE = Vec16<T>::N;
+  constexpr int KS = BK / 4;            /* MFMA steps per slice */
+  constexpr int VE = Vec16<T>::N;       /* elements per 16 bytes */
+  /* MFMA steps read together where LDS is K-contiguous (more registers, fewer reads) */
+  constexpr int KV = (VE < KS ? VE : KS) < KVMAX ? (VE < KS ? VE : KS) : KVMAX;
   constexpr int A_NL = TA ? BM : BK, A_LEN = TA ? BK : BM, A_LS = A_LEN + VE;
   constexpr int B_NL = TB ? BK : BN, B_LEN = TB ? BN : BK, B_LS = B_LEN + VE;
-  static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0, "tile shape");
+  static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0 && KS % KV == 0, "tile shape");
+  typedef typename Mfma16<T>::acc_t acc_t;
+  typedef typename Vec16<T>::type vec_t;
+  typedef T kvec_t __attribute__((ext_vector_type(KV)));
 
   __shared__ __align__(16) T sA[2][A_NL * A_LS];
   __shared__ __align__(16) T sB[2][B_NL * B_LS];
@@ -54,19 +75,39 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
   const int wm = wave % WM, wn = wave / WM;
   const int li = lane & 15, kg = lane >> 4;
   int tm, tn;
-  tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, &tm, &tn);
+  if (FAST || edge_all) {
+    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, &tm, &tn);
+  } else {
+    const int row_tiles = (M % BM) ? tiles_n + ((N % BN) ? 1 : 0) : 0;
+    if ((int)blockIdx.x < row_tiles) {
+      tm = tiles_m;
+      tn = blockIdx.x;
+    } else {
+      tm = blockIdx.x - row_tiles;
+      tn = tiles_n;
+    }
+  }
   const int m0 = tm * BM, n0 = tn * BN;
 
   TileCopy<T, A_NL, A_LEN, NT> ca;
   TileCopy<T, B_NL, B_LEN, NT> cb;
-  auto load_slice = [&](int k0) {
-    if (!TA) ca.load(A + m0 + k0 * lda, lda, K - k0, M - m0, vecA, tid);
-    else     ca.load(A + k0 + m0 * lda, lda, M - m0, K - k0, vecA, tid);
-    if (!TB) cb.load(B + k0 + n0 * ldb, ldb, N - n0, K - k0, vecB, tid);
-    else     cb.load(B + n0 + k0 * ldb, ldb, K - k0, N - n0, vecB, tid);
+  auto abase = [&](int k0) { return TA ? A + k0 + m0 * lda : A + m0 + k0 * lda; };
+  auto bbase = [&](int k0) { return TB ? B + n0 + k0 * ldb : B + k0 + n0 * ldb; };
+  auto load_safe = [&](int k0) {
+    if (!TA) ca.load(abase(k0), lda, K - k0, M - m0, vecA, tid);
+    else     ca.load(abase(k0), lda, M - m0, K - k0, vecA, tid);
+    if (!TB) cb.load(bbase(k0), ldb, N - n0, K - k0, vecB, tid);
+    else     cb.load(bbase(k0), ldb, K - k0, N - n0, vecB, tid);
+  };
+  auto load_fast = [&](int k0) {
+    ca.load_vec(abase(k0), lda, tid);
+    cb.load_vec(bbase(k0), ldb, tid);
+  };
+  auto store = [&](int buf) {
+    ca.template store<A_LS>(sA[buf], tid);
+    cb.template store<B_LS>(sB[buf], tid);
   };
 
-  typedef typename Mfma16<T>::acc_t acc_t;
   acc_t acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -75,42 +116,101 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
 
-  const int nk = (K + BK - 1) / BK;
-  load_slice(0);
-  ca.template store<A_LS>(sA[0], tid);
-  cb.template store<B_LS>(sB[0], tid);
-  __syncthreads();
-
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) load_slice((kt + 1) * BK);
-    const T *as = sA[cur];
-    const T *bs = sB[cur];
+  /* KV consecutive MFMA steps of one slice from LDS buffer `buf`. */
+  auto compute = [&](int buf) {
+    const T *as = sA[buf];
+    const T *bs = sB[buf];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      const int kx = kg * KS + s;
-      T af[FM], bf[FN];
+    for (int s0 = 0; s0 < KS; s0 += KV) {
+      const int kx = kg * KS + s0;
+      T af[KV][FM], bf[KV][FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         const int row = wm * WTM + i * 16 + li;
-        af[i] = TA ? as[row * A_LS + kx] : as[kx * A_LS + row];
+        if (TA) {
+          const kvec_t v = *reinterpret_cast<const kvec_t *>(as + row * A_LS + kx);
+#pragma unroll
+          for (int v2 = 0; v2 < KV; ++v2) af[v2][i] = v[v2];
+        } else {
+#pragma unroll
+          for (int v2 = 0; v2 < KV; ++v2) af[v2][i] = as[(kx + v2) * A_LS + row];
+        }
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
         const int col = wn * WTN + j * 16 + li;
-        bf[j] = TB ? bs[kx * B_LS + col] : bs[col * B_LS + kx];
+        if (!TB) {
+          const kvec_t v = *reinterpret_cast<const kvec_t *>(bs + col * B_LS + kx);
+#pragma unroll
+          for (int v2 = 0; v2 < KV; ++v2) bf[v2][j] = v[v2];
+        } else {
+#pragma unroll
+          for (int v2 = 0; v2 < KV; ++v2) bf[v2][j] = bs[(kx + v2) * B_LS + col];
+        }
       }
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+      for (int v2 = 0; v2 < KV; ++v2)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
-          acc[i][j] = Mfma16<T>::mma(af[i], bf[j], acc[i][j]);
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = Mfma16<T>::mma(af[v2][i], bf[v2][j], acc[i][j]);
     }
-    if (kt + 1 < nk) {
-      ca.template store<A_LS>(sA[cur ^ 1], tid);
-      cb.template store<B_LS>(sB[cur ^ 1], tid);
-    }
+  };
+
+  const int nfull = K / BK, nk = (K + BK - 1) / BK;
+  if (FAST && nfull >= 1) {
+    load_fast(0);
+    store(0);
     __syncthreads();
+    for (int kt = 0; kt + 1 < nfull; ++kt) {
+      load_fast((kt + 1) * BK);
+      compute(kt & 1);
+      store((kt + 1) & 1);
+      {
+        constexpr int NLD = TileCopy<T, A_NL, A_LEN, NT>::PER + TileCopy<T, B_NL, B_LEN, NT>::PER;
+        constexpr int NR = (TA ? FM : FM * KV) + (TB ? FN * KV : FN); /* LDS reads per KV steps */
+        constexpr int NMF = KV * FM * FN;                             /* MFMAs per KV steps */
+        __builtin_amdgcn_sched_group_barrier(0x020, NLD, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
+#pragma unroll
+        for (int s = KV; s < KS; s += KV) {
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            __builtin_amdgcn_sched_group_barrier(0x008, NMF / NR, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          }
+          __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NR) * NR, 0);
+        }
+#pragma unroll
+        for (int w = 0; w < NLD; ++w) {
+          __builtin_amdgcn_sched_group_barrier(0x008, NMF / NLD, 0);
+          __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NLD) * NLD, 0);
+      }
+      __syncthreads();
+    }
+    const int last = nfull - 1;
+    if (nk > nfull) {
+      load_safe(nfull * BK); /* the K remainder, zero-filled past K */
+      compute(last & 1);
+      store(nfull & 1);
+      __syncthreads();
+      compute(nfull & 1);
+    } else {
+      compute(last & 1);
+    }
+  } else {
+    load_safe(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      if (kt + 1 < nk) load_safe((kt + 1) * BK);
+      compute(kt & 1);
+      if (kt + 1 < nk) store((kt + 1) & 1);
+      __syncthreads();
+    }
   }
 
   /* Epilogue: rows m..ldc-1 and columns >= n are never touched, matching the
@@ -143,35 +243,56 @@ scale_kernel(T *__restrict__ C, int64_t ldc, int M, int N, T beta) {
   }
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN>
-hipError_t launch_cfg(const gemmx_problem *p, hipStream_t s) {
-  const int tiles_m = (p->m + BM - 1) / BM, tiles_n = (p->n + BN - 1) / BN;
-  const dim3 grid(tiles_m * tiles_n), block(64 * WM * WN);
+/* Interior tiles go to the FAST kernel, the rest to the predicated one; both
+ * launches are on stream s and touch disjoint tiles of C. */
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
+hipError_t launch_trans(const gemmx_problem *p, hipStream_t s) {
   constexpr int VE = Vec16<T>::N;
   const int vecA = ((uintptr_t)p->a % 16 == 0) && (p->lda % VE == 0);
   const int vecB = ((uintptr_t)p->b % 16 == 0) && (p->ldb % VE == 0);
   const T *A = (const T *)p->a, *B = (const T *)p->b;
   T *C = (T *)p->c;
   const T alpha = (T)p->alpha, beta = (T)p->beta;
-#define GEMMX_LAUNCH(TA, TB)                                                  \
-  hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN>), grid, \
-                     block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb,    \
-                     p->ldc, alpha, beta, tiles_m, tiles_n, vecA, vecB)
-  if (!p->ta && !p->tb) GEMMX_LAUNCH(false, false);
-  else if (!p->ta && p->tb) GEMMX_LAUNCH(false, true);
-  else if (p->ta && !p->tb) GEMMX_LAUNCH(true, false);
-  else GEMMX_LAUNCH(true, true);
-#undef GEMMX_LAUNCH
+  const dim3 block(64 * WM * WN);
+  const int tiles_m = (p->m + BM - 1) / BM, tiles_n = (p->n + BN - 1) / BN;
+  const int full_m = p->m / BM, full_n = p->n / BN;
+  if (vecA && vecB && full_m > 0 && full_n > 0) {
+    hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
+                       dim3(full_m * full_n), block, 0, s, A, B, C, p->m, p->n, p->k, p->lda,
+                       p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0);
+    const int edges = tiles_m * tiles_n - full_m * full_n;
+    if (edges > 0)
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
+                         dim3(edges), block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb,
+                         p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0);
+  } else {
+    hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
+                       dim3(tiles_m * tiles_n), block, 0, s, A, B, C, p->m, p->n, p->k,
+                       p->lda, p->ldb, p->ldc, alpha, beta, tiles_m, tiles_n, vecA, vecB, 1);
+  }
   return hipGetLastError();
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
+hipError_t launch_cfg(const gemmx_problem *p, hipStream_t s) {
+  if (!p->ta && !p->tb) return launch_trans<T, false, false, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
+  if (!p->ta && p->tb) return launch_trans<T, false, true, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
+  if (p->ta && !p->tb) return launch_trans<T, true, false, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
+  return launch_trans<T, true, true, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
 }
 
 template <typename T>
 hipError_t launch_native(const gemmx_problem *p, int cfg, hipStream_t s) {
+  /* BM, BN, BK, waves (M x N), min waves per SIMD, K steps per LDS read */
   switch (cfg) {
-  case GEMMX_NCFG_64x64: return launch_cfg<T, 64, 64, 16, 2, 2>(p, s);
-  case GEMMX_NCFG_256x128: return launch_cfg<T, 256, 128, 16, 4, 2>(p, s);
-  case GEMMX_NCFG_128x128:
-  default: return launch_cfg<T, 128, 128, 16, 2, 2>(p, s);
+  case 1: return launch_cfg<T, 64, 64, 16, 2, 2, 2, 1>(p, s);
+  case 2: return launch_cfg<T, 256, 128, 16, 4, 2, 2, 1>(p, s);
+  case 3: return launch_cfg<T, 128, 128, 32, 2, 2, 1, 4>(p, s);
+  case 4: return launch_cfg<T, 128, 128, 16, 2, 2, 1, 4>(p, s);
+  case 5: return launch_cfg<T, 128, 128, 16, 4, 2, 2, 1>(p, s);
+  case 6: return launch_cfg<T, 256, 128, 16, 4, 2, 1, 4>(p, s);
+  case 0:
+  default: return launch_cfg<T, 128, 128, 16, 2, 2, 2, 1>(p, s);
   }
 }
 

@@ -351,6 +351,27 @@ EXPORT int sgemm_x_set_mode(int mode) {
   return set_error(GEMMX_ERR_ARG, "sgemm_x_set_mode: mode %d", mode);
 }
 
+/* Tile configuration of the native kernels: cfg >= 0 forces one for every
+ * call, -1 returns to the tuning table.  Table reload: path or NULL. */
+EXPORT int dgemm_x_set_tile_config(int cfg) {
+  pthread_once(&g_once, init_once);
+  if (gemmx_tune_force(8, cfg) != 0) return set_error(GEMMX_ERR_ARG, "tile config %d", cfg);
+  return GEMMX_OK;
+}
+EXPORT int sgemm_x_set_tile_config(int cfg) {
+  pthread_once(&g_once, init_once);
+  if (gemmx_tune_force(4, cfg) != 0) return set_error(GEMMX_ERR_ARG, "tile config %d", cfg);
+  return GEMMX_OK;
+}
+EXPORT int dgemm_x_load_tuning(const char *path) {
+  pthread_once(&g_once, init_once);
+  return gemmx_tune_load(path);
+}
+EXPORT int sgemm_x_load_tuning(const char *path) {
+  pthread_once(&g_once, init_once);
+  return gemmx_tune_load(path);
+}
+
 EXPORT int dgemm_x_last_error(const char **msg) {
   if (msg) *msg = t_msg;
   return t_err;

@@ -102,6 +102,15 @@ struct TileCopy {
     }
   }
 
+  /* Interior tile with aligned operands: every chunk is one 16-byte load. */
+  __device__ __forceinline__ void load_vec(const T *__restrict__ g, int64_t ld, int tid) {
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const int ch = tid + c * NT, line = ch / CPL, off = (ch % CPL) * VE;
+      r[c] = *reinterpret_cast<const vec_t *>(g + line * ld + off);
+    }
+  }
+
   template <int LS>
   __device__ __forceinline__ void store(T *s, int tid) const {
 #pragma unroll

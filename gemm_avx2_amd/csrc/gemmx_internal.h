@@ -36,10 +36,12 @@ typedef struct gemmx_problem {
 /* Kernel configurations of the native MFMA tile kernels.  The tuning table
  * (gemmx_tune.c) picks one per (dtype, trans, shape bucket). */
 enum {
-  GEMMX_NCFG_128x128 = 0, /* 128x128x16 tile, 4 waves of 64x64 */
-  GEMMX_NCFG_64x64 = 1,   /* 64x64x16 tile, 4 waves of 32x32 */
-  GEMMX_NCFG_256x128 = 2, /* 256x128x16 tile, 8 waves of 64x64 */
-  GEMMX_NCFG_COUNT = 3
+  GEMMX_NCFG_128x128 = 0, /* 128x128x16, 4 waves of 64x64, >= 2 waves/SIMD */
+  GEMMX_NCFG_64x64 = 1,   /* 64x64x16, 4 waves of 32x32 */
+  GEMMX_NCFG_256x128 = 2, /* 256x128x16, 8 waves of 64x64, >= 2 waves/SIMD */
+  /* 3: 128x128x32 1 wave/SIMD; 4: 128x128x16 1 wave/SIMD; 5: 128x128x16 with
+   * 8 waves of 32x64; 6: 256x128x16 1 wave/SIMD (see gemm_native.hip) */
+  GEMMX_NCFG_COUNT = 7
 };
 
 /* Native tile kernels (gemm_native.hip).  alpha != 0 and k > 0 here; the

@@ -20,7 +20,7 @@
 #define NB 12 /* buckets 4..15 */
 static signed char g_table[2][4][NB][NB][NB]; /* [f64/f32][trans][m][n][k] */
 static int g_entries;
-static int g_force = -1; /* GEMMX_NATIVE_CFG: one configuration for every call */
+static int g_force[2] = {-1, -1}; /* [f64/f32]: one configuration for every call */
 static char g_path[1024];
 
 int gemmx_tune_bucket(int dim) {
@@ -70,7 +70,7 @@ void gemmx_tune_init(void) {
   const char *force = getenv("GEMMX_NATIVE_CFG");
   if (force && *force) {
     int c = atoi(force);
-    if (c >= 0 && c < GEMMX_NCFG_COUNT) g_force = c;
+    if (c >= 0 && c < GEMMX_NCFG_COUNT) g_force[0] = g_force[1] = c;
   }
   const char *env = getenv("GEMMX_TUNE_FILE");
   if (env && *env) {
@@ -88,11 +88,25 @@ void gemmx_tune_init(void) {
 }
 
 int gemmx_tune_entries(void) { return g_entries; }
+
+int gemmx_tune_force(int es, int cfg) {
+  if (cfg < -1 || cfg >= GEMMX_NCFG_COUNT) return -1;
+  g_force[es == 8 ? 0 : 1] = cfg;
+  return 0;
+}
+
+int gemmx_tune_load(const char *path) {
+  memset(g_table, 0, sizeof g_table);
+  g_entries = 0;
+  g_path[0] = 0;
+  if (path && *path) load_table(path);
+  return g_entries;
+}
 const char *gemmx_tune_path(void) { return g_path; }
 
 int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {
-  if (g_force >= 0) return g_force;
   const int d = es == 8 ? 0 : 1, t = (ta ? 2 : 0) + (tb ? 1 : 0);
+  if (g_force[d] >= 0) return g_force[d];
   const int c = g_table[d][t][gemmx_tune_bucket(m) - 4][gemmx_tune_bucket(n) - 4]
                        [gemmx_tune_bucket(k) - 4];
   if (c > 0) return c - 1;

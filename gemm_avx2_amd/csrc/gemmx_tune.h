@@ -25,6 +25,11 @@ int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k);
 int gemmx_tune_bucket(int dim);
 /* Number of entries loaded from the table file (0 when none). */
 int gemmx_tune_entries(void);
+/* Forces one configuration for every call of the given element size (8 or
+ * 4); -1 returns to the table.  Returns 0, or -1 for an unknown cfg. */
+int gemmx_tune_force(int es, int cfg);
+/* Loads (or, with NULL, clears) a table at run time; returns entries read. */
+int gemmx_tune_load(const char *path);
 /* Path of the table that was loaded, or "" when none. */
 const char *gemmx_tune_path(void);
 

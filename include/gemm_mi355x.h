@@ -71,6 +71,19 @@ void sgemm_(const char *transa, const char *transb, const int *m, const int *n,
 int dgemm_x_set_mode(int mode, int slices);
 int sgemm_x_set_mode(int mode);
 
+/* Tile configuration of the native kernels, in place of the reference's
+ * compile-time tuning headers (dgemm_tune.h / sgemm_tune.h, clamped in
+ * src/gemm_set_parameters.h:10-61): cfg in 0..GEMMX_TILE_CONFIGS-1 forces one
+ * configuration for every call, -1 returns to the tuning table.
+ * *_x_load_tuning reads a table written by the autotuner (path; NULL clears
+ * it) and returns the number of entries read; the table is shared by both
+ * element types.  Also GEMMX_TUNE_FILE and GEMMX_NATIVE_CFG at first use. */
+#define GEMMX_TILE_CONFIGS 7
+int dgemm_x_set_tile_config(int cfg);
+int sgemm_x_set_tile_config(int cfg);
+int dgemm_x_load_tuning(const char *path);
+int sgemm_x_load_tuning(const char *path);
+
 /* Last error of the calling thread: returns its code and, when msg is not
  * NULL, a static description.  Cleared by the next successful call. */
 int dgemm_x_last_error(const char **msg);

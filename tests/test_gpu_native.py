@@ -240,3 +240,33 @@ def test_errors_leave_c_untouched(cuda):
     gx.dgemm("N", "N", 4, 4, 4, 1.0, A, 4, A, 4, 0.0, C, 4)
     assert gx.last_error("d")[0] == gx.OK
     np.testing.assert_array_equal(C, 4.0)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_every_tile_config(cuda, dt):
+    """Each compiled tile configuration, forced through the C ABI, on shapes
+    that hit interior tiles (branch-free path), edge tiles, a K remainder and
+    all transposes."""
+    import torch
+    d = "d" if dt == np.float64 else "s"
+    rng = np.random.default_rng(23)
+    try:
+        for cfg in range(gx.TILE_CONFIGS):
+            gx.set_tile_config(d, cfg)
+            for (m, n, k) in [(256, 256, 64), (300, 270, 77), (512, 129, 48)]:
+                for ta, tb in TRANS:
+                    ra, ca = stored_shape(ta, m, k)
+                    rb, cb = stored_shape(tb, k, n)
+                    A = np.asfortranarray(rng.uniform(-1, 1, (ra, ca)).astype(dt))
+                    B = np.asfortranarray(rng.uniform(-1, 1, (rb, cb)).astype(dt))
+                    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(dt))
+                    R = C.copy(order="F")
+                    oracle.gemm(ta, tb, m, n, k, 0.75, A, ra, B, rb, -1.0, R, m)
+                    dA, dB, dC = to_device(A, torch), to_device(B, torch), to_device(C, torch)
+                    _gemm_dev(dt, ta, tb, m, n, k, 0.75, dA, ra, dB, rb, -1.0, dC, m)
+                    torch.cuda.synchronize()
+                    G = from_device(dC, C.shape)
+                    bound = abs_bound(A, B, C, ta, tb, m, n, k, 0.75, -1.0)
+                    assert max_comp(G, R, bound) <= _tol(dt, k), (cfg, m, n, k, ta, tb)
+    finally:
+        gx.set_tile_config(d, -1)
