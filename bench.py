@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--profile-only", choices=["native", "ozaki", "sgemm", "bf16x3"],
+    ap.add_argument("--profile-only", choices=["native", "ozaki", "sgemm", "bf16x3", "batch"],
                     help="run just one form (for rocprofv3)")
     return ap.parse_args()
 
@@ -154,6 +154,62 @@ def cpu_baseline():
             "kind": kind, "sample": sample, "seconds": round(best, 3)}
 
 
+# ---------------------------------------------------------------- grouped batch
+BATCH_COUNT, BATCH_LO, BATCH_HI = 4096, 16, 1024
+
+
+def batch_workload(torch, gx, dtype, stream, seed=11):
+    """4096 independent GEMMs, m, n, k log-uniform in [16, 1024], transa and
+    transb uniform over {N, T}, alpha = 1.5, beta = -0.5, operands packed
+    back to back (ld = stored rows) in three device buffers.  Returns the
+    plan and the single-call argument lists of the same problems."""
+    rng = np.random.default_rng(seed)
+    tdt = torch.float64 if dtype == "d" else torch.float32
+    shapes = []
+    offs = [0, 0, 0]
+    for _ in range(BATCH_COUNT):
+        m, n, k = (int(round(math.exp(rng.uniform(math.log(BATCH_LO), math.log(BATCH_HI)))))
+                   for _ in range(3))
+        ta, tb = "NT"[rng.integers(2)], "NT"[rng.integers(2)]
+        shapes.append((ta, tb, m, n, k, tuple(offs)))
+        # 16-byte aligned starts for every operand
+        for j, size in enumerate((m * k, k * n, m * n)):
+            offs[j] += (size + 1) // 2 * 2
+    bufs = [torch.empty(max(o, 1), dtype=tdt, device="cuda") for o in offs]
+    for j, b in enumerate(bufs):
+        gx.fill(b, b.numel(), 1, b.numel(), seed, j, stream=stream)
+    es = bufs[0].element_size()
+    specs = []
+    for ta, tb, m, n, k, (oa, ob, oc) in shapes:
+        ra = m if ta == "N" else k
+        rb = k if tb == "N" else n
+        specs.append((ta, tb, m, n, k, ALPHA, bufs[0].data_ptr() + oa * es, ra,
+                      bufs[1].data_ptr() + ob * es, rb, BETA, bufs[2].data_ptr() + oc * es, m))
+    return gx.BatchPlan(dtype, specs), specs, bufs
+
+
+def bench_batch(torch, gx, stream, steps, warmup):
+    out = {"problems": BATCH_COUNT, "sizes": f"log-uniform [{BATCH_LO}, {BATCH_HI}]",
+           "trans": "mixed N/T"}
+    for d, key in (("d", "f64"), ("s", "f32")):
+        plan, specs, bufs = batch_workload(torch, gx, d, stream)
+        _, per = time_launches(torch, lambda: plan.run(stream), steps, warmup)
+        ms = float(np.mean(per))
+        single = gx.dgemm_dev if d == "d" else gx.sgemm_dev
+
+        def loop():
+            for sp in specs:
+                single(*sp, stream=stream)
+        _, per_l = time_launches(torch, loop, 2, 1)
+        ms_l = float(np.mean(per_l))
+        out[key] = {"tflops": round(plan.flops / ms / 1e9, 2), "ms": round(ms, 3),
+                    "gflop": round(plan.flops / 1e9, 2),
+                    "single_call_loop_tflops": round(plan.flops / ms_l / 1e9, 2)}
+        plan.destroy()
+        del bufs
+    return out
+
+
 # ---------------------------------------------------------------- main
 def main():
     args = parse()
@@ -189,6 +245,12 @@ def main():
         gx.dgemm_ozaki_dev("N", "N", M, N, K, ALPHA, dA, LD, dB, LD, BETA, dC, LD, s,
                            ws, ws.numel(), stream=stream)
 
+    if args.profile_only == "batch":
+        del ws, dA, dB, dC, C0
+        plan, _, bufs = batch_workload(torch, gx, "d", stream)
+        wall, per = time_launches(torch, lambda: plan.run(stream), args.steps, args.warmup)
+        print(json.dumps({"profile_only": "batch", "ms": per, "gflop": plan.flops / 1e9}))
+        return
     if args.profile_only:
         if args.profile_only in ("native", "ozaki"):
             fn = native if args.profile_only == "native" else ozaki
@@ -272,7 +334,8 @@ def main():
         _, per = time_launches(torch, fb, 3, 1)
         sg["bf16x3_NN"] = round(flops / float(np.mean(per)) / 1e9, 2)
         result["sgemm_tflops"] = sg
-        del sA, sB, sC
+        del sA, sB, sC, wsb
+        result["batch"] = bench_batch(torch, gx, stream, max(3, args.steps // 2), 1)
 
     value = flops / best_ms / 1e9
     ms_step = ms_n if best_name == "native" else best_ms

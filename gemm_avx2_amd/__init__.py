@@ -26,7 +26,7 @@ __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "last_error", "fill",
-    "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
+    "BatchPlan", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
 ]
 
 LIB_DIR = os.environ.get("GEMMX_LIB_DIR") or os.path.join(
@@ -96,6 +96,25 @@ def _declare(lib, dtype):
             ctypes.c_void_p]
 
 
+def _declare_batch(lib, dtype):
+    fl = ctypes.c_double if dtype == "d" else ctypes.c_float
+    f = getattr(lib, f"{dtype}gemm_x_batch_create")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_char_p,
+                  ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p, ctypes.POINTER(fl),
+                  ctypes.POINTER(ctypes.c_void_p), _c_int_p, ctypes.POINTER(ctypes.c_void_p),
+                  _c_int_p, ctypes.POINTER(fl), ctypes.POINTER(ctypes.c_void_p), _c_int_p]
+    f = getattr(lib, f"{dtype}gemm_x_batch_run")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    f = getattr(lib, f"{dtype}gemm_x_batch_flops")
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_void_p]
+    f = getattr(lib, f"{dtype}gemm_x_batch_destroy")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p]
+
+
 def load(name: str):
     """Loads lib/<name> ('DGEMM.so', 'SGEMM.so', 'libgemmx_gen.so') once."""
     if name in _libs:
@@ -107,8 +126,10 @@ def load(name: str):
     lib = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
     if name == "DGEMM.so":
         _declare(lib, "d")
+        _declare_batch(lib, "d")
     elif name == "SGEMM.so":
         _declare(lib, "s")
+        _declare_batch(lib, "s")
     elif name == "libgemmx_gen.so":
         for fn, t in (("gemmx_fill_f64", ctypes.c_void_p), ("gemmx_fill_f32", ctypes.c_void_p)):
             f = getattr(lib, fn)
@@ -229,6 +250,59 @@ def sgemm_bf16x3_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ld
                                 _stream(stream))
     if rc:
         _check("s", rc)
+
+
+class BatchPlan:
+    """A grouped batch of independent GEMMs on device pointers, captured once
+    as a hipGraph on the current device and replayed by run().  `problems` is
+    a sequence of (transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c,
+    ldc) with a/b/c device tensors or addresses; each problem has the
+    semantics of dgemm_/sgemm_.  No reference counterpart (the reference
+    computes one GEMM per call, src/gemm_driver.c:118)."""
+
+    def __init__(self, dtype: str, problems):
+        self.dtype = dtype
+        self._lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+        fl = ctypes.c_double if dtype == "d" else ctypes.c_float
+        cnt = len(problems)
+        I = ctypes.c_int * max(cnt, 1)
+        F = fl * max(cnt, 1)
+        P = ctypes.c_void_p * max(cnt, 1)
+        cols = list(zip(*problems)) if cnt else [()] * 13
+        ta = b"".join(_ch(t) for t in cols[0]) or b"N"
+        tb = b"".join(_ch(t) for t in cols[1]) or b"N"
+        m, n, k = (I(*cols[i]) for i in (2, 3, 4))
+        alpha, beta = F(*cols[5]), F(*cols[10])
+        a, b, c = (P(*[_addr(x) for x in cols[i]]) for i in (6, 8, 11))
+        lda, ldb, ldc = (I(*cols[i]) for i in (7, 9, 12))
+        self._plan = ctypes.c_void_p()
+        rc = getattr(self._lib, f"{dtype}gemm_x_batch_create")(
+            ctypes.byref(self._plan), cnt, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+        if rc:
+            self._plan = None
+            _check(dtype, rc)
+
+    @property
+    def flops(self) -> float:
+        return float(getattr(self._lib, f"{self.dtype}gemm_x_batch_flops")(self._plan))
+
+    def run(self, stream=None):
+        if self._plan is None:
+            raise GemmError(ERR_ARG, "batch plan destroyed")
+        rc = getattr(self._lib, f"{self.dtype}gemm_x_batch_run")(self._plan, _stream(stream))
+        if rc:
+            _check(self.dtype, rc)
+
+    def destroy(self):
+        if self._plan is not None:
+            getattr(self._lib, f"{self.dtype}gemm_x_batch_destroy")(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 TILE_CONFIGS = 7

@@ -28,8 +28,9 @@ BUILD = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["gemm_native.hip", "gemm_split.hip", "gemm_products.hip", "gemm_emulated.hip"]
-C_SOURCES = ["gemmx_api.c", "gemmx_tune.c"]
+HIP_SOURCES = ["gemm_native.hip", "gemm_split.hip", "gemm_products.hip", "gemm_emulated.hip",
+               "gemm_batch.hip"]
+C_SOURCES = ["gemmx_api.c", "gemmx_tune.c", "gemmx_batch.c"]
 GEN_SOURCES = ["gemmx_gen.hip"]
 
 HIPCC_FLAGS = [

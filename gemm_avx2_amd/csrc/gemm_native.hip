@@ -37,6 +37,7 @@
 
 #include "gemmx_device.h"
 #include "gemmx_internal.h"
+#include "gemmx_tile.h"
 
 using namespace gemmx;
 
@@ -54,26 +55,7 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
                    T *__restrict__ C, int M, int N, int K, int64_t lda,
                    int64_t ldb, int64_t ldc, T alpha, T beta, int tiles_m,
                    int tiles_n, int vecA, int vecB, int edge_all) {
-  constexpr int NT = 64 * WM * WN;
-  constexpr int WTM = BM / WM, WTN = BN / WN;
-  constexpr int FM = WTM / 16, FN = WTN / 16;
-  constexpr int KS = BK / 4;            /* MFMA steps per slice */
-  constexpr int VE = Vec16<T>::N;       /* elements per 16 bytes */
-  /* MFMA steps read together where LDS is K-contiguous (more registers, fewer reads) */
-  constexpr int KV = (VE < KS ? VE : KS) < KVMAX ? (VE < KS ? VE : KS) : KVMAX;
-  constexpr int A_NL = TA ? BM : BK, A_LEN = TA ? BK : BM, A_LS = A_LEN + VE;
-  constexpr int B_NL = TB ? BK : BN, B_LEN = TB ? BN : BK, B_LS = B_LEN + VE;
-  static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0 && KS % KV == 0, "tile shape");
-  typedef typename Mfma16<T>::acc_t acc_t;
-  typedef typename Vec16<T>::type vec_t;
-  typedef T kvec_t __attribute__((ext_vector_type(KV)));
-
-  __shared__ __align__(16) T sA[2][A_NL * A_LS];
-  __shared__ __align__(16) T sB[2][B_NL * B_LS];
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WM, wn = wave / WM;
-  const int li = lane & 15, kg = lane >> 4;
+  __shared__ __align__(16) T lds[tile_lds_elems<T, TA, TB, BM, BN, BK>()];
   int tm, tn;
   if (FAST || edge_all) {
     tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, &tm, &tn);
@@ -87,149 +69,9 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
       tn = tiles_n;
     }
   }
-  const int m0 = tm * BM, n0 = tn * BN;
-
-  TileCopy<T, A_NL, A_LEN, NT> ca;
-  TileCopy<T, B_NL, B_LEN, NT> cb;
-  auto abase = [&](int k0) { return TA ? A + k0 + m0 * lda : A + m0 + k0 * lda; };
-  auto bbase = [&](int k0) { return TB ? B + n0 + k0 * ldb : B + k0 + n0 * ldb; };
-  auto load_safe = [&](int k0) {
-    if (!TA) ca.load(abase(k0), lda, K - k0, M - m0, vecA, tid);
-    else     ca.load(abase(k0), lda, M - m0, K - k0, vecA, tid);
-    if (!TB) cb.load(bbase(k0), ldb, N - n0, K - k0, vecB, tid);
-    else     cb.load(bbase(k0), ldb, K - k0, N - n0, vecB, tid);
-  };
-  auto load_fast = [&](int k0) {
-    ca.load_vec(abase(k0), lda, tid);
-    cb.load_vec(bbase(k0), ldb, tid);
-  };
-  auto store = [&](int buf) {
-    ca.template store<A_LS>(sA[buf], tid);
-    cb.template store<B_LS>(sB[buf], tid);
-  };
-
-  acc_t acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[i][j][r] = T(0);
-
-  /* KV consecutive MFMA steps of one slice from LDS buffer `buf`. */
-  auto compute = [&](int buf) {
-    const T *as = sA[buf];
-    const T *bs = sB[buf];
-#pragma unroll
-    for (int s0 = 0; s0 < KS; s0 += KV) {
-      const int kx = kg * KS + s0;
-      T af[KV][FM], bf[KV][FN];
-#pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        const int row = wm * WTM + i * 16 + li;
-        if (TA) {
-          const kvec_t v = *reinterpret_cast<const kvec_t *>(as + row * A_LS + kx);
-#pragma unroll
-          for (int v2 = 0; v2 < KV; ++v2) af[v2][i] = v[v2];
-        } else {
-#pragma unroll
-          for (int v2 = 0; v2 < KV; ++v2) af[v2][i] = as[(kx + v2) * A_LS + row];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const int col = wn * WTN + j * 16 + li;
-        if (!TB) {
-          const kvec_t v = *reinterpret_cast<const kvec_t *>(bs + col * B_LS + kx);
-#pragma unroll
-          for (int v2 = 0; v2 < KV; ++v2) bf[v2][j] = v[v2];
-        } else {
-#pragma unroll
-          for (int v2 = 0; v2 < KV; ++v2) bf[v2][j] = bs[(kx + v2) * B_LS + col];
-        }
-      }
-#pragma unroll
-      for (int v2 = 0; v2 < KV; ++v2)
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            acc[i][j] = Mfma16<T>::mma(af[v2][i], bf[v2][j], acc[i][j]);
-    }
-  };
-
-  const int nfull = K / BK, nk = (K + BK - 1) / BK;
-  if (FAST && nfull >= 1) {
-    load_fast(0);
-    store(0);
-    __syncthreads();
-    for (int kt = 0; kt + 1 < nfull; ++kt) {
-      load_fast((kt + 1) * BK);
-      compute(kt & 1);
-      store((kt + 1) & 1);
-      {
-        constexpr int NLD = TileCopy<T, A_NL, A_LEN, NT>::PER + TileCopy<T, B_NL, B_LEN, NT>::PER;
-        constexpr int NR = (TA ? FM : FM * KV) + (TB ? FN * KV : FN); /* LDS reads per KV steps */
-        constexpr int NMF = KV * FM * FN;                             /* MFMAs per KV steps */
-        __builtin_amdgcn_sched_group_barrier(0x020, NLD, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
-#pragma unroll
-        for (int s = KV; s < KS; s += KV) {
-#pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            __builtin_amdgcn_sched_group_barrier(0x008, NMF / NR, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          }
-          __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NR) * NR, 0);
-        }
-#pragma unroll
-        for (int w = 0; w < NLD; ++w) {
-          __builtin_amdgcn_sched_group_barrier(0x008, NMF / NLD, 0);
-          __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-        }
-        __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NLD) * NLD, 0);
-      }
-      __syncthreads();
-    }
-    const int last = nfull - 1;
-    if (nk > nfull) {
-      load_safe(nfull * BK); /* the K remainder, zero-filled past K */
-      compute(last & 1);
-      store(nfull & 1);
-      __syncthreads();
-      compute(nfull & 1);
-    } else {
-      compute(last & 1);
-    }
-  } else {
-    load_safe(0);
-    store(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-      if (kt + 1 < nk) load_safe((kt + 1) * BK);
-      compute(kt & 1);
-      if (kt + 1 < nk) store((kt + 1) & 1);
-      __syncthreads();
-    }
-  }
-
-  /* Epilogue: rows m..ldc-1 and columns >= n are never touched, matching the
-   * reference's masked stores (src/kernel/gemm_kernel_unroll6.S:145-155). */
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn * WTN + j * 16 + li;
-      const int row0 = m0 + wm * WTM + i * 16;
-      if (col < N) {
-        T *cp = C + row0 + col * ldc;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int dr = Mfma16<T>::row(kg, r);
-          if (row0 + dr < M) cp[dr] = fma_t(alpha, acc[i][j][r], beta * cp[dr]);
-        }
-      }
-    }
+  gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, FAST>(A, B, C, M, N, K, lda, ldb, ldc,
+                                                        alpha, beta, tm * BM, tn * BN, vecA,
+                                                        vecB, lds);
 }
 
 template <typename T>

@@ -34,11 +34,8 @@ static __thread char t_msg[320];
 static pthread_mutex_t g_report_lock = PTHREAD_MUTEX_INITIALIZER;
 static unsigned g_reported;
 
-static int set_error(int code, const char *fmt, ...) {
-  va_list ap;
-  va_start(ap, fmt);
+static int set_error_v(int code, const char *fmt, va_list ap) {
   vsnprintf(t_msg, sizeof t_msg, fmt, ap);
-  va_end(ap);
   t_err = code;
   const char *q = getenv("GEMMX_QUIET");
   if (!(q && *q && *q != '0')) {
@@ -52,10 +49,29 @@ static int set_error(int code, const char *fmt, ...) {
   return code;
 }
 
+static int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  set_error_v(code, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
 static void clear_error(void) {
   t_err = GEMMX_OK;
   t_msg[0] = 0;
 }
+
+/* The same, for the other host sources of the library (gemmx_batch.c). */
+int gemmx_report(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  set_error_v(code, fmt, ap);
+  va_end(ap);
+  return code;
+}
+void gemmx_clear_error(void) { clear_error(); }
+int gemmx_last_code(void) { return t_err; }
 
 static int hip_fail(hipError_t e, const char *what) {
   return set_error(e == hipErrorOutOfMemory ? GEMMX_ERR_ALLOC : GEMMX_ERR_LAUNCH,
@@ -110,6 +126,8 @@ static int current_device(int *dev) {
     return set_error(GEMMX_ERR_UNSUPPORTED, "device ordinal %d out of range", *dev);
   return GEMMX_OK;
 }
+
+int gemmx_current_device(int *dev) { return current_device(dev); }
 
 /* Grows the device's scratch to `bytes`; caller holds g_dev[dev].lock and
  * has no work in flight on the old scratch. */

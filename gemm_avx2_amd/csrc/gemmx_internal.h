@@ -44,6 +44,14 @@ enum {
   GEMMX_NCFG_COUNT = 7
 };
 
+/* Host-side error state and device lookup shared by the host sources
+ * (gemmx_api.c).  gemmx_report records the per-thread last error and
+ * returns `code`. */
+int gemmx_report(int code, const char *fmt, ...);
+void gemmx_clear_error(void);
+int gemmx_last_code(void);
+int gemmx_current_device(int *dev);
+
 /* Native tile kernels (gemm_native.hip).  alpha != 0 and k > 0 here; the
  * host layer handles the alpha == 0 / k == 0 / empty cases. */
 hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, hipStream_t s);
@@ -54,6 +62,35 @@ hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta,
                            hipStream_t s);
 hipError_t gemmx_scale_f32(float *c, int64_t ldc, int m, int n, float beta,
                            hipStream_t s);
+
+/* Grouped batch (gemm_batch.hip): one problem of a persistent grouped
+ * launch.  Problems are stored heaviest first; tiles of problem p are
+ * tile_begin .. tile_begin + tiles_m*tiles_n - 1 of the launch. */
+enum {
+  GEMMX_BF_TA = 1,      /* transpose A */
+  GEMMX_BF_TB = 2,      /* transpose B */
+  GEMMX_BF_VEC_A = 4,   /* A loadable with 16-byte loads */
+  GEMMX_BF_VEC_B = 8,
+  GEMMX_BF_SCALE = 16   /* alpha == 0 or k == 0: C = beta*C only */
+};
+typedef struct gemmx_batch_desc {
+  const void *a;
+  const void *b;
+  void *c;
+  int64_t lda, ldb, ldc;
+  double alpha, beta;
+  int m, n, k, flags;
+  int tiles_m, tiles_n, tile_begin, pad;
+} gemmx_batch_desc;
+
+/* Tile edge of the batch kernel and the number of its workgroups that fit on
+ * the current device at once (the persistent grid). */
+int gemmx_batch_tile(void);
+int gemmx_batch_grid(int es);
+/* Zeroes the tile ticket, then runs the persistent grouped kernel. */
+hipError_t gemmx_batch_launch(int es, const gemmx_batch_desc *d_desc,
+                              const int *d_tile_prob, int total_tiles,
+                              unsigned *d_counter, int grid, hipStream_t s);
 
 /* Emulated DGEMM, Ozaki-I INT8 slices (gemm_slices.hip).  `work` is device
  * scratch of gemmx_ozaki_workspace() bytes. */

@@ -121,6 +121,39 @@ int sgemm_x_bf16x3_dev(char transa, char transb, int m, int n, int k,
                        int ldb, float beta, float *c, int ldc, void *work,
                        size_t work_bytes, void *stream);
 
+/* Grouped batch on device pointers.  No counterpart in the reference, which
+ * computes one GEMM per call (wjc404/GEMM_AVX2 src/gemm_driver.c:118); each
+ * problem i has the semantics of dgemm_/sgemm_ with transa[i], transb[i],
+ * m[i] ... ldc[i] (alpha/beta by value in arrays, a/b/c device pointers).
+ *
+ * _create checks every problem, orders them heaviest first, uploads the
+ * descriptors and captures one persistent grouped launch into a hipGraph on
+ * the current device; the argument arrays may be freed afterwards.  _run
+ * replays it on `stream` (NULL = null stream; the current device must be the
+ * plan's) and returns without synchronising; replays of one plan must not
+ * overlap.  Problems with m == 0, n == 0, or with no product and beta == 1
+ * are dropped; C blocks of different problems must not overlap.  _flops
+ * gives 2*m*n*k summed over the problems that have a product.  _destroy
+ * releases the plan (after its last replay has finished). */
+int dgemm_x_batch_create(void **plan, int count, const char *transa,
+                         const char *transb, const int *m, const int *n,
+                         const int *k, const double *alpha,
+                         const double *const *a, const int *lda,
+                         const double *const *b, const int *ldb,
+                         const double *beta, double *const *c, const int *ldc);
+int dgemm_x_batch_run(void *plan, void *stream);
+double dgemm_x_batch_flops(const void *plan);
+int dgemm_x_batch_destroy(void *plan);
+int sgemm_x_batch_create(void **plan, int count, const char *transa,
+                         const char *transb, const int *m, const int *n,
+                         const int *k, const float *alpha,
+                         const float *const *a, const int *lda,
+                         const float *const *b, const int *ldb,
+                         const float *beta, float *const *c, const int *ldc);
+int sgemm_x_batch_run(void *plan, void *stream);
+double sgemm_x_batch_flops(const void *plan);
+int sgemm_x_batch_destroy(void *plan);
+
 #ifdef __cplusplus
 }
 #endif

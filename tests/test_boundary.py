@@ -104,3 +104,22 @@ def test_mode_switches():
     wb = gx.sgemm_bf16x3_workspace(1000, 700, 333)
     # hi and lo BF16 slices of A and B, K padded to 64 elements
     assert wb >= 2 * 2 * (1000 + 700) * 384
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_batch_checks_arguments_before_the_device(dtype):
+    """Grouped batch (no reference counterpart): per-problem argument checks
+    name the problem; a valid batch on a machine without a device reports
+    ERR_NODEVICE (on a GPU machine it builds)."""
+    bad = [("N", "N", 4, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 4),
+           ("T", "N", 4, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 3)]
+    with pytest.raises(gx.GemmError) as ei:
+        gx.BatchPlan(dtype, bad)
+    assert ei.value.code == gx.ERR_ARG and "problem 1" in str(ei.value)
+    with pytest.raises(gx.GemmError) as ei:
+        gx.BatchPlan(dtype, [("N", "N", -1, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 4)])
+    assert ei.value.code == gx.ERR_ARG
+    if not _has_gpu():
+        with pytest.raises(gx.GemmError) as ei:
+            gx.BatchPlan(dtype, bad[:1])
+        assert ei.value.code == gx.ERR_NODEVICE
