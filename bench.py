@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--profile-only", choices=["native", "ozaki", "sgemm", "bf16x3", "batch"],
+    ap.add_argument("--profile-only",
+                    choices=["native", "ozaki", "sgemm", "bf16x3", "batch", "copy"],
                     help="run just one form (for rocprofv3)")
     return ap.parse_args()
 
@@ -245,6 +246,13 @@ def main():
         gx.dgemm_ozaki_dev("N", "N", M, N, K, ALPHA, dA, LD, dB, LD, BETA, dC, LD, s,
                            ws, ws.numel(), stream=stream)
 
+    if args.profile_only == "copy":
+        # a plain device copy of one 8200 x 8192 FP64 operand (537 MB read,
+        # 537 MB written): the calibration for the HBM byte counters
+        dst = torch.empty_like(dA)
+        wall, per = time_launches(torch, lambda: dst.copy_(dA), args.steps, args.warmup)
+        print(json.dumps({"profile_only": "copy", "ms": per, "bytes": dA.numel() * 8}))
+        return
     if args.profile_only == "batch":
         del ws, dA, dB, dC, C0
         plan, _, bufs = batch_workload(torch, gx, "d", stream)

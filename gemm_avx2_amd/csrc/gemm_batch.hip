@@ -25,6 +25,13 @@ constexpr int BT_THREADS = 64 * BT_WM * BT_WN;
 
 __global__ void batch_reset_kernel(unsigned *counter) { *counter = 0u; }
 
+/* Operand pointers read from the descriptors are generic; marking them as
+ * global lets every load through them be a global (not flat) load. */
+template <typename T>
+__device__ __forceinline__ T *as_global(T *p) {
+  return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(BT_THREADS, 2)
 gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
@@ -44,9 +51,9 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
     const int tm = local % q.tiles_m, tn = local / q.tiles_m;
     const int m0 = tm * BT_BM, n0 = tn * BT_BN;
     const T alpha = (T)q.alpha, beta = (T)q.beta;
-    const T *A = (const T *)q.a;
-    const T *B = (const T *)q.b;
-    T *C = (T *)q.c;
+    const T *A = as_global((const T *)q.a);
+    const T *B = as_global((const T *)q.b);
+    T *C = as_global((T *)q.c);
     const int f = q.flags;
     if (f & GEMMX_BF_SCALE) {
       /* alpha == 0 or k == 0: C = beta*C on this tile (beta != 1 here) */

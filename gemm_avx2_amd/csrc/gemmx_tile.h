@@ -70,8 +70,10 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
   typedef typename Vec16<T>::type vec_t;
   typedef T kvec_t __attribute__((ext_vector_type(KV)));
 
-  T *const sA[2] = {lds, lds + A_NL * A_LS};
-  T *const sB[2] = {lds + 2 * A_NL * A_LS, lds + 2 * A_NL * A_LS + B_NL * B_LS};
+  // Stage pointers by arithmetic on `lds` (an array of pointers indexed at
+  // run time would live in scratch and turn every LDS access into a flat one).
+  auto sA = [&](int st) { return lds + st * (A_NL * A_LS); };
+  auto sB = [&](int st) { return lds + 2 * A_NL * A_LS + st * (B_NL * B_LS); };
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -92,8 +94,8 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     cb.load_vec(bbase(k0), ldb, tid);
   };
   auto store = [&](int buf) {
-    ca.template store<A_LS>(sA[buf], tid);
-    cb.template store<B_LS>(sB[buf], tid);
+    ca.template store<A_LS>(sA(buf), tid);
+    cb.template store<B_LS>(sB(buf), tid);
   };
 
   acc_t acc[FM][FN];
@@ -106,8 +108,8 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
 
   /* KV consecutive MFMA steps of one slice from LDS buffer `buf`. */
   auto compute = [&](int buf) {
-    const T *as = sA[buf];
-    const T *bs = sB[buf];
+    const T *as = sA(buf);
+    const T *bs = sB(buf);
 #pragma unroll
     for (int s0 = 0; s0 < KS; s0 += KV) {
       const int kx = kg * KS + s0;
