@@ -155,6 +155,24 @@ def cpu_baseline():
             "kind": kind, "sample": sample, "seconds": round(best, 3)}
 
 
+# ---------------------------------------------------------------- HBM traffic
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profiled_traffic(summary):
+    """HBM bytes per launch of a kernel, as recorded in its rocprofv3
+    summary under profiles/ (FETCH_SIZE and WRITE_SIZE runs scaled by the
+    plain-copy calibration); None when the summary is absent."""
+    try:
+        with open(os.path.join(PROFILES, summary)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 # ---------------------------------------------------------------- grouped batch
 BATCH_COUNT, BATCH_LO, BATCH_HI = 4096, 16, 1024
 
@@ -366,7 +384,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(flops / ev_n / 1e9, 3),
                      "peak": PEAK["f64"], "unit": "TFLOP/s",
                      "frac": round(flops / ev_n / 1e9 / PEAK["f64"], 4),
-                     "traffic": None, "kernel": "gemm_native_kernel<double> (native FP64 MFMA)"},
+                     "traffic": profiled_traffic("dgemm_native_nn_8192.md"),
+                     "kernel": "gemm_native_kernel<double> (native FP64 MFMA)"},
     }
     line.update(result)
     if not args.no_cpu:

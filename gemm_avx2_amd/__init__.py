@@ -25,8 +25,8 @@ import os
 __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
-    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "last_error", "fill",
-    "BatchPlan", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
+    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "tune_lookup", "last_error", "fill",
+    "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
 ]
 
 LIB_DIR = os.environ.get("GEMMX_LIB_DIR") or os.path.join(
@@ -113,6 +113,9 @@ def _declare_batch(lib, dtype):
     f = getattr(lib, f"{dtype}gemm_x_batch_destroy")
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_void_p]
+    f = getattr(lib, f"{dtype}gemm_x_batch_set_config")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int]
 
 
 def load(name: str):
@@ -305,6 +308,18 @@ class BatchPlan:
             pass
 
 
+BATCH_CONFIGS = 4
+
+
+def set_batch_config(dtype: str, cfg: int):
+    """Batch kernel tile configuration for plans created afterwards ('d' or
+    's'); -1 returns to GEMMX_BATCH_CFG / the default."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    rc = getattr(lib, f"{dtype}gemm_x_batch_set_config")(cfg)
+    if rc:
+        _check(dtype, rc)
+
+
 TILE_CONFIGS = 7
 
 
@@ -318,10 +333,20 @@ def set_tile_config(dtype: str, cfg: int):
 
 
 def load_tuning(path, dtype: str = "d") -> int:
-    """Loads a tuning table into the library of `dtype`; returns entries read."""
+    """Loads a tuning table into the library of `dtype`; returns entries read,
+    or -1 when the file is unreadable or of another format version or arch."""
     lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
     return int(getattr(lib, f"{dtype}gemm_x_load_tuning")(
         None if path is None else str(path).encode()))
+
+
+def tune_lookup(dtype, transa, transb, m, n, k) -> int:
+    """Tile configuration a call with these arguments would use."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    f = getattr(lib, f"{dtype}gemm_x_tune_lookup")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    return int(f(_ch(transa), _ch(transb), m, n, k))
 
 
 def set_dgemm_mode(mode: int, slices: int = 8):

@@ -1,0 +1,225 @@
+"""Autotuner: measures every native tile configuration on a set of shapes
+and writes the tuning table the libraries read at first use.
+
+It takes the place of the reference's hand-edited compile-time tuning
+headers (wjc404/GEMM_AVX2 dgemm_tune.h:6-10, sgemm_tune.h:7-11, README.md:13-15:
+edit UNROLL_N / block sizes, rebuild, time with an external benchmark).  Here
+the choice is a run-time table keyed by (element type, transposes, shape
+bucket), bucket = floor(log2(dim)) clamped to 4..15, so one build serves every
+shape and retuning needs no rebuild.
+
+    python -m gemm_avx2_amd.autotune [--out PATH] [--dtypes d,s]
+        [--trans NN,NT,TN,TT] [--set quick|full] [--reps N] [--resume]
+
+For each (dtype, trans, shape) every configuration runs on device-resident
+U[-1,1) operands (alpha = 1.5, beta = -0.5); the fastest by median event time
+is written, after its result is checked against configuration 0's on the same
+inputs (|difference| within 64 k u (|alpha| k + |beta|), the bound for
+U[-1,1) inputs).  With --resume, keys already in the output table are kept and
+not measured again.  The table is written in sorted key order with the
+measured times as comments.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import platform
+import re
+import sys
+import time
+
+FORMAT = 1
+TILE_CONFIGS = 7  # compiled native tile configurations (GEMMX_TILE_CONFIGS)
+DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
+                           "gemm_mi355x_tune.txt")
+TRANS = ("NN", "NT", "TN", "TT")
+ALPHA, BETA = 1.5, -0.5
+
+# Shapes per set; every shape fills the table entry of its bucket.
+SHAPES = {
+    "quick": [(s, s, s) for s in (256, 1024, 4096, 8192)],
+    "full": [(s, s, s) for s in (128, 256, 512, 1024, 2048, 4096, 8192, 16384)]
+    + [(8192, 8192, 256), (8192, 8192, 1024), (8192, 256, 8192), (256, 8192, 8192),
+       (16384, 1024, 1024), (1024, 16384, 1024), (4096, 4096, 16384), (2048, 2048, 8192),
+       (65536, 256, 256), (512, 512, 65536)],
+}
+
+
+def bucket(dim: int) -> int:
+    """floor(log2(dim)) clamped to 4..15, as the libraries compute it."""
+    b = max(int(dim), 1).bit_length() - 1
+    return min(max(b, 4), 15)
+
+
+def key_of(dtype: str, trans: str, m: int, n: int, k: int):
+    return ("f64" if dtype == "d" else "f32", trans, bucket(m), bucket(n), bucket(k))
+
+
+# ------------------------------------------------------------------ table I/O
+_ENTRY = re.compile(r"^\s*(f64|f32)\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)\s*(#.*)?$")
+
+
+def read_table(path):
+    """(entries {key: cfg}, comments {key: text}, arch or None); later lines
+    replace earlier ones, like the libraries' reader."""
+    entries, notes, arch = {}, {}, None
+    if not os.path.exists(path):
+        return entries, notes, arch
+    with open(path) as f:
+        for line in f:
+            body = line.split("#", 1)[0].split()
+            if len(body) == 2 and body[0] == "format" and int(body[1]) != FORMAT:
+                raise ValueError(f"{path}: table format {body[1]}, expected {FORMAT}")
+            if len(body) == 2 and body[0] == "arch":
+                arch = body[1]
+                continue
+            m = _ENTRY.match(line)
+            if m:
+                key = (m.group(1), m.group(2), int(m.group(3)), int(m.group(4)), int(m.group(5)))
+                cfg = int(m.group(6))
+                # the libraries skip entries outside the buckets or configurations
+                if not all(4 <= b <= 15 for b in key[2:]) or not 0 <= cfg < TILE_CONFIGS:
+                    continue
+                entries[key] = cfg
+                notes[key] = (m.group(7) or "").lstrip("# ").rstrip()
+    return entries, notes, arch
+
+
+def write_table(path, entries, notes, arch, header=()):
+    """Writes the table in sorted key order (the same entries always give
+    the same file)."""
+    lines = ["# gemm_mi355x tuning table, written by gemm_avx2_amd.autotune"]
+    lines += [f"# {h}" for h in header]
+    lines += [f"format {FORMAT}"]
+    if arch:
+        lines += [f"arch {arch}"]
+    lines += ["# dtype trans mbucket nbucket kbucket cfg"]
+    order = {"f64": 0, "f32": 1}
+    for key in sorted(entries, key=lambda k: (order[k[0]], TRANS.index(k[1]), k[2], k[3], k[4])):
+        note = notes.get(key, "")
+        lines.append(f"{key[0]} {key[1]} {key[2]} {key[3]} {key[4]} {entries[key]}"
+                     + (f"  # {note}" if note else ""))
+    tmp = path + ".tmp"
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(tmp, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    os.replace(tmp, path)
+
+
+def pending(wanted, entries, resume):
+    """(dtype, trans, m, n, k) still to measure, one per table key: all of
+    them, or with resume those whose key is not in the table yet."""
+    out, seen = [], set()
+    for w in wanted:
+        key = key_of(*w)
+        if key in seen or (resume and key in entries):
+            continue
+        seen.add(key)
+        out.append(w)
+    return out
+
+
+# ------------------------------------------------------------------ measurement
+def _median(xs):
+    xs = sorted(xs)
+    n = len(xs)
+    return xs[n // 2] if n % 2 else 0.5 * (xs[n // 2 - 1] + xs[n // 2])
+
+
+def measure(torch, gx, dtype, trans, m, n, k, reps):
+    """Median event time (ms) of every configuration, and the index of the
+    fastest whose result agrees with configuration 0's."""
+    tdt = torch.float64 if dtype == "d" else torch.float32
+    u = 2.0 ** -53 if dtype == "d" else 2.0 ** -24
+    ra, ca = (m, k) if trans[0] == "N" else (k, m)
+    rb, cb = (k, n) if trans[1] == "N" else (n, k)
+    A = torch.empty(ra * ca, dtype=tdt, device="cuda")
+    B = torch.empty(rb * cb, dtype=tdt, device="cuda")
+    C0 = torch.empty(m * n, dtype=tdt, device="cuda")
+    gx.fill(A, ra, ca, ra, 97, 0)
+    gx.fill(B, rb, cb, rb, 97, 1)
+    gx.fill(C0, m, n, m, 97, 2)
+    C = torch.empty_like(C0)
+    dev = gx.dgemm_dev if dtype == "d" else gx.sgemm_dev
+    stream = torch.cuda.current_stream()
+
+    def call():
+        dev(trans[0], trans[1], m, n, k, ALPHA, A, ra, B, rb, BETA, C, m, stream=stream)
+
+    times, ref, ok = [], None, []
+    bound = 64 * k * u * (abs(ALPHA) * k + abs(BETA))
+    try:
+        for cfg in range(gx.TILE_CONFIGS):
+            gx.set_tile_config(dtype, cfg)
+            C.copy_(C0)
+            call()
+            torch.cuda.synchronize()
+            if ref is None:
+                ref = C.clone()
+                ok.append(True)
+            else:
+                ok.append(bool((C - ref).abs().max().item() <= 2 * bound))
+            ev = []
+            for _ in range(reps):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                call()
+                e1.record(stream)
+                e1.synchronize()
+                ev.append(e0.elapsed_time(e1))
+            times.append(_median(ev))
+    finally:
+        gx.set_tile_config(dtype, -1)
+    best = min((t, c) for c, t in enumerate(times) if ok[c])[1]
+    return times, ok, best
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--dtypes", default="d,s")
+    ap.add_argument("--trans", default=",".join(TRANS))
+    ap.add_argument("--set", default="quick", choices=sorted(SHAPES))
+    ap.add_argument("--shapes", default=None, help="m,n,k;m,n,k;... instead of --set")
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--resume", action="store_true")
+    args = ap.parse_args(argv)
+
+    import torch
+    import gemm_avx2_amd as gx
+
+    if not torch.cuda.is_available():
+        raise SystemExit("autotune: no GPU visible")
+    props = torch.cuda.get_device_properties(0)
+    arch = getattr(props, "gcnArchName", "unknown").split(":")[0]
+    shapes = ([tuple(int(v) for v in s.split(",")) for s in args.shapes.split(";")]
+              if args.shapes else SHAPES[args.set])
+    entries, notes, old_arch = read_table(args.out) if args.resume else ({}, {}, None)
+    if old_arch and old_arch != arch:
+        raise SystemExit(f"autotune: {args.out} is for {old_arch}, this device is {arch}")
+    header = [f"device: {props.name}, {props.multi_processor_count} CUs, arch {arch}",
+              f"hip: {torch.version.hip}; torch: {torch.__version__}; host: {platform.node()}",
+              f"written: {time.strftime('%Y-%m-%d %H:%M:%S')}; set: {args.shapes or args.set}; "
+              f"reps: {args.reps}"]
+    wanted = []
+    for d in args.dtypes.split(","):
+        for t in args.trans.split(","):
+            for (m, n, k) in shapes:
+                wanted.append((d, t, m, n, k))
+    todo = pending(wanted, entries, args.resume)
+    for i, (d, t, m, n, k) in enumerate(todo):
+        times, ok, best = measure(torch, gx, d, t, m, n, k, args.reps)
+        key = key_of(d, t, m, n, k)
+        entries[key] = best
+        notes[key] = f"{m}x{n}x{k} ms: " + " ".join(
+            f"c{c}={tm:.3f}" + ("" if ok[c] else "!") for c, tm in enumerate(times))
+        write_table(args.out, entries, notes, arch, header)  # after every shape: resumable
+        print(f"[{i + 1}/{len(todo)}] {key[0]} {t} {m}x{n}x{k}: cfg {best} "
+              f"({times[best]:.3f} ms; cfg0 {times[0]:.3f} ms)", flush=True)
+    print(f"autotune: {len(entries)} entries in {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

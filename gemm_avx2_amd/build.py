@@ -4,6 +4,7 @@ Outputs (gemm_avx2_amd/lib/):
   DGEMM.so         dgemm_ and the dgemm_x_* extensions
   SGEMM.so         sgemm_ and the sgemm_x_* extensions
   libgemmx_gen.so  device-side input generator used by bench.py and the tests
+  gemm_mi355x_tune.txt  the shipped gfx950 tuning table (tuning/), if present
 
 Each HIP source is compiled once with hipcc for gfx950; the host C layer is
 compiled with gcc; the two BLAS libraries are linked from the same objects and
@@ -32,6 +33,8 @@ HIP_SOURCES = ["gemm_native.hip", "gemm_split.hip", "gemm_products.hip", "gemm_e
                "gemm_batch.hip"]
 C_SOURCES = ["gemmx_api.c", "gemmx_tune.c", "gemmx_batch.c"]
 GEN_SOURCES = ["gemmx_gen.hip"]
+# tuning table measured on MI355X by gemm_avx2_amd.autotune (--set full)
+SHIPPED_TABLE = os.path.join(HERE, "tuning", "gemm_mi355x_tune_gfx950.txt")
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
@@ -104,6 +107,12 @@ def build(jobs: int = 8, verbose: bool = False, lib_dir: str = LIB, build_dir: s
           *[obj(s) for s in GEN_SOURCES], f"-L{ROCM}/lib", "-lamdhip64"], verbose)
     os.replace(gen + ".tmp", gen)
     outputs["libgemmx_gen.so"] = gen
+    # the shipped tuning table goes beside the libraries, where they look first
+    # (a table written there by the autotuner is replaced on every build)
+    if os.path.exists(SHIPPED_TABLE):
+        table = os.path.join(LIB, "gemm_mi355x_tune.txt")
+        shutil.copyfile(SHIPPED_TABLE, table)
+        outputs["gemm_mi355x_tune.txt"] = table
     return outputs
 
 

@@ -7,10 +7,13 @@
  *
  * The grid is as large as the device holds at once; each workgroup takes the
  * next tile with an atomic ticket until the tickets run out, so every wave
- * reaches the exit.  Tiles are numbered heaviest problem first (the host
- * sorts), which with dynamic tickets approximates longest-processing-time
- * scheduling.  A tile is computed by the same gemm_tile as the single-GEMM
- * kernels, instantiated for the four transpose pairs and chosen per tile.
+ * reaches the exit.  The ticket for the next tile is requested when a tile
+ * starts, so its latency hides behind the tile's work.  Tiles are numbered
+ * heaviest problem first (the host sorts), which with dynamic tickets
+ * approximates longest-processing-time scheduling.  A tile is computed by
+ * the same gemm_tile as the single-GEMM kernels: the branch-free loop for
+ * interior tiles of 16-byte loadable operands, the bounds-checked loop for
+ * edge tiles, instantiated for the four transpose pairs and chosen per tile.
  */
 #include "gemmx_device.h"
 #include "gemmx_internal.h"
@@ -20,36 +23,46 @@ using namespace gemmx;
 
 namespace {
 
-constexpr int BT_BM = 64, BT_BN = 64, BT_BK = 16, BT_WM = 2, BT_WN = 2;
-constexpr int BT_THREADS = 64 * BT_WM * BT_WN;
+constexpr int BT_THREADS = 256; /* 4 waves in every configuration */
+
+/* Tile configurations of the batch kernel (BM x BN x BK, waves WM x WN). */
+template <int CFG> struct BatchCfg;
+template <> struct BatchCfg<0> { static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2; };
+template <> struct BatchCfg<1> { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };
+template <> struct BatchCfg<2> { static constexpr int BM = 128, BN = 64, BK = 16, WM = 2, WN = 2; };
+template <> struct BatchCfg<3> { static constexpr int BM = 32, BN = 32, BK = 32, WM = 2, WN = 2; };
 
 __global__ void batch_reset_kernel(unsigned *counter) { *counter = 0u; }
 
-/* Operand pointers read from the descriptors are generic; marking them as
- * global lets every load through them be a global (not flat) load. */
+/* Operand pointers read from the descriptors are generic; building them in
+ * the global address space lets every load through them be a global (not
+ * flat) load. */
 template <typename T>
 __device__ __forceinline__ T *as_global(T *p) {
   return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
 }
 
-template <typename T>
+template <typename T, int CFG>
 __global__ void __launch_bounds__(BT_THREADS, 2)
 gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
                   const int *__restrict__ tile_prob, int total,
                   unsigned *__restrict__ counter) {
-  __shared__ __align__(16) T lds[tile_lds_max<T, BT_BM, BT_BN, BT_BK>()];
+  typedef BatchCfg<CFG> G;
+  static_assert(64 * G::WM * G::WN == BT_THREADS, "batch block size");
+  __shared__ __align__(16) T lds[tile_lds_max<T, G::BM, G::BN, G::BK>()];
   __shared__ int ticket;
-  for (;;) {
-    if (threadIdx.x == 0) ticket = (int)atomicAdd(counter, 1u);
-    __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(ticket);
-    __syncthreads(); /* everyone has read the ticket before it is replaced */
-    if (t >= total) break;
+  if (threadIdx.x == 0) ticket = (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  int t = __builtin_amdgcn_readfirstlane(ticket);
+  while (t < total) {
+    /* request the next tile now; the value is needed only after this one */
+    unsigned next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
     const int p = __builtin_amdgcn_readfirstlane(tile_prob[t]);
     const gemmx_batch_desc &q = desc[p];
     const int local = t - q.tile_begin;
     const int tm = local % q.tiles_m, tn = local / q.tiles_m;
-    const int m0 = tm * BT_BM, n0 = tn * BT_BN;
+    const int m0 = tm * G::BM, n0 = tn * G::BN;
     const T alpha = (T)q.alpha, beta = (T)q.beta;
     const T *A = as_global((const T *)q.a);
     const T *B = as_global((const T *)q.b);
@@ -57,56 +70,90 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
     const int f = q.flags;
     if (f & GEMMX_BF_SCALE) {
       /* alpha == 0 or k == 0: C = beta*C on this tile (beta != 1 here) */
-      for (int e = threadIdx.x; e < BT_BM * BT_BN; e += BT_THREADS) {
-        const int i = m0 + e % BT_BM, j = n0 + e / BT_BM;
+      for (int e = threadIdx.x; e < G::BM * G::BN; e += BT_THREADS) {
+        const int i = m0 + e % G::BM, j = n0 + e / G::BM;
         if (i < q.m && j < q.n) C[i + j * q.ldc] *= beta;
       }
     } else {
       const int va = (f & GEMMX_BF_VEC_A) != 0, vb = (f & GEMMX_BF_VEC_B) != 0;
-#define GEMMX_TILE(TA, TB)                                                              \
-  gemm_tile<T, TA, TB, BT_BM, BT_BN, BT_BK, BT_WM, BT_WN, 1, false>(                    \
+      /* interior tiles with 16-byte loadable operands take the branch-free
+       * loop; edge tiles the bounds-checked one */
+      const int fast = va && vb && m0 + G::BM <= q.m && n0 + G::BN <= q.n;
+#define GEMMX_TILE(TA, TB, FAST)                                                        \
+  gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, FAST>(                     \
       A, B, C, q.m, q.n, q.k, q.lda, q.ldb, q.ldc, alpha, beta, m0, n0, va, vb, lds)
-      switch (f & (GEMMX_BF_TA | GEMMX_BF_TB)) {
-      case 0: GEMMX_TILE(false, false); break;
-      case GEMMX_BF_TB: GEMMX_TILE(false, true); break;
-      case GEMMX_BF_TA: GEMMX_TILE(true, false); break;
-      default: GEMMX_TILE(true, true); break;
+      switch ((f & (GEMMX_BF_TA | GEMMX_BF_TB)) | (fast ? 4 : 0)) {
+      case 0: GEMMX_TILE(false, false, false); break;
+      case GEMMX_BF_TB: GEMMX_TILE(false, true, false); break;
+      case GEMMX_BF_TA: GEMMX_TILE(true, false, false); break;
+      case GEMMX_BF_TA | GEMMX_BF_TB: GEMMX_TILE(true, true, false); break;
+      case 4: GEMMX_TILE(false, false, true); break;
+      case 4 | GEMMX_BF_TB: GEMMX_TILE(false, true, true); break;
+      case 4 | GEMMX_BF_TA: GEMMX_TILE(true, false, true); break;
+      default: GEMMX_TILE(true, true, true); break;
       }
 #undef GEMMX_TILE
     }
-    __syncthreads(); /* LDS of this tile is free before the next tile's stores */
+    /* every wave is done with this tile's LDS and has read `ticket` */
+    __syncthreads();
+    if (threadIdx.x == 0) ticket = (int)next;
+    __syncthreads();
+    t = __builtin_amdgcn_readfirstlane(ticket);
   }
 }
 
-template <typename T>
+template <typename T, int CFG>
 int batch_grid() {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_batch_kernel<T>, BT_THREADS, 0) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_batch_kernel<T, CFG>, BT_THREADS,
+                                                   0) != hipSuccess)
     return 0;
   return cus * (per > 0 ? per : 1);
 }
 
-} // namespace
-
-extern "C" int gemmx_batch_tile(void) { return BT_BM; }
-
-extern "C" int gemmx_batch_grid(int es) {
-  return es == 8 ? batch_grid<double>() : batch_grid<float>();
+template <typename T, int CFG>
+void launch(const gemmx_batch_desc *d, const int *tp, int total, unsigned *ctr, int grid,
+            hipStream_t s) {
+  hipLaunchKernelGGL((gemm_batch_kernel<T, CFG>), dim3(grid), dim3(BT_THREADS), 0, s, d, tp,
+                     total, ctr);
 }
 
-extern "C" hipError_t gemmx_batch_launch(int es, const gemmx_batch_desc *d_desc,
+} // namespace
+
+extern "C" int gemmx_batch_tile(int cfg, int *bm, int *bn) {
+  switch (cfg) {
+  case 0: *bm = BatchCfg<0>::BM; *bn = BatchCfg<0>::BN; return 0;
+  case 1: *bm = BatchCfg<1>::BM; *bn = BatchCfg<1>::BN; return 0;
+  case 2: *bm = BatchCfg<2>::BM; *bn = BatchCfg<2>::BN; return 0;
+  case 3: *bm = BatchCfg<3>::BM; *bn = BatchCfg<3>::BN; return 0;
+  default: return -1;
+  }
+}
+
+extern "C" int gemmx_batch_grid(int es, int cfg) {
+  switch (cfg) {
+  case 0: return es == 8 ? batch_grid<double, 0>() : batch_grid<float, 0>();
+  case 1: return es == 8 ? batch_grid<double, 1>() : batch_grid<float, 1>();
+  case 2: return es == 8 ? batch_grid<double, 2>() : batch_grid<float, 2>();
+  case 3: return es == 8 ? batch_grid<double, 3>() : batch_grid<float, 3>();
+  default: return 0;
+  }
+}
+
+extern "C" hipError_t gemmx_batch_launch(int es, int cfg, const gemmx_batch_desc *d_desc,
                                          const int *d_tile_prob, int total_tiles,
                                          unsigned *d_counter, int grid, hipStream_t s) {
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, d_counter);
-  if (es == 8)
-    hipLaunchKernelGGL(gemm_batch_kernel<double>, dim3(grid), dim3(BT_THREADS), 0, s, d_desc,
-                       d_tile_prob, total_tiles, d_counter);
-  else
-    hipLaunchKernelGGL(gemm_batch_kernel<float>, dim3(grid), dim3(BT_THREADS), 0, s, d_desc,
-                       d_tile_prob, total_tiles, d_counter);
+  const bool d = es == 8;
+  switch (cfg) {
+  case 0: (d ? launch<double, 0> : launch<float, 0>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
+  case 1: (d ? launch<double, 1> : launch<float, 1>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
+  case 2: (d ? launch<double, 2> : launch<float, 2>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
+  case 3: (d ? launch<double, 3> : launch<float, 3>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
+  default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }

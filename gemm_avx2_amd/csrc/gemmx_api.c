@@ -112,7 +112,10 @@ static void init_once(void) {
     g_ndev = 0;
     (void)hipGetLastError();
   }
-  gemmx_tune_init();
+  /* the tuning table is checked against the arch of device 0 */
+  hipDeviceProp_t prop;
+  const int have_arch = g_ndev > 0 && hipGetDeviceProperties(&prop, 0) == hipSuccess;
+  gemmx_tune_init(have_arch ? prop.gcnArchName : NULL);
 }
 
 static int current_device(int *dev) {
@@ -388,6 +391,21 @@ EXPORT int dgemm_x_load_tuning(const char *path) {
 EXPORT int sgemm_x_load_tuning(const char *path) {
   pthread_once(&g_once, init_once);
   return gemmx_tune_load(path);
+}
+
+/* The tile configuration a call with these arguments would use (forced
+ * configuration, else the table entry of its shape bucket, else the size
+ * heuristic).  Needs no device. */
+static int tune_lookup(int es, char transa, char transb, int m, int n, int k) {
+  pthread_once(&g_once, init_once);
+  if (m < 0 || n < 0 || k < 0) return set_error(GEMMX_ERR_ARG, "negative dimension"), -1;
+  return gemmx_tune_pick(es, !notrans(transa), !notrans(transb), m, n, k);
+}
+EXPORT int dgemm_x_tune_lookup(char transa, char transb, int m, int n, int k) {
+  return tune_lookup(8, transa, transb, m, n, k);
+}
+EXPORT int sgemm_x_tune_lookup(char transa, char transb, int m, int n, int k) {
+  return tune_lookup(4, transa, transb, m, n, k);
 }
 
 EXPORT int dgemm_x_last_error(const char **msg) {

@@ -20,7 +20,7 @@
 #define EXPORT __attribute__((visibility("default")))
 
 struct gemmx_batch {
-  int es, device, count, total_tiles, grid;
+  int es, cfg, device, count, total_tiles, grid;
   double flops;
   gemmx_batch_desc *d_desc;
   int *d_tile_prob;
@@ -54,6 +54,31 @@ static void destroy(struct gemmx_batch *p) {
 
 static int notrans(char c) { return c == 'N' || c == 'n'; }
 
+/* Batch kernel configuration per element size (index 0: FP64, 1: FP32);
+ * -1 = GEMMX_BATCH_CFG from the environment, else the default. */
+static int g_batch_cfg[2] = {-1, -1};
+static const int k_batch_default[2] = {0, 0};
+
+static int batch_cfg(int es) {
+  const int i = es == 8 ? 0 : 1;
+  if (g_batch_cfg[i] >= 0) return g_batch_cfg[i];
+  const char *e = getenv("GEMMX_BATCH_CFG");
+  if (e && *e) {
+    const int c = atoi(e);
+    if (c >= 0 && c < GEMMX_BATCH_CONFIGS) return c;
+  }
+  return k_batch_default[i];
+}
+
+static int set_batch_cfg(int es, int cfg) {
+  gemmx_clear_error();
+  if (cfg < -1 || cfg >= GEMMX_BATCH_CONFIGS)
+    return gemmx_report(GEMMX_ERR_ARG, "batch configuration %d not in -1..%d", cfg,
+                        GEMMX_BATCH_CONFIGS - 1);
+  g_batch_cfg[es == 8 ? 0 : 1] = cfg;
+  return GEMMX_OK;
+}
+
 /* alpha/beta/a/b/c arrays hold doubles and double* (es 8) or floats and
  * float* (es 4). */
 static int create(int es, void **out, int count, const char *transa, const char *transb,
@@ -67,7 +92,10 @@ static int create(int es, void **out, int count, const char *transa, const char 
   if (count > 0 && (!transa || !transb || !m || !n || !k || !alpha || !a || !lda || !b ||
                     !ldb || !beta || !c || !ldc))
     return gemmx_report(GEMMX_ERR_ARG, "batch: NULL argument array");
-  const int tile = gemmx_batch_tile();
+  const int cfg = batch_cfg(es);
+  int tbm = 0, tbn = 0;
+  if (gemmx_batch_tile(cfg, &tbm, &tbn) != 0)
+    return gemmx_report(GEMMX_ERR_ARG, "batch: unknown configuration %d", cfg);
   gemmx_batch_desc *h = (gemmx_batch_desc *)calloc(count > 0 ? count : 1, sizeof *h);
   order_t *ord = (order_t *)calloc(count > 0 ? count : 1, sizeof *ord);
   struct gemmx_batch *p = (struct gemmx_batch *)calloc(1, sizeof *p);
@@ -78,6 +106,7 @@ static int create(int es, void **out, int count, const char *transa, const char 
     goto fail;
   }
   p->es = es;
+  p->cfg = cfg;
   for (int i = 0; i < count; i++) {
     const double al = es == 8 ? ((const double *)alpha)[i] : (double)((const float *)alpha)[i];
     const double be = es == 8 ? ((const double *)beta)[i] : (double)((const float *)beta)[i];
@@ -115,8 +144,8 @@ static int create(int es, void **out, int count, const char *transa, const char 
                ((((uintptr_t)a[i] % 16) == 0 && lda[i] % ve == 0) ? GEMMX_BF_VEC_A : 0) |
                ((((uintptr_t)b[i] % 16) == 0 && ldb[i] % ve == 0) ? GEMMX_BF_VEC_B : 0) |
                (product ? 0 : GEMMX_BF_SCALE);
-    d->tiles_m = (m[i] + tile - 1) / tile;
-    d->tiles_n = (n[i] + tile - 1) / tile;
+    d->tiles_m = (m[i] + tbm - 1) / tbm;
+    d->tiles_n = (n[i] + tbn - 1) / tbn;
     ord[used].work = product ? (double)k[i] * m[i] * n[i] : 0.0;
     ord[used].idx = used;
     if (product) p->flops += 2.0 * m[i] * n[i] * k[i];
@@ -169,7 +198,7 @@ static int create(int es, void **out, int count, const char *transa, const char 
     rc = gemmx_report(GEMMX_ERR_LAUNCH, "batch: descriptor upload failed: %s", hipGetErrorString(e));
     goto fail;
   }
-  p->grid = gemmx_batch_grid(es);
+  p->grid = gemmx_batch_grid(es, cfg);
   if (p->grid <= 0) {
     rc = gemmx_report(GEMMX_ERR_LAUNCH, "batch: occupancy query failed");
     goto fail;
@@ -183,7 +212,7 @@ static int create(int es, void **out, int count, const char *transa, const char 
     goto fail;
   }
   const hipError_t le = p->total_tiles > 0
-                           ? gemmx_batch_launch(es, p->d_desc, p->d_tile_prob, p->total_tiles,
+                           ? gemmx_batch_launch(es, cfg, p->d_desc, p->d_tile_prob, p->total_tiles,
                                                 p->d_counter, p->grid, p->capture)
                            : hipSuccess;
   e = hipStreamEndCapture(p->capture, &p->graph);
@@ -247,6 +276,9 @@ EXPORT double dgemm_x_batch_flops(const void *plan) {
 EXPORT double sgemm_x_batch_flops(const void *plan) {
   return plan ? ((const struct gemmx_batch *)plan)->flops : 0.0;
 }
+
+EXPORT int dgemm_x_batch_set_config(int cfg) { return set_batch_cfg(8, cfg); }
+EXPORT int sgemm_x_batch_set_config(int cfg) { return set_batch_cfg(4, cfg); }
 
 EXPORT int dgemm_x_batch_destroy(void *plan) {
   destroy((struct gemmx_batch *)plan);

@@ -83,12 +83,15 @@ typedef struct gemmx_batch_desc {
   int tiles_m, tiles_n, tile_begin, pad;
 } gemmx_batch_desc;
 
-/* Tile edge of the batch kernel and the number of its workgroups that fit on
- * the current device at once (the persistent grid). */
-int gemmx_batch_tile(void);
-int gemmx_batch_grid(int es);
+/* Batch kernel configurations (gemm_batch.hip): 0 64x64x16, 1 64x64x32,
+ * 2 128x64x16, 3 32x32x32.  gemmx_batch_tile gives the tile of `cfg` (-1 for
+ * an unknown one); gemmx_batch_grid the number of its workgroups that fit on
+ * the current device at once (the persistent grid, 0 on failure). */
+#define GEMMX_BATCH_CONFIGS 4
+int gemmx_batch_tile(int cfg, int *bm, int *bn);
+int gemmx_batch_grid(int es, int cfg);
 /* Zeroes the tile ticket, then runs the persistent grouped kernel. */
-hipError_t gemmx_batch_launch(int es, const gemmx_batch_desc *d_desc,
+hipError_t gemmx_batch_launch(int es, int cfg, const gemmx_batch_desc *d_desc,
                               const int *d_tile_prob, int total_tiles,
                               unsigned *d_counter, int grid, hipStream_t s);
 

@@ -22,6 +22,7 @@ static signed char g_table[2][4][NB][NB][NB]; /* [f64/f32][trans][m][n][k] */
 static int g_entries;
 static int g_force[2] = {-1, -1}; /* [f64/f32]: one configuration for every call */
 static char g_path[1024];
+static char g_arch[64];           /* current device arch, "" = unknown */
 
 int gemmx_tune_bucket(int dim) {
   int b = 0;
@@ -42,31 +43,66 @@ static int trans_index(const char *t) {
   return -1;
 }
 
-static void load_table(const char *path) {
+static void clear_table(void) {
+  memset(g_table, 0, sizeof g_table);
+  g_entries = 0;
+  g_path[0] = 0;
+}
+
+/* Reads a table; on a format or arch mismatch nothing is kept. */
+static int load_table(const char *path) {
   FILE *f = fopen(path, "r");
-  if (!f) return;
+  if (!f) return -1;
+  static signed char staged[2][4][NB][NB][NB];
+  memset(staged, 0, sizeof staged);
   char line[256];
-  while (fgets(line, sizeof line, f)) {
+  int format = 0, entries = 0, ok = 1;
+  while (ok && fgets(line, sizeof line, f)) {
     char *hash = strchr(line, '#');
     if (hash) *hash = 0;
+    char w0[32], w1[64];
+    if (sscanf(line, "%31s %63s", w0, w1) == 2) {
+      if (strcmp(w0, "format") == 0) {
+        format = atoi(w1);
+        if (format != GEMMX_TUNE_FORMAT) ok = 0;
+        continue;
+      }
+      if (strcmp(w0, "arch") == 0) {
+        if (g_arch[0] && strcmp(w1, g_arch) != 0) ok = 0;
+        continue;
+      }
+    }
     char dt[8], tr[8];
     int mb, nb, kb, cfg;
     if (sscanf(line, "%7s %7s %d %d %d %d", dt, tr, &mb, &nb, &kb, &cfg) != 6) continue;
+    if (format != GEMMX_TUNE_FORMAT) {
+      ok = 0; /* entries before (or without) a format directive */
+      break;
+    }
     int d = strcmp(dt, "f64") == 0 ? 0 : strcmp(dt, "f32") == 0 ? 1 : -1;
     int t = trans_index(tr);
     if (d < 0 || t < 0 || mb < 4 || mb > 15 || nb < 4 || nb > 15 || kb < 4 || kb > 15) continue;
     if (cfg < 0 || cfg >= GEMMX_NCFG_COUNT) continue;
-    g_table[d][t][mb - 4][nb - 4][kb - 4] = (signed char)(cfg + 1);
-    g_entries++;
+    staged[d][t][mb - 4][nb - 4][kb - 4] = (signed char)(cfg + 1);
+    entries++;
   }
   fclose(f);
+  if (!ok || format != GEMMX_TUNE_FORMAT) return -1;
+  memcpy(g_table, staged, sizeof g_table);
+  g_entries = entries;
   snprintf(g_path, sizeof g_path, "%s", path);
+  return entries;
 }
 
-void gemmx_tune_init(void) {
-  memset(g_table, 0, sizeof g_table);
-  g_entries = 0;
-  g_path[0] = 0;
+void gemmx_tune_init(const char *arch) {
+  clear_table();
+  g_arch[0] = 0;
+  if (arch) {
+    /* "gfx950:sramecc+:xnack-" -> "gfx950" */
+    snprintf(g_arch, sizeof g_arch, "%s", arch);
+    char *colon = strchr(g_arch, ':');
+    if (colon) *colon = 0;
+  }
   const char *force = getenv("GEMMX_NATIVE_CFG");
   if (force && *force) {
     int c = atoi(force);
@@ -74,7 +110,7 @@ void gemmx_tune_init(void) {
   }
   const char *env = getenv("GEMMX_TUNE_FILE");
   if (env && *env) {
-    load_table(env);
+    (void)load_table(env);
     return;
   }
   Dl_info info;
@@ -83,7 +119,7 @@ void gemmx_tune_init(void) {
     snprintf(buf, sizeof buf, "%s", info.dli_fname);
     char path[1100];
     snprintf(path, sizeof path, "%s/gemm_mi355x_tune.txt", dirname(buf));
-    load_table(path);
+    (void)load_table(path);
   }
 }
 
@@ -96,12 +132,13 @@ int gemmx_tune_force(int es, int cfg) {
 }
 
 int gemmx_tune_load(const char *path) {
-  memset(g_table, 0, sizeof g_table);
-  g_entries = 0;
-  g_path[0] = 0;
-  if (path && *path) load_table(path);
-  return g_entries;
+  if (!path || !*path) {
+    clear_table();
+    return 0;
+  }
+  return load_table(path);
 }
+
 const char *gemmx_tune_path(void) { return g_path; }
 
 int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {

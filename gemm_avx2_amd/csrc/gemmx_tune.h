@@ -8,10 +8,14 @@
  * text table written by the autotuner (gemm_avx2_amd/autotune.py).
  *
  * Table file: GEMMX_TUNE_FILE, else gemm_mi355x_tune.txt beside the loaded
- * library.  One entry per line, '#' starts a comment:
+ * library.  '#' starts a comment.  Directives:
+ *     format 1            table format version (required, first directive)
+ *     arch <gfxNNN>       the table is ignored on a device of another arch
+ * and one entry per line:
  *     <dtype:f64|f32> <trans:NN|NT|TN|TT> <mbucket> <nbucket> <kbucket> <cfg>
  * where a bucket is floor(log2(dim)) clamped to 4..15 and cfg is a
- * configuration id.  Missing entries fall back to a size heuristic.
+ * configuration id.  A later entry for the same key replaces an earlier one.
+ * Missing entries fall back to a size heuristic.
  */
 #ifndef GEMMX_TUNE_H
 #define GEMMX_TUNE_H
@@ -20,7 +24,11 @@
 extern "C" {
 #endif
 
-void gemmx_tune_init(void);
+#define GEMMX_TUNE_FORMAT 1
+
+/* `arch` is the current device's architecture name ("gfx950"), or NULL when
+ * there is no device (then a table's arch directive is not checked). */
+void gemmx_tune_init(const char *arch);
 int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k);
 int gemmx_tune_bucket(int dim);
 /* Number of entries loaded from the table file (0 when none). */
@@ -28,7 +36,8 @@ int gemmx_tune_entries(void);
 /* Forces one configuration for every call of the given element size (8 or
  * 4); -1 returns to the table.  Returns 0, or -1 for an unknown cfg. */
 int gemmx_tune_force(int es, int cfg);
-/* Loads (or, with NULL, clears) a table at run time; returns entries read. */
+/* Loads (or, with NULL, clears) a table at run time; returns entries read,
+ * or -1 when the file cannot be read or has a different format or arch. */
 int gemmx_tune_load(const char *path);
 /* Path of the table that was loaded, or "" when none. */
 const char *gemmx_tune_path(void);

@@ -76,13 +76,19 @@ int sgemm_x_set_mode(int mode);
  * src/gemm_set_parameters.h:10-61): cfg in 0..GEMMX_TILE_CONFIGS-1 forces one
  * configuration for every call, -1 returns to the tuning table.
  * *_x_load_tuning reads a table written by the autotuner (path; NULL clears
- * it) and returns the number of entries read; the table is shared by both
- * element types.  Also GEMMX_TUNE_FILE and GEMMX_NATIVE_CFG at first use. */
+ * it) and returns the number of entries read, or -1 (previous table kept)
+ * when the file is unreadable or its format version or arch does not match;
+ * the table is shared by both element types.  GEMMX_TUNE_FILE (else
+ * gemm_mi355x_tune.txt beside the library) and GEMMX_NATIVE_CFG are read at
+ * first use.  *_x_tune_lookup returns the configuration a call with these
+ * arguments would use (no device needed), or -1 for negative sizes. */
 #define GEMMX_TILE_CONFIGS 7
 int dgemm_x_set_tile_config(int cfg);
 int sgemm_x_set_tile_config(int cfg);
 int dgemm_x_load_tuning(const char *path);
 int sgemm_x_load_tuning(const char *path);
+int dgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);
+int sgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);
 
 /* Last error of the calling thread: returns its code and, when msg is not
  * NULL, a static description.  Cleared by the next successful call. */
@@ -153,6 +159,12 @@ int sgemm_x_batch_create(void **plan, int count, const char *transa,
 int sgemm_x_batch_run(void *plan, void *stream);
 double sgemm_x_batch_flops(const void *plan);
 int sgemm_x_batch_destroy(void *plan);
+/* Tile configuration of the batch kernel for plans created afterwards
+ * (0 64x64x16, 1 64x64x32, 2 128x64x16, 3 32x32x32; -1 = the environment's
+ * GEMMX_BATCH_CFG, else the default).  No reference counterpart. */
+#define GEMMX_BATCH_CONFIGS 4
+int dgemm_x_batch_set_config(int cfg);
+int sgemm_x_batch_set_config(int cfg);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,28 @@ def test_batch_matches_oracle_and_replays(cuda, dt):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_every_batch_config(cuda, dt):
+    """Each compiled batch tile configuration on 60 mixed problems."""
+    import torch
+    d = "d" if dt == np.float64 else "s"
+    try:
+        for cfg in range(gx.BATCH_CONFIGS):
+            gx.set_batch_config(d, cfg)
+            rng = np.random.default_rng(300 + cfg)
+            probs = _problems(rng, 60, 1, 200, dt)
+            for p in probs:
+                _upload(p, torch, dt)
+            plan = gx.BatchPlan(d, [_spec(p) for p in probs])
+            plan.run()
+            torch.cuda.synchronize()
+            for p in probs:
+                _check(p, _expected(p), from_device(p["dC"], p["C"].shape), dt)
+            plan.destroy()
+    finally:
+        gx.set_batch_config(d, -1)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_batch_agrees_with_single_calls_at_full_size(cuda, dt):
     """4096 problems with sizes log-uniform in [16, 1024] and mixed
     transposes, as in the batch benchmark: every problem against the same

@@ -1,0 +1,130 @@
+"""The tuning table that replaces the reference's compile-time tuning headers
+(wjc404/GEMM_AVX2 dgemm_tune.h:6-10, sgemm_tune.h:7-11, clamped in
+src/gemm_set_parameters.h:10-61): on the CPU, the libraries' reader and the
+autotuner's writer agree, the same table always gives the same kernel choice
+(determinism), forced configurations and format checks behave as documented.
+No device is needed: *_x_tune_lookup only consults the table."""
+import os
+
+import pytest
+
+import gemm_avx2_amd as gx
+from gemm_avx2_amd import autotune as at
+
+DEFAULT = os.path.join(gx.LIB_DIR, "gemm_mi355x_tune.txt")
+
+
+@pytest.fixture
+def restore_tables():
+    yield
+    for d in ("d", "s"):
+        gx.set_tile_config(d, -1)
+        gx.load_tuning(DEFAULT if os.path.exists(DEFAULT) else None, d)
+
+
+def _entries():
+    return {("f64", "NN", 13, 13, 13): 6, ("f64", "TN", 8, 8, 8): 1, ("f64", "NT", 10, 12, 9): 3,
+            ("f32", "NN", 13, 13, 13): 2, ("f32", "TT", 4, 15, 15): 5}
+
+
+def test_write_read_roundtrip_is_deterministic(tmp_path):
+    p1, p2 = str(tmp_path / "a.txt"), str(tmp_path / "b.txt")
+    e = _entries()
+    notes = {k: f"note {i}" for i, k in enumerate(e)}
+    at.write_table(p1, e, notes, "gfx950", ["header"])
+    # insertion order must not matter
+    at.write_table(p2, dict(reversed(list(e.items()))), notes, "gfx950", ["header"])
+    assert open(p1).read() == open(p2).read()
+    got, got_notes, arch = at.read_table(p1)
+    assert got == e and arch == "gfx950" and got_notes == notes
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_library_lookup_follows_the_table(tmp_path, dtype, restore_tables):
+    path = str(tmp_path / "t.txt")
+    e = _entries()
+    at.write_table(path, e, {}, None)
+    assert gx.load_tuning(path, dtype) == len(e)
+    want = "f64" if dtype == "d" else "f32"
+    for (dt, tr, mb, nb, kb), cfg in e.items():
+        if dt != want:
+            continue
+        for scale in (0, 1):  # lowest and highest size in each bucket
+            m, n, k = ((1 << b) if not scale else (1 << (b + 1)) - 1 for b in (mb, nb, kb))
+            if mb == 4:
+                m = 3  # buckets clamp: anything below 16 is bucket 4
+            assert gx.tune_lookup(dtype, tr[0], tr[1], m, n, k) == cfg
+            # only the first character counts, and lower case is the same
+            assert gx.tune_lookup(dtype, tr[0].lower(), "C" if tr[1] == "T" else "n",
+                                  m, n, k) == cfg
+            # the same table always gives the same choice
+            assert gx.tune_lookup(dtype, tr[0], tr[1], m, n, k) == cfg
+    # outside the table: the size heuristic (128x128 tiles when there are
+    # enough of them for every CU, else 64x64)
+    assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 64) == 0
+    assert gx.tune_lookup(dtype, "N", "N", 100, 100, 100) == 1
+    assert gx.tune_lookup(dtype, "N", "N", -1, 1, 1) == -1
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_forced_configuration_overrides_the_table(tmp_path, dtype, restore_tables):
+    path = str(tmp_path / "t.txt")
+    at.write_table(path, _entries(), {}, None)
+    gx.load_tuning(path, dtype)
+    gx.set_tile_config(dtype, 4)
+    assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == 4
+    assert gx.tune_lookup(dtype, "T", "N", 300, 300, 300) == 4
+    gx.set_tile_config(dtype, -1)
+    assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == (6 if dtype == "d" else 2)
+    with pytest.raises(gx.GemmError):
+        gx.set_tile_config(dtype, gx.TILE_CONFIGS)
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_reader_rejects_other_formats_and_keeps_the_previous_table(tmp_path, dtype,
+                                                                    restore_tables):
+    good = str(tmp_path / "good.txt")
+    at.write_table(good, _entries(), {}, None)
+    assert gx.load_tuning(good, dtype) == 5
+    bad_format = tmp_path / "bad1.txt"
+    bad_format.write_text("format 99\nf64 NN 13 13 13 1\nf32 NN 13 13 13 1\n")
+    no_format = tmp_path / "bad2.txt"
+    no_format.write_text("f64 NN 13 13 13 1\nf32 NN 13 13 13 1\n")
+    for bad in (bad_format, no_format, tmp_path / "missing.txt"):
+        assert gx.load_tuning(str(bad), dtype) == -1
+        assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == (6 if dtype == "d" else 2)
+    with pytest.raises(ValueError):
+        at.read_table(str(bad_format))
+    # invalid lines are skipped, a later entry replaces an earlier one
+    mixed = tmp_path / "mixed.txt"
+    mixed.write_text("format 1\n# comment\nf64 NN 13 13 13 2\nf64 NN 13 13 13 5  # later\n"
+                     "f32 NN 13 13 13 2\nf32 NN 13 13 13 5\n"
+                     "f64 NN 3 13 13 1\nf64 NN 13 13 13 99\nf16 NN 13 13 13 1\n"
+                     "f32 NN 3 13 13 1\nf32 NN 13 13 13 99\n")
+    assert gx.load_tuning(str(mixed), dtype) == 4
+    assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == 5
+    py, _, _ = at.read_table(str(mixed))
+    assert py[("f64", "NN", 13, 13, 13)] == 5
+    assert gx.load_tuning(None, dtype) == 0
+    assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == 0
+
+
+def test_resume_skips_measured_keys():
+    wanted = [("d", "NN", 8192, 8192, 8192), ("d", "NN", 9000, 9000, 9000),
+              ("d", "TN", 256, 256, 256), ("s", "NN", 8192, 8192, 8192)]
+    have = {at.key_of("d", "NN", 8192, 8192, 8192): 0}
+    # one shape per key (9000 falls in the 8192 bucket)
+    assert at.pending(wanted, {}, False) == [wanted[0], wanted[2], wanted[3]]
+    assert at.pending(wanted, have, True) == [wanted[2], wanted[3]]
+    assert at.bucket(1) == 4 and at.bucket(16) == 4 and at.bucket(31) == 4
+    assert at.bucket(32) == 5 and at.bucket(8191) == 12 and at.bucket(1 << 20) == 15
+
+
+def test_configuration_counts_agree():
+    import re
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "include", "gemm_mi355x.h")).read()
+    n = int(re.search(r"#define GEMMX_TILE_CONFIGS (\d+)", hdr).group(1))
+    nb = int(re.search(r"#define GEMMX_BATCH_CONFIGS (\d+)", hdr).group(1))
+    assert gx.TILE_CONFIGS == at.TILE_CONFIGS == n
+    assert gx.BATCH_CONFIGS == nb
