@@ -3,19 +3,26 @@
 
 Metric: "DGEMM/SGEMM TFLOP/s 8192^3: emulated vs native at matched FP64
 error".  One step is one 8192^3 DGEMM through the library's device-pointer
-entry point (dgemm_x_dev, or dgemm_x_ozaki_dev for the emulated form) with
-inputs already resident in HBM:
-    transa = transb = 'N', m = n = k = 8192, lda = ldb = ldc = 8200,
-    alpha = 1.5, beta = -0.5, A/B/C ~ U[-1,1) from the counter-based generator.
-TFLOP/s counts the algorithmic 2*m*n*k flops for every form (slice products
-of the emulated form are not counted).  The headline `value` is the faster of
-native FP64 MFMA and emulated (Ozaki-I INT8) DGEMM among the forms whose
-error against an exact product is no larger than native FP64's on the same
-inputs; both are reported, with their errors, under "dgemm".
+entry point (dgemm_x_dev for native FP64 MFMA, dgemm_x_ozaki_dev for the
+emulated Ozaki-I INT8 form) with inputs already resident in HBM:
+    transa = transb = 'N', m = n = k = 8192, lda = ldb = ldc = 8192,
+    alpha = 1, beta = 0, A and B entries (u - 0.5) exp(phi g), u ~ U[0,1),
+    g ~ N(0,1), phi in {0, 1, 4}, from the counter-based generator.
+TFLOP/s counts the algorithmic 2mnk flops for every form (slice products of
+the emulated form are not counted).  The error of every form is measured
+against exactly rounded entries (error-free products summed exactly) on 144
+sampled entries for each phi.  The emulated form qualifies with the smallest
+slice count whose relative Frobenius error is no larger than native FP64's
+for every phi; the headline `value` is the faster of native and the
+qualifying emulated form, timed on the phi = 0 inputs.
 
-Also reported (not the headline): SGEMM native/BF16x3 and the other transpose
-pairs at 8192^3, the rooflines, and the reference's AVX2/OpenMP DGEMM timed on
-this host's cores on a bounded slice of the same workload.
+Also reported (not the headline): native DGEMM/SGEMM at 8192^3 in all four
+transpose pairs (ld 8200, alpha 1.5, beta -0.5) and the ragged
+8191 x 8189 x 8187 case; DGEMM 1024^3; split-BF16x3 SGEMM against native
+SGEMM at 16384^3 NN and 65536 x 256 x 256 TN on uniform and log-normal
+inputs; the 4096-problem grouped batch; dgemm_ on host arrays (transfers
+included); the rooflines; and the reference's AVX2/OpenMP dgemm_ on this
+host's cores over the headline workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -36,8 +43,9 @@ sys.path.insert(0, ROOT)
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "i8": 5000.0}  # dense, TFLOP/s (TOP/s)
 HBM_PEAK = 8000.0  # GB/s
 M = N = K = 8192
-LD = 8200
-ALPHA, BETA = 1.5, -0.5
+PHIS = (0.0, 1.0, 4.0)
+SLICES = (3, 4, 5, 6, 7, 8)
+SEED = 1
 
 
 def parse():
@@ -50,6 +58,7 @@ def parse():
     ap.add_argument("--profile-only",
                     choices=["native", "ozaki", "sgemm", "bf16x3", "batch", "copy"],
                     help="run just one form (for rocprofv3)")
+    ap.add_argument("--slices", type=int, default=8, help="slice count for --profile-only ozaki")
     return ap.parse_args()
 
 
@@ -73,10 +82,15 @@ def time_launches(torch, fn, steps, warmup):
     return wall, [a.elapsed_time(b) for a, b in evs]
 
 
+def median(xs):
+    return float(np.median(np.asarray(xs)))
+
+
 # ---------------------------------------------------------------- exact reference
 def _two_prod(a, b):
     """Dekker/Veltkamp TwoProduct in numpy: a*b = p + e exactly."""
     c = 134217729.0
+
     def split(x):
         t = c * x
         hi = t - (t - x)
@@ -89,8 +103,8 @@ def _two_prod(a, b):
 
 
 def exact_entries(arows, bcols, c0, alpha, beta):
-    """Correctly rounded alpha*sum(a*b) + beta*c0 per sampled entry (math.fsum
-    over the exact products); arows (r, k), bcols (k, c) -> (r, c)."""
+    """alpha*sum(a*b) + beta*c0 per sampled entry, the sum correctly rounded
+    (math.fsum over the exact products); arows (r, k), bcols (k, c)."""
     out = np.empty((arows.shape[0], bcols.shape[1]))
     for i in range(arows.shape[0]):
         for j in range(bcols.shape[1]):
@@ -102,29 +116,49 @@ def exact_entries(arows, bcols, c0, alpha, beta):
     return out
 
 
-def sampled_error(torch, dA, dB, dC, c0_samples, rows, cols, ta="N", tb="N"):
-    """Max componentwise and relative-Frobenius error of C (after one call
-    from the original C) at the sampled rows x cols."""
-    A2 = dA.view(-1, LD)  # (cols of stored A, LD)
-    B2 = dB.view(-1, LD)
-    r = torch.tensor(rows, device="cuda")
-    c = torch.tensor(cols, device="cuda")
-    arows = A2[:, r].T.contiguous()[:, :K].double().cpu().numpy()    # (len rows, K)
-    bcols = B2[c, :K].T.contiguous().double().cpu().numpy()         # (K, len cols)
-    got = dC.view(-1, LD)[c][:, r].T.double().cpu().numpy()          # (rows, cols)
-    ref = exact_entries(arows, bcols, c0_samples, ALPHA, BETA)
-    scale = np.abs(arows) @ np.abs(bcols) * abs(ALPHA) + abs(BETA) * np.abs(c0_samples)
-    err = np.abs(got - ref)
-    return {"max_comp": float(np.max(err / scale)),
-            "rel_fro": float(np.linalg.norm(err) / np.linalg.norm(ref))}
+class Samples:
+    """Sampled entries (rows x cols) of one column-major problem on device
+    buffers: op(A) rows, op(B) columns and the original C, taken once."""
+
+    def __init__(self, torch, ta, tb, m, n, k, dA, lda, dB, ldb, dC, ldc, seed=0, count=12):
+        rng = np.random.default_rng(seed)
+        self.rows = sorted(rng.choice(m, min(count, m), replace=False).tolist())
+        self.cols = sorted(rng.choice(n, min(count, n), replace=False).tolist())
+        r = torch.tensor(self.rows, device="cuda")
+        c = torch.tensor(self.cols, device="cuda")
+        A2, B2 = dA.view(-1, lda), dB.view(-1, ldb)
+        # op(A)(i, p): A(i, p) = A2[p, i] (N) or A(p, i) = A2[i, p] (T)
+        arows = A2[:k, r].T if ta == "N" else A2[r, :k]
+        # op(B)(p, j): B(p, j) = B2[j, p] (N) or B(j, p) = B2[p, j] (T)
+        bcols = B2[c, :k].T if tb == "N" else B2[:k, c]
+        self.arows = arows.contiguous().double().cpu().numpy()
+        self.bcols = bcols.contiguous().double().cpu().numpy()
+        self.ldc = ldc
+        self.c0 = self.take(torch, dC)
+
+    def take(self, torch, dC):
+        r = torch.tensor(self.rows, device="cuda")
+        c = torch.tensor(self.cols, device="cuda")
+        return dC.view(-1, self.ldc)[c][:, r].T.contiguous().double().cpu().numpy()
+
+    def error(self, torch, dC, alpha, beta):
+        """Max componentwise and relative Frobenius error of C (after one call
+        from the original C) at the sampled entries."""
+        got = self.take(torch, dC)
+        ref = exact_entries(self.arows, self.bcols, self.c0, alpha, beta)
+        scale = np.abs(self.arows) @ np.abs(self.bcols) * abs(alpha) + abs(beta) * np.abs(self.c0)
+        err = np.abs(got - ref)
+        return {"max_comp": float(np.max(err / scale)),
+                "rel_fro": float(np.linalg.norm(err) / np.linalg.norm(ref))}
 
 
 # ---------------------------------------------------------------- CPU baseline
 def cpu_baseline():
     """The reference's own dgemm_ (compiled from its sources into oracle/_ref)
-    on this host's cores, over a bounded slice of the workload: m = k = 8192,
-    n = 1024 (one eighth of the 8192^3 flops), ld 8200, alpha 1.5, beta -0.5.
-    Falls back to the oracle's Dot2 port on a smaller slice."""
+    on this host's cores over the headline workload itself (8192^3 NN, ld
+    8192, alpha 1, beta 0, phi = 0 inputs), best of 2.  Falls back to the
+    oracle's Dot2 port on a 1024 x 128 x 1024 slice when the reference build
+    is absent."""
     import ctypes
     import oracle
     cores = len(os.sched_getaffinity(0))
@@ -132,26 +166,27 @@ def cpu_baseline():
     os.environ.setdefault("OMP_PROC_BIND", "close")
     fn = oracle.ref("d")
     if fn is not None:
-        m, n, k = M, 1024, K
-        kind, sample = "reference", f"dgemm_ NN m=k=8192 n=1024 ld=8200 (1/8 of 8192^3), best of 2"
+        m, n, k, ld = M, N, K, M
+        kind = "reference"
+        sample = "dgemm_ NN 8192^3 ld 8192 alpha 1 beta 0, phi 0 (the whole workload), best of 2"
     else:
-        m, n, k = 1024, 128, 1024
+        m, n, k, ld = 1024, 128, 1024, 1024
         kind, sample = "port", "oracle Dot2 port, NN 1024x128x1024, best of 2"
         fn = oracle.lib().oracle_dgemm
-    A = oracle.fill(m, k, LD, 0, 0)
-    B = oracle.fill(k, n, LD, 0, 1)
-    C = oracle.fill(m, n, LD, 0, 2)
+    A = oracle.fill(m, k, ld, SEED, 0, np.float64, 1, 0.0)
+    B = oracle.fill(k, n, ld, SEED, 1, np.float64, 1, 0.0)
+    C = oracle.fill(m, n, ld, SEED, 2)
     I = ctypes.c_int
     best = float("inf")
     for _ in range(2):
         t0 = time.perf_counter()
         fn(b"N", b"N", ctypes.byref(I(m)), ctypes.byref(I(n)), ctypes.byref(I(k)),
-           ctypes.byref(ctypes.c_double(ALPHA)), A.ctypes.data, ctypes.byref(I(LD)),
-           B.ctypes.data, ctypes.byref(I(LD)), ctypes.byref(ctypes.c_double(BETA)),
-           C.ctypes.data, ctypes.byref(I(LD)))
+           ctypes.byref(ctypes.c_double(1.0)), A.ctypes.data, ctypes.byref(I(ld)),
+           B.ctypes.data, ctypes.byref(I(ld)), ctypes.byref(ctypes.c_double(0.0)),
+           C.ctypes.data, ctypes.byref(I(ld)))
         best = min(best, time.perf_counter() - t0)
     return {"value": round(2.0 * m * n * k / best / 1e12, 4), "unit": "TFLOP/s",
-            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)) if kind == "reference" else cores,
+            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)) if kind == "reference" else 1,
             "kind": kind, "sample": sample, "seconds": round(best, 3)}
 
 
@@ -173,15 +208,169 @@ def profiled_traffic(summary):
     return None
 
 
-# ---------------------------------------------------------------- grouped batch
+# ---------------------------------------------------------------- headline
+class Headline:
+    """8192^3 NN, ld 8192, alpha 1, beta 0 on phi-distributed inputs."""
+
+    def __init__(self, torch, gx, stream, slices=max(SLICES)):
+        self.torch, self.gx, self.stream = torch, gx, stream
+        t = torch.float64
+        self.A = torch.empty(M * K, dtype=t, device="cuda")
+        self.B = torch.empty(K * N, dtype=t, device="cuda")
+        self.C = torch.empty(M * N, dtype=t, device="cuda")
+        self.ws = torch.empty(gx.dgemm_ozaki_workspace(M, N, K, slices),
+                              dtype=torch.uint8, device="cuda")
+        self.set_phi(0.0)
+
+    def set_phi(self, phi):
+        gx, s = self.gx, self.stream
+        gx.fill(self.A, M, K, M, SEED, 0, 1, phi, stream=s)
+        gx.fill(self.B, K, N, K, SEED, 1, 1, phi, stream=s)
+        gx.fill(self.C, M, N, M, SEED, 2, stream=s)
+        self.C0 = self.C.clone()
+        self.torch.cuda.synchronize()
+
+    def native(self):
+        self.gx.dgemm_dev("N", "N", M, N, K, 1.0, self.A, M, self.B, K, 0.0, self.C, M,
+                          stream=self.stream)
+
+    def ozaki(self, s):
+        self.gx.dgemm_ozaki_dev("N", "N", M, N, K, 1.0, self.A, M, self.B, K, 0.0, self.C, M,
+                                s, self.ws, self.ws.numel(), stream=self.stream)
+
+    def error(self, fn, samples):
+        self.C.copy_(self.C0)
+        fn()
+        self.torch.cuda.synchronize()
+        return samples.error(self.torch, self.C, 1.0, 0.0)
+
+
+def run_headline(torch, gx, stream, args):
+    h = Headline(torch, gx, stream)
+    flops = 2.0 * M * N * K
+    wall_n, per_n = time_launches(torch, h.native, args.steps, args.warmup)
+    native = {"ms_event": median(per_n), "ms_wall": wall_n / args.steps, "error": {}}
+    slices = SLICES if not args.quick else (7, 8)
+    ozaki = {s: {"ms_event": None, "error": {}} for s in slices}
+    for s in slices:
+        _, per = time_launches(torch, lambda s=s: h.ozaki(s), max(3, args.steps // 2), 1)
+        ozaki[s]["ms_event"] = median(per)
+    for phi in PHIS:
+        h.set_phi(phi)
+        smp = Samples(torch, "N", "N", M, N, K, h.A, M, h.B, K, h.C, M, seed=int(phi) + 1)
+        native["error"][f"phi{phi:g}"] = h.error(h.native, smp)
+        for s in slices:
+            ozaki[s]["error"][f"phi{phi:g}"] = h.error(lambda s=s: h.ozaki(s), smp)
+    qual = [s for s in slices
+            if all(ozaki[s]["error"][p]["rel_fro"] <= native["error"][p]["rel_fro"]
+                   for p in native["error"])]
+    s_star = min(qual) if qual else None
+    best = ("native", native["ms_event"])
+    if s_star is not None and ozaki[s_star]["ms_event"] < native["ms_event"]:
+        best = (f"ozaki_s{s_star}", ozaki[s_star]["ms_event"])
+    out = {"native": dict(native, tflops=round(flops / native["ms_event"] / 1e9, 3)),
+           "ozaki": {f"s{s}": dict(ozaki[s], tflops=round(flops / ozaki[s]["ms_event"] / 1e9, 3))
+                     for s in slices},
+           "qualifying_slices": s_star}
+    return out, best, native["ms_wall"]
+
+
+# ---------------------------------------------------------------- other workloads
+def run_trans_and_ragged(torch, gx, stream):
+    """Native DGEMM and SGEMM, 8192^3 in all four transpose pairs and the
+    ragged 8191 x 8189 x 8187 NN, ld 8200, alpha 1.5, beta -0.5, U[-1,1)."""
+    LD, al, be = 8200, 1.5, -0.5
+    out = {}
+    for d, tdt, dev in (("f64", torch.float64, gx.dgemm_dev), ("f32", torch.float32, gx.sgemm_dev)):
+        A = torch.empty(LD * K, dtype=tdt, device="cuda")
+        B = torch.empty(LD * K, dtype=tdt, device="cuda")
+        C = torch.empty(LD * K, dtype=tdt, device="cuda")
+        for t, i in ((A, 0), (B, 1), (C, 2)):
+            gx.fill(t, 8192, 8192, LD, SEED, i, stream=stream)
+        C0 = C.clone()
+        res = {}
+        for (ta, tb, m, n, k) in (("N", "N", M, N, K), ("N", "T", M, N, K), ("T", "N", M, N, K),
+                                  ("T", "T", M, N, K), ("N", "N", 8191, 8189, 8187)):
+            def f(ta=ta, tb=tb, m=m, n=n, k=k):
+                dev(ta, tb, m, n, k, al, A, LD, B, LD, be, C, LD, stream=stream)
+            _, per = time_launches(torch, f, 3, 1)
+            smp = Samples(torch, ta, tb, m, n, k, A, LD, B, LD, C0, LD, seed=3)
+            C.copy_(C0)
+            f()
+            torch.cuda.synchronize()
+            err = smp.error(torch, C, al, be)
+            name = ta + tb if m == M else f"{ta}{tb}_{m}x{n}x{k}"
+            res[name] = {"tflops": round(2.0 * m * n * k / median(per) / 1e9, 2),
+                         "rel_fro": err["rel_fro"], "max_comp": err["max_comp"]}
+        out[d] = res
+        del A, B, C, C0
+    return out
+
+
+def run_small(torch, gx, stream):
+    """DGEMM 1024^3 NN, ld 1024, alpha 1, beta 0, U[-1,1) from seed 0."""
+    n = 1024
+    A, B, C = (torch.empty(n * n, dtype=torch.float64, device="cuda") for _ in range(3))
+    for t, i in ((A, 0), (B, 1), (C, 2)):
+        gx.fill(t, n, n, n, 0, i, stream=stream)
+
+    def f():
+        gx.dgemm_dev("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n, stream=stream)
+    _, per = time_launches(torch, f, 20, 3)
+    smp = Samples(torch, "N", "N", n, n, n, A, n, B, n, C, n, seed=4)
+    f()
+    torch.cuda.synchronize()
+    return {"tflops": round(2.0 * n ** 3 / median(per) / 1e9, 2), "ms": round(median(per), 4),
+            **smp.error(torch, C, 1.0, 0.0)}
+
+
+def run_bf16x3(torch, gx, stream):
+    """split-BF16x3 against native SGEMM: 16384^3 NN and 65536 x 256 x 256 TN,
+    alpha 1, beta 0, U[-1,1) and log-normal sigma = 2 entries; errors against
+    exactly rounded entries."""
+    out = {}
+    for (ta, tb, m, n, k) in (("N", "N", 16384, 16384, 16384), ("T", "N", 65536, 256, 256)):
+        ra, ca = (m, k) if ta == "N" else (k, m)
+        rb, cb = (k, n) if tb == "N" else (n, k)
+        A = torch.empty(ra * ca, dtype=torch.float32, device="cuda")
+        B = torch.empty(rb * cb, dtype=torch.float32, device="cuda")
+        C = torch.empty(m * n, dtype=torch.float32, device="cuda")
+        ws = torch.empty(gx.sgemm_bf16x3_workspace(m, n, k), dtype=torch.uint8, device="cuda")
+
+        def nat():
+            gx.sgemm_dev(ta, tb, m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, stream=stream)
+
+        def emu():
+            gx.sgemm_bf16x3_dev(ta, tb, m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, ws, ws.numel(),
+                                stream=stream)
+        res = {}
+        for dist, name, param in ((0, "uniform", 0.0), (2, "lognormal2", 2.0)):
+            gx.fill(A, ra, ca, ra, SEED, 0, dist, param, stream=stream)
+            gx.fill(B, rb, cb, rb, SEED, 1, dist, param, stream=stream)
+            gx.fill(C, m, n, m, SEED, 2, stream=stream)
+            smp = Samples(torch, ta, tb, m, n, k, A, ra, B, rb, C, m, seed=5)
+            r = {}
+            for form, fn in (("native", nat), ("bf16x3", emu)):
+                _, per = time_launches(torch, fn, 3, 1)
+                fn()
+                torch.cuda.synchronize()
+                r[form] = {"tflops": round(2.0 * m * n * k / median(per) / 1e9, 2),
+                           **smp.error(torch, C, 1.0, 0.0)}
+            res[name] = r
+        out[f"{ta}{tb}_{m}x{n}x{k}"] = res
+        del A, B, C, ws
+    return out
+
+
 BATCH_COUNT, BATCH_LO, BATCH_HI = 4096, 16, 1024
 
 
 def batch_workload(torch, gx, dtype, stream, seed=11):
-    """4096 independent GEMMs, m, n, k log-uniform in [16, 1024], transa and
-    transb uniform over {N, T}, alpha = 1.5, beta = -0.5, operands packed
-    back to back (ld = stored rows) in three device buffers.  Returns the
-    plan and the single-call argument lists of the same problems."""
+    """4096 independent GEMMs, m, n, k log-uniform integers in [16, 1024],
+    transa and transb uniform over {N, T}, alpha ~ U[0.5, 2], beta ~ U[-1, 1],
+    ld = stored rows, operands packed back to back in three device buffers
+    filled from the generator.  Returns the plan, the single-call argument
+    lists of the same problems and the buffers."""
     rng = np.random.default_rng(seed)
     tdt = torch.float64 if dtype == "d" else torch.float32
     shapes = []
@@ -190,7 +379,8 @@ def batch_workload(torch, gx, dtype, stream, seed=11):
         m, n, k = (int(round(math.exp(rng.uniform(math.log(BATCH_LO), math.log(BATCH_HI)))))
                    for _ in range(3))
         ta, tb = "NT"[rng.integers(2)], "NT"[rng.integers(2)]
-        shapes.append((ta, tb, m, n, k, tuple(offs)))
+        alpha, beta = float(rng.uniform(0.5, 2.0)), float(rng.uniform(-1.0, 1.0))
+        shapes.append((ta, tb, m, n, k, alpha, beta, tuple(offs)))
         # 16-byte aligned starts for every operand
         for j, size in enumerate((m * k, k * n, m * n)):
             offs[j] += (size + 1) // 2 * 2
@@ -199,34 +389,92 @@ def batch_workload(torch, gx, dtype, stream, seed=11):
         gx.fill(b, b.numel(), 1, b.numel(), seed, j, stream=stream)
     es = bufs[0].element_size()
     specs = []
-    for ta, tb, m, n, k, (oa, ob, oc) in shapes:
+    for ta, tb, m, n, k, alpha, beta, (oa, ob, oc) in shapes:
         ra = m if ta == "N" else k
         rb = k if tb == "N" else n
-        specs.append((ta, tb, m, n, k, ALPHA, bufs[0].data_ptr() + oa * es, ra,
-                      bufs[1].data_ptr() + ob * es, rb, BETA, bufs[2].data_ptr() + oc * es, m))
+        specs.append((ta, tb, m, n, k, alpha, bufs[0].data_ptr() + oa * es, ra,
+                      bufs[1].data_ptr() + ob * es, rb, beta, bufs[2].data_ptr() + oc * es, m))
     return gx.BatchPlan(dtype, specs), specs, bufs
 
 
-def bench_batch(torch, gx, stream, steps, warmup):
+def run_batch(torch, gx, stream, steps, warmup):
     out = {"problems": BATCH_COUNT, "sizes": f"log-uniform [{BATCH_LO}, {BATCH_HI}]",
-           "trans": "mixed N/T"}
+           "trans": "mixed N/T", "alpha": "U[0.5,2]", "beta": "U[-1,1]"}
     for d, key in (("d", "f64"), ("s", "f32")):
         plan, specs, bufs = batch_workload(torch, gx, d, stream)
         _, per = time_launches(torch, lambda: plan.run(stream), steps, warmup)
-        ms = float(np.mean(per))
+        ms = median(per)
         single = gx.dgemm_dev if d == "d" else gx.sgemm_dev
 
         def loop():
             for sp in specs:
                 single(*sp, stream=stream)
         _, per_l = time_launches(torch, loop, 2, 1)
-        ms_l = float(np.mean(per_l))
         out[key] = {"tflops": round(plan.flops / ms / 1e9, 2), "ms": round(ms, 3),
                     "gflop": round(plan.flops / 1e9, 2),
-                    "single_call_loop_tflops": round(plan.flops / ms_l / 1e9, 2)}
+                    "single_call_loop_tflops": round(plan.flops / median(per_l) / 1e9, 2)}
         plan.destroy()
         del bufs
     return out
+
+
+def run_host_pointer(torch, gx):
+    """dgemm_ on host (pageable numpy) arrays at the headline shape: the wall
+    clock of the whole call, staging copies to and from the GPU included."""
+    import oracle
+    A = oracle.fill(M, K, M, SEED, 0, np.float64, 1, 0.0)
+    B = oracle.fill(K, N, K, SEED, 1, np.float64, 1, 0.0)
+    C = oracle.fill(M, N, M, SEED, 2)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        gx.dgemm("N", "N", M, N, K, 1.0, A, M, B, K, 0.0, C, M)
+        times.append(time.perf_counter() - t0)
+    t = min(times[1:])
+    return {"ms_wall": round(t * 1e3, 2),
+            "tflops_pcie_inclusive": round(2.0 * M * N * K / t / 1e12, 2),
+            "note": "pageable host arrays; H2D of A, B, C and D2H of C included"}
+
+
+# ---------------------------------------------------------------- profiling entry
+def profile_only(torch, gx, stream, args):
+    keep = []
+    if args.profile_only in ("native", "ozaki"):
+        h = Headline(torch, gx, stream, slices=args.slices)
+        fn = h.native if args.profile_only == "native" else (lambda: h.ozaki(args.slices))
+    elif args.profile_only == "copy":
+        # a plain device copy of one 8192 x 8192 FP64 buffer (537 MB read,
+        # 537 MB written): the calibration for the HBM byte counters
+        src = torch.empty(M * N, dtype=torch.float64, device="cuda")
+        gx.fill(src, M, N, M, SEED, 0, stream=stream)
+        dst = torch.empty_like(src)
+
+        def fn():
+            dst.copy_(src)
+    elif args.profile_only == "batch":
+        plan, _, bufs = batch_workload(torch, gx, "d", stream)
+        keep = bufs
+
+        def fn():
+            plan.run(stream)
+    else:
+        LD = 8200
+        sA, sB, sC = (torch.empty(LD * K, dtype=torch.float32, device="cuda") for _ in range(3))
+        for t, i in ((sA, 0), (sB, 1), (sC, 2)):
+            gx.fill(t, 8192, 8192, LD, SEED, i, stream=stream)
+        if args.profile_only == "sgemm":
+            def fn():
+                gx.sgemm_dev("N", "N", M, N, K, 1.5, sA, LD, sB, LD, -0.5, sC, LD, stream=stream)
+        else:
+            wsb = torch.empty(gx.sgemm_bf16x3_workspace(M, N, K), dtype=torch.uint8,
+                              device="cuda")
+
+            def fn():
+                gx.sgemm_bf16x3_dev("N", "N", M, N, K, 1.5, sA, LD, sB, LD, -0.5, sC, LD,
+                                    wsb, wsb.numel(), stream=stream)
+    _, per = time_launches(torch, fn, args.steps, args.warmup)
+    print(json.dumps({"profile_only": args.profile_only, "ms": per}))
+    del keep
 
 
 # ---------------------------------------------------------------- main
@@ -234,159 +482,60 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(every box has one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+              "(every box here has one MI355X)", file=sys.stderr)
         sys.exit(2)
     import torch
     import gemm_avx2_amd as gx
 
-    dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
-    flops = 2.0 * M * N * K
-
-    def alloc(dt):
-        return torch.empty(LD * N, dtype=dt, device=dev)
-
-    # FP64 operands for the headline
-    dA, dB, dC = alloc(torch.float64), alloc(torch.float64), alloc(torch.float64)
-    gx.fill(dA, M, K, LD, 1, 0, stream=stream)
-    gx.fill(dB, K, N, LD, 1, 1, stream=stream)
-    gx.fill(dC, M, N, LD, 1, 2, stream=stream)
-    C0 = dC.clone()
-    torch.cuda.synchronize()
-
-    def native():
-        gx.dgemm_dev("N", "N", M, N, K, ALPHA, dA, LD, dB, LD, BETA, dC, LD, stream=stream)
-
-    slices = int(os.environ.get("GEMMX_BENCH_SLICES", "8"))
-    ws = torch.empty(gx.dgemm_ozaki_workspace(M, N, K, slices), dtype=torch.uint8, device=dev)
-
-    def ozaki(s=slices):
-        gx.dgemm_ozaki_dev("N", "N", M, N, K, ALPHA, dA, LD, dB, LD, BETA, dC, LD, s,
-                           ws, ws.numel(), stream=stream)
-
-    if args.profile_only == "copy":
-        # a plain device copy of one 8200 x 8192 FP64 operand (537 MB read,
-        # 537 MB written): the calibration for the HBM byte counters
-        dst = torch.empty_like(dA)
-        wall, per = time_launches(torch, lambda: dst.copy_(dA), args.steps, args.warmup)
-        print(json.dumps({"profile_only": "copy", "ms": per, "bytes": dA.numel() * 8}))
-        return
-    if args.profile_only == "batch":
-        del ws, dA, dB, dC, C0
-        plan, _, bufs = batch_workload(torch, gx, "d", stream)
-        wall, per = time_launches(torch, lambda: plan.run(stream), args.steps, args.warmup)
-        print(json.dumps({"profile_only": "batch", "ms": per, "gflop": plan.flops / 1e9}))
-        return
     if args.profile_only:
-        if args.profile_only in ("native", "ozaki"):
-            fn = native if args.profile_only == "native" else ozaki
-        else:
-            del ws, dA, dB, dC, C0
-            sA, sB, sC = alloc(torch.float32), alloc(torch.float32), alloc(torch.float32)
-            for t, r, c, i in ((sA, M, K, 0), (sB, K, N, 1), (sC, M, N, 2)):
-                gx.fill(t, r, c, LD, 1, i, stream=stream)
-            if args.profile_only == "sgemm":
-                def fn():
-                    gx.sgemm_dev("N", "N", M, N, K, ALPHA, sA, LD, sB, LD, BETA, sC, LD,
-                                 stream=stream)
-            else:
-                wsb = torch.empty(gx.sgemm_bf16x3_workspace(M, N, K), dtype=torch.uint8,
-                                  device=dev)
-
-                def fn():
-                    gx.sgemm_bf16x3_dev("N", "N", M, N, K, ALPHA, sA, LD, sB, LD, BETA, sC,
-                                        LD, wsb, wsb.numel(), stream=stream)
-        wall, per = time_launches(torch, fn, args.steps, args.warmup)
-        print(json.dumps({"profile_only": args.profile_only, "ms": per}))
+        profile_only(torch, gx, stream, args)
         return
 
-    result = {}
-    # ---- headline candidates: native and emulated DGEMM
-    wall_n, per_n = time_launches(torch, native, args.steps, args.warmup)
-    ms_n = wall_n / args.steps
-    ev_n = float(np.mean(per_n))
-    # accuracy on sampled entries, one call from the original C
-    rng = np.random.default_rng(0)
-    rows = sorted(rng.choice(M, 12, replace=False).tolist())
-    cols = sorted(rng.choice(N, 12, replace=False).tolist())
-    c0s = C0.view(-1, LD)[torch.tensor(cols, device=dev)][:, torch.tensor(rows, device=dev)].T
-    c0s = c0s.double().cpu().numpy()
-    dC.copy_(C0)
-    native()
-    torch.cuda.synchronize()
-    err_n = sampled_error(torch, dA, dB, dC, c0s, rows, cols)
-    dgemm = {"native": {"tflops": flops / ev_n / 1e9, "ms_event": ev_n, "ms_wall": ms_n,
-                        "error_vs_exact": err_n}}
-    best_name, best_ms = "native", ev_n
+    flops = 2.0 * M * N * K
+    head, (best_name, best_ms), native_wall = run_headline(torch, gx, stream, args)
+    result = {"dgemm": head}
     if not args.quick:
-        wall_o, per_o = time_launches(torch, ozaki, max(3, args.steps // 2), 1)
-        ev_o = float(np.mean(per_o))
-        dC.copy_(C0)
-        ozaki()
-        torch.cuda.synchronize()
-        err_o = sampled_error(torch, dA, dB, dC, c0s, rows, cols)
-        dgemm[f"ozaki_s{slices}"] = {"tflops": flops / ev_o / 1e9, "ms_event": ev_o,
-                                     "error_vs_exact": err_o}
-        if err_o["rel_fro"] <= err_n["rel_fro"] and ev_o < best_ms:
-            best_name, best_ms = f"ozaki_s{slices}", ev_o
-    result["dgemm"] = dgemm
+        for key, fn in (("native_8192_ld8200", lambda: run_trans_and_ragged(torch, gx, stream)),
+                        ("dgemm_1024", lambda: run_small(torch, gx, stream)),
+                        ("sgemm_bf16x3", lambda: run_bf16x3(torch, gx, stream)),
+                        ("batch", lambda: run_batch(torch, gx, stream, max(3, args.steps // 2), 1)),
+                        ("dgemm_host_pointers", lambda: run_host_pointer(torch, gx))):
+            result[key] = fn()
+            torch.cuda.empty_cache()
 
-    # ---- the other transpose pairs and SGEMM (reported, not the headline)
-    if not args.quick:
-        trans = {}
-        for ta, tb in (("N", "T"), ("T", "N"), ("T", "T")):
-            def f(ta=ta, tb=tb):
-                gx.dgemm_dev(ta, tb, M, N, K, ALPHA, dA, LD, dB, LD, BETA, dC, LD, stream=stream)
-            _, per = time_launches(torch, f, 3, 1)
-            trans[ta + tb] = round(flops / float(np.mean(per)) / 1e9, 2)
-        result["dgemm_native_trans_tflops"] = trans
-        del ws
-        sA, sB, sC = alloc(torch.float32), alloc(torch.float32), alloc(torch.float32)
-        gx.fill(sA, M, K, LD, 1, 0, stream=stream)
-        gx.fill(sB, K, N, LD, 1, 1, stream=stream)
-        gx.fill(sC, M, N, LD, 1, 2, stream=stream)
-        sg = {}
-        for ta, tb in (("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")):
-            def f(ta=ta, tb=tb):
-                gx.sgemm_dev(ta, tb, M, N, K, ALPHA, sA, LD, sB, LD, BETA, sC, LD, stream=stream)
-            _, per = time_launches(torch, f, 3, 1)
-            sg["native_" + ta + tb] = round(flops / float(np.mean(per)) / 1e9, 2)
-
-        wsb = torch.empty(gx.sgemm_bf16x3_workspace(M, N, K), dtype=torch.uint8, device=dev)
-
-        def fb():
-            gx.sgemm_bf16x3_dev("N", "N", M, N, K, ALPHA, sA, LD, sB, LD, BETA, sC, LD,
-                                wsb, wsb.numel(), stream=stream)
-        _, per = time_launches(torch, fb, 3, 1)
-        sg["bf16x3_NN"] = round(flops / float(np.mean(per)) / 1e9, 2)
-        result["sgemm_tflops"] = sg
-        del sA, sB, sC, wsb
-        result["batch"] = bench_batch(torch, gx, stream, max(3, args.steps // 2), 1)
-
-    value = flops / best_ms / 1e9
-    ms_step = ms_n if best_name == "native" else best_ms
+    ev_n = head["native"]["ms_event"]
     line = {
         "metric": "DGEMM/SGEMM TFLOP/s 8192^3: emulated vs native at matched FP64 error",
-        "value": round(value, 3),
+        "value": round(flops / best_ms / 1e9, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(native_wall if best_name == "native" else best_ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (counter-based U[-1,1) generator, seed 1)",
-        "config": {"workload": f"DGEMM {best_name} NN m=n=k=8192 ld=8200 alpha=1.5 beta=-0.5",
-                   "m": M, "n": N, "k": K, "ld": LD, "alpha": ALPHA, "beta": BETA,
-                   "mode": best_name},
+        "data": "synthetic (counter-based generator, seed 1; A, B (u-0.5)exp(phi g), phi 0 timed)",
+        "config": {"workload": f"DGEMM {best_name} NN m=n=k=8192 ld=8192 alpha=1 beta=0",
+                   "m": M, "n": N, "k": K, "ld": M, "alpha": 1.0, "beta": 0.0,
+                   "phi": list(PHIS), "mode": best_name},
         "roofline": {"bound": "mfma", "achieved": round(flops / ev_n / 1e9, 3),
                      "peak": PEAK["f64"], "unit": "TFLOP/s",
                      "frac": round(flops / ev_n / 1e9 / PEAK["f64"], 4),
                      "traffic": profiled_traffic("dgemm_native_nn_8192.md"),
                      "kernel": "gemm_native_kernel<double> (native FP64 MFMA)"},
     }
+    if best_name != "native":
+        s = int(best_name.split("_s")[1])
+        ops = s * (s + 1) / 2 * flops  # INT8 slice-pair products
+        line["roofline"] = {"bound": "mfma", "achieved": round(ops / best_ms / 1e9, 1),
+                            "peak": PEAK["i8"], "unit": "TOP/s",
+                            "frac": round(ops / best_ms / 1e9 / PEAK["i8"], 4),
+                            "traffic": profiled_traffic(f"dgemm_ozaki_s{s}_nn_8192.md"),
+                            "kernel": "slice_products_kernel (INT8 MFMA); achieved over the "
+                                      "whole emulated call"}
     line.update(result)
     if not args.no_cpu:
         try:
