@@ -45,6 +45,7 @@ HBM_PEAK = 8000.0  # GB/s
 M = N = K = 8192
 PHIS = (0.0, 1.0, 4.0)
 SLICES = (3, 4, 5, 6, 7, 8)
+MODULI = (14, 16, 18, 19, 20, 22)
 SEED = 1
 
 
@@ -56,9 +57,10 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-only",
-                    choices=["native", "ozaki", "sgemm", "bf16x3", "batch", "copy"],
+                    choices=["native", "ozaki", "ozaki2", "sgemm", "bf16x3", "batch", "copy"],
                     help="run just one form (for rocprofv3)")
-    ap.add_argument("--slices", type=int, default=8, help="slice count for --profile-only ozaki")
+    ap.add_argument("--slices", type=int, default=8,
+                    help="slice count (ozaki) or modulus count (ozaki2) for --profile-only")
     return ap.parse_args()
 
 
@@ -212,13 +214,14 @@ def profiled_traffic(summary):
 class Headline:
     """8192^3 NN, ld 8192, alpha 1, beta 0 on phi-distributed inputs."""
 
-    def __init__(self, torch, gx, stream, slices=max(SLICES)):
+    def __init__(self, torch, gx, stream, slices=max(SLICES), moduli=max(MODULI)):
         self.torch, self.gx, self.stream = torch, gx, stream
         t = torch.float64
         self.A = torch.empty(M * K, dtype=t, device="cuda")
         self.B = torch.empty(K * N, dtype=t, device="cuda")
         self.C = torch.empty(M * N, dtype=t, device="cuda")
-        self.ws = torch.empty(gx.dgemm_ozaki_workspace(M, N, K, slices),
+        self.ws = torch.empty(max(gx.dgemm_ozaki_workspace(M, N, K, slices),
+                                  gx.dgemm_ozaki2_workspace(M, N, K, moduli)),
                               dtype=torch.uint8, device="cuda")
         self.set_phi(0.0)
 
@@ -237,6 +240,10 @@ class Headline:
     def ozaki(self, s):
         self.gx.dgemm_ozaki_dev("N", "N", M, N, K, 1.0, self.A, M, self.B, K, 0.0, self.C, M,
                                 s, self.ws, self.ws.numel(), stream=self.stream)
+
+    def ozaki2(self, nmod):
+        self.gx.dgemm_ozaki2_dev("N", "N", M, N, K, 1.0, self.A, M, self.B, K, 0.0, self.C, M,
+                                 nmod, self.ws, self.ws.numel(), stream=self.stream)
 
     def error(self, fn, samples):
         self.C.copy_(self.C0)
@@ -439,9 +446,10 @@ def run_host_pointer(torch, gx):
 # ---------------------------------------------------------------- profiling entry
 def profile_only(torch, gx, stream, args):
     keep = []
-    if args.profile_only in ("native", "ozaki"):
-        h = Headline(torch, gx, stream, slices=args.slices)
-        fn = h.native if args.profile_only == "native" else (lambda: h.ozaki(args.slices))
+    if args.profile_only in ("native", "ozaki", "ozaki2"):
+        h = Headline(torch, gx, stream, slices=max(SLICES), moduli=max(MODULI))
+        fn = {"native": h.native, "ozaki": lambda: h.ozaki(args.slices),
+              "ozaki2": lambda: h.ozaki2(args.slices)}[args.profile_only]
     elif args.profile_only == "copy":
         # a plain device copy of one 8192 x 8192 FP64 buffer (537 MB read,
         # 537 MB written): the calibration for the HBM byte counters

@@ -24,15 +24,16 @@ import os
 
 __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
-    "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
+    "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
+    "dgemm_ozaki2_bits", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "tune_lookup", "last_error", "fill",
-    "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3",
+    "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2",
 ]
 
 LIB_DIR = os.environ.get("GEMMX_LIB_DIR") or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), "lib")
 
-MODE_NATIVE, MODE_OZAKI, MODE_BF16X3 = 0, 1, 2
+MODE_NATIVE, MODE_OZAKI, MODE_BF16X3, MODE_OZAKI2 = 0, 1, 2, 3
 OK, ERR_ARG, ERR_NODEVICE, ERR_ALLOC, ERR_LAUNCH, ERR_UNSUPPORTED = range(6)
 
 
@@ -83,6 +84,12 @@ def _declare(lib, dtype):
             ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
             ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
             ctypes.c_size_t, ctypes.c_void_p]
+        lib.dgemm_x_ozaki2_workspace.restype = ctypes.c_size_t
+        lib.dgemm_x_ozaki2_workspace.argtypes = [ctypes.c_int] * 4
+        lib.dgemm_x_ozaki2_bits.restype = ctypes.c_int
+        lib.dgemm_x_ozaki2_bits.argtypes = [ctypes.c_int] * 2
+        lib.dgemm_x_ozaki2_dev.restype = ctypes.c_int
+        lib.dgemm_x_ozaki2_dev.argtypes = lib.dgemm_x_ozaki_dev.argtypes
     else:
         lib.sgemm_x_set_mode.restype = ctypes.c_int
         lib.sgemm_x_set_mode.argtypes = [ctypes.c_int]
@@ -236,6 +243,25 @@ def dgemm_ozaki_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc
     rc = lib.dgemm_x_ozaki_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda,
                                _addr(b), ldb, beta, _addr(c), ldc, slices, _addr(work),
                                work_bytes, _stream(stream))
+    if rc:
+        _check("d", rc)
+
+
+def dgemm_ozaki2_workspace(m, n, k, moduli) -> int:
+    return int(load("DGEMM.so").dgemm_x_ozaki2_workspace(m, n, k, moduli))
+
+
+def dgemm_ozaki2_bits(moduli, k) -> int:
+    return int(load("DGEMM.so").dgemm_x_ozaki2_bits(moduli, k))
+
+
+def dgemm_ozaki2_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, moduli,
+                     work=None, work_bytes=0, stream=None):
+    """Emulated DGEMM (Ozaki-II, CRT over INT8 residues) on device pointers."""
+    lib = load("DGEMM.so")
+    rc = lib.dgemm_x_ozaki2_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda,
+                                _addr(b), ldb, beta, _addr(c), ldc, moduli, _addr(work),
+                                work_bytes, _stream(stream))
     if rc:
         _check("d", rc)
 

@@ -13,9 +13,17 @@
  * or the same FP64 operation as the oracle's restatement, so results are
  * bit-identical to it.
  *
+ * Ozaki-II (DGEMM): split into residues of P-bit scaled integers modulo
+ * nmod pairwise-coprime moduli <= 256 -> one INT8 GEMM per modulus, its
+ * INT32 sum stored modulo the modulus (K chunked so that kc * 128^2 < 2^31)
+ * -> exact CRT reconstruction of the integer product here, rounded once to
+ * FP64 and scaled; bit-identical to the oracle's exact integer restatement.
+ *
  * split-BF16x3 (SGEMM): split -> pairs (lo,hi), (hi,lo), (hi,hi) in one FP32
  * accumulator -> C = fma(alpha, acc, beta*C) in the products epilogue.
  */
+#include <string.h>
+
 #include "gemmx_device.h"
 #include "gemmx_internal.h"
 #include "gemmx_slices.h"
@@ -78,7 +86,352 @@ SliceGroups ozaki_groups(int s) {
   return G;
 }
 
+/* ---- Ozaki-II reconstruction ------------------------------------------
+ * The residues c_j = X mod m_j of the exact integer X = sum_p a_ip b_pj give
+ * X by the Chinese remainder theorem: with M = prod m_j, M_j = M / m_j and
+ * w_j = M_j^-1 mod m_j, X = (sum_j ((c_j w_j) mod m_j) M_j) mod M, taken in
+ * (-M/2, M/2) (|X| < M/2 by the choice of P).  The sum is formed exactly
+ * with 24-bit multiply-adds of the byte y_j into 16-bit limbs of M_j (limb
+ * sums stay below 2^29), carried into 192-bit integers (32-bit limbs),
+ * reduced modulo M, rounded to 53 bits (ties to even) and scaled by
+ * 2^(e_i + f_j - 2P) with one ldexp. */
+constexpr int OZ2_LIMBS = 6;    /* 32-bit limbs: M < 2^171 */
+constexpr int OZ2_LIMBS16 = 11; /* 16-bit limbs of M_j */
+
+struct Oz2Crt {
+  int nmod, P;
+  unsigned Mj16[OZ2_MAX_MODULI][OZ2_LIMBS16]; /* M / m_j, 16-bit limbs */
+  int w[OZ2_MAX_MODULI];                       /* (M / m_j)^-1 mod m_j */
+  int mod[OZ2_MAX_MODULI];                     /* m_j */
+  float rcp[OZ2_MAX_MODULI];                   /* 1 / m_j, rounded */
+  unsigned M[OZ2_LIMBS], H[OZ2_LIMBS];         /* M and floor(M / 2) */
+  double rMd;                                  /* 1 / M, rounded */
+};
+
+/* Host: the constants of the first nmod moduli and P for K chunks of kc. */
+bool oz2_constants(int nmod, int kc, Oz2Crt *c) {
+  if (nmod < OZ2_MIN_MODULI || nmod > OZ2_MAX_MODULI || kc < 1) return false;
+  memset(c, 0, sizeof *c);
+  c->nmod = nmod;
+  c->M[0] = 1;
+  for (int j = 0; j < nmod; ++j) {
+    unsigned long long carry = 0;
+    for (int l = 0; l < OZ2_LIMBS; ++l) {
+      const unsigned long long v = (unsigned long long)c->M[l] * OZ2_MODULI[j] + carry;
+      c->M[l] = (unsigned)v;
+      carry = v >> 32;
+    }
+  }
+  int L = 0; /* floor(log2 M) */
+  for (int l = OZ2_LIMBS - 1; l >= 0; --l)
+    if (c->M[l]) {
+      L = 32 * l + 31 - __builtin_clz(c->M[l]);
+      break;
+    }
+  int kbits = 0;
+  while ((1LL << kbits) <= kc) ++kbits;
+  /* |X| <= kc 2^2P < 2^(kbits+2P) <= 2^(L-1) <= M/2; capped so that the
+   * scaled integers stay below 2^78 (five 16-bit limbs in the split) */
+  c->P = (L - 1 - kbits) / 2;
+  if (c->P > OZ2_MAX_BITS) c->P = OZ2_MAX_BITS;
+  for (int j = 0; j < nmod; ++j) {
+    const int m = OZ2_MODULI[j];
+    unsigned mj[OZ2_LIMBS];
+    unsigned long long rem = 0;
+    for (int l = OZ2_LIMBS - 1; l >= 0; --l) { /* M_j = M / m, exact */
+      const unsigned long long cur = (rem << 32) | c->M[l];
+      mj[l] = (unsigned)(cur / m);
+      rem = cur % m;
+    }
+    for (int l = 0; l < OZ2_LIMBS16; ++l)
+      c->Mj16[j][l] = (mj[l / 2] >> (16 * (l % 2))) & 0xffffu;
+    if (mj[OZ2_LIMBS - 1] >> 16) return false; /* M_j < 2^176 */
+    unsigned long long r = 0; /* M_j mod m */
+    for (int l = OZ2_LIMBS - 1; l >= 0; --l) r = ((r << 32) | mj[l]) % m;
+    int inv = 0;
+    for (int t = 1; t < m; ++t)
+      if ((r * t) % m == 1) {
+        inv = t;
+        break;
+      }
+    if (!inv) return false;
+    c->w[j] = inv;
+  }
+  for (int l = 0; l < OZ2_LIMBS; ++l)
+    c->H[l] = (c->M[l] >> 1) | (l + 1 < OZ2_LIMBS ? c->M[l + 1] << 31 : 0u);
+  double md = 0.0;
+  for (int l = OZ2_LIMBS - 1; l >= 0; --l) md = md * 4294967296.0 + (double)c->M[l];
+  c->rMd = 1.0 / md;
+  for (int j = 0; j < nmod; ++j) {
+    c->mod[j] = OZ2_MODULI[j];
+    c->rcp[j] = 1.0f / (float)OZ2_MODULI[j];
+  }
+  return true;
+}
+
+/* Bits [s, s+64) of a 192-bit number. */
+__device__ __forceinline__ unsigned long long bits64(const unsigned (&x)[OZ2_LIMBS], int s) {
+  unsigned long long r = 0;
+#pragma unroll
+  for (int l = 0; l < OZ2_LIMBS; ++l) {
+    const int pos = 32 * l - s; /* where limb l lands in the result */
+    if (pos >= 64 || pos <= -32) continue;
+    const unsigned long long v = x[l];
+    r |= pos >= 0 ? v << pos : v >> -pos;
+  }
+  return r;
+}
+
+/* X from the carried 16-bit limb sums, reduced modulo M, rounded and scaled;
+ * then the alpha/beta update of C(i, j). */
+__device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], const Oz2Crt &crt,
+                                           int e, int f, double *cp, double alpha,
+                                           double beta) {
+  unsigned x[OZ2_LIMBS];
+  {
+    unsigned carry = 0, half = 0;
+#pragma unroll
+    for (int l = 0; l < OZ2_LIMBS16; ++l) {
+      const unsigned v = acc[l] + carry; /* < 2^29 + 2^13 */
+      carry = v >> 16;
+      if (l % 2 == 0) {
+        half = v & 0xffffu;
+      } else {
+        x[l / 2] = half | (v << 16);
+      }
+    }
+    x[OZ2_LIMBS - 1] = half | (carry << 16); /* limb 10 and the final carry */
+  }
+  /* X = X' mod M: subtract q M with q from FP64 (off by at most one),
+   * then correct */
+  double xd = 0.0;
+#pragma unroll
+  for (int l = OZ2_LIMBS - 1; l >= 0; --l) xd = xd * 4294967296.0 + (double)x[l];
+  const long long q = (long long)(xd * crt.rMd);
+  long long borrow = 0;
+#pragma unroll
+  for (int l = 0; l < OZ2_LIMBS; ++l) {
+    const long long d = (long long)x[l] - (long long)((unsigned long long)q * crt.M[l]) + borrow;
+    x[l] = (unsigned)d;
+    borrow = d >> 32; /* arithmetic: floor(d / 2^32) */
+  }
+  if (borrow < 0) { /* X < 0: add M */
+    unsigned long long c2 = 0;
+#pragma unroll
+    for (int l = 0; l < OZ2_LIMBS; ++l) {
+      const unsigned long long v = (unsigned long long)x[l] + crt.M[l] + c2;
+      x[l] = (unsigned)v;
+      c2 = v >> 32;
+    }
+  }
+  bool ge = true; /* X >= M: subtract M */
+#pragma unroll
+  for (int l = OZ2_LIMBS - 1; l >= 0; --l) {
+    if (x[l] != crt.M[l]) {
+      ge = x[l] > crt.M[l];
+      break;
+    }
+  }
+  if (ge) {
+    long long b2 = 0;
+#pragma unroll
+    for (int l = 0; l < OZ2_LIMBS; ++l) {
+      const long long d = (long long)x[l] - (long long)crt.M[l] + b2;
+      x[l] = (unsigned)d;
+      b2 = d >> 32;
+    }
+  }
+  /* symmetric range: X > M/2 stands for X - M */
+  bool gt = false;
+#pragma unroll
+  for (int l = OZ2_LIMBS - 1; l >= 0; --l) {
+    if (x[l] != crt.H[l]) {
+      gt = x[l] > crt.H[l];
+      break;
+    }
+  }
+  const bool neg = gt;
+  if (neg) {
+    long long b3 = 0;
+#pragma unroll
+    for (int l = 0; l < OZ2_LIMBS; ++l) {
+      const long long d = (long long)crt.M[l] - (long long)x[l] + b3;
+      x[l] = (unsigned)d;
+      b3 = d >> 32;
+    }
+  }
+  /* round |X| to 53 bits, ties to even */
+  int top = -1;
+#pragma unroll
+  for (int l = OZ2_LIMBS - 1; l >= 0; --l)
+    if (top < 0 && x[l]) top = 32 * l + 31 - __builtin_clz(x[l]);
+  double mag = 0.0;
+  int shift = 0;
+  if (top >= 0) {
+    unsigned long long mant;
+    if (top <= 52) {
+      mant = (unsigned long long)x[0] | ((unsigned long long)x[1] << 32);
+    } else {
+      shift = top - 52;
+      mant = bits64(x, shift) & ((1ull << 53) - 1);
+      const int hb = shift - 1;
+      const bool half = (bits64(x, hb) & 1ull) != 0;
+      bool sticky = false;
+#pragma unroll
+      for (int l = 0; l < OZ2_LIMBS; ++l) {
+        const int lo = 32 * l;
+        if (lo >= hb) continue;
+        const unsigned mask = hb - lo >= 32 ? 0xffffffffu : ((1u << (hb - lo)) - 1u);
+        sticky |= (x[l] & mask) != 0;
+      }
+      if (half && (sticky || (mant & 1))) {
+        ++mant;
+        if (mant == (1ull << 53)) {
+          mant >>= 1;
+          ++shift;
+        }
+      }
+    }
+    mag = (double)mant;
+  }
+  double scaled;
+  if (e == OZ_NONFINITE || f == OZ_NONFINITE) {
+    scaled = __builtin_nan("");
+  } else {
+    scaled = ldexp(mag, shift + e + f - 2 * crt.P);
+    scaled = neg ? -scaled : scaled;
+  }
+  *cp = __builtin_fma(alpha, scaled, beta * *cp);
+}
+
+/* Four consecutive elements (column-major index t) per thread: the modulus
+ * loop reads each plane's bytes of the four with one load and each M_j limb
+ * once for the four. */
+__global__ void __launch_bounds__(256)
+ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
+                    const int *__restrict__ ea, const int *__restrict__ fb,
+                    double *__restrict__ C, int64_t ldc, int M, int N, double alpha,
+                    double beta) {
+  constexpr int E = 4;
+  const int64_t total = (int64_t)M * N;
+  const int64_t quads = (total + E - 1) / E;
+  const bool vec = (total & (E - 1)) == 0;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < quads;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t0 = u * E;
+    /* X'_e = sum_g y_g M_g with y_g = (c_g w_g) mod m_g, < nmod * M */
+    unsigned acc[E][OZ2_LIMBS16];
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+#pragma unroll
+      for (int l = 0; l < OZ2_LIMBS16; ++l) acc[q][l] = 0u;
+#pragma unroll 1
+    for (int g = 0; g < crt.nmod; ++g) {
+      const int m = crt.mod[g];
+      const float rcp = crt.rcp[g];
+      const unsigned w = (unsigned)crt.w[g];
+      const uint8_t *rp = R + (int64_t)g * total + t0;
+      unsigned c4 = 0;
+      if (vec) {
+        c4 = *reinterpret_cast<const unsigned *>(rp);
+      } else {
+#pragma unroll
+        for (int q = 0; q < E; ++q)
+          if (t0 + q < total) c4 |= (unsigned)rp[q] << (8 * q);
+      }
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const unsigned cw = __umul24((c4 >> (8 * q)) & 0xffu, w);
+        const int qt = (int)((float)cw * rcp); /* cw < 2^16: off by <= 1 */
+        int yr = (int)cw - qt * m;
+        yr += yr < 0 ? m : 0;
+        yr -= yr >= m ? m : 0;
+#pragma unroll
+        for (int l = 0; l < OZ2_LIMBS16; ++l)
+          acc[q][l] += __umul24((unsigned)yr, crt.Mj16[g][l]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int64_t t = t0 + q;
+      if (t < total) {
+        const int64_t j = t / M, i = t - j * M;
+        oz2_finish(acc[q], crt, ea[i], fb[j], C + i + j * ldc, alpha, beta);
+      }
+    }
+  }
+}
+
+struct Oz2Layout {
+  size_t ea, fb, mb, as, bs, r8, total;
+  int64_t kpad;
+};
+Oz2Layout oz2_layout(int m, int n, int k, int nmod) {
+  Oz2Layout L;
+  L.kpad = (int64_t)rup((size_t)k, SL_BK);
+  L.ea = 0;
+  L.fb = rup((size_t)m * 4, 256);
+  L.mb = L.fb + rup((size_t)n * 4, 256);
+  L.as = L.mb + rup((size_t)(m > n ? m : n) * 8, 256);
+  L.bs = L.as + rup((size_t)nmod * m * L.kpad, 256);
+  L.r8 = L.bs + rup((size_t)nmod * n * L.kpad, 256);
+  L.total = L.r8 + rup((size_t)nmod * m * n, 256);
+  return L;
+}
+
 } // namespace
+
+extern "C" size_t gemmx_ozaki2_workspace(int m, int n, int k, int moduli) {
+  return oz2_layout(m, n, k, moduli).total;
+}
+
+extern "C" int gemmx_ozaki2_bits(int moduli, int k) {
+  Oz2Crt c;
+  return oz2_constants(moduli, k < OZ2_KCHUNK ? (k > 0 ? k : 1) : OZ2_KCHUNK, &c) ? c.P : -1;
+}
+
+extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *work,
+                                       hipStream_t st) {
+  Oz2Crt crt;
+  const int kc = p->k < OZ2_KCHUNK ? p->k : OZ2_KCHUNK;
+  if (!work || !oz2_constants(nmod, kc, &crt)) return hipErrorInvalidValue;
+  const Oz2Layout L = oz2_layout(p->m, p->n, p->k, nmod);
+  char *w = (char *)work;
+  int *ea = (int *)(w + L.ea), *fb = (int *)(w + L.fb);
+  int8_t *as = (int8_t *)(w + L.as), *bs = (int8_t *)(w + L.bs);
+  uint8_t *R = (uint8_t *)(w + L.r8);
+  unsigned long long *mb = (unsigned long long *)(w + L.mb);
+  hipError_t e = ozaki2_prepare((const double *)p->a, p->lda, p->ta != 0, p->m, p->k, L.kpad,
+                                nmod, crt.P, ea, mb, as, st);
+  if (e != hipSuccess) return e;
+  e = ozaki2_prepare((const double *)p->b, p->ldb, p->tb == 0, p->n, p->k, L.kpad, nmod,
+                     crt.P, fb, mb, bs, st);
+  if (e != hipSuccess) return e;
+  /* group g: plane g of A times plane g of B, stored modulo m_g */
+  SliceGroups G;
+  G.ngroups = 0;
+  for (int g = 0; g < nmod; ++g) {
+    const int zero = 0;
+    G.add(OZ2_MODULI[g], 1, &zero, &zero);
+  }
+  const int kp = (int)L.kpad;
+  const int64_t total = (int64_t)p->m * p->n;
+  int64_t rblocks = (total + 4 * 256 - 1) / (4 * 256); /* four elements per thread */
+  if (rblocks > 65536) rblocks = 65536;
+  for (int kb0 = 0; kb0 < kp; kb0 += OZ2_KCHUNK) {
+    const int kb1 = kb0 + OZ2_KCHUNK < kp ? kb0 + OZ2_KCHUNK : kp;
+    e = slice_products(SP_I8_MOD, as, bs, p->m, p->n, L.kpad, kb0, kb1, G, R, NULL, 0, 0.f,
+                       0.f, st);
+    if (e != hipSuccess) return e;
+    /* chunks after the first add onto C: beta = 1 leaves C as it is */
+    hipLaunchKernelGGL(ozaki2_recon_kernel, dim3((unsigned)rblocks), dim3(256), 0, st, R, crt,
+                       ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha,
+                       kb0 == 0 ? p->beta : 1.0);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 extern "C" int gemmx_ozaki_kchunk(int slices) {
   const int64_t lim = (int64_t)2147483647 / ((int64_t)slices * 127 * 127);
@@ -114,7 +467,7 @@ extern "C" hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int s, void *work,
   if (rblocks > 65536) rblocks = 65536;
   for (int kb0 = 0; kb0 < kp; kb0 += kc) {
     const int kb1 = kb0 + kc < kp ? kb0 + kc : kp;
-    e = slice_products(false, as, bs, p->m, p->n, L.kpad, kb0, kb1, groups, S, NULL, 0,
+    e = slice_products(SP_I8_SUM, as, bs, p->m, p->n, L.kpad, kb0, kb1, groups, S, NULL, 0,
                        0.f, 0.f, st);
     if (e != hipSuccess) return e;
     /* chunks after the first add onto C: beta = 1 leaves C as it is */
@@ -148,7 +501,7 @@ extern "C" hipError_t gemmx_bf16x3_f32(const gemmx_problem *p, void *work, hipSt
   const int tt[3] = {1, 0, 0}, uu[3] = {0, 1, 0};
   G.add(0, 3, tt, uu);
   const int64_t kpad_bytes = kpad * 2;
-  return slice_products(true, (const int8_t *)as, (const int8_t *)bs, p->m, p->n, kpad_bytes,
+  return slice_products(SP_BF16, (const int8_t *)as, (const int8_t *)bs, p->m, p->n, kpad_bytes,
                         0, (int)kpad_bytes, G, NULL, (float *)p->c, p->ldc, (float)p->alpha,
                         (float)p->beta, st);
 }

@@ -6,7 +6,8 @@
  * slice pairs and K stages, staging K-contiguous slice tiles through LDS
  * (double-buffered, the next stage's global loads in flight during the
  * current stage's MFMAs), and multiplies them with
- *   INT8: v_mfma_i32_16x16x64_i8   -> exact INT32 sums
+ *   INT8: v_mfma_i32_16x16x64_i8   -> exact INT32 sums (Ozaki-I weight sums,
+ *         or Ozaki-II residue products stored modulo their modulus)
  *   BF16: v_mfma_f32_16x16x32_bf16 -> FP32 sums
  * Both consume 64 K-bytes per lane-group step: lane l holds row l%16 and
  * bytes [16*(l/16), 16*(l/16)+16) of each 64-byte K chunk, for A and B
@@ -71,12 +72,13 @@ struct SliceTileCopy {
   }
 };
 
-template <bool BF16, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
+template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
 __global__ void __launch_bounds__(64 * WM * WN)
 slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
                       int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
-                      SliceGroups G, int tiles_m, int tiles_n, int *__restrict__ S,
+                      SliceGroups G, int tiles_m, int tiles_n, void *__restrict__ S,
                       float *__restrict__ C, int64_t ldc, float alpha, float beta) {
+  constexpr bool BF16 = KIND == SP_BF16;
   constexpr int NT = 64 * WM * WN;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 16, FN = WTN / 16;
@@ -113,7 +115,9 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
   const int8_t *b_tile = Bs + (int64_t)n0 * kpad_bytes;
   auto load_iter = [&](int q) {
     const int pl = q / nkt, kt = kt_begin + q - pl * nkt;
-    const int t = (tpack >> (4 * pl)) & 15, u = (upack >> (4 * pl)) & 15;
+    /* residue products: group g multiplies plane g by plane g */
+    const int t = KIND == SP_I8_MOD ? g : (tpack >> (4 * pl)) & 15;
+    const int u = KIND == SP_I8_MOD ? g : (upack >> (4 * pl)) & 15;
     const int *ga = reinterpret_cast<const int *>(a_tile + (int64_t)t * M * kpad_bytes) + kt * LEN;
     const int *gb = reinterpret_cast<const int *>(b_tile + (int64_t)u * N * kpad_bytes) + kt * LEN;
     ca.load(ga, ld_i, M - m0, tid);
@@ -195,8 +199,31 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (row + r < M) cp[r] = __builtin_fmaf(alpha, (float)acc[i][j][r], beta * cp[r]);
+      } else if (KIND == SP_I8_MOD) {
+        /* |sum| < 2^31; q = floor(sum / m) from FP64 (off by at most one,
+         * corrected below), so the stored residue is exact */
+        const int mod = G.gout[g];
+        const double rcp = 1.0 / (double)mod;
+        unsigned bytes = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int v = acc[i][j][r];
+          int q = (int)floor((double)v * rcp);
+          int rem = v - q * mod;
+          rem += rem < 0 ? mod : 0;
+          rem -= rem >= mod ? mod : 0;
+          bytes |= (unsigned)rem << (8 * r);
+        }
+        uint8_t *sp = (uint8_t *)S + (int64_t)g * M * N + row + (int64_t)col * M;
+        if (row + 3 < M && (M & 3) == 0) {
+          *reinterpret_cast<unsigned *>(sp) = bytes;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (row + r < M) sp[r] = (uint8_t)(bytes >> (8 * r));
+        }
       } else {
-        int *sp = S + (int64_t)G.gout[g] * M * N + row + (int64_t)col * M;
+        int *sp = (int *)S + (int64_t)G.gout[g] * M * N + row + (int64_t)col * M;
         if (row + 3 < M && (M & 3) == 0) {
           *reinterpret_cast<i4_t *>(sp) = *reinterpret_cast<i4_t *>(&acc[i][j]);
         } else {
@@ -208,11 +235,11 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
     }
 }
 
-template <bool BF16, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
+template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
 hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
-                      int kb0, int kb1, const SliceGroups &G, int *S, float *C,
+                      int kb0, int kb1, const SliceGroups &G, void *S, float *C,
                       int64_t ldc, float alpha, float beta, hipStream_t st) {
-  auto kern = slice_products_kernel<BF16, BM, BN, WM, WN, BKB, PIPE>;
+  auto kern = slice_products_kernel<KIND, BM, BN, WM, WN, BKB, PIPE>;
   const size_t lds = (size_t)2 * (BM + BN) * (BKB / 4 + 4) * 4;
   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -238,33 +265,40 @@ int slice_cfg() {
   return cfg;
 }
 
-template <bool BF16>
+template <int KIND>
 hipError_t products(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
-                    int kb0, int kb1, const SliceGroups &G, int *S, float *C,
+                    int kb0, int kb1, const SliceGroups &G, void *S, float *C,
                     int64_t ldc, float alpha, float beta, hipStream_t st) {
   switch (slice_cfg()) {
   case 1:
-    return launch_cfg<BF16, 256, 256, 2, 2, 64, false>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+    return launch_cfg<KIND, 256, 256, 2, 2, 64, false>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
                                                        ldc, alpha, beta, st);
   case 2:
-    return launch_cfg<BF16, 256, 256, 2, 4, 128, false>(as, bs, m, n, kpad, kb0, kb1, G, S,
+    return launch_cfg<KIND, 256, 256, 2, 4, 128, false>(as, bs, m, n, kpad, kb0, kb1, G, S,
                                                         C, ldc, alpha, beta, st);
   default:
-    return launch_cfg<BF16, 256, 256, 2, 4, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+    return launch_cfg<KIND, 256, 256, 2, 4, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
                                                        ldc, alpha, beta, st);
   }
 }
 
 } // namespace
 
-hipError_t slice_products(bool bf16, const int8_t *as, const int8_t *bs, int m, int n,
+hipError_t slice_products(int kind, const int8_t *as, const int8_t *bs, int m, int n,
                           int64_t kpad_bytes, int kb0, int kb1, const SliceGroups &g,
-                          int *S, float *C, int64_t ldc, float alpha, float beta,
+                          void *S, float *C, int64_t ldc, float alpha, float beta,
                           hipStream_t st) {
-  return bf16 ? products<true>(as, bs, m, n, kpad_bytes, kb0, kb1, g, S, C, ldc, alpha,
-                               beta, st)
-              : products<false>(as, bs, m, n, kpad_bytes, kb0, kb1, g, S, C, ldc, alpha,
-                                beta, st);
+  switch (kind) {
+  case SP_BF16:
+    return products<SP_BF16>(as, bs, m, n, kpad_bytes, kb0, kb1, g, S, C, ldc, alpha, beta,
+                             st);
+  case SP_I8_MOD:
+    return products<SP_I8_MOD>(as, bs, m, n, kpad_bytes, kb0, kb1, g, S, C, ldc, alpha, beta,
+                               st);
+  default:
+    return products<SP_I8_SUM>(as, bs, m, n, kpad_bytes, kb0, kb1, g, S, C, ldc, alpha, beta,
+                               st);
+  }
 }
 
 } // namespace gemmx

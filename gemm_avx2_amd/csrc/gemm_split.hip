@@ -6,6 +6,8 @@
  * Ozaki-I (FP64 -> INT8): per vector the exponent e with max|x| < 2^e, then s
  * signed 7-bit digits of x*2^-e by truncation toward zero,
  * x*2^-e = sum_t d_t 2^-7t + r, |d_t| <= 127; every step is exact in FP64.
+ * Ozaki-II (FP64 -> INT8 residues): per vector the same exponent e, then the
+ * residues of rint(x * 2^(P-e)) modulo each modulus (exact integer work).
  * BF16x3 (FP32 -> 2 x BF16): hi = bf16_rne(x), lo = bf16_rne(x - hi), the
  * difference being exact in FP32.
  * No reference counterpart (the reference packs A and B for its AVX2
@@ -175,6 +177,155 @@ ozaki_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
     *reinterpret_cast<i4_t *>(out + ((int64_t)t * nvec + v) * kpad + p0 + 16 * q) = dig[t];
 }
 
+/* ---- Ozaki-II residues ------------------------------------------------
+ * x = (-1)^neg * mant * 2^q exactly (mant < 2^53); the scaled integer is
+ * rint(x * 2^(P - e)) = rint(mant * 2^sh), sh = q + P - e <= P - 52 <= 25.
+ * Its magnitude (mant << sh, or rint(mant / 2^-sh) ties to even) is cut into
+ * five 16-bit limbs L_l, so that for each modulus m
+ *   |a| mod m = (sum_l L_l * (2^16l mod m)) mod m,
+ * a sum of 24-bit products below 2^27 reduced once with an FP32 quotient
+ * (off by at most one, then corrected).  The residue of a is negated for a
+ * negative a and stored as a signed byte in [-m/2, m/2). */
+struct Oz2Limbs {
+  unsigned l[5];
+  int neg;
+};
+
+__device__ __forceinline__ Oz2Limbs oz2_limbs(double x, int e, int P) {
+  Oz2Limbs r;
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+  r.neg = (int)(bits >> 63);
+  const int be = (int)((bits >> 52) & 0x7ff);
+  unsigned long long mant = bits & 0xfffffffffffffull;
+  int q = -1074;
+  if (be != 0) {
+    mant |= 1ull << 52;
+    q = be - 1075;
+  }
+  const int sh = q + P - e;
+  unsigned long long lo;
+  unsigned hi = 0;
+  if (sh >= 0) { /* sh <= 25: at most 78 bits */
+    lo = mant << sh;
+    hi = sh ? (unsigned)(mant >> (64 - sh)) : 0u;
+  } else {
+    const int s = -sh;
+    lo = 0;
+    if (s <= 63) {
+      lo = mant >> s;
+      const unsigned long long rem = mant & ((1ull << s) - 1), half = 1ull << (s - 1);
+      lo += (rem > half || (rem == half && (lo & 1))) ? 1 : 0;
+    }
+  }
+  r.l[0] = (unsigned)lo & 0xffffu;
+  r.l[1] = (unsigned)(lo >> 16) & 0xffffu;
+  r.l[2] = (unsigned)(lo >> 32) & 0xffffu;
+  r.l[3] = (unsigned)(lo >> 48);
+  r.l[4] = hi & 0xffffu; /* < 2^14 */
+  return r;
+}
+
+constexpr unsigned pow2_mod(int e, int m) {
+  unsigned r = 1u % (unsigned)m;
+  for (int i = 0; i < e; ++i) r = (r * 2u) % (unsigned)m;
+  return r;
+}
+
+template <int J>
+__device__ __forceinline__ unsigned oz2_residue_byte(const Oz2Limbs &a) {
+  constexpr int m = OZ2_MODULI[J];
+  int r;
+  if (m == 256) {
+    r = (int)(a.l[0] & 255u);
+  } else {
+    const unsigned s = a.l[0] + __umul24(a.l[1], pow2_mod(16, m)) +
+                       __umul24(a.l[2], pow2_mod(32, m)) + __umul24(a.l[3], pow2_mod(48, m)) +
+                       __umul24(a.l[4], pow2_mod(64, m));
+    const int qt = (int)((float)s * (1.0f / (float)m));
+    r = (int)s - qt * m;
+    r += r < 0 ? m : 0;
+    r -= r >= m ? m : 0;
+  }
+  if (a.neg && r) r = m - r;
+  if (r > (m - 1) / 2) r -= m;
+  return (unsigned)r & 0xffu;
+}
+
+/* All nmod residue bytes of one element into byte q of each plane word. */
+template <int J>
+__device__ __forceinline__ void oz2_encode(const Oz2Limbs &a, int nmod, i4_t (&res)[OZ2_MAX_MODULI],
+                                           int q) {
+  if (J < nmod) res[J][q >> 2] |= (int)(oz2_residue_byte<J>(a) << (8 * (q & 3)));
+  if constexpr (J + 1 < OZ2_MAX_MODULI) oz2_encode<J + 1>(a, nmod, res, q);
+}
+
+/* nmod residue planes of 16 consecutive K entries of one vector per thread
+ * (as ozaki_split_kernel). */
+__global__ void __launch_bounds__(256)
+ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int nvec,
+                    int len, int64_t kpad, const int *__restrict__ ex, int v_fast,
+                    int nmod, int P, int8_t *__restrict__ out) {
+  const int64_t nblk = kpad / 16;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (tid >= nblk * nvec) return;
+  int64_t v, blk;
+  if (v_fast) { v = tid % nvec; blk = tid / nvec; }
+  else        { v = tid / nblk; blk = tid % nblk; }
+  const int e = ex[v];
+  i4_t res[OZ2_MAX_MODULI];
+#pragma unroll
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
+#pragma unroll 2
+  for (int q = 0; q < 16; ++q) {
+    const int64_t p = blk * 16 + q;
+    const double val = (p < len && e != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0;
+    oz2_encode<0>(oz2_limbs(val, e, P), nmod, res, q);
+  }
+#pragma unroll
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j)
+    if (j < nmod)
+      *reinterpret_cast<i4_t *>(out + ((int64_t)j * nvec + v) * kpad + blk * 16) = res[j];
+}
+
+/* The same for vectors adjacent in memory, through the LDS transpose of
+ * ozaki_split_strided_kernel. */
+__global__ void __launch_bounds__(256)
+ozaki2_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec, int len,
+                            int64_t kpad, const int *__restrict__ ex, int nmod, int P,
+                            int8_t *__restrict__ out) {
+  constexpr int ROW = 129;
+  __shared__ double tile[32 * ROW];
+  const int v0 = blockIdx.x * 32;
+  const int64_t p0 = (int64_t)blockIdx.y * 128;
+  {
+    const int vl = threadIdx.x & 31, pr = threadIdx.x >> 5;
+    const int v = v0 + vl;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int p = pr + 8 * j;
+      const double val = (v < nvec && p0 + p < len) ? x[v + (p0 + p) * eld] : 0.0;
+      tile[vl * ROW + (p % 16) * 8 + p / 16] = val;
+    }
+  }
+  __syncthreads();
+  const int vl = threadIdx.x >> 3, q = threadIdx.x & 7;
+  const int v = v0 + vl;
+  if (v >= nvec) return;
+  const int e = ex[v];
+  i4_t res[OZ2_MAX_MODULI];
+#pragma unroll
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
+#pragma unroll 2
+  for (int i = 0; i < 16; ++i) {
+    const double val = e != OZ_NONFINITE ? tile[vl * ROW + i * 8 + q] : 0.0;
+    oz2_encode<0>(oz2_limbs(val, e, P), nmod, res, i);
+  }
+#pragma unroll
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j)
+    if (j < nmod)
+      *reinterpret_cast<i4_t *>(out + ((int64_t)j * nvec + v) * kpad + p0 + 16 * q) = res[j];
+}
+
 __device__ __forceinline__ unsigned bf16_rne_bits(float x) {
   unsigned u = __float_as_uint(x);
   if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0x007fffffu) ? 0x40u : 0u);
@@ -280,6 +431,29 @@ hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return ozaki_split_strided(s, x, ld, nvec, k, kpad, ex, slices, st);
+}
+
+hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec, int k,
+                          int64_t kpad, int nmod, int P, int *ex, unsigned long long *mbits,
+                          int8_t *res, hipStream_t st) {
+  if (contiguous) {
+    hipLaunchKernelGGL(exp_contig_kernel, dim3((nvec + 3) / 4), dim3(256), 0, st, x,
+                       ld, nvec, k, ex);
+    const int64_t threads = (kpad / 16) * nvec;
+    hipLaunchKernelGGL(ozaki2_split_kernel, dim3((unsigned)((threads + 255) / 256)),
+                       dim3(256), 0, st, x, ld, (int64_t)1, nvec, k, kpad, ex, 0, nmod, P,
+                       res);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, mbits,
+                     nvec);
+  hipLaunchKernelGGL(absmax_strided_kernel, dim3((nvec + 63) / 64, (k + 255) / 256),
+                     dim3(256), 0, st, x, ld, nvec, k, mbits);
+  hipLaunchKernelGGL(exp_from_bits_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
+                     mbits, nvec, ex);
+  hipLaunchKernelGGL(ozaki2_split_strided_kernel, dim3((nvec + 31) / 32, (unsigned)(kpad / 128)),
+                     dim3(256), 0, st, x, ld, nvec, k, kpad, ex, nmod, P, res);
+  return hipGetLastError();
 }
 
 hipError_t bf16_prepare(const float *x, int64_t ld, bool contiguous, int nvec,

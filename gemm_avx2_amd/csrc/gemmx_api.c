@@ -98,7 +98,14 @@ static int g_smode = GEMMX_MODE_NATIVE;
 static void init_once(void) {
   for (int i = 0; i < GEMMX_MAX_DEV; i++) pthread_mutex_init(&g_dev[i].lock, NULL);
   const char *dm = getenv("GEMMX_DGEMM_MODE");
-  if (dm && strncmp(dm, "ozaki", 5) == 0) {
+  if (dm && strncmp(dm, "ozaki2", 6) == 0) {
+    g_dmode = GEMMX_MODE_OZAKI2;
+    g_dslices = 20;
+    if (dm[6] == ':') {
+      int s = atoi(dm + 7);
+      if (s >= 8 && s <= 22) g_dslices = s;
+    }
+  } else if (dm && strncmp(dm, "ozaki", 5) == 0) {
     g_dmode = GEMMX_MODE_OZAKI;
     if (dm[5] == ':') {
       int s = atoi(dm + 6);
@@ -221,6 +228,10 @@ static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
     e = gemmx_ozaki_f64(p, slices, work, s);
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "ozaki kernel launch");
   }
+  if (es == 8 && mode == GEMMX_MODE_OZAKI2) {
+    e = gemmx_ozaki2_f64(p, slices, work, s);
+    return e == hipSuccess ? GEMMX_OK : hip_fail(e, "ozaki2 kernel launch");
+  }
   if (es == 4 && mode == GEMMX_MODE_BF16X3) {
     e = gemmx_bf16x3_f32(p, work, s);
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "bf16x3 kernel launch");
@@ -235,6 +246,8 @@ static size_t emu_ws(const call_t *x, int mode, int slices) {
   if (x->alpha == 0.0 || x->k == 0 || x->m == 0 || x->n == 0) return 0;
   if (x->es == 8 && mode == GEMMX_MODE_OZAKI)
     return gemmx_ozaki_workspace(x->m, x->n, x->k, slices);
+  if (x->es == 8 && mode == GEMMX_MODE_OZAKI2)
+    return gemmx_ozaki2_workspace(x->m, x->n, x->k, slices);
   if (x->es == 4 && mode == GEMMX_MODE_BF16X3)
     return gemmx_bf16x3_workspace(x->m, x->n, x->k);
   return 0;
@@ -355,7 +368,8 @@ EXPORT int dgemm_x_set_mode(int mode, int slices) {
     g_dmode = mode;
     return GEMMX_OK;
   }
-  if (mode == GEMMX_MODE_OZAKI && slices >= 3 && slices <= 8) {
+  if ((mode == GEMMX_MODE_OZAKI && slices >= 3 && slices <= 8) ||
+      (mode == GEMMX_MODE_OZAKI2 && slices >= 8 && slices <= 22)) {
     g_dmode = mode;
     g_dslices = slices;
     return GEMMX_OK;
@@ -479,6 +493,29 @@ EXPORT int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
   }
   call_t x = {8, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
   return dev_call(&x, GEMMX_MODE_OZAKI, slices, work, work_bytes, stream);
+}
+
+EXPORT size_t dgemm_x_ozaki2_workspace(int m, int n, int k, int moduli) {
+  if (m <= 0 || n <= 0 || k <= 0 || moduli < 8 || moduli > 22) return 0;
+  return gemmx_ozaki2_workspace(m, n, k, moduli);
+}
+
+EXPORT int dgemm_x_ozaki2_bits(int moduli, int k) {
+  if (k <= 0) return -1;
+  return gemmx_ozaki2_bits(moduli, k);
+}
+
+EXPORT int dgemm_x_ozaki2_dev(char transa, char transb, int m, int n, int k,
+                              double alpha, const double *a, int lda,
+                              const double *b, int ldb, double beta, double *c,
+                              int ldc, int moduli, void *work, size_t work_bytes,
+                              void *stream) {
+  if (moduli < 8 || moduli > 22) {
+    clear_error();
+    return set_error(GEMMX_ERR_ARG, "moduli=%d outside 8..22", moduli);
+  }
+  call_t x = {8, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return dev_call(&x, GEMMX_MODE_OZAKI2, moduli, work, work_bytes, stream);
 }
 
 EXPORT size_t sgemm_x_bf16x3_workspace(int m, int n, int k) {

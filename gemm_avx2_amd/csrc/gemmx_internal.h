@@ -102,6 +102,15 @@ int gemmx_ozaki_kchunk(int slices);
 hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int slices, void *work,
                            hipStream_t s);
 
+/* Emulated DGEMM, Ozaki-II: residues modulo `moduli` (8..22) pairwise-coprime
+ * integers <= 256 (gemm_emulated.hip).  `work` is device scratch of
+ * gemmx_ozaki2_workspace() bytes; gemmx_ozaki2_bits gives the integer bits P
+ * used for K = k (-1 for an unsupported modulus count). */
+size_t gemmx_ozaki2_workspace(int m, int n, int k, int moduli);
+int gemmx_ozaki2_bits(int moduli, int k);
+hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int moduli, void *work,
+                            hipStream_t s);
+
 /* Emulated SGEMM, split-BF16x3 (gemm_slices.hip).  `work` is device scratch
  * of gemmx_bf16x3_workspace() bytes. */
 size_t gemmx_bf16x3_workspace(int m, int n, int k);

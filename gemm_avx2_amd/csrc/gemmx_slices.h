@@ -61,13 +61,37 @@ hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
 hipError_t bf16_prepare(const float *x, int64_t ld, bool contiguous, int nvec, int k,
                         int64_t kpad, unsigned short *out, hipStream_t st);
 
+/* Ozaki-II: residues of rint(x * 2^(P - e)) modulo OZ2_MODULI[0..nmod-1]
+ * (pairwise coprime, taken greedily downward from 256), each stored as a
+ * signed byte in [-m/2, m/2), one plane per modulus in the slice layout. */
+constexpr int OZ2_MAX_MODULI = 22;
+constexpr int OZ2_MIN_MODULI = 8;
+constexpr int OZ2_KCHUNK = 65536; /* K bytes per chunk: kc * 128^2 < 2^31 */
+constexpr int OZ2_MAX_BITS = 77;  /* cap on P: scaled integers below 2^78 */
+constexpr int OZ2_MODULI[OZ2_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                            229, 227, 223, 217, 211, 199, 197, 193,
+                                            191, 181, 179, 173, 167, 163};
+
+/* Exponents (as ozaki_prepare) and nmod residue planes of an FP64 operand
+ * for P-bit scaled integers. */
+hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec, int k,
+                          int64_t kpad, int nmod, int P, int *ex, unsigned long long *mbits,
+                          int8_t *res, hipStream_t st);
+
+/* Kinds of slice products. */
+enum SliceKind {
+  SP_I8_SUM = 0, /* INT8 slices, group g's INT32 sums to S + gout[g]*m*n */
+  SP_BF16 = 1,   /* BF16 slices, one group, C = fma(alpha, acc, beta*C) */
+  SP_I8_MOD = 2  /* INT8 residues, group g multiplies plane g by plane g and
+                    stores the sum modulo gout[g] as a byte to
+                    (uint8_t *)S + g*m*n */
+};
+
 /* Products of slice pairs over the K byte range [kb0, kb1) (multiples of
- * SL_BK).  INT8 (bf16 = false): group g's INT32 sums go to
- * S + gout[g]*m*n (column-major, ld m).  BF16 (bf16 = true): a single group;
- * C = fma(alpha, acc, beta*C) with leading dimension ldc. */
-hipError_t slice_products(bool bf16, const int8_t *as, const int8_t *bs, int m, int n,
+ * SL_BK); outputs column-major with leading dimension m (S) or ldc (C). */
+hipError_t slice_products(int kind, const int8_t *as, const int8_t *bs, int m, int n,
                           int64_t kpad_bytes, int kb0, int kb1, const SliceGroups &g,
-                          int *S, float *C, int64_t ldc, float alpha, float beta,
+                          void *S, float *C, int64_t ldc, float alpha, float beta,
                           hipStream_t st);
 
 } // namespace gemmx

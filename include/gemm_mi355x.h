@@ -63,11 +63,15 @@ void sgemm_(const char *transa, const char *transb, const int *m, const int *n,
 
 /* Arithmetic used by dgemm_ / sgemm_ (process-wide, default native).
  * Also set by the environment at first use:
- *   GEMMX_DGEMM_MODE = native | ozaki[:slices]     (slices 3..8, default 8)
- *   GEMMX_SGEMM_MODE = native | bf16x3                                     */
+ *   GEMMX_DGEMM_MODE = native | ozaki[:slices] (slices 3..8, default 8)
+ *                    | ozaki2[:moduli]         (moduli 8..22, default 20)
+ *   GEMMX_SGEMM_MODE = native | bf16x3
+ * For GEMMX_MODE_OZAKI2 the second argument of dgemm_x_set_mode is the
+ * modulus count. */
 #define GEMMX_MODE_NATIVE 0
 #define GEMMX_MODE_OZAKI 1  /* FP64 by INT8 slices on INT8 MFMA */
 #define GEMMX_MODE_BF16X3 2 /* FP32 by split-BF16x3 on BF16 MFMA */
+#define GEMMX_MODE_OZAKI2 3 /* FP64 by INT8 residues (CRT) on INT8 MFMA */
 int dgemm_x_set_mode(int mode, int slices);
 int sgemm_x_set_mode(int mode);
 
@@ -116,6 +120,23 @@ int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
                       double alpha, const double *a, int lda, const double *b,
                       int ldb, double beta, double *c, int ldc, int slices,
                       void *work, size_t work_bytes, void *stream);
+
+/* Emulated DGEMM on device pointers, Ozaki-II: every row of op(A) and
+ * column of op(B) is scaled by a power of two to P-bit integers
+ * rint(x 2^(P-e)); their exact integer product is formed from residues
+ * modulo `moduli` (8..22) pairwise-coprime integers <= 256 (one INT8 GEMM
+ * per modulus on the matrix cores, exact in INT32) by the Chinese remainder
+ * theorem, then rounded once to FP64 and scaled back.  P grows with the
+ * modulus count (dgemm_x_ozaki2_bits: 24 bits at 8 moduli, 70 at 20 and 77
+ * at 22 for k <= 8192, at most 77; -1 when unsupported); entries smaller than 2^(e-P)
+ * relative to their row's or column's largest are lost.  `work` as for
+ * dgemm_x_ozaki_dev, of at least dgemm_x_ozaki2_workspace() bytes. */
+size_t dgemm_x_ozaki2_workspace(int m, int n, int k, int moduli);
+int dgemm_x_ozaki2_bits(int moduli, int k);
+int dgemm_x_ozaki2_dev(char transa, char transb, int m, int n, int k,
+                       double alpha, const double *a, int lda, const double *b,
+                       int ldb, double beta, double *c, int ldc, int moduli,
+                       void *work, size_t work_bytes, void *stream);
 
 /* Emulated SGEMM on device pointers: split-BF16x3 (a = hi + lo in BF16,
  * C ~ hi*hi + hi*lo + lo*hi) on BF16 MFMA with FP32 accumulation.  `work` is

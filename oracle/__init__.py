@@ -8,7 +8,7 @@ Two things live here:
   * ``liboracle.so`` (gemm_oracle.c): a restatement of the reference's
     semantics (wjc404/GEMM_AVX2 src/gemm_driver.c:118-197) with Dot2
     accumulation, the counter-based input generator, and bit-exact
-    restatements of the emulated forms (Ozaki-I, split-BF16x3);
+    restatements of the emulated forms (Ozaki-I, Ozaki-II, split-BF16x3);
   * ``_ref/DGEMM.so`` / ``_ref/SGEMM.so``: the reference itself, compiled from
     its sources by oracle/Makefile when they are available.
 """
@@ -83,6 +83,21 @@ def lib():
     L.oracle_ozaki_kchunk.argtypes = [ctypes.c_int]
     L.oracle_ozaki_dgemm.restype = None
     L.oracle_ozaki_dgemm.argtypes = [
+        ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.oracle_ozaki2_modulus.restype = ctypes.c_int
+    L.oracle_ozaki2_modulus.argtypes = [ctypes.c_int]
+    L.oracle_ozaki2_bits.restype = ctypes.c_int
+    L.oracle_ozaki2_bits.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.oracle_ozaki2_scaled.restype = None
+    L.oracle_ozaki2_scaled.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_uint64),
+                                       ctypes.POINTER(ctypes.c_int64)]
+    L.oracle_ozaki2_round.restype = ctypes.c_double
+    L.oracle_ozaki2_round.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]
+    L.oracle_ozaki2_dgemm.restype = None
+    L.oracle_ozaki2_dgemm.argtypes = [
         ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
@@ -170,6 +185,33 @@ def sampled(dtype, transa, transb, m, n, k, alpha, beta, seed, is_, js, dist=0, 
 def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, slices):
     lib().oracle_ozaki_dgemm(transa.encode()[:1], transb.encode()[:1], m, n, k, alpha, _ptr(a),
                              lda, _ptr(b), ldb, beta, _ptr(c), ldc, slices)
+
+
+def ozaki2_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, moduli):
+    lib().oracle_ozaki2_dgemm(transa.encode()[:1], transb.encode()[:1], m, n, k, alpha,
+                              _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc, moduli)
+
+
+def ozaki2_modulus(j):
+    return int(lib().oracle_ozaki2_modulus(j))
+
+
+def ozaki2_bits(moduli, kc):
+    return int(lib().oracle_ozaki2_bits(moduli, kc))
+
+
+def ozaki2_scaled(x, e, P):
+    """rint(x * 2^(P-e)) as a Python int."""
+    lo, hi = ctypes.c_uint64(), ctypes.c_int64()
+    lib().oracle_ozaki2_scaled(float(x), e, P, ctypes.byref(lo), ctypes.byref(hi))
+    return (hi.value << 64) | lo.value
+
+
+def ozaki2_round(value, scale):
+    """A signed integer (|value| < 2^255) rounded to 53 bits and scaled."""
+    v = value & ((1 << 256) - 1)
+    limbs = (ctypes.c_uint64 * 4)(*[(v >> (64 * t)) & ((1 << 64) - 1) for t in range(4)])
+    return float(lib().oracle_ozaki2_round(limbs, scale))
 
 
 def bf16x3_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):

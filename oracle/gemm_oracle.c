@@ -481,3 +481,246 @@ void oracle_bf16x3_sgemm(char transa, char transb, int m, int n, int k,
     free(bl);
   }
 }
+
+/* ---------------------------------------------------------------------- */
+/* Ozaki-II (CRT).  No reference counterpart: restates the emulated DGEMM  */
+/* of the library's modular form.  Per row i of op(A) the exponent e_i      */
+/* (oracle_ozaki_exponent) and the integers a_ip = rint(a * 2^(P - e_i)),   */
+/* |a_ip| <= 2^P; likewise b_pj per column j.  The GPU forms the exact      */
+/* integer X_ij = sum_p a_ip b_pj from residues modulo pairwise-coprime     */
+/* moduli (INT8 residue products on the matrix cores, then the Chinese      */
+/* remainder theorem); P is chosen so that |X| < M/2.  This restatement     */
+/* computes X directly with 256-bit integer arithmetic instead (no CRT), so */
+/* it checks the GPU's reconstruction independently.  The result is         */
+/* RNE(X) to 53 bits, scaled by 2^(e_i + f_j - 2P) with one ldexp.          */
+/* ---------------------------------------------------------------------- */
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+int oracle_ozaki2_modulus(int j) {
+  /* the j-th of the pairwise-coprime integers taken greedily downward from
+   * 256 */
+  static int list[OZAKI2_MAX_MODULI];
+  static int count;
+  if (!count) {
+    for (int c = 256; c > 1 && count < OZAKI2_MAX_MODULI; c--) {
+      int ok = 1;
+      for (int t = 0; t < count && ok; t++) {
+        int x = c, y = list[t];
+        while (y) {
+          int r = x % y;
+          x = y;
+          y = r;
+        }
+        ok = (x == 1);
+      }
+      if (ok) list[count++] = c;
+    }
+  }
+  return (j >= 0 && j < count) ? list[j] : -1;
+}
+
+int oracle_ozaki2_bits(int nmod, int kc) {
+  if (nmod < OZAKI2_MIN_MODULI || nmod > OZAKI2_MAX_MODULI || kc < 1) return -1;
+  /* L = floor(log2(M)), M = product of the first nmod moduli, exactly */
+  uint32_t limb[8] = {0};
+  limb[0] = 1;
+  for (int j = 0; j < nmod; j++) {
+    uint64_t carry = 0;
+    for (int t = 0; t < 8; t++) {
+      uint64_t v = (uint64_t)limb[t] * (uint64_t)oracle_ozaki2_modulus(j) + carry;
+      limb[t] = (uint32_t)v;
+      carry = v >> 32;
+    }
+  }
+  int L = 0;
+  for (int t = 7; t >= 0; t--)
+    if (limb[t]) {
+      L = 32 * t + 31 - __builtin_clz(limb[t]);
+      break;
+    }
+  int kbits = 0; /* bit length of kc: kc < 2^kbits */
+  while ((1L << kbits) <= kc) kbits++;
+  /* |X| <= kc 2^(2P) < 2^(kbits + 2P) <= 2^(L-1) <= M/2; at most
+   * OZAKI2_MAX_BITS, so that |a| < 2^(53 + 25) */
+  const int P = (L - 1 - kbits) / 2;
+  return P < OZAKI2_MAX_BITS ? P : OZAKI2_MAX_BITS;
+}
+
+static i128 ozaki2_scaled(double x, int e, int P) {
+  if (x == 0.0) return 0;
+  uint64_t bits;
+  memcpy(&bits, &x, 8);
+  const int neg = (int)(bits >> 63);
+  const int be = (int)((bits >> 52) & 0x7ff);
+  uint64_t mant = bits & 0xfffffffffffffull;
+  int q;
+  if (be == 0) {
+    q = -1074; /* subnormal: x = mant * 2^-1074 */
+  } else {
+    mant |= 1ull << 52;
+    q = be - 1075;
+  }
+  const int sh = q + P - e; /* x * 2^(P-e) = mant * 2^sh */
+  u128 mag;
+  if (sh >= 0) {
+    mag = (u128)mant << sh;
+  } else {
+    const int s = -sh;
+    if (s > 63) {
+      mag = 0; /* mant < 2^53 <= 2^(s-1): rounds to 0 */
+    } else {
+      uint64_t qv = mant >> s, rem = mant & ((1ull << s) - 1), half = 1ull << (s - 1);
+      if (rem > half || (rem == half && (qv & 1))) qv++;
+      mag = qv;
+    }
+  }
+  return neg ? -(i128)mag : (i128)mag;
+}
+
+void oracle_ozaki2_scaled(double x, int e, int P, uint64_t *lo, int64_t *hi) {
+  const i128 v = ozaki2_scaled(x, e, P);
+  *lo = (uint64_t)(u128)v;
+  *hi = (int64_t)(v >> 64);
+}
+
+/* 256-bit two's complement accumulator, little-endian 64-bit limbs */
+static void acc_add_u128_shifted(uint64_t acc[4], u128 v, int word, int negate) {
+  uint64_t add[4] = {0, 0, 0, 0};
+  if (word < 4) add[word] = (uint64_t)v;
+  if (word + 1 < 4) add[word + 1] = (uint64_t)(v >> 64);
+  if (negate) {
+    /* acc -= add */
+    uint64_t borrow = 0;
+    for (int t = 0; t < 4; t++) {
+      const u128 d = (u128)acc[t] - add[t] - borrow;
+      acc[t] = (uint64_t)d;
+      borrow = (uint64_t)(d >> 64) ? 1 : 0;
+    }
+  } else {
+    uint64_t carry = 0;
+    for (int t = 0; t < 4; t++) {
+      const u128 s = (u128)acc[t] + add[t] + carry;
+      acc[t] = (uint64_t)s;
+      carry = (uint64_t)(s >> 64);
+    }
+  }
+}
+
+static void acc_add_product(uint64_t acc[4], i128 a, i128 b) {
+  if (a == 0 || b == 0) return;
+  const int neg = (a < 0) != (b < 0);
+  const u128 ua = a < 0 ? (u128)(-a) : (u128)a, ub = b < 0 ? (u128)(-b) : (u128)b;
+  const uint64_t a0 = (uint64_t)ua, a1 = (uint64_t)(ua >> 64);
+  const uint64_t b0 = (uint64_t)ub, b1 = (uint64_t)(ub >> 64);
+  acc_add_u128_shifted(acc, (u128)a0 * b0, 0, neg);
+  acc_add_u128_shifted(acc, (u128)a0 * b1, 1, neg);
+  acc_add_u128_shifted(acc, (u128)a1 * b0, 1, neg);
+  acc_add_u128_shifted(acc, (u128)a1 * b1, 2, neg);
+}
+
+double oracle_ozaki2_round(const uint64_t limbs[4], int scale) {
+  uint64_t v[4] = {limbs[0], limbs[1], limbs[2], limbs[3]};
+  const int neg = (int)(v[3] >> 63);
+  if (neg) { /* magnitude = -v */
+    uint64_t carry = 1;
+    for (int t = 0; t < 4; t++) {
+      const u128 s = (u128)(~v[t]) + carry;
+      v[t] = (uint64_t)s;
+      carry = (uint64_t)(s >> 64);
+    }
+  }
+  int top = -1;
+  for (int t = 3; t >= 0; t--)
+    if (v[t]) {
+      top = 64 * t + 63 - __builtin_clzll(v[t]);
+      break;
+    }
+  if (top < 0) return 0.0;
+  uint64_t mant;
+  int shift = top - 52 > 0 ? top - 52 : 0;
+  if (shift == 0) {
+    mant = v[0];
+  } else {
+    /* mant = v >> shift (53 bits), rem = low shift bits compared with half */
+    const int w = shift / 64, b = shift % 64;
+    mant = v[w] >> b;
+    if (b && w + 1 < 4) mant |= v[w + 1] << (64 - b);
+    mant &= (1ull << 53) - 1;
+    const int hw = (shift - 1) / 64, hb = (shift - 1) % 64; /* the half bit */
+    const int half_bit = (int)((v[hw] >> hb) & 1);
+    int sticky = 0;
+    for (int t = 0; t < hw; t++) sticky |= v[t] != 0;
+    if (hb) sticky |= (v[hw] & ((1ull << hb) - 1)) != 0;
+    if (half_bit && (sticky || (mant & 1))) {
+      mant++;
+      if (mant == (1ull << 53)) {
+        mant >>= 1;
+        shift++;
+      }
+    }
+  }
+  const double r = ldexp((double)mant, shift + scale);
+  return neg ? -r : r;
+}
+
+void oracle_ozaki2_dgemm(char transa, char transb, int m, int n, int k,
+                         double alpha, const double *a, int lda,
+                         const double *b, int ldb, double beta, double *c,
+                         int ldc, int nmod) {
+  const int na = is_notrans(transa), nb = is_notrans(transb);
+  if (m <= 0 || n <= 0) return;
+  if (alpha == 0.0 || k == 0) {
+    if (beta != 1.0)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) c[i + (int64_t)j * ldc] *= beta;
+    return;
+  }
+  const int kc = k < OZAKI2_KCHUNK ? k : OZAKI2_KCHUNK;
+  const int P = oracle_ozaki2_bits(nmod, kc);
+  int *ea = (int *)malloc(sizeof(int) * m);
+  int *fb = (int *)malloc(sizeof(int) * n);
+  i128 *ia = (i128 *)malloc(sizeof(i128) * (size_t)m * k); /* [i][p] */
+  i128 *ib = (i128 *)malloc(sizeof(i128) * (size_t)n * k); /* [j][p] */
+  for (int i = 0; i < m; i++)
+    ea[i] = na ? oracle_ozaki_exponent(a + i, k, lda)
+               : oracle_ozaki_exponent(a + (int64_t)i * lda, k, 1);
+  for (int j = 0; j < n; j++)
+    fb[j] = nb ? oracle_ozaki_exponent(b + (int64_t)j * ldb, k, 1)
+               : oracle_ozaki_exponent(b + j, k, ldb);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < m; i++)
+    for (int p = 0; p < k; p++) {
+      const double v = na ? a[i + (int64_t)p * lda] : a[p + (int64_t)i * lda];
+      ia[(size_t)i * k + p] = ea[i] == OZAKI_NONFINITE ? 0 : ozaki2_scaled(v, ea[i], P);
+    }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int p = 0; p < k; p++) {
+      const double v = nb ? b[p + (int64_t)j * ldb] : b[j + (int64_t)p * ldb];
+      ib[(size_t)j * k + p] = fb[j] == OZAKI_NONFINITE ? 0 : ozaki2_scaled(v, fb[j], P);
+    }
+  for (int k0 = 0; k0 < k; k0 += kc) {
+    const int k1 = (k0 + kc < k) ? k0 + kc : k;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < n; j++) {
+      for (int i = 0; i < m; i++) {
+        uint64_t acc[4] = {0, 0, 0, 0};
+        for (int p = k0; p < k1; p++)
+          acc_add_product(acc, ia[(size_t)i * k + p], ib[(size_t)j * k + p]);
+        double *cij = c + i + (int64_t)j * ldc;
+        const double c1 = (k0 == 0 && beta != 1.0) ? beta * *cij : *cij;
+        double scaled;
+        if (ea[i] == OZAKI_NONFINITE || fb[j] == OZAKI_NONFINITE)
+          scaled = NAN;
+        else
+          scaled = oracle_ozaki2_round(acc, ea[i] + fb[j] - 2 * P);
+        *cij = fma(alpha, scaled, c1);
+      }
+    }
+  }
+  free(ea);
+  free(fb);
+  free(ia);
+  free(ib);
+}

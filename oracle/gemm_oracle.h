@@ -22,9 +22,10 @@
  *     src/kernel/gemm_kernel_unroll6.S:70-125).
  *   - the counter-based input generator (SplitMix64 on (seed, id, index)).
  *   - the emulated forms this build adds (no reference counterpart):
- *     Ozaki-I INT8 slicing of FP64 and the split-BF16x3 form of FP32.  These
- *     restate the algorithm the GPU kernels implement, bit for bit where the
- *     GPU arithmetic is exact (integer slice products, FP64 recombination).
+ *     Ozaki-I INT8 slicing of FP64, the Ozaki-II CRT form of FP64 and the
+ *     split-BF16x3 form of FP32.  These restate the algorithm the GPU kernels
+ *     implement, bit for bit where the GPU arithmetic is exact (integer slice
+ *     and residue products, FP64 recombination, exact CRT reconstruction).
  *
  * Pinning (see tests/test_oracle.py): golden outputs of the reference built
  * from source (tests/golden/), the closed-form 12x12x12 known-answer test of
@@ -104,6 +105,31 @@ void oracle_ozaki_dgemm(char transa, char transb, int m, int n, int k,
                         double alpha, const double *a, int lda,
                         const double *b, int ldb, double beta, double *c,
                         int ldc, int slices);
+
+/* ---- Ozaki-II, CRT over INT8 residues (emulated DGEMM restatement) ---- */
+#define OZAKI2_MIN_MODULI 8
+#define OZAKI2_MAX_MODULI 22
+#define OZAKI2_KCHUNK 65536 /* K chunk: kc * 128^2 < 2^31 */
+#define OZAKI2_MAX_BITS 77   /* cap on P */
+/* The j-th modulus: pairwise-coprime integers taken greedily downward from
+ * 256 (256, 255, 253, 251, 247, ...); -1 past the end. */
+int oracle_ozaki2_modulus(int j);
+/* Bits P of the scaled integers for nmod moduli and K chunk kc: the largest
+ * P with bitlen(kc) + 2P + 1 <= floor(log2(M)), M the product of the moduli,
+ * capped at OZAKI2_MAX_BITS; -1 for nmod outside
+ * [OZAKI2_MIN_MODULI, OZAKI2_MAX_MODULI]. */
+int oracle_ozaki2_bits(int nmod, int kc);
+/* rint(x * 2^(P-e)) (ties to even) as a signed 128-bit integer (lo, hi). */
+void oracle_ozaki2_scaled(double x, int e, int P, uint64_t *lo, int64_t *hi);
+/* A signed 256-bit integer (little-endian two's complement 64-bit limbs)
+ * rounded to 53 bits (ties to even) and scaled by 2^scale with one ldexp. */
+double oracle_ozaki2_round(const uint64_t limbs[4], int scale);
+/* The emulated product, same semantics as oracle_ozaki_dgemm, computed with
+ * exact 256-bit integer dot products (no residues). */
+void oracle_ozaki2_dgemm(char transa, char transb, int m, int n, int k,
+                         double alpha, const double *a, int lda,
+                         const double *b, int ldb, double beta, double *c,
+                         int ldc, int nmod);
 
 /* ---- split-BF16x3 (emulated SGEMM restatement) ------------------------ */
 uint16_t oracle_bf16_rne(float x);

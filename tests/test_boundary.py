@@ -123,3 +123,15 @@ def test_batch_checks_arguments_before_the_device(dtype):
         with pytest.raises(gx.GemmError) as ei:
             gx.BatchPlan(dtype, bad[:1])
         assert ei.value.code == gx.ERR_NODEVICE
+
+
+def test_ozaki2_bits_match_the_oracle():
+    """The library's integer width P for Ozaki-II (host arithmetic, no
+    device) equals the oracle's for every modulus count and chunk length."""
+    import oracle
+    for nmod in range(6, 25):
+        for k in (1, 100, 8191, 8192, 65535, 65536, 65537, 10 ** 6):
+            want = oracle.ozaki2_bits(nmod, min(k, 65536))
+            assert gx.dgemm_ozaki2_bits(nmod, k) == want, (nmod, k)
+    assert gx.dgemm_ozaki2_workspace(64, 64, 64, 7) == 0
+    assert gx.dgemm_ozaki2_workspace(64, 64, 64, 20) > 20 * 64 * 128 * 2

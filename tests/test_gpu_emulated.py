@@ -104,3 +104,89 @@ def test_bf16x3(cuda, ta, tb):
         assert max_comp(G[:m], R3[:m], bound) <= 64 * k * U32, (m, n, k)
         assert max_comp(G[:m], R64[:m], bound) <= 3 * 2.0 ** -16 + 64 * k * U32, (m, n, k)
         assert np.all(G[m:] == 555.0)
+
+
+def _ozaki2_gpu(torch, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nmod):
+    dA, dB, dC = to_device(A, torch), to_device(B, torch), to_device(C, torch)
+    gx.dgemm_ozaki2_dev(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc, nmod)
+    torch.cuda.synchronize()
+    return from_device(dC, C.shape)
+
+
+@pytest.mark.parametrize("nmod", [8, 12, 16, 20, 22])
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_ozaki2_bit_exact(cuda, ta, tb, nmod):
+    """Residue split, INT8 residue products and CRT reconstruction against the
+    oracle's exact 256-bit integer restatement: identical bits."""
+    import torch
+    rng = np.random.default_rng(200 + nmod)
+    for (m, n, k, pad) in [(1, 1, 1, 0), (17, 9, 33, 1), (130, 129, 200, 2), (64, 256, 513, 0),
+                           (257, 31, 129, 3)]:
+        ra, ca = stored_shape(ta, m, k)
+        rb, cb = stored_shape(tb, k, n)
+        A = np.asfortranarray(rng.uniform(-1, 1, (ra + pad, ca)) * np.exp(4 * rng.standard_normal((ra + pad, ca))))
+        B = np.asfortranarray(rng.uniform(-1, 1, (rb + pad, cb)) * np.exp(2 * rng.standard_normal((rb + pad, cb))))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + pad, n)))
+        C[m:] = 555.0
+        R = C.copy(order="F")
+        oracle.ozaki2_gemm(ta, tb, m, n, k, 1.5, A, ra + pad, B, rb + pad, -0.5, R, m + pad, nmod)
+        G = _ozaki2_gpu(torch, ta, tb, m, n, k, 1.5, A, ra + pad, B, rb + pad, -0.5, C, m + pad, nmod)
+        np.testing.assert_array_equal(G, R)
+
+
+def test_ozaki2_chunks_subnormals_and_semantics(cuda):
+    import torch
+    rng = np.random.default_rng(9)
+    # K beyond one 65536 chunk
+    m, n, k = 3, 5, 65536 + 300
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    R = C.copy(order="F")
+    oracle.ozaki2_gemm("N", "N", m, n, k, 0.75, A, m, B, k, 2.0, R, m, 20)
+    G = _ozaki2_gpu(torch, "N", "N", m, n, k, 0.75, A, m, B, k, 2.0, C, m, 20)
+    np.testing.assert_array_equal(G, R)
+    # tiny and subnormal entries, results near the bottom of the range
+    A = np.asfortranarray(rng.uniform(-1, 1, (40, 50)) * 2.0 ** -1000)
+    A[3, :5] = 5e-324
+    B = np.asfortranarray(rng.uniform(-1, 1, (50, 30)) * 2.0 ** -60)
+    C = np.zeros((40, 30), order="F")
+    R = C.copy(order="F")
+    oracle.ozaki2_gemm("N", "N", 40, 30, 50, 1.0, A, 40, B, 50, 0.0, R, 40, 16)
+    G = _ozaki2_gpu(torch, "N", "N", 40, 30, 50, 1.0, A, 40, B, 50, 0.0, C, 40, 16)
+    np.testing.assert_array_equal(G, R)
+    # Inf in a row gives NaN in that row; alpha = 0 reads neither operand
+    A = np.ones((5, 6), order="F")
+    A[2, 3] = np.inf
+    B = np.ones((6, 4), order="F")
+    G = _ozaki2_gpu(torch, "N", "N", 5, 4, 6, 1.0, A, 5, B, 6, 0.0, np.zeros((5, 4), order="F"), 5, 12)
+    assert np.all(np.isnan(G[2])) and np.all(G[[0, 1, 3, 4]] == 6.0)
+    G = _ozaki2_gpu(torch, "N", "N", 5, 4, 6, 0.0, A, 5, B, 6, 2.0, np.full((5, 4), 3.0, order="F"), 5, 12)
+    assert np.all(G == 6.0)
+    with pytest.raises(gx.GemmError):
+        gx.dgemm_ozaki2_dev("N", "N", 5, 4, 6, 1.0, 0, 5, 0, 6, 0.0, 0, 5, 7)
+
+
+def test_ozaki2_accuracy_matches_native_at_20_moduli(cuda):
+    """On entries with a wide exponent range ((u-0.5) exp(4g)) the error
+    against Dot2 falls with the modulus count and at 20 moduli is no larger
+    than native FP64's."""
+    import torch
+    rng = np.random.default_rng(6)
+    m, n, k = 96, 80, 1000
+    A = np.asfortranarray((rng.uniform(0, 1, (m, k)) - 0.5) * np.exp(4 * rng.standard_normal((m, k))))
+    B = np.asfortranarray((rng.uniform(0, 1, (k, n)) - 0.5) * np.exp(4 * rng.standard_normal((k, n))))
+    R = np.zeros((m, n), order="F")
+    oracle.gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m)
+    dA, dB = to_device(A, torch), to_device(B, torch)
+    dC = to_device(np.zeros((m, n), order="F"), torch)
+    gx.dgemm_dev("N", "N", m, n, k, 1.0, dA, m, dB, k, 0.0, dC, m)
+    torch.cuda.synchronize()
+    native = rel_fro(from_device(dC, (m, n)), R)
+    errs = []
+    for nmod in (10, 14, 18, 20, 22):
+        G = _ozaki2_gpu(torch, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0,
+                        np.zeros((m, n), order="F"), m, nmod)
+        errs.append(rel_fro(G, R))
+    assert all(b <= a for a, b in zip(errs, errs[1:])), errs
+    assert errs[3] <= native, (errs, native)

@@ -10,9 +10,11 @@ accumulation.  It is checked here against
   * invariants that a dropped term, a wrong sign, a transposed operand or a
     wrong beta rule would break,
 and, when oracle/_ref/ is built, against the reference on extra shapes.
-The emulated restatements (Ozaki-I, split-BF16x3) are pinned by exact
-integer cases, by a library BF16 rounding routine and by their error against
-Dot2 shrinking as the slice count grows.
+The emulated restatements (Ozaki-I, Ozaki-II, split-BF16x3) are pinned by
+exact integer cases, by exact rational arithmetic (Python's Fraction: the
+scaled integers, the rounding, the correctly rounded product when the inputs
+fit), by a library BF16 rounding routine and by their error against Dot2
+shrinking as the slice or modulus count grows.
 """
 import ctypes
 import math
@@ -318,6 +320,130 @@ def test_ozaki_nonfinite_rows_and_kchunk():
     C = np.zeros((2, 3), order="F")
     oracle.ozaki_gemm("N", "N", 2, 3, k, 1.0, A, 2, B, k, 0.0, C, 2, 8)
     assert rel_fro(C, R) < 1e-14
+
+
+# --------------------------------------------------------------- Ozaki-II
+def test_ozaki2_moduli_are_the_greedy_coprime_list():
+    mods = [oracle.ozaki2_modulus(j) for j in range(22)]
+    assert oracle.ozaki2_modulus(22) == -1 and mods[0] == 256
+    assert all(a > b for a, b in zip(mods, mods[1:]))
+    for i, a in enumerate(mods):
+        assert all(math.gcd(a, b) == 1 for b in mods[:i])
+    # greedy: every skipped integer in (mods[-1], 256] shares a factor with a
+    # larger modulus
+    for c in range(mods[-1] + 1, 257):
+        if c not in mods:
+            assert any(math.gcd(c, b) > 1 for b in mods if b > c), c
+
+
+@pytest.mark.parametrize("kc", [1, 37, 8192, 8193, 65536])
+def test_ozaki2_bits_keep_the_integer_product_inside_the_crt_range(kc):
+    for nmod in range(8, 23):
+        P = oracle.ozaki2_bits(nmod, kc)
+        M = math.prod(oracle.ozaki2_modulus(j) for j in range(nmod))
+        # safety: k terms of magnitude <= 2^(2P) stay below M/2
+        assert kc * 2 ** (2 * P) * 2 < M, (nmod, kc, P)
+        # maximality of the bit condition bitlen(kc) + 2P + 1 <= floor(log2 M),
+        # up to the cap of 77 bits
+        assert P == 77 or kc.bit_length() + 2 * (P + 1) + 1 > M.bit_length() - 1
+        assert P <= 77
+    assert oracle.ozaki2_bits(7, kc) == -1 and oracle.ozaki2_bits(23, kc) == -1
+
+
+def test_ozaki2_scaled_integers_are_rounded_to_nearest_even():
+    from fractions import Fraction
+    rng = np.random.default_rng(21)
+    xs = list(rng.standard_normal(200) * np.exp(4 * rng.standard_normal(200)))
+    xs += [0.0, -0.0, 5e-324, -2.5e-310, 1.5, -2.5, 0.5 + 2.0 ** -52, 2.0 ** 60]
+    for x in xs:
+        if x == 0.0:
+            e = 0
+        else:
+            e = math.frexp(abs(x))[1] + int(rng.integers(0, 30))  # row max >= |x|
+        for P in (24, 53, 60, 77):
+            want = round(Fraction(x) * Fraction(2) ** (P - e))  # ties to even
+            assert oracle.ozaki2_scaled(x, e, P) == want, (x, e, P)
+            assert abs(want) <= 2 ** P
+
+
+def test_ozaki2_round_is_rne_then_one_ldexp():
+    from fractions import Fraction
+    rng = np.random.default_rng(22)
+    vals = [0, 1, -1, 2 ** 53 + 1, 2 ** 53 + 3, -(2 ** 54 + 2), 2 ** 200 + 2 ** 147,
+            2 ** 200 + 2 ** 147 + 1, 3 * 2 ** 147, -(2 ** 254)]
+    vals += [int(rng.integers(-2 ** 62, 2 ** 62)) << int(rng.integers(0, 190)) for _ in range(300)]
+    vals += [int(rng.integers(-2 ** 62, 2 ** 62)) * (2 ** 64) + int(rng.integers(0, 2 ** 62))
+             for _ in range(100)]
+    for v in vals:
+        for scale in (0, -140, 37):
+            # normal range: one correctly rounded conversion
+            assert oracle.ozaki2_round(v, scale) == float(Fraction(v) * Fraction(2) ** scale), v
+    # results below the normal range: RNE to 53 bits first, then ldexp
+    v = (2 ** 53 - 1) * 2 ** 60 + 2 ** 59 + 1
+    first = float(Fraction(v))
+    assert oracle.ozaki2_round(v, -1130) == math.ldexp(first, -1130)
+
+
+def test_ozaki2_is_the_correctly_rounded_product_when_inputs_fit():
+    """Entries j / 2^20 with |j| < 2^20 become exact integers for every
+    modulus count (P - e >= 20), so the result is the exact dot product
+    rounded once: math.fsum of the (exact) products."""
+    rng = np.random.default_rng(23)
+    for nmod in (8, 12, 16, 22):
+        for ta in "NT":
+            for tb in "NT":
+                m, n, k = 6, 5, 301
+                ra, ca = stored_shape(ta, m, k)
+                rb, cb = stored_shape(tb, k, n)
+                A = np.asfortranarray(rng.integers(-2 ** 20 + 1, 2 ** 20, (ra, ca)) / 2.0 ** 20)
+                B = np.asfortranarray(rng.integers(-2 ** 20 + 1, 2 ** 20, (rb, cb)) / 2.0 ** 20)
+                C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+                oracle.ozaki2_gemm(ta, tb, m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, nmod)
+                oa, ob = op(A, ta, m, k), op(B, tb, k, n)
+                for i in range(m):
+                    for j in range(n):
+                        assert C[i, j] == math.fsum(oa[i] * ob[:, j]), (nmod, ta, tb, i, j)
+
+
+def test_ozaki2_semantics_error_and_chunks():
+    rng = np.random.default_rng(24)
+    # beta != 1 first (beta == 0 keeps NaN), alpha == 0 reads neither operand
+    A = np.ones((3, 4), order="F")
+    B = np.ones((4, 2), order="F")
+    C = np.full((3, 2), np.nan, order="F")
+    oracle.ozaki2_gemm("N", "N", 3, 2, 4, 1.0, A, 3, B, 4, 0.0, C, 3, 16)
+    assert np.all(np.isnan(C))
+    C = np.full((3, 2), 2.0, order="F")
+    oracle.ozaki2_gemm("N", "N", 3, 2, 4, 0.0, A, 3, B, 4, 3.0, C, 3, 16)
+    assert np.all(C == 6.0)
+    A[1, 2] = np.inf
+    C = np.zeros((3, 2), order="F")
+    oracle.ozaki2_gemm("N", "N", 3, 2, 4, 1.0, A, 3, B, 4, 0.0, C, 3, 16)
+    assert np.all(np.isnan(C[1])) and np.all(C[[0, 2]] == 4.0)
+    # error against Dot2 falls with the modulus count on wide exponent ranges
+    m, n, k = 20, 16, 400
+    A = np.asfortranarray(rng.uniform(-0.5, 0.5, (m, k)) * np.exp(4 * rng.standard_normal((m, k))))
+    B = np.asfortranarray(rng.uniform(-0.5, 0.5, (k, n)) * np.exp(4 * rng.standard_normal((k, n))))
+    R = np.zeros((m, n), order="F")
+    oracle.gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m)
+    errs = []
+    for nmod in (8, 10, 12, 14, 16, 18, 20, 22):
+        C = np.zeros((m, n), order="F")
+        oracle.ozaki2_gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m, nmod)
+        errs.append(rel_fro(C, R))
+    assert all(b <= a for a, b in zip(errs, errs[1:])), errs
+    assert errs[0] > 1e-6 and errs[-1] < 1e-15, errs
+    # K beyond one chunk: chunk sums rounded and added in order onto beta*C
+    from fractions import Fraction
+    k = 65536 + 100
+    a = rng.integers(-2 ** 10, 2 ** 10, (1, k)) / 2.0 ** 10
+    b = rng.integers(-2 ** 10, 2 ** 10, (k, 1)) / 2.0 ** 10
+    A, B = np.asfortranarray(a), np.asfortranarray(b)
+    C = np.array([[0.25]], order="F")
+    oracle.ozaki2_gemm("N", "N", 1, 1, k, 1.0, A, 1, B, k, -2.0, C, 1, 22)
+    s0 = float(sum(Fraction(x) * Fraction(y) for x, y in zip(a[0, :65536], b[:65536, 0])))
+    s1 = float(sum(Fraction(x) * Fraction(y) for x, y in zip(a[0, 65536:], b[65536:, 0])))
+    assert C[0, 0] == (s0 + (-0.5)) + s1
 
 
 # --------------------------------------------------------------- BF16x3
