@@ -25,7 +25,7 @@ import os
 __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
-    "dgemm_ozaki2_bits", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
+    "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "tune_lookup", "last_error", "fill",
     "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2",
 ]
@@ -264,6 +264,27 @@ def dgemm_ozaki2_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ld
                                 work_bytes, _stream(stream))
     if rc:
         _check("d", rc)
+
+
+def set_phase_timing(on: bool = True):
+    """Per-thread phase timing of the emulated DGEMMs (see phase_times)."""
+    lib = load("DGEMM.so")
+    lib.dgemm_x_set_phase_timing.restype = ctypes.c_int
+    lib.dgemm_x_set_phase_timing.argtypes = [ctypes.c_int]
+    lib.dgemm_x_set_phase_timing(1 if on else 0)
+
+
+def phase_times():
+    """(split, products, recombination) ms of this thread's last emulated
+    DGEMM issued with phase timing on; waits for it."""
+    lib = load("DGEMM.so")
+    lib.dgemm_x_phase_times.restype = ctypes.c_int
+    lib.dgemm_x_phase_times.argtypes = [ctypes.POINTER(ctypes.c_float)]
+    out = (ctypes.c_float * 3)()
+    rc = lib.dgemm_x_phase_times(out)
+    if rc:
+        _check("d", rc)
+    return tuple(float(v) for v in out)
 
 
 def sgemm_bf16x3_workspace(m, n, k) -> int:

@@ -391,7 +391,7 @@ extern "C" int gemmx_ozaki2_bits(int moduli, int k) {
 }
 
 extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *work,
-                                       hipStream_t st) {
+                                       hipStream_t st, hipEvent_t *ev) {
   Oz2Crt crt;
   const int kc = p->k < OZ2_KCHUNK ? p->k : OZ2_KCHUNK;
   if (!work || !oz2_constants(nmod, kc, &crt)) return hipErrorInvalidValue;
@@ -401,12 +401,14 @@ extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *w
   int8_t *as = (int8_t *)(w + L.as), *bs = (int8_t *)(w + L.bs);
   uint8_t *R = (uint8_t *)(w + L.r8);
   unsigned long long *mb = (unsigned long long *)(w + L.mb);
+  if (ev) (void)hipEventRecord(ev[0], st);
   hipError_t e = ozaki2_prepare((const double *)p->a, p->lda, p->ta != 0, p->m, p->k, L.kpad,
                                 nmod, crt.P, ea, mb, as, st);
   if (e != hipSuccess) return e;
   e = ozaki2_prepare((const double *)p->b, p->ldb, p->tb == 0, p->n, p->k, L.kpad, nmod,
                      crt.P, fb, mb, bs, st);
   if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[1], st);
   /* group g: plane g of A times plane g of B, stored modulo m_g */
   SliceGroups G;
   G.ngroups = 0;
@@ -423,6 +425,7 @@ extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *w
     e = slice_products(SP_I8_MOD, as, bs, p->m, p->n, L.kpad, kb0, kb1, G, R, NULL, 0, 0.f,
                        0.f, st);
     if (e != hipSuccess) return e;
+    if (ev && kb0 + OZ2_KCHUNK >= kp) (void)hipEventRecord(ev[2], st);
     /* chunks after the first add onto C: beta = 1 leaves C as it is */
     hipLaunchKernelGGL(ozaki2_recon_kernel, dim3((unsigned)rblocks), dim3(256), 0, st, R, crt,
                        ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha,
@@ -430,6 +433,7 @@ extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *w
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  if (ev) (void)hipEventRecord(ev[3], st);
   return hipSuccess;
 }
 
@@ -443,7 +447,7 @@ extern "C" size_t gemmx_ozaki_workspace(int m, int n, int k, int slices) {
 }
 
 extern "C" hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int s, void *work,
-                                      hipStream_t st) {
+                                      hipStream_t st, hipEvent_t *ev) {
   if (s < 3 || s > 8 || !work) return hipErrorInvalidValue;
   const OzLayout L = oz_layout(p->m, p->n, p->k, s);
   char *w = (char *)work;
@@ -453,12 +457,14 @@ extern "C" hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int s, void *work,
   /* op(A) rows: transa N -> row i at a + i, stride lda; T -> contiguous.
    * op(B) columns: transb N -> contiguous at b + j*ldb; T -> stride ldb. */
   unsigned long long *mb = (unsigned long long *)(w + L.mb);
+  if (ev) (void)hipEventRecord(ev[0], st);
   hipError_t e = ozaki_prepare((const double *)p->a, p->lda, p->ta != 0, p->m, p->k,
                                L.kpad, s, ea, mb, as, st);
   if (e != hipSuccess) return e;
   e = ozaki_prepare((const double *)p->b, p->ldb, p->tb == 0, p->n, p->k, L.kpad, s,
                     fb, mb, bs, st);
   if (e != hipSuccess) return e;
+  if (ev) (void)hipEventRecord(ev[1], st);
   const SliceGroups groups = ozaki_groups(s);
   const int kc = gemmx_ozaki_kchunk(s); /* bytes, a multiple of SL_BK */
   const int kp = (int)L.kpad;
@@ -470,6 +476,7 @@ extern "C" hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int s, void *work,
     e = slice_products(SP_I8_SUM, as, bs, p->m, p->n, L.kpad, kb0, kb1, groups, S, NULL, 0,
                        0.f, 0.f, st);
     if (e != hipSuccess) return e;
+    if (ev && kb0 + kc >= kp) (void)hipEventRecord(ev[2], st);
     /* chunks after the first add onto C: beta = 1 leaves C as it is */
     hipLaunchKernelGGL(ozaki_recombine_kernel, dim3((unsigned)rblocks), dim3(256), 0, st,
                        S, s, ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha,
@@ -477,6 +484,7 @@ extern "C" hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int s, void *work,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  if (ev) (void)hipEventRecord(ev[3], st);
   return hipSuccess;
 }
 

@@ -180,6 +180,35 @@ static int is_device_ptr(const void *p) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Phase timing of the emulated DGEMMs (per thread, off by default).        */
+/* ------------------------------------------------------------------------ */
+static __thread int t_timing;
+static __thread int t_ev_dev = -1;
+static __thread int t_ev_used;
+static __thread hipEvent_t t_ev[4];
+
+/* The calling thread's four events on the current device when timing is on,
+ * else NULL. */
+static hipEvent_t *phase_events(void) {
+  if (!t_timing) return NULL;
+  int dev;
+  if (hipGetDevice(&dev) != hipSuccess) return NULL;
+  if (t_ev_dev != dev) {
+    if (t_ev_dev >= 0)
+      for (int i = 0; i < 4; i++) (void)hipEventDestroy(t_ev[i]);
+    t_ev_dev = -1;
+    for (int i = 0; i < 4; i++)
+      if (hipEventCreate(&t_ev[i]) != hipSuccess) {
+        for (int j = 0; j < i; j++) (void)hipEventDestroy(t_ev[j]);
+        return NULL;
+      }
+    t_ev_dev = dev;
+  }
+  t_ev_used = 1;
+  return t_ev;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Arguments.                                                               */
 /* ------------------------------------------------------------------------ */
 static int notrans(char c) { return c == 'N' || c == 'n'; }
@@ -225,11 +254,11 @@ static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "scale kernel launch");
   }
   if (es == 8 && mode == GEMMX_MODE_OZAKI) {
-    e = gemmx_ozaki_f64(p, slices, work, s);
+    e = gemmx_ozaki_f64(p, slices, work, s, phase_events());
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "ozaki kernel launch");
   }
   if (es == 8 && mode == GEMMX_MODE_OZAKI2) {
-    e = gemmx_ozaki2_f64(p, slices, work, s);
+    e = gemmx_ozaki2_f64(p, slices, work, s, phase_events());
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "ozaki2 kernel launch");
   }
   if (es == 4 && mode == GEMMX_MODE_BF16X3) {
@@ -493,6 +522,27 @@ EXPORT int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
   }
   call_t x = {8, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
   return dev_call(&x, GEMMX_MODE_OZAKI, slices, work, work_bytes, stream);
+}
+
+EXPORT int dgemm_x_set_phase_timing(int on) {
+  pthread_once(&g_once, init_once);
+  t_timing = on != 0;
+  t_ev_used = 0;
+  return GEMMX_OK;
+}
+
+EXPORT int dgemm_x_phase_times(float *ms) {
+  clear_error();
+  if (!ms) return set_error(GEMMX_ERR_ARG, "phase_times: NULL output");
+  if (!t_ev_used || t_ev_dev < 0)
+    return set_error(GEMMX_ERR_ARG, "phase_times: no timed emulated call on this thread");
+  hipError_t e = hipEventSynchronize(t_ev[3]);
+  if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+  for (int i = 0; i < 3; i++) {
+    e = hipEventElapsedTime(&ms[i], t_ev[i], t_ev[i + 1]);
+    if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+  }
+  return GEMMX_OK;
 }
 
 EXPORT size_t dgemm_x_ozaki2_workspace(int m, int n, int k, int moduli) {

@@ -100,7 +100,7 @@ hipError_t gemmx_batch_launch(int es, int cfg, const gemmx_batch_desc *d_desc,
 size_t gemmx_ozaki_workspace(int m, int n, int k, int slices);
 int gemmx_ozaki_kchunk(int slices);
 hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int slices, void *work,
-                           hipStream_t s);
+                           hipStream_t s, hipEvent_t *ev);
 
 /* Emulated DGEMM, Ozaki-II: residues modulo `moduli` (8..22) pairwise-coprime
  * integers <= 256 (gemm_emulated.hip).  `work` is device scratch of
@@ -109,7 +109,11 @@ hipError_t gemmx_ozaki_f64(const gemmx_problem *p, int slices, void *work,
 size_t gemmx_ozaki2_workspace(int m, int n, int k, int moduli);
 int gemmx_ozaki2_bits(int moduli, int k);
 hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int moduli, void *work,
-                            hipStream_t s);
+                            hipStream_t s, hipEvent_t *ev);
+/* ev (NULL, or four events of the current device): the emulated DGEMMs
+ * record ev[0] before the split, ev[1] after it, ev[2] after the products
+ * (with more than one K chunk, after the last chunk's recombination) and
+ * ev[3] at the end, on stream s. */
 
 /* Emulated SGEMM, split-BF16x3 (gemm_slices.hip).  `work` is device scratch
  * of gemmx_bf16x3_workspace() bytes. */

@@ -138,6 +138,15 @@ int dgemm_x_ozaki2_dev(char transa, char transb, int m, int n, int k,
                        int ldb, double beta, double *c, int ldc, int moduli,
                        void *work, size_t work_bytes, void *stream);
 
+/* Phase timing of the emulated DGEMMs (Ozaki-I and Ozaki-II), for the
+ * calling thread: after dgemm_x_set_phase_timing(1), each emulated call
+ * records HIP events on its stream; dgemm_x_phase_times waits for the last
+ * call's events and gives, in ms, the split, the slice/residue products
+ * (with more than one K chunk: products and recombination together) and the
+ * recombination.  Off by default; costs four event records per call. */
+int dgemm_x_set_phase_timing(int on);
+int dgemm_x_phase_times(float *ms);
+
 /* Emulated SGEMM on device pointers: split-BF16x3 (a = hi + lo in BF16,
  * C ~ hi*hi + hi*lo + lo*hi) on BF16 MFMA with FP32 accumulation.  `work` is
  * device scratch of at least sgemm_x_bf16x3_workspace() bytes, or NULL to
