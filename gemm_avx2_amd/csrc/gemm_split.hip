@@ -251,12 +251,15 @@ __device__ __forceinline__ unsigned oz2_residue_byte(const Oz2Limbs &a) {
   return (unsigned)r & 0xffu;
 }
 
-/* All nmod residue bytes of one element into byte q of each plane word. */
-template <int J>
-__device__ __forceinline__ void oz2_encode(const Oz2Limbs &a, int nmod, i4_t (&res)[OZ2_MAX_MODULI],
-                                           int q) {
-  if (J < nmod) res[J][q >> 2] |= (int)(oz2_residue_byte<J>(a) << (8 * (q & 3)));
-  if constexpr (J + 1 < OZ2_MAX_MODULI) oz2_encode<J + 1>(a, nmod, res, q);
+/* The residue bytes of four consecutive entries, packed into word W of each
+ * of the nmod plane vectors (all indices compile-time constants). */
+template <int J, int W>
+__device__ __forceinline__ void oz2_encode4(const Oz2Limbs (&a)[4], int nmod,
+                                            i4_t (&res)[OZ2_MAX_MODULI]) {
+  if (J < nmod)
+    res[J][W] = (int)(oz2_residue_byte<J>(a[0]) | (oz2_residue_byte<J>(a[1]) << 8) |
+                      (oz2_residue_byte<J>(a[2]) << 16) | (oz2_residue_byte<J>(a[3]) << 24));
+  if constexpr (J + 1 < OZ2_MAX_MODULI) oz2_encode4<J + 1, W>(a, nmod, res);
 }
 
 /* nmod residue planes of 16 consecutive K entries of one vector per thread
@@ -275,12 +278,19 @@ ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int 
   i4_t res[OZ2_MAX_MODULI];
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
-#pragma unroll 2
-  for (int q = 0; q < 16; ++q) {
-    const int64_t p = blk * 16 + q;
-    const double val = (p < len && e != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0;
-    oz2_encode<0>(oz2_limbs(val, e, P), nmod, res, q);
-  }
+  Oz2Limbs a[4];
+#define GEMMX_OZ2_WORD(W)                                                                \
+  for (int q = 0; q < 4; ++q) {                                                          \
+    const int64_t p = blk * 16 + 4 * (W) + q;                                            \
+    a[q] = oz2_limbs((p < len && e != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0, \
+                     e, P);                                                              \
+  }                                                                                      \
+  oz2_encode4<0, W>(a, nmod, res);
+  GEMMX_OZ2_WORD(0)
+  GEMMX_OZ2_WORD(1)
+  GEMMX_OZ2_WORD(2)
+  GEMMX_OZ2_WORD(3)
+#undef GEMMX_OZ2_WORD
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j)
     if (j < nmod)
@@ -315,11 +325,16 @@ ozaki2_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
   i4_t res[OZ2_MAX_MODULI];
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
-#pragma unroll 2
-  for (int i = 0; i < 16; ++i) {
-    const double val = e != OZ_NONFINITE ? tile[vl * ROW + i * 8 + q] : 0.0;
-    oz2_encode<0>(oz2_limbs(val, e, P), nmod, res, i);
-  }
+  Oz2Limbs a[4];
+#define GEMMX_OZ2_WORD(W)                                                                \
+  for (int i = 0; i < 4; ++i)                                                            \
+    a[i] = oz2_limbs(e != OZ_NONFINITE ? tile[vl * ROW + (4 * (W) + i) * 8 + q] : 0.0, e, P); \
+  oz2_encode4<0, W>(a, nmod, res);
+  GEMMX_OZ2_WORD(0)
+  GEMMX_OZ2_WORD(1)
+  GEMMX_OZ2_WORD(2)
+  GEMMX_OZ2_WORD(3)
+#undef GEMMX_OZ2_WORD
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j)
     if (j < nmod)
