@@ -88,24 +88,22 @@ SliceGroups ozaki_groups(int s) {
 
 /* ---- Ozaki-II reconstruction ------------------------------------------
  * The residues c_j = X mod m_j of the exact integer X = sum_p a_ip b_pj give
- * X by the Chinese remainder theorem: with M = prod m_j, M_j = M / m_j and
- * w_j = M_j^-1 mod m_j, X = (sum_j ((c_j w_j) mod m_j) M_j) mod M, taken in
- * (-M/2, M/2) (|X| < M/2 by the choice of P).  The sum is formed exactly
- * with 24-bit multiply-adds of the byte y_j into 16-bit limbs of M_j (limb
- * sums stay below 2^29), carried into 192-bit integers (32-bit limbs),
- * reduced modulo M, rounded to 53 bits (ties to even) and scaled by
- * 2^(e_i + f_j - 2P) with one ldexp. */
+ * X by the Chinese remainder theorem: with M = prod m_j, M_j = M / m_j,
+ * w_j = M_j^-1 mod m_j and W_j = (w_j M_j) mod M, X = (sum_j c_j W_j) mod M,
+ * taken in (-M/2, M/2) (|X| < M/2 by the choice of P).  The sum
+ * (< nmod 2^8 M < 2^185) is formed exactly with 24-bit multiply-adds of the
+ * byte c_j into 16-bit limbs of W_j (limb sums stay below 2^29), carried
+ * into 192-bit integers (32-bit limbs), reduced modulo M with an FP64
+ * quotient and one correction, rounded to 53 bits (ties to even) and scaled
+ * by 2^(e_i + f_j - 2P) with one ldexp. */
 constexpr int OZ2_LIMBS = 6;    /* 32-bit limbs: M < 2^171 */
 constexpr int OZ2_LIMBS16 = 11; /* 16-bit limbs of M_j */
 
 struct Oz2Crt {
   int nmod, P;
-  unsigned Mj16[OZ2_MAX_MODULI][OZ2_LIMBS16]; /* M / m_j, 16-bit limbs */
-  int w[OZ2_MAX_MODULI];                       /* (M / m_j)^-1 mod m_j */
-  int mod[OZ2_MAX_MODULI];                     /* m_j */
-  float rcp[OZ2_MAX_MODULI];                   /* 1 / m_j, rounded */
-  unsigned M[OZ2_LIMBS], H[OZ2_LIMBS];         /* M and floor(M / 2) */
-  double rMd;                                  /* 1 / M, rounded */
+  unsigned W16[OZ2_MAX_MODULI][OZ2_LIMBS16]; /* W_j = (w_j M_j) mod M, 16-bit limbs */
+  unsigned M[OZ2_LIMBS], H[OZ2_LIMBS];        /* M and floor(M / 2) */
+  double rMd;                                 /* 1 / M, rounded */
 };
 
 /* Host: the constants of the first nmod moduli and P for K chunks of kc. */
@@ -143,9 +141,6 @@ bool oz2_constants(int nmod, int kc, Oz2Crt *c) {
       mj[l] = (unsigned)(cur / m);
       rem = cur % m;
     }
-    for (int l = 0; l < OZ2_LIMBS16; ++l)
-      c->Mj16[j][l] = (mj[l / 2] >> (16 * (l % 2))) & 0xffffu;
-    if (mj[OZ2_LIMBS - 1] >> 16) return false; /* M_j < 2^176 */
     unsigned long long r = 0; /* M_j mod m */
     for (int l = OZ2_LIMBS - 1; l >= 0; --l) r = ((r << 32) | mj[l]) % m;
     int inv = 0;
@@ -155,17 +150,23 @@ bool oz2_constants(int nmod, int kc, Oz2Crt *c) {
         break;
       }
     if (!inv) return false;
-    c->w[j] = inv;
+    /* W_j = inv * M_j (< m M_j = M, so no reduction is needed) */
+    unsigned wj[OZ2_LIMBS];
+    unsigned long long carry = 0;
+    for (int l = 0; l < OZ2_LIMBS; ++l) {
+      const unsigned long long v = (unsigned long long)mj[l] * inv + carry;
+      wj[l] = (unsigned)v;
+      carry = v >> 32;
+    }
+    if (carry || (wj[OZ2_LIMBS - 1] >> 16)) return false; /* W_j < 2^176 */
+    for (int l = 0; l < OZ2_LIMBS16; ++l)
+      c->W16[j][l] = (wj[l / 2] >> (16 * (l % 2))) & 0xffffu;
   }
   for (int l = 0; l < OZ2_LIMBS; ++l)
     c->H[l] = (c->M[l] >> 1) | (l + 1 < OZ2_LIMBS ? c->M[l + 1] << 31 : 0u);
   double md = 0.0;
   for (int l = OZ2_LIMBS - 1; l >= 0; --l) md = md * 4294967296.0 + (double)c->M[l];
   c->rMd = 1.0 / md;
-  for (int j = 0; j < nmod; ++j) {
-    c->mod[j] = OZ2_MODULI[j];
-    c->rcp[j] = 1.0f / (float)OZ2_MODULI[j];
-  }
   return true;
 }
 
@@ -200,10 +201,10 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
         x[l / 2] = half | (v << 16);
       }
     }
-    x[OZ2_LIMBS - 1] = half | (carry << 16); /* limb 10 and the final carry */
+    x[OZ2_LIMBS - 1] = half | (carry << 16); /* limb 10 and the final carry (< 2^14) */
   }
-  /* X = X' mod M: subtract q M with q from FP64 (off by at most one),
-   * then correct */
+  /* X = X' mod M: subtract q M with q < nmod 2^8 from FP64 (off by at most
+   * one), then correct */
   double xd = 0.0;
 #pragma unroll
   for (int l = OZ2_LIMBS - 1; l >= 0; --l) xd = xd * 4294967296.0 + (double)x[l];
@@ -319,7 +320,7 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
   for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < quads;
        u += (int64_t)gridDim.x * blockDim.x) {
     const int64_t t0 = u * E;
-    /* X'_e = sum_g y_g M_g with y_g = (c_g w_g) mod m_g, < nmod * M */
+    /* X'_e = sum_g c_g W_g < nmod 2^8 M */
     unsigned acc[E][OZ2_LIMBS16];
 #pragma unroll
     for (int q = 0; q < E; ++q)
@@ -327,9 +328,6 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
       for (int l = 0; l < OZ2_LIMBS16; ++l) acc[q][l] = 0u;
 #pragma unroll 1
     for (int g = 0; g < crt.nmod; ++g) {
-      const int m = crt.mod[g];
-      const float rcp = crt.rcp[g];
-      const unsigned w = (unsigned)crt.w[g];
       const uint8_t *rp = R + (int64_t)g * total + t0;
       unsigned c4 = 0;
       if (vec) {
@@ -341,14 +339,9 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
       }
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const unsigned cw = __umul24((c4 >> (8 * q)) & 0xffu, w);
-        const int qt = (int)((float)cw * rcp); /* cw < 2^16: off by <= 1 */
-        int yr = (int)cw - qt * m;
-        yr += yr < 0 ? m : 0;
-        yr -= yr >= m ? m : 0;
+        const unsigned cg = (c4 >> (8 * q)) & 0xffu;
 #pragma unroll
-        for (int l = 0; l < OZ2_LIMBS16; ++l)
-          acc[q][l] += __umul24((unsigned)yr, crt.Mj16[g][l]);
+        for (int l = 0; l < OZ2_LIMBS16; ++l) acc[q][l] += __umul24(cg, crt.W16[g][l]);
       }
     }
 #pragma unroll

@@ -72,7 +72,7 @@ struct SliceTileCopy {
   }
 };
 
-template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
+template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE, int PAD>
 __global__ void __launch_bounds__(64 * WM * WN)
 slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
                       int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
@@ -83,7 +83,7 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 16, FN = WTN / 16;
   constexpr int LEN = BKB / 4; /* ints per LDS line: one K stage of BKB bytes */
-  constexpr int LS = LEN + 4;  /* +16 bytes against bank conflicts */
+  constexpr int LS = LEN + PAD; /* PAD ints against bank conflicts */
   constexpr int KSTEPS = BKB / 64;
   typedef SliceMfma<BF16> MM;
   typedef typename MM::acc_t acc_t;
@@ -235,12 +235,12 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
     }
 }
 
-template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE>
+template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE, int PAD = 4>
 hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
                       int kb0, int kb1, const SliceGroups &G, void *S, float *C,
                       int64_t ldc, float alpha, float beta, hipStream_t st) {
-  auto kern = slice_products_kernel<KIND, BM, BN, WM, WN, BKB, PIPE>;
-  const size_t lds = (size_t)2 * (BM + BN) * (BKB / 4 + 4) * 4;
+  auto kern = slice_products_kernel<KIND, BM, BN, WM, WN, BKB, PIPE, PAD>;
+  const size_t lds = (size_t)2 * (BM + BN) * (BKB / 4 + PAD) * 4;
   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -254,13 +254,15 @@ hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
 /* Tile configuration: 0 = 256x256 with 8 waves of 128x64 and 128-byte K
  * stages, MFMAs interleaved with the fragment reads and LDS writes;
  * 1 = 256x256 with 4 waves of 128x128 and 64-byte K stages; 2 = as 0 with
- * the compiler's own order.  GEMMX_SLICE_CFG overrides the default. */
+ * the compiler's own order; 3 = as 0 with 32 bytes of line padding; 4 = as 0
+ * with 4 waves of 128x128; 5 = 256x128 with 4 waves of 128x64; 6 = as 0
+ * with 8 waves of 64x128.  Default 3; GEMMX_SLICE_CFG overrides it. */
 int slice_cfg() {
   static int cfg = -2;
   if (cfg == -2) {
     const char *e = getenv("GEMMX_SLICE_CFG");
-    cfg = (e && *e) ? atoi(e) : 0;
-    if (cfg < 0 || cfg > 2) cfg = 0;
+    cfg = (e && *e) ? atoi(e) : 3;
+    if (cfg < 0 || cfg > 6) cfg = 3;
   }
   return cfg;
 }
@@ -276,6 +278,18 @@ hipError_t products(const int8_t *as, const int8_t *bs, int m, int n, int64_t kp
   case 2:
     return launch_cfg<KIND, 256, 256, 2, 4, 128, false>(as, bs, m, n, kpad, kb0, kb1, G, S,
                                                         C, ldc, alpha, beta, st);
+  case 3:
+    return launch_cfg<KIND, 256, 256, 2, 4, 128, true, 8>(as, bs, m, n, kpad, kb0, kb1, G, S,
+                                                          C, ldc, alpha, beta, st);
+  case 4:
+    return launch_cfg<KIND, 256, 256, 2, 2, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+                                                       ldc, alpha, beta, st);
+  case 5:
+    return launch_cfg<KIND, 256, 128, 2, 2, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+                                                       ldc, alpha, beta, st);
+  case 6:
+    return launch_cfg<KIND, 256, 256, 4, 2, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
+                                                       ldc, alpha, beta, st);
   default:
     return launch_cfg<KIND, 256, 256, 2, 4, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
                                                        ldc, alpha, beta, st);

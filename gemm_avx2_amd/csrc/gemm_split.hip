@@ -180,19 +180,22 @@ ozaki_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
 /* ---- Ozaki-II residues ------------------------------------------------
  * x = (-1)^neg * mant * 2^q exactly (mant < 2^53); the scaled integer is
  * rint(x * 2^(P - e)) = rint(mant * 2^sh), sh = q + P - e <= P - 52 <= 25.
- * Its magnitude (mant << sh, or rint(mant / 2^-sh) ties to even) is cut into
- * five 16-bit limbs L_l, so that for each modulus m
- *   |a| mod m = (sum_l L_l * (2^16l mod m)) mod m,
- * a sum of 24-bit products below 2^27 reduced once with an FP32 quotient
- * (off by at most one, then corrected).  The residue of a is negated for a
- * negative a and stored as a signed byte in [-m/2, m/2). */
-struct Oz2Limbs {
-  unsigned l[5];
+ * Its magnitude (mant << sh, or rint(mant / 2^-sh) ties to even; < 2^78) is
+ * split as h * 2^39 + l with h, l < 2^39, both exact in FP64, so that for
+ * each modulus m
+ *   |a| mod m = (h * (2^39 mod m) + l) mod m,
+ * with v = h * (2^39 mod m) + l < 2^48 exact in FP64, v - rint(v / m) * m
+ * exact (one fma), brought into [-(m-1)/2, (m-1)/2] by one correction each
+ * way, and negated for a negative a.  m = 256 takes the low byte.  The
+ * residue is stored as a signed byte. */
+struct Oz2Val {
+  double h, l;   /* |a| = h 2^39 + l */
+  unsigned low;  /* |a| mod 2^32 */
   int neg;
 };
 
-__device__ __forceinline__ Oz2Limbs oz2_limbs(double x, int e, int P) {
-  Oz2Limbs r;
+__device__ __forceinline__ Oz2Val oz2_val(double x, int e, int P) {
+  Oz2Val r;
   const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
   r.neg = (int)(bits >> 63);
   const int be = (int)((bits >> 52) & 0x7ff);
@@ -203,25 +206,26 @@ __device__ __forceinline__ Oz2Limbs oz2_limbs(double x, int e, int P) {
     q = be - 1075;
   }
   const int sh = q + P - e;
-  unsigned long long lo;
-  unsigned hi = 0;
-  if (sh >= 0) { /* sh <= 25: at most 78 bits */
-    lo = mant << sh;
-    hi = sh ? (unsigned)(mant >> (64 - sh)) : 0u;
+  constexpr unsigned long long M39 = (1ull << 39) - 1;
+  unsigned long long hi, lo;
+  if (sh >= 0) { /* sh <= 25 */
+    hi = mant >> (39 - sh);
+    lo = (mant << sh) & M39;
+    r.low = (unsigned)(mant << sh);
   } else {
     const int s = -sh;
-    lo = 0;
+    unsigned long long v = 0;
     if (s <= 63) {
-      lo = mant >> s;
+      v = mant >> s;
       const unsigned long long rem = mant & ((1ull << s) - 1), half = 1ull << (s - 1);
-      lo += (rem > half || (rem == half && (lo & 1))) ? 1 : 0;
+      v += (rem > half || (rem == half && (v & 1))) ? 1 : 0;
     }
+    hi = v >> 39;
+    lo = v & M39;
+    r.low = (unsigned)v;
   }
-  r.l[0] = (unsigned)lo & 0xffffu;
-  r.l[1] = (unsigned)(lo >> 16) & 0xffffu;
-  r.l[2] = (unsigned)(lo >> 32) & 0xffffu;
-  r.l[3] = (unsigned)(lo >> 48);
-  r.l[4] = hi & 0xffffu; /* < 2^14 */
+  r.h = (double)hi;
+  r.l = (double)lo;
   return r;
 }
 
@@ -232,29 +236,28 @@ constexpr unsigned pow2_mod(int e, int m) {
 }
 
 template <int J>
-__device__ __forceinline__ unsigned oz2_residue_byte(const Oz2Limbs &a) {
+__device__ __forceinline__ unsigned oz2_residue_byte(const Oz2Val &a) {
   constexpr int m = OZ2_MODULI[J];
   int r;
   if (m == 256) {
-    r = (int)(a.l[0] & 255u);
+    r = (int)(a.low & 255u);
+    r = a.neg ? -r : r;
   } else {
-    const unsigned s = a.l[0] + __umul24(a.l[1], pow2_mod(16, m)) +
-                       __umul24(a.l[2], pow2_mod(32, m)) + __umul24(a.l[3], pow2_mod(48, m)) +
-                       __umul24(a.l[4], pow2_mod(64, m));
-    const int qt = (int)((float)s * (1.0f / (float)m));
-    r = (int)s - qt * m;
-    r += r < 0 ? m : 0;
-    r -= r >= m ? m : 0;
+    constexpr double c39 = (double)pow2_mod(39, m), rcp = 1.0 / (double)m;
+    const double v = __builtin_fma(a.h, c39, a.l);
+    const double qd = __builtin_rint(v * rcp);
+    r = (int)__builtin_fma(-qd, (double)m, v);
+    r -= r > (m - 1) / 2 ? m : 0;
+    r += r < -(m - 1) / 2 ? m : 0;
+    r = a.neg ? -r : r;
   }
-  if (a.neg && r) r = m - r;
-  if (r > (m - 1) / 2) r -= m;
   return (unsigned)r & 0xffu;
 }
 
 /* The residue bytes of four consecutive entries, packed into word W of each
  * of the nmod plane vectors (all indices compile-time constants). */
 template <int J, int W>
-__device__ __forceinline__ void oz2_encode4(const Oz2Limbs (&a)[4], int nmod,
+__device__ __forceinline__ void oz2_encode4(const Oz2Val (&a)[4], int nmod,
                                             i4_t (&res)[OZ2_MAX_MODULI]) {
   if (J < nmod)
     res[J][W] = (int)(oz2_residue_byte<J>(a[0]) | (oz2_residue_byte<J>(a[1]) << 8) |
@@ -278,11 +281,11 @@ ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int 
   i4_t res[OZ2_MAX_MODULI];
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
-  Oz2Limbs a[4];
+  Oz2Val a[4];
 #define GEMMX_OZ2_WORD(W)                                                                \
   for (int q = 0; q < 4; ++q) {                                                          \
     const int64_t p = blk * 16 + 4 * (W) + q;                                            \
-    a[q] = oz2_limbs((p < len && e != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0, \
+    a[q] = oz2_val((p < len && e != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0, \
                      e, P);                                                              \
   }                                                                                      \
   oz2_encode4<0, W>(a, nmod, res);
@@ -325,10 +328,10 @@ ozaki2_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
   i4_t res[OZ2_MAX_MODULI];
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
-  Oz2Limbs a[4];
+  Oz2Val a[4];
 #define GEMMX_OZ2_WORD(W)                                                                \
   for (int i = 0; i < 4; ++i)                                                            \
-    a[i] = oz2_limbs(e != OZ_NONFINITE ? tile[vl * ROW + (4 * (W) + i) * 8 + q] : 0.0, e, P); \
+    a[i] = oz2_val(e != OZ_NONFINITE ? tile[vl * ROW + (4 * (W) + i) * 8 + q] : 0.0, e, P); \
   oz2_encode4<0, W>(a, nmod, res);
   GEMMX_OZ2_WORD(0)
   GEMMX_OZ2_WORD(1)
