@@ -26,14 +26,14 @@ __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
-    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "tune_lookup", "last_error", "fill",
-    "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2",
+    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
+    "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2", "MODE_AUTO",
 ]
 
 LIB_DIR = os.environ.get("GEMMX_LIB_DIR") or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), "lib")
 
-MODE_NATIVE, MODE_OZAKI, MODE_BF16X3, MODE_OZAKI2 = 0, 1, 2, 3
+MODE_NATIVE, MODE_OZAKI, MODE_BF16X3, MODE_OZAKI2, MODE_AUTO = 0, 1, 2, 3, 4
 OK, ERR_ARG, ERR_NODEVICE, ERR_ALLOC, ERR_LAUNCH, ERR_UNSUPPORTED = range(6)
 
 
@@ -394,6 +394,18 @@ def tune_lookup(dtype, transa, transb, m, n, k) -> int:
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     return int(f(_ch(transa), _ch(transb), m, n, k))
+
+
+def tune_lookup_emulated(transa, transb, m, n, k):
+    """(mode, count) dgemm_ uses in MODE_AUTO for these arguments."""
+    lib = load("DGEMM.so")
+    f = lib.dgemm_x_tune_lookup_emulated
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_int)]
+    c = ctypes.c_int()
+    mode = int(f(_ch(transa), _ch(transb), m, n, k, ctypes.byref(c)))
+    return mode, c.value
 
 
 def set_dgemm_mode(mode: int, slices: int = 8):

@@ -15,9 +15,12 @@ For each (dtype, trans, shape) every configuration runs on device-resident
 U[-1,1) operands (alpha = 1.5, beta = -0.5); the fastest by median event time
 is written, after its result is checked against configuration 0's on the same
 inputs (|difference| within 64 k u (|alpha| k + |beta|), the bound for
-U[-1,1) inputs).  With --resume, keys already in the output table are kept and
-not measured again.  The table is written in sorted key order with the
-measured times as comments.
+U[-1,1) inputs).  For DGEMM shapes with min(m, n, k) >= 1024 it also picks
+the arithmetic of dgemm_'s automatic mode: Ozaki-II with the smallest
+modulus count whose error matches native FP64's on phi in {0, 1, 4} inputs,
+when that is faster than native (an `emu` entry).  With --resume, keys already
+in the output table are kept and not measured again.  The table is written in
+sorted key order with the measured times as comments.
 """
 from __future__ import annotations
 
@@ -57,6 +60,13 @@ def key_of(dtype: str, trans: str, m: int, n: int, k: int):
 
 # ------------------------------------------------------------------ table I/O
 _ENTRY = re.compile(r"^\s*(f64|f32)\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)\s*(#.*)?$")
+_EMU = re.compile(r"^\s*emu\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(native|ozaki|ozaki2)"
+                  r"\s+(\d+)\s*(#.*)?$")
+_EMU_RANGE = {"native": (0, 0), "ozaki": (3, 8), "ozaki2": (8, 22)}
+
+
+def emu_key(trans, m, n, k):
+    return ("emu", trans, bucket(m), bucket(n), bucket(k))
 
 
 def read_table(path):
@@ -72,6 +82,15 @@ def read_table(path):
                 raise ValueError(f"{path}: table format {body[1]}, expected {FORMAT}")
             if len(body) == 2 and body[0] == "arch":
                 arch = body[1]
+                continue
+            m = _EMU.match(line)
+            if m:
+                key = ("emu", m.group(1), int(m.group(2)), int(m.group(3)), int(m.group(4)))
+                scheme, cnt = m.group(5), int(m.group(6))
+                lo, hi = _EMU_RANGE[scheme]
+                if all(4 <= b <= 15 for b in key[2:]) and lo <= cnt <= hi:
+                    entries[key] = (scheme, cnt)
+                    notes[key] = (m.group(7) or "").lstrip("# ").rstrip()
                 continue
             m = _ENTRY.match(line)
             if m:
@@ -93,11 +112,14 @@ def write_table(path, entries, notes, arch, header=()):
     lines += [f"format {FORMAT}"]
     if arch:
         lines += [f"arch {arch}"]
-    lines += ["# dtype trans mbucket nbucket kbucket cfg"]
-    order = {"f64": 0, "f32": 1}
+    lines += ["# dtype trans mbucket nbucket kbucket cfg",
+              "# emu trans mbucket nbucket kbucket scheme count   (DGEMM arithmetic, auto mode)"]
+    order = {"f64": 0, "f32": 1, "emu": 2}
     for key in sorted(entries, key=lambda k: (order[k[0]], TRANS.index(k[1]), k[2], k[3], k[4])):
         note = notes.get(key, "")
-        lines.append(f"{key[0]} {key[1]} {key[2]} {key[3]} {key[4]} {entries[key]}"
+        val = entries[key]
+        val = f"{val[0]} {val[1]}" if key[0] == "emu" else str(val)
+        lines.append(f"{key[0]} {key[1]} {key[2]} {key[3]} {key[4]} {val}"
                      + (f"  # {note}" if note else ""))
     tmp = path + ".tmp"
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
@@ -175,6 +197,105 @@ def measure(torch, gx, dtype, trans, m, n, k, reps):
     return times, ok, best
 
 
+# ------------------------------------------------------------------ emulated DGEMM
+PHIS = (0.0, 1.0, 4.0)
+MODULI = (14, 16, 17, 18, 19, 20, 22)
+
+
+def _two_prod(a, b):
+    """Dekker/Veltkamp TwoProduct: a*b = p + e exactly."""
+    c = 134217729.0
+
+    def split(x):
+        t = c * x
+        hi = t - (t - x)
+        return hi, x - hi
+    p = a * b
+    ah, al = split(a)
+    bh, bl = split(b)
+    return p, ((ah * bh - p) + ah * bl + al * bh) + al * bl
+
+
+def _exact_dots(arows, bcols):
+    """Correctly rounded dot products of every row of arows with every column
+    of bcols (error-free products summed with math.fsum)."""
+    import math
+    import numpy as np
+    out = np.empty((arows.shape[0], bcols.shape[1]))
+    for i in range(arows.shape[0]):
+        for j in range(bcols.shape[1]):
+            p, e = _two_prod(arows[i], bcols[:, j])
+            out[i, j] = math.fsum(np.concatenate([p, e]))
+    return out
+
+
+def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI):
+    """DGEMM arithmetic for one shape: native FP64, or Ozaki-II with the
+    smallest modulus count whose relative error (12 x 12 sampled entries
+    against exactly rounded dot products, alpha = 1, beta = 0) is no larger
+    than native's for entries (u - 0.5) exp(phi g), phi in {0, 1, 4}, if it
+    is faster.  Returns (scheme, count, note)."""
+    import numpy as np
+    ra, ca = (m, k) if trans[0] == "N" else (k, m)
+    rb, cb = (k, n) if trans[1] == "N" else (n, k)
+    A = torch.empty(ra * ca, dtype=torch.float64, device="cuda")
+    B = torch.empty(rb * cb, dtype=torch.float64, device="cuda")
+    C = torch.empty(m * n, dtype=torch.float64, device="cuda")
+    ws = torch.empty(gx.dgemm_ozaki2_workspace(m, n, k, max(moduli)), dtype=torch.uint8,
+                     device="cuda")
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(3)
+    rows = torch.tensor(sorted(rng.choice(m, min(12, m), replace=False).tolist()), device="cuda")
+    cols = torch.tensor(sorted(rng.choice(n, min(12, n), replace=False).tolist()), device="cuda")
+
+    def native():
+        gx.dgemm_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, stream=stream)
+
+    def emulated(nm):
+        gx.dgemm_ozaki2_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, nm, ws,
+                            ws.numel(), stream=stream)
+
+    def timed(fn):
+        fn()
+        ev = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ev.append(e0.elapsed_time(e1))
+        return _median(ev)
+
+    def error(fn, ref):
+        fn()
+        torch.cuda.synchronize()
+        got = C.view(-1, m)[cols][:, rows].T.double().cpu().numpy()
+        return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+
+    errs_nat, errs = [], {nm: [] for nm in moduli}
+    for phi in PHIS:
+        gx.fill(A, ra, ca, ra, 41, 0, 1, phi)
+        gx.fill(B, rb, cb, rb, 41, 1, 1, phi)
+        torch.cuda.synchronize()
+        A2, B2 = A.view(-1, ra), B.view(-1, rb)
+        arows = (A2[:k, rows].T if trans[0] == "N" else A2[rows, :k]).double().cpu().numpy()
+        bcols = (B2[cols, :k].T if trans[1] == "N" else B2[:k, cols]).double().cpu().numpy()
+        ref = _exact_dots(arows, bcols)
+        errs_nat.append(error(native, ref))
+        for nm in moduli:
+            errs[nm].append(error(lambda nm=nm: emulated(nm), ref))
+    t_nat = timed(native)
+    qual = [nm for nm in moduli if all(e <= en for e, en in zip(errs[nm], errs_nat))]
+    if not qual:
+        return "native", 0, f"{m}x{n}x{k} native {t_nat:.3f} ms; no modulus count matches its error"
+    nm = min(qual)
+    t_emu = timed(lambda: emulated(nm))
+    note = f"{m}x{n}x{k} native {t_nat:.3f} ms, ozaki2 n{nm} {t_emu:.3f} ms"
+    return ("ozaki2", nm, note) if t_emu < t_nat else ("native", 0, note)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=DEFAULT_OUT)
@@ -184,6 +305,9 @@ def main(argv=None):
     ap.add_argument("--shapes", default=None, help="m,n,k;m,n,k;... instead of --set")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--resume", action="store_true")
+    ap.add_argument("--emulated", choices=["on", "off"], default="on",
+                    help="also choose the DGEMM arithmetic (native or Ozaki-II) per bucket for "
+                         "shapes with min(m, n, k) >= 1024")
     args = ap.parse_args(argv)
 
     import torch
@@ -217,6 +341,23 @@ def main(argv=None):
         write_table(args.out, entries, notes, arch, header)  # after every shape: resumable
         print(f"[{i + 1}/{len(todo)}] {key[0]} {t} {m}x{n}x{k}: cfg {best} "
               f"({times[best]:.3f} ms; cfg0 {times[0]:.3f} ms)", flush=True)
+    if args.emulated == "on" and "d" in args.dtypes.split(","):
+        etodo, seen = [], set()
+        for t in args.trans.split(","):
+            for (m, n, k) in shapes:
+                key = emu_key(t, m, n, k)
+                if min(m, n, k) < 1024 or key in seen or (args.resume and key in entries):
+                    continue
+                seen.add(key)
+                etodo.append((t, m, n, k))
+        for i, (t, m, n, k) in enumerate(etodo):
+            scheme, cnt, note = measure_emulated(torch, gx, t, m, n, k, args.reps)
+            key = emu_key(t, m, n, k)
+            entries[key] = (scheme, cnt)
+            notes[key] = note
+            write_table(args.out, entries, notes, arch, header)
+            print(f"[emu {i + 1}/{len(etodo)}] {t} {m}x{n}x{k}: {scheme} {cnt} ({note})",
+                  flush=True)
     print(f"autotune: {len(entries)} entries in {args.out}")
     return 0
 

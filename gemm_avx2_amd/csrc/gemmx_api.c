@@ -98,7 +98,9 @@ static int g_smode = GEMMX_MODE_NATIVE;
 static void init_once(void) {
   for (int i = 0; i < GEMMX_MAX_DEV; i++) pthread_mutex_init(&g_dev[i].lock, NULL);
   const char *dm = getenv("GEMMX_DGEMM_MODE");
-  if (dm && strncmp(dm, "ozaki2", 6) == 0) {
+  if (dm && strcmp(dm, "auto") == 0) {
+    g_dmode = GEMMX_MODE_AUTO;
+  } else if (dm && strncmp(dm, "ozaki2", 6) == 0) {
     g_dmode = GEMMX_MODE_OZAKI2;
     g_dslices = 20;
     if (dm[6] == ':') {
@@ -287,6 +289,10 @@ static void fortran_call(call_t *x, int mode, int slices) {
   clear_error();
   if (validate(x) != GEMMX_OK) return;
   if (x->m == 0 || x->n == 0) return;
+  if (mode == GEMMX_MODE_AUTO) /* per-shape arithmetic from the tuning table */
+    mode = x->es == 8 ? gemmx_tune_pick_emulated(!notrans(x->ta), !notrans(x->tb), x->m, x->n,
+                                                 x->k, &slices)
+                      : GEMMX_MODE_NATIVE;
   int dev;
   if (current_device(&dev) != GEMMX_OK) return;
 
@@ -393,7 +399,7 @@ EXPORT void sgemm_(const char *transa, const char *transb, const int *m,
 /* ------------------------------------------------------------------------ */
 EXPORT int dgemm_x_set_mode(int mode, int slices) {
   pthread_once(&g_once, init_once);
-  if (mode == GEMMX_MODE_NATIVE) {
+  if (mode == GEMMX_MODE_NATIVE || mode == GEMMX_MODE_AUTO) {
     g_dmode = mode;
     return GEMMX_OK;
   }
@@ -434,6 +440,18 @@ EXPORT int dgemm_x_load_tuning(const char *path) {
 EXPORT int sgemm_x_load_tuning(const char *path) {
   pthread_once(&g_once, init_once);
   return gemmx_tune_load(path);
+}
+
+/* The arithmetic dgemm_ in the automatic mode would use for these
+ * arguments (GEMMX_MODE_*), with the slice or modulus count in *count.
+ * Needs no device. */
+EXPORT int dgemm_x_tune_lookup_emulated(char transa, char transb, int m, int n, int k,
+                                        int *count) {
+  pthread_once(&g_once, init_once);
+  int c = 0;
+  const int mode = gemmx_tune_pick_emulated(!notrans(transa), !notrans(transb), m, n, k, &c);
+  if (count) *count = c;
+  return mode;
 }
 
 /* The tile configuration a call with these arguments would use (forced

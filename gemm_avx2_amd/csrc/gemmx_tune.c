@@ -15,10 +15,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "../../include/gemm_mi355x.h"
 #include "gemmx_internal.h"
 
 #define NB 12 /* buckets 4..15 */
 static signed char g_table[2][4][NB][NB][NB]; /* [f64/f32][trans][m][n][k] */
+/* DGEMM arithmetic per bucket: 0 = no entry, else (mode + 1) << 8 | count */
+static short g_emu[4][NB][NB][NB];
 static int g_entries;
 static int g_force[2] = {-1, -1}; /* [f64/f32]: one configuration for every call */
 static char g_path[1024];
@@ -45,6 +48,7 @@ static int trans_index(const char *t) {
 
 static void clear_table(void) {
   memset(g_table, 0, sizeof g_table);
+  memset(g_emu, 0, sizeof g_emu);
   g_entries = 0;
   g_path[0] = 0;
 }
@@ -54,7 +58,9 @@ static int load_table(const char *path) {
   FILE *f = fopen(path, "r");
   if (!f) return -1;
   static signed char staged[2][4][NB][NB][NB];
+  static short staged_emu[4][NB][NB][NB];
   memset(staged, 0, sizeof staged);
+  memset(staged_emu, 0, sizeof staged_emu);
   char line[256];
   int format = 0, entries = 0, ok = 1;
   while (ok && fgets(line, sizeof line, f)) {
@@ -72,8 +78,27 @@ static int load_table(const char *path) {
         continue;
       }
     }
-    char dt[8], tr[8];
+    char dt[8], tr[8], sc[16];
     int mb, nb, kb, cfg;
+    if (sscanf(line, "emu %7s %d %d %d %15s %d", tr, &mb, &nb, &kb, sc, &cfg) == 6) {
+      if (format != GEMMX_TUNE_FORMAT) {
+        ok = 0;
+        break;
+      }
+      const int t = trans_index(tr);
+      const int mode = strcmp(sc, "native") == 0   ? GEMMX_MODE_NATIVE
+                       : strcmp(sc, "ozaki") == 0  ? GEMMX_MODE_OZAKI
+                       : strcmp(sc, "ozaki2") == 0 ? GEMMX_MODE_OZAKI2
+                                                   : -1;
+      const int cmin = mode == GEMMX_MODE_OZAKI ? 3 : mode == GEMMX_MODE_OZAKI2 ? 8 : 0;
+      const int cmax = mode == GEMMX_MODE_OZAKI ? 8 : mode == GEMMX_MODE_OZAKI2 ? 22 : 0;
+      if (t < 0 || mode < 0 || mb < 4 || mb > 15 || nb < 4 || nb > 15 || kb < 4 || kb > 15 ||
+          cfg < cmin || cfg > cmax)
+        continue;
+      staged_emu[t][mb - 4][nb - 4][kb - 4] = (short)(((mode + 1) << 8) | cfg);
+      entries++;
+      continue;
+    }
     if (sscanf(line, "%7s %7s %d %d %d %d", dt, tr, &mb, &nb, &kb, &cfg) != 6) continue;
     if (format != GEMMX_TUNE_FORMAT) {
       ok = 0; /* entries before (or without) a format directive */
@@ -89,6 +114,7 @@ static int load_table(const char *path) {
   fclose(f);
   if (!ok || format != GEMMX_TUNE_FORMAT) return -1;
   memcpy(g_table, staged, sizeof g_table);
+  memcpy(g_emu, staged_emu, sizeof g_emu);
   g_entries = entries;
   snprintf(g_path, sizeof g_path, "%s", path);
   return entries;
@@ -140,6 +166,18 @@ int gemmx_tune_load(const char *path) {
 }
 
 const char *gemmx_tune_path(void) { return g_path; }
+
+int gemmx_tune_pick_emulated(int ta, int tb, int m, int n, int k, int *count) {
+  const int t = (ta ? 2 : 0) + (tb ? 1 : 0);
+  const int v = g_emu[t][gemmx_tune_bucket(m) - 4][gemmx_tune_bucket(n) - 4]
+                     [gemmx_tune_bucket(k) - 4];
+  if (!v) {
+    *count = 0;
+    return GEMMX_MODE_NATIVE;
+  }
+  *count = v & 0xff;
+  return (v >> 8) - 1;
+}
 
 int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {
   const int d = es == 8 ? 0 : 1, t = (ta ? 2 : 0) + (tb ? 1 : 0);

@@ -14,8 +14,11 @@
  * and one entry per line:
  *     <dtype:f64|f32> <trans:NN|NT|TN|TT> <mbucket> <nbucket> <kbucket> <cfg>
  * where a bucket is floor(log2(dim)) clamped to 4..15 and cfg is a
- * configuration id.  A later entry for the same key replaces an earlier one.
- * Missing entries fall back to a size heuristic.
+ * configuration id, or, for the arithmetic of DGEMM in the automatic mode,
+ *     emu <trans> <mbucket> <nbucket> <kbucket> <native|ozaki|ozaki2> <count>
+ * (count: slices 3..8 for ozaki, moduli 8..22 for ozaki2, 0 for native).
+ * A later entry for the same key replaces an earlier one.  Missing entries
+ * fall back to a size heuristic (tile) or native arithmetic.
  */
 #ifndef GEMMX_TUNE_H
 #define GEMMX_TUNE_H
@@ -30,6 +33,9 @@ extern "C" {
  * there is no device (then a table's arch directive is not checked). */
 void gemmx_tune_init(const char *arch);
 int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k);
+/* Arithmetic of a DGEMM in the automatic mode: GEMMX_MODE_NATIVE, _OZAKI or
+ * _OZAKI2, with the slice or modulus count in *count. */
+int gemmx_tune_pick_emulated(int ta, int tb, int m, int n, int k, int *count);
 int gemmx_tune_bucket(int dim);
 /* Number of entries loaded from the table file (0 when none). */
 int gemmx_tune_entries(void);

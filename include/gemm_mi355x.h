@@ -65,6 +65,9 @@ void sgemm_(const char *transa, const char *transb, const int *m, const int *n,
  * Also set by the environment at first use:
  *   GEMMX_DGEMM_MODE = native | ozaki[:slices] (slices 3..8, default 8)
  *                    | ozaki2[:moduli]         (moduli 8..22, default 20)
+ *                    | auto  (the tuning table's `emu` entry of the shape's
+ *                             bucket, written by the autotuner; native when
+ *                             there is none)
  *   GEMMX_SGEMM_MODE = native | bf16x3
  * For GEMMX_MODE_OZAKI2 the second argument of dgemm_x_set_mode is the
  * modulus count. */
@@ -72,6 +75,7 @@ void sgemm_(const char *transa, const char *transb, const int *m, const int *n,
 #define GEMMX_MODE_OZAKI 1  /* FP64 by INT8 slices on INT8 MFMA */
 #define GEMMX_MODE_BF16X3 2 /* FP32 by split-BF16x3 on BF16 MFMA */
 #define GEMMX_MODE_OZAKI2 3 /* FP64 by INT8 residues (CRT) on INT8 MFMA */
+#define GEMMX_MODE_AUTO 4   /* dgemm_ only: per-shape choice from the tuning table */
 int dgemm_x_set_mode(int mode, int slices);
 int sgemm_x_set_mode(int mode);
 
@@ -92,6 +96,11 @@ int sgemm_x_set_tile_config(int cfg);
 int dgemm_x_load_tuning(const char *path);
 int sgemm_x_load_tuning(const char *path);
 int dgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);
+/* The arithmetic dgemm_ uses in GEMMX_MODE_AUTO for these arguments
+ * (GEMMX_MODE_NATIVE, _OZAKI or _OZAKI2; count = slices or moduli, 0 for
+ * native); no device needed. */
+int dgemm_x_tune_lookup_emulated(char transa, char transb, int m, int n, int k,
+                                 int *count);
 int sgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);
 
 /* Last error of the calling thread: returns its code and, when msg is not

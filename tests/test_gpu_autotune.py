@@ -52,3 +52,40 @@ def test_autotune_table_is_used_and_correct(cuda, tmp_path, capsys):
         import os
         for d in ("d", "s"):
             gx.load_tuning(at.DEFAULT_OUT if os.path.exists(at.DEFAULT_OUT) else None, d)
+
+
+def test_autotune_emulated_choice_drives_auto_mode(cuda, tmp_path):
+    """The autotuner's DGEMM arithmetic choice for a 1024^3 bucket is what
+    dgemm_ uses in the automatic mode: bit-identical to the oracle's Ozaki-II
+    restatement when it chose Ozaki-II, within the native tolerance
+    otherwise."""
+    import os
+    out = str(tmp_path / "tune.txt")
+    assert at.main(["--out", out, "--shapes", "1024,1024,1024", "--trans", "NN",
+                    "--dtypes", "d", "--reps", "2"]) == 0
+    entries, notes, _ = at.read_table(out)
+    scheme, cnt = entries[("emu", "NN", 10, 10, 10)]
+    assert scheme in ("native", "ozaki2")
+    rng = np.random.default_rng(12)
+    m = n = k = 1024
+    A = np.asfortranarray((rng.uniform(0, 1, (m, k)) - 0.5) * np.exp(rng.standard_normal((m, k))))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    try:
+        assert gx.load_tuning(out, "d") == len(entries)
+        assert gx.tune_lookup_emulated("N", "N", m, n, k) == (
+            gx.MODE_OZAKI2 if scheme == "ozaki2" else gx.MODE_NATIVE, cnt)
+        gx.set_dgemm_mode(gx.MODE_AUTO, 0)
+        G = C.copy(order="F")
+        gx.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, G, m)
+        R = C.copy(order="F")
+        if scheme == "ozaki2":
+            oracle.ozaki2_gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m, cnt)
+            np.testing.assert_array_equal(G, R)
+        else:
+            oracle.gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m)
+            bound = abs_bound(A, B, C, "N", "N", m, n, k, 1.0, 0.0)
+            assert max_comp(G, R, bound) <= 64 * k * U64
+    finally:
+        gx.set_dgemm_mode(gx.MODE_NATIVE)
+        gx.load_tuning(at.DEFAULT_OUT if os.path.exists(at.DEFAULT_OUT) else None, "d")

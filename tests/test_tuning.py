@@ -128,3 +128,31 @@ def test_configuration_counts_agree():
     nb = int(re.search(r"#define GEMMX_BATCH_CONFIGS (\d+)", hdr).group(1))
     assert gx.TILE_CONFIGS == at.TILE_CONFIGS == n
     assert gx.BATCH_CONFIGS == nb
+
+
+def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
+    """`emu` lines (written by the autotuner) choose dgemm_'s arithmetic in
+    the automatic mode per shape bucket; the same table always gives the same
+    choice, invalid counts are skipped, and the Python reader agrees."""
+    path = str(tmp_path / "t.txt")
+    e = dict(_entries())
+    e[("emu", "NN", 13, 13, 13)] = ("ozaki2", 18)
+    e[("emu", "TN", 12, 12, 12)] = ("native", 0)
+    e[("emu", "NT", 14, 14, 10)] = ("ozaki", 6)
+    at.write_table(path, e, {}, None)
+    got, _, _ = at.read_table(path)
+    assert got == e
+    assert gx.load_tuning(path, "d") == len(e)
+    assert gx.tune_lookup_emulated("N", "N", 8192, 9000, 16000) == (gx.MODE_OZAKI2, 18)
+    assert gx.tune_lookup_emulated("n", "n", 8192, 9000, 16000) == (gx.MODE_OZAKI2, 18)
+    assert gx.tune_lookup_emulated("T", "N", 4096, 4096, 4096) == (gx.MODE_NATIVE, 0)
+    assert gx.tune_lookup_emulated("N", "T", 16384, 16384, 1024) == (gx.MODE_OZAKI, 6)
+    assert gx.tune_lookup_emulated("T", "T", 8192, 8192, 8192) == (gx.MODE_NATIVE, 0)
+    # the native tile entries are unaffected
+    assert gx.tune_lookup("d", "N", "N", 8192, 8192, 8192) == 6
+    bad = tmp_path / "bad.txt"
+    bad.write_text("format 1\nemu NN 13 13 13 ozaki2 40\nemu NN 13 13 13 ozaki 2\n"
+                   "emu NN 13 13 13 fp16 4\nemu NN 13 13 13 ozaki2 20\n")
+    assert gx.load_tuning(str(bad), "d") == 1
+    assert gx.tune_lookup_emulated("N", "N", 8192, 8192, 8192) == (gx.MODE_OZAKI2, 20)
+    assert at.read_table(str(bad))[0] == {("emu", "NN", 13, 13, 13): ("ozaki2", 20)}
