@@ -31,7 +31,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["gemm_native.hip", "gemm_split.hip", "gemm_products.hip", "gemm_emulated.hip",
                "gemm_batch.hip"]
-C_SOURCES = ["gemmx_api.c", "gemmx_tune.c", "gemmx_batch.c"]
+C_SOURCES = ["gemmx_api.c", "gemmx_tune.c", "gemmx_batch.c", "gemmx_stage.c"]
 GEN_SOURCES = ["gemmx_gen.hip"]
 # tuning table measured on MI355X by gemm_avx2_amd.autotune (--set full)
 SHIPPED_TABLE = os.path.join(HERE, "tuning", "gemm_mi355x_tune_gfx950.txt")

@@ -329,33 +329,34 @@ static void fortran_call(call_t *x, int mode, int slices) {
   if (total > 0 && get_scratch(dev, total, (void **)&base) != GEMMX_OK) goto out;
   char *pa = base, *pb = pa + bytes_a, *pc = pb + bytes_b, *pw = pc + bytes_c;
   hipError_t e;
+  /* Host operands go through the pinned staging ring (gemmx_stage.c). */
   if (host_a) {
-    e = hipMemcpy2DAsync(pa, lda2 * es, x->a, (size_t)x->lda * es, (size_t)ra * es,
-                         (size_t)ca, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) { hip_fail(e, "copy of A to the device"); goto out; }
+    if (gemmx_stage_h2d(dev, pa, lda2 * es, x->a, (size_t)x->lda * es, (size_t)ra * es,
+                        (size_t)ca, s) != GEMMX_OK)
+      goto out;
     p.a = pa;
   }
   if (host_b) {
-    e = hipMemcpy2DAsync(pb, ldb2 * es, x->b, (size_t)x->ldb * es, (size_t)rb * es,
-                         (size_t)cb, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) { hip_fail(e, "copy of B to the device"); goto out; }
+    if (gemmx_stage_h2d(dev, pb, ldb2 * es, x->b, (size_t)x->ldb * es, (size_t)rb * es,
+                        (size_t)cb, s) != GEMMX_OK)
+      goto out;
     p.b = pb;
   }
   if (host_c) {
     /* C is always read: beta == 0 still multiplies (NaN propagates). */
-    e = hipMemcpy2DAsync(pc, ldc2 * es, x->c, (size_t)x->ldc * es, (size_t)x->m * es,
-                         (size_t)x->n, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) { hip_fail(e, "copy of C to the device"); goto out; }
+    if (gemmx_stage_h2d(dev, pc, ldc2 * es, x->c, (size_t)x->ldc * es, (size_t)x->m * es,
+                        (size_t)x->n, s) != GEMMX_OK)
+      goto out;
     p.c = pc;
   }
   if (launch_device(&p, x->es, mode, slices, bytes_w ? pw : NULL, s) != GEMMX_OK) {
     (void)hipStreamSynchronize(s);
     goto out;
   }
-  if (host_c) {
-    e = hipMemcpy2DAsync(x->c, (size_t)x->ldc * es, pc, ldc2 * es, (size_t)x->m * es,
-                         (size_t)x->n, hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) { hip_fail(e, "copy of C to the host"); (void)hipStreamSynchronize(s); goto out; }
+  if (host_c && gemmx_stage_d2h(dev, x->c, (size_t)x->ldc * es, pc, ldc2 * es,
+                                 (size_t)x->m * es, (size_t)x->n, s) != GEMMX_OK) {
+    (void)hipStreamSynchronize(s);
+    goto out;
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) hip_fail(e, "GEMM execution");

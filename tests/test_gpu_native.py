@@ -270,3 +270,27 @@ def test_every_tile_config(cuda, dt):
                     assert max_comp(G, R, bound) <= _tol(dt, k), (cfg, m, n, k, ta, tb)
     finally:
         gx.set_tile_config(d, -1)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_host_pointer_staging_ring(cuda, dt, monkeypatch):
+    """Host operands above the staging threshold go through the ring of
+    pinned chunks (1 MB chunks here, so A, B and C each take many chunks and
+    reuse the buffers of the previous operand); odd leading dimensions and
+    transposes; rows m..ldc-1 of C untouched."""
+    monkeypatch.setenv("GEMMX_STAGE_CHUNK_MB", "1")
+    rng = np.random.default_rng(31)
+    for (ta, tb, m, n, k, pad) in [("N", "N", 1500, 1300, 1100, 3), ("T", "T", 1030, 1700, 1200, 1)]:
+        ra, ca = stored_shape(ta, m, k)
+        rb, cb = stored_shape(tb, k, n)
+        A = np.asfortranarray(rng.uniform(-1, 1, (ra + pad, ca)).astype(dt))
+        B = np.asfortranarray(rng.uniform(-1, 1, (rb + pad, cb)).astype(dt))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + pad, n)).astype(dt))
+        C[m:] = 321.0
+        C0 = C.copy(order="F")
+        R = C.copy(order="F")
+        oracle.gemm(ta, tb, m, n, k, 0.5, A, ra + pad, B, rb + pad, 2.0, R, m + pad)
+        _gemm_host(dt, ta, tb, m, n, k, 0.5, A, ra + pad, B, rb + pad, 2.0, C, m + pad)
+        bound = abs_bound(A, B, C0, ta, tb, m, n, k, 0.5, 2.0)
+        assert max_comp(C[:m], R[:m], bound) <= _tol(dt, k), (ta, tb)
+        assert np.all(C[m:] == 321.0)
