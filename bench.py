@@ -46,7 +46,7 @@ HBM_PEAK = 8000.0  # GB/s
 M = N = K = 8192
 PHIS = (0.0, 1.0, 4.0)
 SLICES = (3, 4, 5, 6, 7, 8)
-MODULI = (14, 16, 18, 19, 20, 22)
+MODULI = (14, 15, 16, 17, 18, 20, 22)
 SEED = 1
 
 
@@ -266,7 +266,7 @@ def run_headline(torch, gx, stream, args):
     forms = {}  # name -> (call, info)
     for s in (SLICES if not args.quick else (8,)):
         forms[f"ozaki_s{s}"] = (lambda s=s: h.ozaki(s), {"scheme": "ozaki1", "count": s})
-    for nm in (MODULI if not args.quick else (18, 19, 20)):
+    for nm in (MODULI if not args.quick else (15, 16, 17, 18)):
         forms[f"ozaki2_n{nm}"] = (lambda nm=nm: h.ozaki2(nm), {"scheme": "ozaki2", "count": nm,
                                                                "bits": gx.dgemm_ozaki2_bits(nm, K)})
     for name, (fn, info) in forms.items():

@@ -90,25 +90,25 @@ SliceGroups ozaki_groups(int s) {
  * The residues c_j = X mod m_j of the exact integer X = sum_p a_ip b_pj give
  * X by the Chinese remainder theorem: with M = prod m_j, M_j = M / m_j,
  * w_j = M_j^-1 mod m_j and W_j = (w_j M_j) mod M, X = (sum_j c_j W_j) mod M,
- * taken in (-M/2, M/2) (|X| < M/2 by the choice of P).  The sum
+ * taken in (-M/2, M/2) (|X| < M/2 by the vectors' bit counts).  The sum
  * (< nmod 2^8 M < 2^185) is formed exactly with 24-bit multiply-adds of the
  * byte c_j into 16-bit limbs of W_j (limb sums stay below 2^29), carried
  * into 192-bit integers (32-bit limbs), reduced modulo M with an FP64
  * quotient and one correction, rounded to 53 bits (ties to even) and scaled
- * by 2^(e_i + f_j - 2P) with one ldexp. */
+ * by 2^-(D_i + D_j), D = P - e of row i and column j, with one ldexp. */
 constexpr int OZ2_LIMBS = 6;    /* 32-bit limbs: M < 2^171 */
 constexpr int OZ2_LIMBS16 = 11; /* 16-bit limbs of M_j */
 
 struct Oz2Crt {
-  int nmod, P;
+  int nmod, L; /* L = floor(log2 M) */
   unsigned W16[OZ2_MAX_MODULI][OZ2_LIMBS16]; /* W_j = (w_j M_j) mod M, 16-bit limbs */
   unsigned M[OZ2_LIMBS], H[OZ2_LIMBS];        /* M and floor(M / 2) */
   double rMd;                                 /* 1 / M, rounded */
 };
 
-/* Host: the constants of the first nmod moduli and P for K chunks of kc. */
-bool oz2_constants(int nmod, int kc, Oz2Crt *c) {
-  if (nmod < OZ2_MIN_MODULI || nmod > OZ2_MAX_MODULI || kc < 1) return false;
+/* Host: the constants of the first nmod moduli. */
+bool oz2_constants(int nmod, Oz2Crt *c) {
+  if (nmod < OZ2_MIN_MODULI || nmod > OZ2_MAX_MODULI) return false;
   memset(c, 0, sizeof *c);
   c->nmod = nmod;
   c->M[0] = 1;
@@ -120,18 +120,11 @@ bool oz2_constants(int nmod, int kc, Oz2Crt *c) {
       carry = v >> 32;
     }
   }
-  int L = 0; /* floor(log2 M) */
   for (int l = OZ2_LIMBS - 1; l >= 0; --l)
     if (c->M[l]) {
-      L = 32 * l + 31 - __builtin_clz(c->M[l]);
+      c->L = 32 * l + 31 - __builtin_clz(c->M[l]);
       break;
     }
-  int kbits = 0;
-  while ((1LL << kbits) <= kc) ++kbits;
-  /* |X| <= kc 2^2P < 2^(kbits+2P) <= 2^(L-1) <= M/2; capped so that the
-   * scaled integers stay below 2^78 (five 16-bit limbs in the split) */
-  c->P = (L - 1 - kbits) / 2;
-  if (c->P > OZ2_MAX_BITS) c->P = OZ2_MAX_BITS;
   for (int j = 0; j < nmod; ++j) {
     const int m = OZ2_MODULI[j];
     unsigned mj[OZ2_LIMBS];
@@ -186,7 +179,7 @@ __device__ __forceinline__ unsigned long long bits64(const unsigned (&x)[OZ2_LIM
 /* X from the carried 16-bit limb sums, reduced modulo M, rounded and scaled;
  * then the alpha/beta update of C(i, j). */
 __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], const Oz2Crt &crt,
-                                           int e, int f, double *cp, double alpha,
+                                           int da, int db, double *cp, double alpha,
                                            double beta) {
   unsigned x[OZ2_LIMBS];
   {
@@ -296,10 +289,10 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
     mag = (double)mant;
   }
   double scaled;
-  if (e == OZ_NONFINITE || f == OZ_NONFINITE) {
+  if (da == OZ_NONFINITE || db == OZ_NONFINITE) {
     scaled = __builtin_nan("");
   } else {
-    scaled = ldexp(mag, shift + e + f - 2 * crt.P);
+    scaled = ldexp(mag, shift - da - db);
     scaled = neg ? -scaled : scaled;
   }
   *cp = __builtin_fma(alpha, scaled, beta * *cp);
@@ -356,7 +349,7 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
 }
 
 struct Oz2Layout {
-  size_t ea, fb, mb, as, bs, r8, total;
+  size_t ea, fb, mb, sq, as, bs, r8, total;
   int64_t kpad;
 };
 Oz2Layout oz2_layout(int m, int n, int k, int nmod) {
@@ -365,7 +358,8 @@ Oz2Layout oz2_layout(int m, int n, int k, int nmod) {
   L.ea = 0;
   L.fb = rup((size_t)m * 4, 256);
   L.mb = L.fb + rup((size_t)n * 4, 256);
-  L.as = L.mb + rup((size_t)(m > n ? m : n) * 8, 256);
+  L.sq = L.mb + rup((size_t)(m > n ? m : n) * 8, 256);
+  L.as = L.sq + rup((size_t)(m > n ? m : n) * 8, 256);
   L.bs = L.as + rup((size_t)nmod * m * L.kpad, 256);
   L.r8 = L.bs + rup((size_t)nmod * n * L.kpad, 256);
   L.total = L.r8 + rup((size_t)nmod * m * n, 256);
@@ -380,26 +374,27 @@ extern "C" size_t gemmx_ozaki2_workspace(int m, int n, int k, int moduli) {
 
 extern "C" int gemmx_ozaki2_bits(int moduli, int k) {
   Oz2Crt c;
-  return oz2_constants(moduli, k < OZ2_KCHUNK ? (k > 0 ? k : 1) : OZ2_KCHUNK, &c) ? c.P : -1;
+  if (k < 1 || !oz2_constants(moduli, &c)) return -1;
+  return ozaki2_bits_for_sum(c.L, (unsigned long long)k << 20); /* every t = 1024 */
 }
 
 extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *work,
                                        hipStream_t st, hipEvent_t *ev) {
   Oz2Crt crt;
-  const int kc = p->k < OZ2_KCHUNK ? p->k : OZ2_KCHUNK;
-  if (!work || !oz2_constants(nmod, kc, &crt)) return hipErrorInvalidValue;
+  if (!work || !oz2_constants(nmod, &crt)) return hipErrorInvalidValue;
   const Oz2Layout L = oz2_layout(p->m, p->n, p->k, nmod);
   char *w = (char *)work;
   int *ea = (int *)(w + L.ea), *fb = (int *)(w + L.fb);
   int8_t *as = (int8_t *)(w + L.as), *bs = (int8_t *)(w + L.bs);
   uint8_t *R = (uint8_t *)(w + L.r8);
   unsigned long long *mb = (unsigned long long *)(w + L.mb);
+  unsigned long long *sq = (unsigned long long *)(w + L.sq);
   if (ev) (void)hipEventRecord(ev[0], st);
   hipError_t e = ozaki2_prepare((const double *)p->a, p->lda, p->ta != 0, p->m, p->k, L.kpad,
-                                nmod, crt.P, ea, mb, as, st);
+                                nmod, crt.L, ea, mb, sq, as, st);
   if (e != hipSuccess) return e;
   e = ozaki2_prepare((const double *)p->b, p->ldb, p->tb == 0, p->n, p->k, L.kpad, nmod,
-                     crt.P, fb, mb, bs, st);
+                     crt.L, fb, mb, sq, bs, st);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[1], st);
   /* group g: plane g of A times plane g of B, stored modulo m_g */

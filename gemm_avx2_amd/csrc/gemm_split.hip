@@ -6,8 +6,9 @@
  * Ozaki-I (FP64 -> INT8): per vector the exponent e with max|x| < 2^e, then s
  * signed 7-bit digits of x*2^-e by truncation toward zero,
  * x*2^-e = sum_t d_t 2^-7t + r, |d_t| <= 127; every step is exact in FP64.
- * Ozaki-II (FP64 -> INT8 residues): per vector the same exponent e, then the
- * residues of rint(x * 2^(P-e)) modulo each modulus (exact integer work).
+ * Ozaki-II (FP64 -> INT8 residues): per vector the same exponent e and a bit
+ * count P from the vector's 2-norm bound, then the residues of
+ * rint(x * 2^(P-e)) modulo each modulus (exact integer work).
  * BF16x3 (FP32 -> 2 x BF16): hi = bf16_rne(x), lo = bf16_rne(x - hi), the
  * difference being exact in FP32.
  * No reference counterpart (the reference packs A and B for its AVX2
@@ -88,6 +89,78 @@ exp_from_bits_kernel(const unsigned long long *__restrict__ maxbits, int nvec,
                      int *__restrict__ out) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < nvec) out[v] = exponent_of_absbits(maxbits[v]);
+}
+
+/* Ozaki-II bound term of one entry: ceil(|x| 2^(10-e)) squared (exact: the
+ * scaling by a power of two and the ceil are exact, t <= 1024). */
+__device__ __forceinline__ unsigned long long oz2_term(double x, int e) {
+  const unsigned long long t = (unsigned long long)ceil(ldexp(fabs(x), 10 - e));
+  return t * t;
+}
+
+__device__ __forceinline__ int oz2_shift(int L, int e, unsigned long long S) {
+  return e == OZ_NONFINITE ? OZ_NONFINITE : ozaki2_bits_for_sum(L, S) - e;
+}
+
+/* Ozaki-II shift D = P - e per vector with elements contiguous: a wave per
+ * vector, the exponent pass of exp_contig_kernel, then the bound sum over
+ * the same (cached) vector. */
+__global__ void __launch_bounds__(256)
+oz2_shift_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len, int L,
+                        int *__restrict__ out) {
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (v >= nvec) return;
+  const double *p = x + v * vld;
+  unsigned long long mx = 0;
+  for (int i = lane; i < len; i += 64) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(p[i]) & 0x7fffffffffffffffull;
+    mx = b > mx ? b : mx;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mx, off, 64);
+    mx = o > mx ? o : mx;
+  }
+  const int e = exponent_of_absbits(mx);
+  unsigned long long S = 0;
+  if (e != OZ_NONFINITE)
+    for (int i = lane; i < len; i += 64) S += oz2_term(p[i], e);
+  for (int off = 32; off > 0; off >>= 1) S += __shfl_xor(S, off, 64);
+  if (lane == 0) out[v] = oz2_shift(L, e, S);
+}
+
+/* Bound sums per vector with vectors adjacent (x[v + p*eld]), in the blocks
+ * of absmax_strided_kernel; ex holds the exponents, sums must start at zero
+ * (integer atomics: the result does not depend on the order). */
+__global__ void __launch_bounds__(256)
+oz2_sums_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec, int len,
+                        const int *__restrict__ ex, unsigned long long *__restrict__ sums) {
+  __shared__ unsigned long long part[4][64];
+  const int vl = threadIdx.x & 63, kq = threadIdx.x >> 6;
+  const int v = blockIdx.x * 64 + vl;
+  const int p0 = blockIdx.y * 256 + kq * 64;
+  unsigned long long S = 0;
+  if (v < nvec) {
+    const int e = ex[v];
+    if (e != OZ_NONFINITE) {
+      const int p1 = p0 + 64 < len ? p0 + 64 : len;
+      for (int p = p0; p < p1; ++p) S += oz2_term(x[v + p * eld], e);
+    }
+  }
+  part[kq][vl] = S;
+  __syncthreads();
+  if (kq == 0 && v < nvec) {
+#pragma unroll
+    for (int q = 1; q < 4; ++q) S += part[q][vl];
+    atomicAdd(sums + v, S);
+  }
+}
+
+/* ex: exponent -> shift D = P - e, from the bound sums. */
+__global__ void __launch_bounds__(256)
+oz2_shift_from_sums_kernel(const unsigned long long *__restrict__ sums, int nvec, int L,
+                           int *__restrict__ ex) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nvec) ex[v] = oz2_shift(L, ex[v], sums[v]);
 }
 
 /* 16 consecutive K entries of one vector per thread -> 16 bytes per slice.
@@ -179,7 +252,8 @@ ozaki_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
 
 /* ---- Ozaki-II residues ------------------------------------------------
  * x = (-1)^neg * mant * 2^q exactly (mant < 2^53); the scaled integer is
- * rint(x * 2^(P - e)) = rint(mant * 2^sh), sh = q + P - e <= P - 52 <= 25.
+ * rint(x * 2^(P - e)) = rint(mant * 2^sh), sh = q + D, D = P - e, and
+ * sh <= P - 53 <= 24 since |x| < 2^e.
  * Its magnitude (mant << sh, or rint(mant / 2^-sh) ties to even; < 2^78) is
  * split as h * 2^39 + l with h, l < 2^39, both exact in FP64, so that for
  * each modulus m
@@ -194,7 +268,7 @@ struct Oz2Val {
   int neg;
 };
 
-__device__ __forceinline__ Oz2Val oz2_val(double x, int e, int P) {
+__device__ __forceinline__ Oz2Val oz2_val(double x, int D) {
   Oz2Val r;
   const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
   r.neg = (int)(bits >> 63);
@@ -205,10 +279,10 @@ __device__ __forceinline__ Oz2Val oz2_val(double x, int e, int P) {
     mant |= 1ull << 52;
     q = be - 1075;
   }
-  const int sh = q + P - e;
+  const int sh = q + D;
   constexpr unsigned long long M39 = (1ull << 39) - 1;
   unsigned long long hi, lo;
-  if (sh >= 0) { /* sh <= 25 */
+  if (sh >= 0) { /* sh <= 24 */
     hi = mant >> (39 - sh);
     lo = (mant << sh) & M39;
     r.low = (unsigned)(mant << sh);
@@ -270,14 +344,14 @@ __device__ __forceinline__ void oz2_encode4(const Oz2Val (&a)[4], int nmod,
 __global__ void __launch_bounds__(256)
 ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int nvec,
                     int len, int64_t kpad, const int *__restrict__ ex, int v_fast,
-                    int nmod, int P, int8_t *__restrict__ out) {
+                    int nmod, int8_t *__restrict__ out) {
   const int64_t nblk = kpad / 16;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (tid >= nblk * nvec) return;
   int64_t v, blk;
   if (v_fast) { v = tid % nvec; blk = tid / nvec; }
   else        { v = tid / nblk; blk = tid % nblk; }
-  const int e = ex[v];
+  const int d = ex[v]; /* shift P - e */
   i4_t res[OZ2_MAX_MODULI];
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
@@ -285,8 +359,7 @@ ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int 
 #define GEMMX_OZ2_WORD(W)                                                                \
   for (int q = 0; q < 4; ++q) {                                                          \
     const int64_t p = blk * 16 + 4 * (W) + q;                                            \
-    a[q] = oz2_val((p < len && e != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0, \
-                     e, P);                                                              \
+    a[q] = oz2_val((p < len && d != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0, d);     \
   }                                                                                      \
   oz2_encode4<0, W>(a, nmod, res);
   GEMMX_OZ2_WORD(0)
@@ -304,7 +377,7 @@ ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int 
  * ozaki_split_strided_kernel. */
 __global__ void __launch_bounds__(256)
 ozaki2_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec, int len,
-                            int64_t kpad, const int *__restrict__ ex, int nmod, int P,
+                            int64_t kpad, const int *__restrict__ ex, int nmod,
                             int8_t *__restrict__ out) {
   constexpr int ROW = 129;
   __shared__ double tile[32 * ROW];
@@ -324,14 +397,14 @@ ozaki2_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
   const int vl = threadIdx.x >> 3, q = threadIdx.x & 7;
   const int v = v0 + vl;
   if (v >= nvec) return;
-  const int e = ex[v];
+  const int d = ex[v]; /* shift P - e */
   i4_t res[OZ2_MAX_MODULI];
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
   Oz2Val a[4];
 #define GEMMX_OZ2_WORD(W)                                                                \
   for (int i = 0; i < 4; ++i)                                                            \
-    a[i] = oz2_val(e != OZ_NONFINITE ? tile[vl * ROW + (4 * (W) + i) * 8 + q] : 0.0, e, P); \
+    a[i] = oz2_val(d != OZ_NONFINITE ? tile[vl * ROW + (4 * (W) + i) * 8 + q] : 0.0, d);    \
   oz2_encode4<0, W>(a, nmod, res);
   GEMMX_OZ2_WORD(0)
   GEMMX_OZ2_WORD(1)
@@ -452,25 +525,28 @@ hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
 }
 
 hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec, int k,
-                          int64_t kpad, int nmod, int P, int *ex, unsigned long long *mbits,
-                          int8_t *res, hipStream_t st) {
+                          int64_t kpad, int nmod, int L, int *ex, unsigned long long *mbits,
+                          unsigned long long *sums, int8_t *res, hipStream_t st) {
   if (contiguous) {
-    hipLaunchKernelGGL(exp_contig_kernel, dim3((nvec + 3) / 4), dim3(256), 0, st, x,
-                       ld, nvec, k, ex);
+    hipLaunchKernelGGL(oz2_shift_contig_kernel, dim3((nvec + 3) / 4), dim3(256), 0, st, x,
+                       ld, nvec, k, L, ex);
     const int64_t threads = (kpad / 16) * nvec;
     hipLaunchKernelGGL(ozaki2_split_kernel, dim3((unsigned)((threads + 255) / 256)),
-                       dim3(256), 0, st, x, ld, (int64_t)1, nvec, k, kpad, ex, 0, nmod, P,
-                       res);
+                       dim3(256), 0, st, x, ld, (int64_t)1, nvec, k, kpad, ex, 0, nmod, res);
     return hipGetLastError();
   }
+  const dim3 red((nvec + 63) / 64, (k + 255) / 256);
   hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, mbits,
                      nvec);
-  hipLaunchKernelGGL(absmax_strided_kernel, dim3((nvec + 63) / 64, (k + 255) / 256),
-                     dim3(256), 0, st, x, ld, nvec, k, mbits);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, sums, nvec);
+  hipLaunchKernelGGL(absmax_strided_kernel, red, dim3(256), 0, st, x, ld, nvec, k, mbits);
   hipLaunchKernelGGL(exp_from_bits_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
                      mbits, nvec, ex);
+  hipLaunchKernelGGL(oz2_sums_strided_kernel, red, dim3(256), 0, st, x, ld, nvec, k, ex, sums);
+  hipLaunchKernelGGL(oz2_shift_from_sums_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
+                     sums, nvec, L, ex);
   hipLaunchKernelGGL(ozaki2_split_strided_kernel, dim3((nvec + 31) / 32, (unsigned)(kpad / 128)),
-                     dim3(256), 0, st, x, ld, nvec, k, kpad, ex, nmod, P, res);
+                     dim3(256), 0, st, x, ld, nvec, k, kpad, ex, nmod, res);
   return hipGetLastError();
 }
 

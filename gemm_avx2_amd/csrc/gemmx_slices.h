@@ -63,7 +63,15 @@ hipError_t bf16_prepare(const float *x, int64_t ld, bool contiguous, int nvec, i
 
 /* Ozaki-II: residues of rint(x * 2^(P - e)) modulo OZ2_MODULI[0..nmod-1]
  * (pairwise coprime, taken greedily downward from 256), each stored as a
- * signed byte in [-m/2, m/2), one plane per modulus in the slice layout. */
+ * signed byte in [-m/2, m/2), one plane per modulus in the slice layout.
+ * Each vector has its own P: with t_p = ceil(|x_p| 2^(10-e)) <= 1024 and
+ * S = sum_p t_p^2 (exact integer, any order), the largest P with
+ * 2P + bitlen(S - 1) <= L + 19, L = floor(log2 M), M the product of the
+ * moduli, capped at OZ2_MAX_BITS.  |rint(x 2^(P-e))| <= t 2^(P-10) (P >= 10)
+ * bounds the vector's 2-norm by 2^(P-10) sqrt(S) <= 2^((L-1)/2), so any dot
+ * product of two scaled vectors stays within 2^(L-1) < M/2 (Cauchy-Schwarz).
+ * Compared with one P from k max-size terms this keeps up to
+ * log2(sqrt(k)) more bits of vectors whose mass sits in few entries. */
 constexpr int OZ2_MAX_MODULI = 22;
 constexpr int OZ2_MIN_MODULI = 8;
 constexpr int OZ2_KCHUNK = 65536; /* K bytes per chunk: kc * 128^2 < 2^31 */
@@ -72,11 +80,19 @@ constexpr int OZ2_MODULI[OZ2_MAX_MODULI] = {256, 255, 253, 251, 247, 241, 239, 2
                                             229, 227, 223, 217, 211, 199, 197, 193,
                                             191, 181, 179, 173, 167, 163};
 
-/* Exponents (as ozaki_prepare) and nmod residue planes of an FP64 operand
- * for P-bit scaled integers. */
+/* Per vector the shift D = P - e (OZ_NONFINITE for a vector holding Inf or
+ * NaN) into ex, and nmod residue planes of an FP64 operand; L as above.
+ * mbits and sums: nvec scratch words each. */
 hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec, int k,
-                          int64_t kpad, int nmod, int P, int *ex, unsigned long long *mbits,
-                          int8_t *res, hipStream_t st);
+                          int64_t kpad, int nmod, int L, int *ex, unsigned long long *mbits,
+                          unsigned long long *sums, int8_t *res, hipStream_t st);
+/* The vector bit count P for bound sum S (host and device). */
+__host__ __device__ inline int ozaki2_bits_for_sum(int L, unsigned long long S) {
+  int b = 0; /* bitlen(S - 1) */
+  for (unsigned long long v = S > 1 ? S - 1 : 0; v; v >>= 1) ++b;
+  const int P = (L + 19 - b) / 2;
+  return P < OZ2_MAX_BITS ? P : OZ2_MAX_BITS;
+}
 
 /* Kinds of slice products. */
 enum SliceKind {

@@ -131,15 +131,21 @@ int dgemm_x_ozaki_dev(char transa, char transb, int m, int n, int k,
                       void *work, size_t work_bytes, void *stream);
 
 /* Emulated DGEMM on device pointers, Ozaki-II: every row of op(A) and
- * column of op(B) is scaled by a power of two to P-bit integers
- * rint(x 2^(P-e)); their exact integer product is formed from residues
- * modulo `moduli` (8..22) pairwise-coprime integers <= 256 (one INT8 GEMM
- * per modulus on the matrix cores, exact in INT32) by the Chinese remainder
- * theorem, then rounded once to FP64 and scaled back.  P grows with the
- * modulus count (dgemm_x_ozaki2_bits: 24 bits at 8 moduli, 70 at 20 and 77
- * at 22 for k <= 8192, at most 77; -1 when unsupported); entries smaller than 2^(e-P)
- * relative to their row's or column's largest are lost.  `work` as for
- * dgemm_x_ozaki_dev, of at least dgemm_x_ozaki2_workspace() bytes. */
+ * column of op(B) is scaled by a power of two to integers rint(x 2^(P-e))
+ * (e: the vector's exponent, max |x| < 2^e); their exact integer product is
+ * formed from residues modulo `moduli` (8..22) pairwise-coprime integers
+ * <= 256 (one INT8 GEMM per modulus on the matrix cores, exact in INT32) by
+ * the Chinese remainder theorem, then rounded once to FP64 and scaled back.
+ * Each vector's bit count P comes from a bound on its 2-norm
+ * (S = sum ceil(|x| 2^(10-e))^2, the largest P <= 77 with
+ * 2P + bitlen(S - 1) <= floor(log2 M) + 19, M the product of the moduli), so
+ * that every integer dot product stays inside the CRT range by
+ * Cauchy-Schwarz: vectors whose mass sits in few entries keep up to
+ * log2(sqrt(k)) more bits.  dgemm_x_ozaki2_bits gives the smallest P over
+ * vectors of k entries (all at the top of their binade: 24 bits at 8
+ * moduli, 55 at 16, 63 at 18 for k = 8192; -1 when unsupported); entries
+ * smaller than 2^(e-P) are lost.  `work` as for dgemm_x_ozaki_dev, of at
+ * least dgemm_x_ozaki2_workspace() bytes. */
 size_t dgemm_x_ozaki2_workspace(int m, int n, int k, int moduli);
 int dgemm_x_ozaki2_bits(int moduli, int k);
 int dgemm_x_ozaki2_dev(char transa, char transb, int m, int n, int k,

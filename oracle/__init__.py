@@ -90,6 +90,9 @@ def lib():
     L.oracle_ozaki2_modulus.argtypes = [ctypes.c_int]
     L.oracle_ozaki2_bits.restype = ctypes.c_int
     L.oracle_ozaki2_bits.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.oracle_ozaki2_vector_bits.restype = ctypes.c_int
+    L.oracle_ozaki2_vector_bits.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int]
     L.oracle_ozaki2_scaled.restype = None
     L.oracle_ozaki2_scaled.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_uint64),
@@ -196,8 +199,15 @@ def ozaki2_modulus(j):
     return int(lib().oracle_ozaki2_modulus(j))
 
 
-def ozaki2_bits(moduli, kc):
-    return int(lib().oracle_ozaki2_bits(moduli, kc))
+def ozaki2_bits(moduli, k):
+    """The smallest per-vector bit count over vectors of k entries."""
+    return int(lib().oracle_ozaki2_bits(moduli, k))
+
+
+def ozaki2_vector_bits(x, moduli):
+    """Bit count P of one vector (a 1-D float64 array)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return int(lib().oracle_ozaki2_vector_bits(_ptr(x), x.size, 1, moduli))
 
 
 def ozaki2_scaled(x, e, P):

@@ -485,14 +485,15 @@ void oracle_bf16x3_sgemm(char transa, char transb, int m, int n, int k,
 /* ---------------------------------------------------------------------- */
 /* Ozaki-II (CRT).  No reference counterpart: restates the emulated DGEMM  */
 /* of the library's modular form.  Per row i of op(A) the exponent e_i      */
-/* (oracle_ozaki_exponent) and the integers a_ip = rint(a * 2^(P - e_i)),   */
-/* |a_ip| <= 2^P; likewise b_pj per column j.  The GPU forms the exact      */
-/* integer X_ij = sum_p a_ip b_pj from residues modulo pairwise-coprime     */
-/* moduli (INT8 residue products on the matrix cores, then the Chinese      */
-/* remainder theorem); P is chosen so that |X| < M/2.  This restatement     */
+/* (oracle_ozaki_exponent), the bit count P_i (oracle_ozaki2_vector_bits)   */
+/* and the integers a_ip = rint(a * 2^(P_i - e_i)); likewise f_j, P_j and   */
+/* b_pj per column j.  The GPU forms the exact integer X_ij = sum_p a_ip    */
+/* b_pj from residues modulo pairwise-coprime moduli (INT8 residue products */
+/* on the matrix cores, then the Chinese remainder theorem); the bit counts */
+/* keep |X| < M/2 by the Cauchy-Schwarz inequality.  This restatement       */
 /* computes X directly with 256-bit integer arithmetic instead (no CRT), so */
-/* it checks the GPU's reconstruction independently.  The result is         */
-/* RNE(X) to 53 bits, scaled by 2^(e_i + f_j - 2P) with one ldexp.          */
+/* it checks the GPU's reconstruction independently.  The result is RNE(X)  */
+/* to 53 bits, scaled by 2^(e_i - P_i + f_j - P_j) with one ldexp.          */
 /* ---------------------------------------------------------------------- */
 typedef __int128 i128;
 typedef unsigned __int128 u128;
@@ -520,9 +521,8 @@ int oracle_ozaki2_modulus(int j) {
   return (j >= 0 && j < count) ? list[j] : -1;
 }
 
-int oracle_ozaki2_bits(int nmod, int kc) {
-  if (nmod < OZAKI2_MIN_MODULI || nmod > OZAKI2_MAX_MODULI || kc < 1) return -1;
-  /* L = floor(log2(M)), M = product of the first nmod moduli, exactly */
+/* floor(log2(M)), M the product of the first nmod moduli, exactly */
+static int ozaki2_log2m(int nmod) {
   uint32_t limb[8] = {0};
   limb[0] = 1;
   for (int j = 0; j < nmod; j++) {
@@ -533,18 +533,41 @@ int oracle_ozaki2_bits(int nmod, int kc) {
       carry = v >> 32;
     }
   }
-  int L = 0;
   for (int t = 7; t >= 0; t--)
-    if (limb[t]) {
-      L = 32 * t + 31 - __builtin_clz(limb[t]);
-      break;
-    }
-  int kbits = 0; /* bit length of kc: kc < 2^kbits */
-  while ((1L << kbits) <= kc) kbits++;
-  /* |X| <= kc 2^(2P) < 2^(kbits + 2P) <= 2^(L-1) <= M/2; at most
-   * OZAKI2_MAX_BITS, so that |a| < 2^(53 + 25) */
-  const int P = (L - 1 - kbits) / 2;
+    if (limb[t]) return 32 * t + 31 - __builtin_clz(limb[t]);
+  return 0;
+}
+
+/* bit length of s - 1 (0 for s <= 1): the smallest b with s <= 2^b */
+static int ceil_log2_u64(uint64_t s) { return s <= 1 ? 0 : 64 - __builtin_clzll(s - 1); }
+
+/* P for a vector whose bound sum is S (see oracle_ozaki2_vector_bits):
+ * the largest P with 2P + bitlen(S - 1) <= floor(log2 M) + 19, capped at
+ * OZAKI2_MAX_BITS.  Then |a_p| <= t_p 2^(P-10) (P >= 10) gives
+ * ||a|| <= 2^(P - 10) sqrt(S) <= 2^((L - 1) / 2) and, for two vectors,
+ * |sum_p a_p b_p| <= ||a|| ||b|| <= 2^(L-1) < M/2. */
+static int ozaki2_bits_for_sum(int L, uint64_t S) {
+  const int P = (L + 19 - ceil_log2_u64(S)) / 2;
   return P < OZAKI2_MAX_BITS ? P : OZAKI2_MAX_BITS;
+}
+
+int oracle_ozaki2_bits(int nmod, int k) {
+  if (nmod < OZAKI2_MIN_MODULI || nmod > OZAKI2_MAX_MODULI || k < 1) return -1;
+  /* every t_p = 1024 */
+  return ozaki2_bits_for_sum(ozaki2_log2m(nmod), (uint64_t)k << 20);
+}
+
+int oracle_ozaki2_vector_bits(const double *x, int count, int stride, int nmod) {
+  if (nmod < OZAKI2_MIN_MODULI || nmod > OZAKI2_MAX_MODULI) return -1;
+  const int e = oracle_ozaki_exponent(x, count, stride);
+  uint64_t S = 0;
+  if (e != OZAKI_NONFINITE)
+    for (int p = 0; p < count; p++) {
+      /* t = ceil(|x| 2^(10-e)) <= 1024: the scaling and ceil are exact */
+      const uint64_t t = (uint64_t)ceil(ldexp(fabs(x[(int64_t)p * stride]), 10 - e));
+      S += t * t;
+    }
+  return ozaki2_bits_for_sum(ozaki2_log2m(nmod), S);
 }
 
 static i128 ozaki2_scaled(double x, int e, int P) {
@@ -677,28 +700,33 @@ void oracle_ozaki2_dgemm(char transa, char transb, int m, int n, int k,
     return;
   }
   const int kc = k < OZAKI2_KCHUNK ? k : OZAKI2_KCHUNK;
-  const int P = oracle_ozaki2_bits(nmod, kc);
-  int *ea = (int *)malloc(sizeof(int) * m);
-  int *fb = (int *)malloc(sizeof(int) * n);
+  int *ea = (int *)malloc(sizeof(int) * m), *pa = (int *)malloc(sizeof(int) * m);
+  int *fb = (int *)malloc(sizeof(int) * n), *pb = (int *)malloc(sizeof(int) * n);
   i128 *ia = (i128 *)malloc(sizeof(i128) * (size_t)m * k); /* [i][p] */
   i128 *ib = (i128 *)malloc(sizeof(i128) * (size_t)n * k); /* [j][p] */
-  for (int i = 0; i < m; i++)
-    ea[i] = na ? oracle_ozaki_exponent(a + i, k, lda)
-               : oracle_ozaki_exponent(a + (int64_t)i * lda, k, 1);
-  for (int j = 0; j < n; j++)
-    fb[j] = nb ? oracle_ozaki_exponent(b + (int64_t)j * ldb, k, 1)
-               : oracle_ozaki_exponent(b + j, k, ldb);
+  for (int i = 0; i < m; i++) {
+    const double *row = na ? a + i : a + (int64_t)i * lda;
+    const int inc = na ? lda : 1;
+    ea[i] = oracle_ozaki_exponent(row, k, inc);
+    pa[i] = oracle_ozaki2_vector_bits(row, k, inc, nmod);
+  }
+  for (int j = 0; j < n; j++) {
+    const double *col = nb ? b + (int64_t)j * ldb : b + j;
+    const int inc = nb ? 1 : ldb;
+    fb[j] = oracle_ozaki_exponent(col, k, inc);
+    pb[j] = oracle_ozaki2_vector_bits(col, k, inc, nmod);
+  }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < m; i++)
     for (int p = 0; p < k; p++) {
       const double v = na ? a[i + (int64_t)p * lda] : a[p + (int64_t)i * lda];
-      ia[(size_t)i * k + p] = ea[i] == OZAKI_NONFINITE ? 0 : ozaki2_scaled(v, ea[i], P);
+      ia[(size_t)i * k + p] = ea[i] == OZAKI_NONFINITE ? 0 : ozaki2_scaled(v, ea[i], pa[i]);
     }
 #pragma omp parallel for schedule(static)
   for (int j = 0; j < n; j++)
     for (int p = 0; p < k; p++) {
       const double v = nb ? b[p + (int64_t)j * ldb] : b[j + (int64_t)p * ldb];
-      ib[(size_t)j * k + p] = fb[j] == OZAKI_NONFINITE ? 0 : ozaki2_scaled(v, fb[j], P);
+      ib[(size_t)j * k + p] = fb[j] == OZAKI_NONFINITE ? 0 : ozaki2_scaled(v, fb[j], pb[j]);
     }
   for (int k0 = 0; k0 < k; k0 += kc) {
     const int k1 = (k0 + kc < k) ? k0 + kc : k;
@@ -714,13 +742,15 @@ void oracle_ozaki2_dgemm(char transa, char transb, int m, int n, int k,
         if (ea[i] == OZAKI_NONFINITE || fb[j] == OZAKI_NONFINITE)
           scaled = NAN;
         else
-          scaled = oracle_ozaki2_round(acc, ea[i] + fb[j] - 2 * P);
+          scaled = oracle_ozaki2_round(acc, ea[i] - pa[i] + fb[j] - pb[j]);
         *cij = fma(alpha, scaled, c1);
       }
     }
   }
   free(ea);
+  free(pa);
   free(fb);
+  free(pb);
   free(ia);
   free(ib);
 }

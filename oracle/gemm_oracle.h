@@ -110,15 +110,22 @@ void oracle_ozaki_dgemm(char transa, char transb, int m, int n, int k,
 #define OZAKI2_MIN_MODULI 8
 #define OZAKI2_MAX_MODULI 22
 #define OZAKI2_KCHUNK 65536 /* K chunk: kc * 128^2 < 2^31 */
-#define OZAKI2_MAX_BITS 77   /* cap on P */
+#define OZAKI2_MAX_BITS 77   /* cap on P: scaled integers below 2^78 */
 /* The j-th modulus: pairwise-coprime integers taken greedily downward from
  * 256 (256, 255, 253, 251, 247, ...); -1 past the end. */
 int oracle_ozaki2_modulus(int j);
-/* Bits P of the scaled integers for nmod moduli and K chunk kc: the largest
- * P with bitlen(kc) + 2P + 1 <= floor(log2(M)), M the product of the moduli,
- * capped at OZAKI2_MAX_BITS; -1 for nmod outside
+/* Bits P of the scaled integers of a vector of `count` entries (stride
+ * `stride`) for nmod moduli: with e its exponent (oracle_ozaki_exponent),
+ * t_p = ceil(|x_p| 2^(10-e)) <= 1024 and S = sum_p t_p^2 (exact), the largest
+ * P with 2P + bitlen(S - 1) <= floor(log2(M)) + 19, M the product of the
+ * moduli, capped at OZAKI2_MAX_BITS.  Any two vectors' integers then have
+ * |dot| <= 2^(floor(log2 M) - 1) < M/2 (Cauchy-Schwarz); S and P do not
+ * depend on the summation order.  -1 for nmod outside
  * [OZAKI2_MIN_MODULI, OZAKI2_MAX_MODULI]. */
-int oracle_ozaki2_bits(int nmod, int kc);
+int oracle_ozaki2_vector_bits(const double *x, int count, int stride, int nmod);
+/* The smallest vector bit count over all vectors of k entries (every
+ * t_p = 1024), -1 as above or for k < 1. */
+int oracle_ozaki2_bits(int nmod, int k);
 /* rint(x * 2^(P-e)) (ties to even) as a signed 128-bit integer (lo, hi). */
 void oracle_ozaki2_scaled(double x, int e, int P, uint64_t *lo, int64_t *hi);
 /* A signed 256-bit integer (little-endian two's complement 64-bit limbs)

@@ -126,12 +126,13 @@ def test_batch_checks_arguments_before_the_device(dtype):
 
 
 def test_ozaki2_bits_match_the_oracle():
-    """The library's integer width P for Ozaki-II (host arithmetic, no
-    device) equals the oracle's for every modulus count and chunk length."""
+    """The library's smallest Ozaki-II integer width P over vectors of k
+    entries (host arithmetic, no device) equals the oracle's for every
+    modulus count and length."""
     import oracle
     for nmod in range(6, 25):
-        for k in (1, 100, 8191, 8192, 65535, 65536, 65537, 10 ** 6):
-            want = oracle.ozaki2_bits(nmod, min(k, 65536))
+        for k in (0, 1, 100, 8191, 8192, 65535, 65536, 65537, 10 ** 6):
+            want = oracle.ozaki2_bits(nmod, k)
             assert gx.dgemm_ozaki2_bits(nmod, k) == want, (nmod, k)
     assert gx.dgemm_ozaki2_workspace(64, 64, 64, 7) == 0
     assert gx.dgemm_ozaki2_workspace(64, 64, 64, 20) > 20 * 64 * 128 * 2
