@@ -23,7 +23,8 @@ import ctypes
 import os
 
 __all__ = [
-    "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm_dev", "sgemm_dev",
+    "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm64", "sgemm64", "dgemm_dev",
+    "sgemm_dev", "dgemm_strided_batched_dev", "sgemm_strided_batched_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
@@ -46,6 +47,7 @@ class GemmError(RuntimeError):
 _libs: dict = {}
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_i64_p = ctypes.POINTER(ctypes.c_int64)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 
@@ -59,11 +61,22 @@ def _declare(lib, dtype):
     f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p, flp,
                   ctypes.c_void_p, _c_int_p, ctypes.c_void_p, _c_int_p, flp, ctypes.c_void_p,
                   _c_int_p]
+    f = getattr(lib, f"{p}gemm_64_")
+    f.restype = None
+    f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_i64_p, _c_i64_p, _c_i64_p, flp,
+                  ctypes.c_void_p, _c_i64_p, ctypes.c_void_p, _c_i64_p, flp, ctypes.c_void_p,
+                  _c_i64_p]
     f = getattr(lib, f"{p}gemm_x_dev")
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, fl,
                   ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, fl,
                   ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    f = getattr(lib, f"{p}gemm_x_strided_batched_dev")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, fl,
+                  ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_int64, fl, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                  ctypes.c_int, ctypes.c_void_p]
     f = getattr(lib, f"{p}gemm_x_last_error")
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.POINTER(ctypes.c_char_p)]
@@ -211,8 +224,56 @@ def sgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=Tr
     _fortran("s", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
 
 
+def _fortran64(dtype, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check):
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    fl = ctypes.c_double if dtype == "d" else ctypes.c_float
+    ta = transa.encode() if isinstance(transa, str) else bytes(transa)
+    tb = transb.encode() if isinstance(transb, str) else bytes(transb)
+    I = ctypes.c_int64
+    getattr(lib, f"{dtype}gemm_64_")(ta, tb, ctypes.byref(I(m)), ctypes.byref(I(n)),
+                                      ctypes.byref(I(k)), ctypes.byref(fl(alpha)), _addr(a),
+                                      ctypes.byref(I(lda)), _addr(b), ctypes.byref(I(ldb)),
+                                      ctypes.byref(fl(beta)), _addr(c), ctypes.byref(I(ldc)))
+    if check:
+        _check(dtype)
+
+
+def dgemm64(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    """dgemm_64_: dgemm_ with 64-bit integer arguments (ILP64)."""
+    _fortran64("d", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def sgemm64(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    """sgemm_64_: sgemm_ with 64-bit integer arguments (ILP64)."""
+    _fortran64("s", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
 def _ch(t):
     return (t.encode() if isinstance(t, str) else bytes(t))[:1]
+
+
+def _strided(dtype, transa, transb, m, n, k, alpha, a, lda, stride_a, b, ldb, stride_b, beta,
+             c, ldc, stride_c, batch, stream):
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    rc = getattr(lib, f"{dtype}gemm_x_strided_batched_dev")(
+        _ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda, stride_a, _addr(b), ldb,
+        stride_b, beta, _addr(c), ldc, stride_c, batch, _stream(stream))
+    if rc:
+        _check(dtype, rc)
+
+
+def dgemm_strided_batched_dev(transa, transb, m, n, k, alpha, a, lda, stride_a, b, ldb,
+                              stride_b, beta, c, ldc, stride_c, batch, stream=None):
+    """`batch` device-pointer DGEMMs of one shape, operands `stride_*`
+    elements apart, enqueued on `stream`."""
+    _strided("d", transa, transb, m, n, k, alpha, a, lda, stride_a, b, ldb, stride_b, beta, c,
+             ldc, stride_c, batch, stream)
+
+
+def sgemm_strided_batched_dev(transa, transb, m, n, k, alpha, a, lda, stride_a, b, ldb,
+                              stride_b, beta, c, ldc, stride_c, batch, stream=None):
+    _strided("s", transa, transb, m, n, k, alpha, a, lda, stride_a, b, ldb, stride_b, beta, c,
+             ldc, stride_c, batch, stream)
 
 
 def dgemm_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):

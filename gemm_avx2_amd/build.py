@@ -1,8 +1,8 @@
 """Builds the gfx950 libraries of this package in-tree.
 
 Outputs (gemm_avx2_amd/lib/):
-  DGEMM.so         dgemm_ and the dgemm_x_* extensions
-  SGEMM.so         sgemm_ and the sgemm_x_* extensions
+  DGEMM.so         dgemm_, its ILP64 form dgemm_64_ and the dgemm_x_* extensions
+  SGEMM.so         sgemm_, sgemm_64_ and the sgemm_x_* extensions
   libgemmx_gen.so  device-side input generator used by bench.py and the tests
   gemm_mi355x_tune.txt  the shipped gfx950 tuning table (tuning/), if present
 
@@ -47,8 +47,8 @@ CC_FLAGS = [
     "-Wno-unused-parameter", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
 ]
 EXPORTS = {
-    "DGEMM.so": ["dgemm_", "dgemm_x_*"],
-    "SGEMM.so": ["sgemm_", "sgemm_x_*"],
+    "DGEMM.so": ["dgemm_", "dgemm_64_", "dgemm_x_*"],
+    "SGEMM.so": ["sgemm_", "sgemm_64_", "sgemm_x_*"],
 }
 
 

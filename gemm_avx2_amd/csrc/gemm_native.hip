@@ -26,6 +26,10 @@
  * the slice; where LDS is K-contiguous, consecutive steps are read with one
  * 16-byte LDS read.
  *
+ * A strided batch of problems of one shape runs in the same launches: grid
+ * row blockIdx.y takes problem (batch base + blockIdx.y), its operands offset
+ * by the per-problem strides.
+ *
  * Workgroups whose tile lies inside C and whose operands allow 16-byte loads
  * run a branch-free loop over the full K slices (the K remainder, if any, is
  * one peeled predicated slice); edge tiles run a predicated loop.  In the
@@ -54,8 +58,12 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW)
 gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
                    T *__restrict__ C, int M, int N, int K, int64_t lda,
                    int64_t ldb, int64_t ldc, T alpha, T beta, int tiles_m,
-                   int tiles_n, int vecA, int vecB, int edge_all) {
+                   int tiles_n, int vecA, int vecB, int edge_all, int64_t sa, int64_t sb,
+                   int64_t sc) {
   __shared__ __align__(16) T lds[tile_lds_elems<T, TA, TB, BM, BN, BK>()];
+  A += blockIdx.y * sa;
+  B += blockIdx.y * sb;
+  C += blockIdx.y * sc;
   int tm, tn;
   if (FAST || edge_all) {
     tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, &tm, &tn);
@@ -76,7 +84,8 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-scale_kernel(T *__restrict__ C, int64_t ldc, int M, int N, T beta) {
+scale_kernel(T *__restrict__ C, int64_t ldc, int M, int N, T beta, int64_t sc) {
+  C += blockIdx.y * sc;
   const int64_t total = (int64_t)M * N;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -86,33 +95,47 @@ scale_kernel(T *__restrict__ C, int64_t ldc, int M, int N, T beta) {
 }
 
 /* Interior tiles go to the FAST kernel, the rest to the predicated one; both
- * launches are on stream s and touch disjoint tiles of C. */
+ * launches are on stream s and touch disjoint tiles of C.  A strided batch
+ * goes in grid rows of at most GEMMX_GRID_Y problems per launch. */
+constexpr int GEMMX_GRID_Y = 65535;
+
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
 hipError_t launch_trans(const gemmx_problem *p, hipStream_t s) {
   constexpr int VE = Vec16<T>::N;
-  const int vecA = ((uintptr_t)p->a % 16 == 0) && (p->lda % VE == 0);
-  const int vecB = ((uintptr_t)p->b % 16 == 0) && (p->ldb % VE == 0);
-  const T *A = (const T *)p->a, *B = (const T *)p->b;
-  T *C = (T *)p->c;
+  const int batch = p->batch > 1 ? p->batch : 1;
+  const int64_t sa = batch > 1 ? p->sa : 0, sb = batch > 1 ? p->sb : 0;
+  const int64_t sc = batch > 1 ? p->sc : 0;
+  const int vecA = ((uintptr_t)p->a % 16 == 0) && (p->lda % VE == 0) && (sa % VE == 0);
+  const int vecB = ((uintptr_t)p->b % 16 == 0) && (p->ldb % VE == 0) && (sb % VE == 0);
   const T alpha = (T)p->alpha, beta = (T)p->beta;
   const dim3 block(64 * WM * WN);
   const int tiles_m = (p->m + BM - 1) / BM, tiles_n = (p->n + BN - 1) / BN;
   const int full_m = p->m / BM, full_n = p->n / BN;
-  if (vecA && vecB && full_m > 0 && full_n > 0) {
-    hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
-                       dim3(full_m * full_n), block, 0, s, A, B, C, p->m, p->n, p->k, p->lda,
-                       p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0);
-    const int edges = tiles_m * tiles_n - full_m * full_n;
-    if (edges > 0)
+  for (int b0 = 0; b0 < batch; b0 += GEMMX_GRID_Y) {
+    const unsigned nb = (unsigned)(batch - b0 < GEMMX_GRID_Y ? batch - b0 : GEMMX_GRID_Y);
+    const T *A = (const T *)p->a + b0 * sa, *B = (const T *)p->b + b0 * sb;
+    T *C = (T *)p->c + b0 * sc;
+    if (vecA && vecB && full_m > 0 && full_n > 0) {
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
+                         dim3(full_m * full_n, nb), block, 0, s, A, B, C, p->m, p->n, p->k,
+                         p->lda, p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0,
+                         sa, sb, sc);
+      const int edges = tiles_m * tiles_n - full_m * full_n;
+      if (edges > 0)
+        hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
+                           dim3(edges, nb), block, 0, s, A, B, C, p->m, p->n, p->k, p->lda,
+                           p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0, sa, sb,
+                           sc);
+    } else {
       hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
-                         dim3(edges), block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb,
-                         p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0);
-  } else {
-    hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
-                       dim3(tiles_m * tiles_n), block, 0, s, A, B, C, p->m, p->n, p->k,
-                       p->lda, p->ldb, p->ldc, alpha, beta, tiles_m, tiles_n, vecA, vecB, 1);
+                         dim3(tiles_m * tiles_n, nb), block, 0, s, A, B, C, p->m, p->n, p->k,
+                         p->lda, p->ldb, p->ldc, alpha, beta, tiles_m, tiles_n, vecA, vecB, 1,
+                         sa, sb, sc);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
@@ -139,14 +162,21 @@ hipError_t launch_native(const gemmx_problem *p, int cfg, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t launch_scale(T *c, int64_t ldc, int m, int n, T beta, hipStream_t s) {
+hipError_t launch_scale(T *c, int64_t ldc, int m, int n, T beta, int batch, int64_t sc,
+                        hipStream_t s) {
   const int64_t total = (int64_t)m * n;
   int64_t blocks = (total + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   if (blocks < 1) return hipSuccess;
-  hipLaunchKernelGGL((scale_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s,
-                     c, ldc, m, n, beta);
-  return hipGetLastError();
+  if (batch < 1) batch = 1;
+  for (int b0 = 0; b0 < batch; b0 += GEMMX_GRID_Y) {
+    const unsigned nb = (unsigned)(batch - b0 < GEMMX_GRID_Y ? batch - b0 : GEMMX_GRID_Y);
+    hipLaunchKernelGGL((scale_kernel<T>), dim3((unsigned)blocks, nb), dim3(256), 0, s,
+                       c + b0 * sc, ldc, m, n, beta, batch > 1 ? sc : (int64_t)0);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 } // namespace
@@ -158,10 +188,10 @@ extern "C" hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, hipStrea
   return launch_native<float>(p, cfg, s);
 }
 extern "C" hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n,
-                                      double beta, hipStream_t s) {
-  return launch_scale<double>(c, ldc, m, n, beta, s);
+                                      double beta, int batch, int64_t sc, hipStream_t s) {
+  return launch_scale<double>(c, ldc, m, n, beta, batch, sc, s);
 }
 extern "C" hipError_t gemmx_scale_f32(float *c, int64_t ldc, int m, int n,
-                                      float beta, hipStream_t s) {
-  return launch_scale<float>(c, ldc, m, n, beta, s);
+                                      float beta, int batch, int64_t sc, hipStream_t s) {
+  return launch_scale<float>(c, ldc, m, n, beta, batch, sc, s);
 }

@@ -14,6 +14,7 @@
  * (native / emulated), and the device-pointer extension entry points.
  */
 #define _GNU_SOURCE
+#include <limits.h>
 #include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -218,7 +219,8 @@ static int notrans(char c) { return c == 'N' || c == 'n'; }
 typedef struct {
   int es; /* element size: 8 or 4 */
   char ta, tb;
-  int m, n, k, lda, ldb, ldc;
+  int m, n, k;
+  int64_t lda, ldb, ldc;
   double alpha, beta;
   const void *a, *b;
   void *c;
@@ -231,13 +233,13 @@ static int validate(const call_t *x) {
     return set_error(GEMMX_ERR_ARG, "negative dimension (m=%d n=%d k=%d)", x->m, x->n, x->k);
   if (x->m == 0 || x->n == 0) return GEMMX_OK;
   if (x->ldc < (x->m > 1 ? x->m : 1))
-    return set_error(GEMMX_ERR_ARG, "ldc=%d < max(1,m=%d)", x->ldc, x->m);
+    return set_error(GEMMX_ERR_ARG, "ldc=%lld < max(1,m=%d)", (long long)x->ldc, x->m);
   if (!x->c) return set_error(GEMMX_ERR_ARG, "C is NULL");
   if (x->alpha != 0.0 && x->k > 0) {
     if (x->lda < (ra > 1 ? ra : 1))
-      return set_error(GEMMX_ERR_ARG, "lda=%d < max(1,%d)", x->lda, ra);
+      return set_error(GEMMX_ERR_ARG, "lda=%lld < max(1,%d)", (long long)x->lda, ra);
     if (x->ldb < (rb > 1 ? rb : 1))
-      return set_error(GEMMX_ERR_ARG, "ldb=%d < max(1,%d)", x->ldb, rb);
+      return set_error(GEMMX_ERR_ARG, "ldb=%lld < max(1,%d)", (long long)x->ldb, rb);
     if (!x->a || !x->b) return set_error(GEMMX_ERR_ARG, "A or B is NULL");
   }
   return GEMMX_OK;
@@ -251,8 +253,10 @@ static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
   hipError_t e;
   if (p->alpha == 0.0 || p->k == 0) {
     if (p->beta == 1.0) return GEMMX_OK;
-    e = es == 8 ? gemmx_scale_f64((double *)p->c, p->ldc, p->m, p->n, p->beta, s)
-                : gemmx_scale_f32((float *)p->c, p->ldc, p->m, p->n, (float)p->beta, s);
+    e = es == 8 ? gemmx_scale_f64((double *)p->c, p->ldc, p->m, p->n, p->beta, p->batch,
+                                  p->sc, s)
+                : gemmx_scale_f32((float *)p->c, p->ldc, p->m, p->n, (float)p->beta, p->batch,
+                                  p->sc, s);
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "scale kernel launch");
   }
   if (es == 8 && mode == GEMMX_MODE_OZAKI) {
@@ -395,6 +399,53 @@ EXPORT void sgemm_(const char *transa, const char *transb, const int *m,
   fortran_call(&x, g_smode, 0);
 }
 
+/* ILP64 forms: the same calls with 64-bit Fortran integers (callers built
+ * with -fdefault-integer-8 and the "_64_" suffix of ILP64 BLAS builds).
+ * Leading dimensions are used at full width; m, n and k must stay below
+ * 2^31 (the kernels' tile grid). */
+static int ilp64_dims(call_t *x, const int64_t *m, const int64_t *n, const int64_t *k) {
+  if (*m > INT_MAX || *n > INT_MAX || *k > INT_MAX || *m < INT_MIN || *n < INT_MIN ||
+      *k < INT_MIN)
+    return set_error(GEMMX_ERR_ARG, "dimension beyond 2^31-1 (m=%lld n=%lld k=%lld)",
+                     (long long)*m, (long long)*n, (long long)*k);
+  x->m = (int)*m;
+  x->n = (int)*n;
+  x->k = (int)*k;
+  return GEMMX_OK;
+}
+
+EXPORT void dgemm_64_(const char *transa, const char *transb, const int64_t *m,
+                      const int64_t *n, const int64_t *k, const double *alpha,
+                      const double *a, const int64_t *lda, const double *b,
+                      const int64_t *ldb, const double *beta, double *c,
+                      const int64_t *ldc) {
+  clear_error();
+  if (!transa || !transb || !m || !n || !k || !alpha || !lda || !ldb || !beta || !ldc) {
+    set_error(GEMMX_ERR_ARG, "dgemm_64_: NULL scalar argument");
+    return;
+  }
+  call_t x = {8, *transa, *transb, 0, 0, 0, *lda, *ldb, *ldc, *alpha, *beta, a, b, c};
+  if (ilp64_dims(&x, m, n, k) != GEMMX_OK) return;
+  pthread_once(&g_once, init_once);
+  fortran_call(&x, g_dmode, g_dslices);
+}
+
+EXPORT void sgemm_64_(const char *transa, const char *transb, const int64_t *m,
+                      const int64_t *n, const int64_t *k, const float *alpha,
+                      const float *a, const int64_t *lda, const float *b,
+                      const int64_t *ldb, const float *beta, float *c,
+                      const int64_t *ldc) {
+  clear_error();
+  if (!transa || !transb || !m || !n || !k || !alpha || !lda || !ldb || !beta || !ldc) {
+    set_error(GEMMX_ERR_ARG, "sgemm_64_: NULL scalar argument");
+    return;
+  }
+  call_t x = {4, *transa, *transb, 0, 0, 0, *lda, *ldb, *ldc, *alpha, *beta, a, b, c};
+  if (ilp64_dims(&x, m, n, k) != GEMMX_OK) return;
+  pthread_once(&g_once, init_once);
+  fortran_call(&x, g_smode, 0);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Extensions.                                                              */
 /* ------------------------------------------------------------------------ */
@@ -523,6 +574,48 @@ EXPORT int sgemm_x_dev(char transa, char transb, int m, int n, int k,
                        int ldb, float beta, float *c, int ldc, void *stream) {
   call_t x = {4, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
   return dev_call(&x, GEMMX_MODE_NATIVE, 0, NULL, 0, stream);
+}
+
+/* Strided batch on device pointers, native arithmetic: `batch` problems of
+ * one shape in one set of launches (grid row per problem). */
+static int strided_call(call_t *x, int64_t sa, int64_t sb, int64_t sc, int batch,
+                        void *stream) {
+  clear_error();
+  if (batch < 0) return set_error(GEMMX_ERR_ARG, "batch count %d < 0", batch);
+  if (validate(x) != GEMMX_OK) return t_err;
+  if (batch == 0 || x->m == 0 || x->n == 0) return GEMMX_OK;
+  if (sa < 0 || sb < 0 || sc < 0)
+    return set_error(GEMMX_ERR_ARG, "negative stride (%lld, %lld, %lld)", (long long)sa,
+                     (long long)sb, (long long)sc);
+  if (batch > 1 && sc < x->ldc * (int64_t)x->n)
+    return set_error(GEMMX_ERR_ARG, "stride_c=%lld < ldc*n=%lld: the C matrices overlap",
+                     (long long)sc, (long long)(x->ldc * (int64_t)x->n));
+  int dev;
+  if (current_device(&dev) != GEMMX_OK) return t_err;
+  gemmx_problem p = {.ta = !notrans(x->ta), .tb = !notrans(x->tb), .m = x->m,
+                     .n = x->n, .k = x->k, .lda = x->lda, .ldb = x->ldb,
+                     .ldc = x->ldc, .alpha = x->alpha, .beta = x->beta,
+                     .a = x->a, .b = x->b, .c = x->c, .batch = batch,
+                     .sa = sa, .sb = sb, .sc = sc};
+  return launch_device(&p, x->es, GEMMX_MODE_NATIVE, 0, NULL, (hipStream_t)stream);
+}
+
+EXPORT int dgemm_x_strided_batched_dev(char transa, char transb, int m, int n, int k,
+                                       double alpha, const double *a, int lda,
+                                       int64_t stride_a, const double *b, int ldb,
+                                       int64_t stride_b, double beta, double *c, int ldc,
+                                       int64_t stride_c, int batch, void *stream) {
+  call_t x = {8, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return strided_call(&x, stride_a, stride_b, stride_c, batch, stream);
+}
+
+EXPORT int sgemm_x_strided_batched_dev(char transa, char transb, int m, int n, int k,
+                                       float alpha, const float *a, int lda,
+                                       int64_t stride_a, const float *b, int ldb,
+                                       int64_t stride_b, float beta, float *c, int ldc,
+                                       int64_t stride_c, int batch, void *stream) {
+  call_t x = {4, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return strided_call(&x, stride_a, stride_b, stride_c, batch, stream);
 }
 
 EXPORT size_t dgemm_x_ozaki_workspace(int m, int n, int k, int slices) {

@@ -31,6 +31,11 @@ typedef struct gemmx_problem {
   const void *a;
   const void *b;
   void *c;
+  /* Strided batch (native arithmetic only): `batch` problems of this shape,
+   * problem i at a + i*sa, b + i*sb, c + i*sc (elements); batch 0 or 1 is
+   * a single problem. */
+  int batch;
+  int64_t sa, sb, sc;
 } gemmx_problem;
 
 /* Kernel configurations of the native MFMA tile kernels.  The tuning table
@@ -69,11 +74,11 @@ size_t gemmx_stage_chunk_bytes(void);
 hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, hipStream_t s);
 hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, hipStream_t s);
 /* C[0:m,0:n] *= beta (beta != 1), the reference's C_MULT_BETA
- * (src/gemm_driver.c:106-116). */
-hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta,
-                           hipStream_t s);
-hipError_t gemmx_scale_f32(float *c, int64_t ldc, int m, int n, float beta,
-                           hipStream_t s);
+ * (src/gemm_driver.c:106-116), for `batch` matrices sc elements apart. */
+hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta, int batch,
+                           int64_t sc, hipStream_t s);
+hipError_t gemmx_scale_f32(float *c, int64_t ldc, int m, int n, float beta, int batch,
+                           int64_t sc, hipStream_t s);
 
 /* Grouped batch (gemm_batch.hip): one problem of a persistent grouped
  * launch.  Problems are stored heaviest first; tiles of problem p are

@@ -14,6 +14,7 @@
 #define GEMM_MI355X_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -48,6 +49,23 @@ void sgemm_(const char *transa, const char *transb, const int *m, const int *n,
             const int *k, const float *alpha, const float *a, const int *lda,
             const float *b, const int *ldb, const float *beta, float *c,
             const int *ldc);
+
+/* ILP64 forms of the same entry points (no counterpart in the reference,
+ * which is LP64 only: src/gemm_driver.c:118 takes int): every integer
+ * argument is a 64-bit integer, as for Fortran callers built with 64-bit
+ * default integers against an ILP64 BLAS ("_64_" symbol suffix).  Same
+ * semantics as dgemm_/sgemm_; leading dimensions are used at full width,
+ * and m, n, k above 2^31 - 1 are rejected (GEMMX_ERR_ARG, C untouched). */
+void dgemm_64_(const char *transa, const char *transb, const int64_t *m,
+               const int64_t *n, const int64_t *k, const double *alpha,
+               const double *a, const int64_t *lda, const double *b,
+               const int64_t *ldb, const double *beta, double *c,
+               const int64_t *ldc);
+void sgemm_64_(const char *transa, const char *transb, const int64_t *m,
+               const int64_t *n, const int64_t *k, const float *alpha,
+               const float *a, const int64_t *lda, const float *b,
+               const int64_t *ldb, const float *beta, float *c,
+               const int64_t *ldc);
 
 /* ------------------------------------------------------------------------
  * Extensions (no reference counterpart).
@@ -118,6 +136,25 @@ int dgemm_x_dev(char transa, char transb, int m, int n, int k, double alpha,
 int sgemm_x_dev(char transa, char transb, int m, int n, int k, float alpha,
                 const float *a, int lda, const float *b, int ldb, float beta,
                 float *c, int ldc, void *stream);
+
+/* Strided batch on device pointers (no counterpart in the reference; the
+ * BLAS-style batched interface): `batch` problems of one shape, problem i
+ * reading A + i*stride_a and B + i*stride_b and updating C + i*stride_c
+ * (strides in elements, >= 0; 0 shares one A or B across the batch;
+ * stride_c >= ldc*n so that no two C overlap), each with the semantics of
+ * dgemm_x_dev, native FP64/FP32 MFMA arithmetic, all in the launches of one
+ * problem (a grid row per problem).  Enqueued on `stream`; returns a
+ * GEMMX_* code. */
+int dgemm_x_strided_batched_dev(char transa, char transb, int m, int n, int k,
+                                double alpha, const double *a, int lda, int64_t stride_a,
+                                const double *b, int ldb, int64_t stride_b, double beta,
+                                double *c, int ldc, int64_t stride_c, int batch,
+                                void *stream);
+int sgemm_x_strided_batched_dev(char transa, char transb, int m, int n, int k,
+                                float alpha, const float *a, int lda, int64_t stride_a,
+                                const float *b, int ldb, int64_t stride_b, float beta,
+                                float *c, int ldc, int64_t stride_c, int batch,
+                                void *stream);
 
 /* Emulated DGEMM on device pointers: Ozaki-I with `slices` signed 7-bit INT8
  * slices per operand (3..8), exact INT32 accumulation on INT8 MFMA, FP64 only

@@ -51,6 +51,35 @@ def test_libraries_load():
 
 
 @pytest.mark.parametrize("dtype", ["d", "s"])
+def test_ilp64_and_strided_batch_argument_checks(dtype):
+    """The ILP64 form rejects dimensions beyond 2^31 - 1 and checks like
+    dgemm_; the strided batch rejects a negative count or stride and C
+    matrices that overlap.  All before any device work (no GPU needed)."""
+    dt = np.float64 if dtype == "d" else np.float32
+    f64 = gx.dgemm64 if dtype == "d" else gx.sgemm64
+    A = np.ones((4, 4), dtype=dt, order="F")
+    C = np.full((4, 4), 7.0, dtype=dt, order="F")
+    for args, word in [((2 ** 31, 4, 4, 1.0, A, 4, A, 4, 0.0, C, 4), "beyond"),
+                       ((4, 4, -2 ** 40, 1.0, A, 4, A, 4, 0.0, C, 4), "beyond"),
+                       ((4, 4, 4, 1.0, A, 3, A, 4, 0.0, C, 4), "lda"),
+                       ((-1, 4, 4, 1.0, A, 4, A, 4, 0.0, C, 4), "negative")]:
+        f64("N", "N", *args, check=False)
+        code, msg = gx.last_error(dtype)
+        assert code == gx.ERR_ARG and word in msg, (code, msg)
+        np.testing.assert_array_equal(C, 7.0)
+    sb = gx.dgemm_strided_batched_dev if dtype == "d" else gx.sgemm_strided_batched_dev
+    fake = 1 << 20  # never dereferenced: every call below fails its checks
+    for args, word in [((4, 4, 4, 1.0, fake, 4, 16, fake, 4, 16, 0.0, fake, 4, 16, -1), "batch"),
+                       ((4, 4, 4, 1.0, fake, 4, 16, fake, 4, 16, 0.0, fake, 4, 15, 2), "overlap"),
+                       ((4, 4, 4, 1.0, fake, 4, -16, fake, 4, 16, 0.0, fake, 4, 16, 2),
+                        "negative stride"),
+                       ((4, 4, 4, 1.0, fake, 3, 16, fake, 4, 16, 0.0, fake, 4, 16, 2), "lda")]:
+        with pytest.raises(gx.GemmError) as ei:
+            sb("N", "N", *args)
+        assert ei.value.code == gx.ERR_ARG and word in str(ei.value), str(ei.value)
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
 def test_invalid_arguments_leave_c_untouched(dtype):
     dt = np.float64 if dtype == "d" else np.float32
     f = gx.dgemm if dtype == "d" else gx.sgemm
