@@ -449,6 +449,53 @@ def run_batch(torch, gx, stream, steps, warmup):
     return out
 
 
+STRIDED = ((1024, 256, 256, 256), (64, 1024, 1024, 1024), (16384, 64, 64, 64))
+
+
+def run_strided(torch, gx, stream, steps, warmup):
+    """Uniform batches (count, m = n = k) in one strided-batched call, NN,
+    alpha 1, beta 0, ld = m, operands back to back; against the grouped plan
+    over the same problems and a loop of single device calls."""
+    out = {}
+    for d, key in (("d", "f64"), ("s", "f32")):
+        tdt = torch.float64 if d == "d" else torch.float32
+        es = 8 if d == "d" else 4
+        sb = gx.dgemm_strided_batched_dev if d == "d" else gx.sgemm_strided_batched_dev
+        single = gx.dgemm_dev if d == "d" else gx.sgemm_dev
+        res = {}
+        for (count, m, n, k) in STRIDED:
+            A = torch.empty(count * m * k, dtype=tdt, device="cuda")
+            B = torch.empty(count * k * n, dtype=tdt, device="cuda")
+            C = torch.empty(count * m * n, dtype=tdt, device="cuda")
+            for j, t in enumerate((A, B, C)):
+                gx.fill(t, t.numel(), 1, t.numel(), SEED, j, stream=stream)
+            flops = 2.0 * count * m * n * k
+
+            def strided():
+                sb("N", "N", m, n, k, 1.0, A, m, m * k, B, k, k * n, 0.0, C, m, m * n, count,
+                   stream=stream)
+            specs = [("N", "N", m, n, k, 1.0, A.data_ptr() + i * m * k * es, m,
+                      B.data_ptr() + i * k * n * es, k, 0.0, C.data_ptr() + i * m * n * es, m)
+                     for i in range(count)]
+            plan = gx.BatchPlan(d, specs)
+
+            def loop():
+                for sp in specs:
+                    single(*sp, stream=stream)
+            _, per_s = time_launches(torch, strided, steps, warmup)
+            _, per_p = time_launches(torch, lambda: plan.run(stream), steps, warmup)
+            _, per_l = time_launches(torch, loop, 2, 1)
+            res[f"{count}x{m}^3"] = {
+                "strided_tflops": round(flops / median(per_s) / 1e9, 2),
+                "strided_ms": round(median(per_s), 3),
+                "grouped_plan_tflops": round(flops / median(per_p) / 1e9, 2),
+                "single_call_loop_tflops": round(flops / median(per_l) / 1e9, 2)}
+            plan.destroy()
+            del A, B, C
+        out[key] = res
+    return out
+
+
 def run_host_pointer(torch, gx):
     """dgemm_ on host (pageable numpy) arrays at the headline shape: the wall
     clock of the whole call, staging copies to and from the GPU included."""
@@ -532,6 +579,8 @@ def main():
                         ("dgemm_1024", lambda: run_small(torch, gx, stream)),
                         ("sgemm_bf16x3", lambda: run_bf16x3(torch, gx, stream)),
                         ("batch", lambda: run_batch(torch, gx, stream, max(3, args.steps // 2), 1)),
+                        ("strided_batch", lambda: run_strided(torch, gx, stream,
+                                                              max(3, args.steps // 2), 1)),
                         ("dgemm_host_pointers", lambda: run_host_pointer(torch, gx))):
             result[key] = fn()
             torch.cuda.empty_cache()
@@ -571,8 +620,10 @@ def main():
                             "frac": round(ops / pm / 1e9 / PEAK["i8"], 4),
                             "traffic": profiled_traffic(
                                 f"dgemm_{info['scheme']}_{cnt}_nn_8192.md"),
-                            "kernel": "slice_products_kernel (INT8 MFMA), product phase "
-                                      "timed with HIP events in the library",
+                            "kernel": ("slice_products_dl_kernel (INT8 MFMA, direct-to-LDS)"
+                                       if info["scheme"] == "ozaki2" else
+                                       "slice_products_kernel (INT8 MFMA)")
+                                      + ", product phase timed with HIP events in the library",
                             "kernel_ms": round(pm, 4)}
     line.update(result)
     if not args.no_cpu:
