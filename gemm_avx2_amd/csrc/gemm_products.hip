@@ -81,6 +81,48 @@ __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, co
                                            int64_t ldc, float alpha, float beta, int M, int N,
                                            int r0, int c0, int li, int kg) {
   constexpr bool BF16 = KIND == SP_BF16;
+  if constexpr (KIND == SP_I8_MOD) {
+    /* Residue of each INT32 sum v: v = hi 2^16 + lo (hi signed, lo < 2^16),
+     * t = hi (2^16 mod m) + lo = v (mod m) with |t| < 2^23, exact in FP32;
+     * q = floor(t / m) from one FP32 reciprocal is off by at most one (the
+     * relative error of t * (1/m) is below 2^-22 and |t / m| < 2^16), and the
+     * remainder is corrected into [0, m).  All integer / FP32 work: the
+     * epilogue runs while the CU's matrix cores wait for the next tile. */
+    const int mod = G.gout[g];
+    const int c16 = 65536 % mod;
+    const float rcp = 1.0f / (float)mod;
+    const bool vec = (M & 3) == 0;
+    uint8_t *const sbase = (uint8_t *)S + (int64_t)g * M * N + r0 + kg * 4 + (int64_t)(c0 + li) * M;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      if (c0 + j * 16 + li >= N) continue;
+      uint8_t *const sj = sbase + (int64_t)(j * 16) * M;
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int row = r0 + i * 16 + kg * 4;
+        unsigned bytes = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int v = acc[i][j][r];
+          const int t = (v >> 16) * c16 + (v & 0xffff);
+          const int q = (int)floorf((float)t * rcp);
+          int rem = t - __mul24(q, mod);
+          rem += rem < 0 ? mod : 0;
+          rem -= rem >= mod ? mod : 0;
+          bytes |= (unsigned)rem << (8 * r);
+        }
+        uint8_t *sp = sj + i * 16;
+        if (vec && row + 3 < M) {
+          *reinterpret_cast<unsigned *>(sp) = bytes;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (row + r < M) sp[r] = (uint8_t)(bytes >> (8 * r));
+        }
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -93,29 +135,6 @@ __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, co
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (row + r < M) cp[r] = __builtin_fmaf(alpha, (float)acc[i][j][r], beta * cp[r]);
-      } else if constexpr (KIND == SP_I8_MOD) {
-        /* |sum| < 2^31; q = floor(sum / m) from FP64 (off by at most one,
-         * corrected below), so the stored residue is exact */
-        const int mod = G.gout[g];
-        const double rcp = 1.0 / (double)mod;
-        unsigned bytes = 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int v = acc[i][j][r];
-          int q = (int)floor((double)v * rcp);
-          int rem = v - q * mod;
-          rem += rem < 0 ? mod : 0;
-          rem -= rem >= mod ? mod : 0;
-          bytes |= (unsigned)rem << (8 * r);
-        }
-        uint8_t *sp = (uint8_t *)S + (int64_t)g * M * N + row + (int64_t)col * M;
-        if (row + 3 < M && (M & 3) == 0) {
-          *reinterpret_cast<unsigned *>(sp) = bytes;
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (row + r < M) sp[r] = (uint8_t)(bytes >> (8 * r));
-        }
       } else {
         int *sp = (int *)S + (int64_t)G.gout[g] * M * N + row + (int64_t)col * M;
         if (row + 3 < M && (M & 3) == 0) {
@@ -263,19 +282,20 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
  * 64-byte global runs per load, ran slower, so the rows stay whole.) */
 __device__ __forceinline__ int dl_swz(int r) { return ((r >> 1) & 2) | ((r >> 3) & 1); }
 
-template <int KIND, int NST, int WM, int WN>
-__global__ void __launch_bounds__(64 * WM * WN)
+template <int KIND, int NST, int WM, int WN, int BN>
+__global__ void __launch_bounds__(64 * WM * WN, 8 / (WM * WN))
 slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
                          int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
                          SliceGroups G, int tiles_m, int tiles_n, void *__restrict__ S,
                          float *__restrict__ C, int64_t ldc, float alpha, float beta) {
-  constexpr int BM = 256, BN = 256, BKB = 64, NW = WM * WN;
+  constexpr int BM = 256, BKB = 64, NW = WM * WN;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 16, FN = WTN / 16;
   constexpr int STAGE = (BM + BN) * BKB; /* bytes per LDS stage */
-  constexpr int LPW = BM * BKB / 1024 / NW; /* 1 KB loads per wave, operand and stage */
-  constexpr int WAITN = (NST - 2) * 2 * LPW; /* loads of NST - 2 stages */
-  static_assert(BM == BN && LPW * NW * 1024 == BM * BKB, "load split");
+  constexpr int LPA = BM * BKB / 1024 / NW; /* 1 KB loads per wave and stage: A */
+  constexpr int LPB = BN * BKB / 1024 / NW; /* and B */
+  constexpr int WAITN = (NST - 2) * (LPA + LPB); /* loads of NST - 2 stages */
+  static_assert(LPA * NW * 1024 == BM * BKB && LPB * NW * 1024 == BN * BKB, "load split");
   constexpr bool BF16 = KIND == SP_BF16;
   typedef SliceMfma<BF16> MM;
   typedef typename MM::acc_t acc_t;
@@ -297,16 +317,19 @@ slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict
   const int iters = pc * nkt;
 
   /* This wave's loads per operand and stage: LDS chunks
-   * c = (LPW wave + u) 64 + lane, row c / 4, position c % 4, holding chunk
+   * c = (LPA wave + u) 64 + lane (LPB for B), row c / 4, position c % 4, holding chunk
    * (c % 4) ^ dl_swz(row) of that row (rows past the matrix are clamped to
    * its last row; their products are never stored). */
-  int64_t a_off[LPW], b_off[LPW];
+  int64_t a_off[LPA], b_off[LPB];
 #pragma unroll
-  for (int u = 0; u < LPW; ++u) {
-    const int c = (LPW * wave + u) * 64 + lane, r = c >> 2, j = (c & 3) ^ dl_swz(r);
-    const int ra = r < M - m0 ? r : M - m0 - 1, rb = r < N - n0 ? r : N - n0 - 1;
-    a_off[u] = ra * kpad_bytes + 16 * j;
-    b_off[u] = rb * kpad_bytes + 16 * j;
+  for (int u = 0; u < LPA; ++u) {
+    const int c = (LPA * wave + u) * 64 + lane, r = c >> 2, j = (c & 3) ^ dl_swz(r);
+    a_off[u] = (r < M - m0 ? r : M - m0 - 1) * kpad_bytes + 16 * j;
+  }
+#pragma unroll
+  for (int u = 0; u < LPB; ++u) {
+    const int c = (LPB * wave + u) * 64 + lane, r = c >> 2, j = (c & 3) ^ dl_swz(r);
+    b_off[u] = (r < N - n0 ? r : N - n0 - 1) * kpad_bytes + 16 * j;
   }
   const int8_t *a_tile = As + (int64_t)m0 * kpad_bytes;
   const int8_t *b_tile = Bs + (int64_t)n0 * kpad_bytes;
@@ -318,14 +341,15 @@ slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict
     const int8_t *gb = b_tile + (int64_t)uu * N * kpad_bytes + kt * BKB;
     char *la = lds + buf * STAGE, *lb = la + BM * BKB;
 #pragma unroll
-    for (int u = 0; u < LPW; ++u) {
+    for (int u = 0; u < LPA; ++u)
       __builtin_amdgcn_global_load_lds((const void *)(ga + a_off[u]),
-                                       (__attribute__((address_space(3))) void *)(la + (LPW * wave + u) * 1024),
+                                       (__attribute__((address_space(3))) void *)(la + (LPA * wave + u) * 1024),
                                        16, 0, 0);
+#pragma unroll
+    for (int u = 0; u < LPB; ++u)
       __builtin_amdgcn_global_load_lds((const void *)(gb + b_off[u]),
-                                       (__attribute__((address_space(3))) void *)(lb + (LPW * wave + u) * 1024),
+                                       (__attribute__((address_space(3))) void *)(lb + (LPB * wave + u) * 1024),
                                        16, 0, 0);
-    }
   };
 
   struct Frag {
@@ -375,7 +399,7 @@ slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict
     /* the stage's loads, then the next fragments' reads spread over the
      * MFMAs */
     constexpr int NR = FM + FN, NMF = FM * FN, PER = NMF / NR;
-    __builtin_amdgcn_sched_group_barrier(0x020, 2 * LPW, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, LPA + LPB, 0);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);
@@ -386,7 +410,7 @@ slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict
 
 #pragma unroll
   for (int s = 0; s < NST; ++s) issue(s < iters ? s : iters - 1, s);
-  asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 1) * 2 * LPW) : "memory");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 1) * (LPA + LPB)) : "memory");
   asm volatile("s_barrier" ::: "memory");
   Frag f0, f1;
   read(0, f0);
@@ -402,16 +426,16 @@ slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict
                            n0 + wn * WTN, li, kg);
 }
 
-template <int KIND, int NST, int WM, int WN>
+template <int KIND, int NST, int WM, int WN, int BN = 256>
 hipError_t launch_dl(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad, int kb0,
                      int kb1, const SliceGroups &G, void *S, float *C, int64_t ldc,
                      float alpha, float beta, hipStream_t st) {
-  auto kern = slice_products_dl_kernel<KIND, NST, WM, WN>;
-  const size_t lds = (size_t)NST * 512 * 64;
+  auto kern = slice_products_dl_kernel<KIND, NST, WM, WN, BN>;
+  const size_t lds = (size_t)NST * (256 + BN) * 64;
   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  const int tiles_m = (m + 255) / 256, tiles_n = (n + 255) / 256;
+  const int tiles_m = (m + 255) / 256, tiles_n = (n + BN - 1) / BN;
   hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n * G.ngroups), dim3(64 * WM * WN), lds, st,
                      as, bs, m, n, kpad, kb0, kb1, G, tiles_m, tiles_n, S, C, ldc, alpha, beta);
   return hipGetLastError();
