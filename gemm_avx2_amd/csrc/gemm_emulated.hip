@@ -163,11 +163,12 @@ bool oz2_constants(int nmod, Oz2Crt *c) {
   return true;
 }
 
-/* Bits [s, s+64) of a 192-bit number. */
-__device__ __forceinline__ unsigned long long bits64(const unsigned (&x)[OZ2_LIMBS], int s) {
+/* Bits [s, s+64) of an NL-limb number. */
+template <int NL>
+__device__ __forceinline__ unsigned long long bits64(const unsigned (&x)[NL], int s) {
   unsigned long long r = 0;
 #pragma unroll
-  for (int l = 0; l < OZ2_LIMBS; ++l) {
+  for (int l = 0; l < NL; ++l) {
     const int pos = 32 * l - s; /* where limb l lands in the result */
     if (pos >= 64 || pos <= -32) continue;
     const unsigned long long v = x[l];
@@ -177,34 +178,43 @@ __device__ __forceinline__ unsigned long long bits64(const unsigned (&x)[OZ2_LIM
 }
 
 /* X from the carried 16-bit limb sums, reduced modulo M, rounded and scaled;
- * then the alpha/beta update of C(i, j). */
-__device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], const Oz2Crt &crt,
+ * then the alpha/beta update of C(i, j).  LW: 16-bit limbs of the W_j (M <
+ * 2^(16 LW)); the sum X' = sum_j c_j W_j < 22 * 2^8 * M needs NL = LW / 2 + 1
+ * 32-bit limbs. */
+template <int LW>
+__device__ __forceinline__ void oz2_finish(const unsigned (&acc)[LW], const Oz2Crt &crt,
                                            int da, int db, double *cp, double alpha,
                                            double beta) {
-  unsigned x[OZ2_LIMBS];
+  constexpr int NL = LW / 2 + 1;
+  unsigned x[NL];
   {
-    unsigned carry = 0, half = 0;
+    /* 16-bit digits of X': the limb sums (< 2^29) carried upward; the final
+     * carry (< 2^14) is digit LW */
+    unsigned d[2 * NL];
+    unsigned carry = 0;
 #pragma unroll
-    for (int l = 0; l < OZ2_LIMBS16; ++l) {
-      const unsigned v = acc[l] + carry; /* < 2^29 + 2^13 */
+    for (int l = 0; l < LW; ++l) {
+      const unsigned v = acc[l] + carry;
+      d[l] = v & 0xffffu;
       carry = v >> 16;
-      if (l % 2 == 0) {
-        half = v & 0xffffu;
-      } else {
-        x[l / 2] = half | (v << 16);
-      }
     }
-    x[OZ2_LIMBS - 1] = half | (carry << 16); /* limb 10 and the final carry (< 2^14) */
+#pragma unroll
+    for (int l = LW; l < 2 * NL; ++l) {
+      d[l] = carry & 0xffffu;
+      carry >>= 16;
+    }
+#pragma unroll
+    for (int l = 0; l < NL; ++l) x[l] = d[2 * l] | (d[2 * l + 1] << 16);
   }
   /* X = X' mod M: subtract q M with q < nmod 2^8 from FP64 (off by at most
    * one), then correct */
   double xd = 0.0;
 #pragma unroll
-  for (int l = OZ2_LIMBS - 1; l >= 0; --l) xd = xd * 4294967296.0 + (double)x[l];
+  for (int l = NL - 1; l >= 0; --l) xd = xd * 4294967296.0 + (double)x[l];
   const long long q = (long long)(xd * crt.rMd);
   long long borrow = 0;
 #pragma unroll
-  for (int l = 0; l < OZ2_LIMBS; ++l) {
+  for (int l = 0; l < NL; ++l) {
     const long long d = (long long)x[l] - (long long)((unsigned long long)q * crt.M[l]) + borrow;
     x[l] = (unsigned)d;
     borrow = d >> 32; /* arithmetic: floor(d / 2^32) */
@@ -212,7 +222,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
   if (borrow < 0) { /* X < 0: add M */
     unsigned long long c2 = 0;
 #pragma unroll
-    for (int l = 0; l < OZ2_LIMBS; ++l) {
+    for (int l = 0; l < NL; ++l) {
       const unsigned long long v = (unsigned long long)x[l] + crt.M[l] + c2;
       x[l] = (unsigned)v;
       c2 = v >> 32;
@@ -220,7 +230,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
   }
   bool ge = true; /* X >= M: subtract M */
 #pragma unroll
-  for (int l = OZ2_LIMBS - 1; l >= 0; --l) {
+  for (int l = NL - 1; l >= 0; --l) {
     if (x[l] != crt.M[l]) {
       ge = x[l] > crt.M[l];
       break;
@@ -229,7 +239,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
   if (ge) {
     long long b2 = 0;
 #pragma unroll
-    for (int l = 0; l < OZ2_LIMBS; ++l) {
+    for (int l = 0; l < NL; ++l) {
       const long long d = (long long)x[l] - (long long)crt.M[l] + b2;
       x[l] = (unsigned)d;
       b2 = d >> 32;
@@ -238,7 +248,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
   /* symmetric range: X > M/2 stands for X - M */
   bool gt = false;
 #pragma unroll
-  for (int l = OZ2_LIMBS - 1; l >= 0; --l) {
+  for (int l = NL - 1; l >= 0; --l) {
     if (x[l] != crt.H[l]) {
       gt = x[l] > crt.H[l];
       break;
@@ -248,7 +258,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
   if (neg) {
     long long b3 = 0;
 #pragma unroll
-    for (int l = 0; l < OZ2_LIMBS; ++l) {
+    for (int l = 0; l < NL; ++l) {
       const long long d = (long long)crt.M[l] - (long long)x[l] + b3;
       x[l] = (unsigned)d;
       b3 = d >> 32;
@@ -257,7 +267,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
   /* round |X| to 53 bits, ties to even */
   int top = -1;
 #pragma unroll
-  for (int l = OZ2_LIMBS - 1; l >= 0; --l)
+  for (int l = NL - 1; l >= 0; --l)
     if (top < 0 && x[l]) top = 32 * l + 31 - __builtin_clz(x[l]);
   double mag = 0.0;
   int shift = 0;
@@ -272,7 +282,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
       const bool half = (bits64(x, hb) & 1ull) != 0;
       bool sticky = false;
 #pragma unroll
-      for (int l = 0; l < OZ2_LIMBS; ++l) {
+      for (int l = 0; l < NL; ++l) {
         const int lo = 32 * l;
         if (lo >= hb) continue;
         const unsigned mask = hb - lo >= 32 ? 0xffffffffu : ((1u << (hb - lo)) - 1u);
@@ -301,6 +311,7 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[OZ2_LIMBS16], c
 /* Four consecutive elements (column-major index t) per thread: the modulus
  * loop reads each plane's bytes of the four with one load and each M_j limb
  * once for the four. */
+template <int LW>
 __global__ void __launch_bounds__(256)
 ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
                     const int *__restrict__ ea, const int *__restrict__ fb,
@@ -314,11 +325,11 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
        u += (int64_t)gridDim.x * blockDim.x) {
     const int64_t t0 = u * E;
     /* X'_e = sum_g c_g W_g < nmod 2^8 M */
-    unsigned acc[E][OZ2_LIMBS16];
+    unsigned acc[E][LW];
 #pragma unroll
     for (int q = 0; q < E; ++q)
 #pragma unroll
-      for (int l = 0; l < OZ2_LIMBS16; ++l) acc[q][l] = 0u;
+      for (int l = 0; l < LW; ++l) acc[q][l] = 0u;
 #pragma unroll 1
     for (int g = 0; g < crt.nmod; ++g) {
       const uint8_t *rp = R + (int64_t)g * total + t0;
@@ -334,7 +345,7 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
       for (int q = 0; q < E; ++q) {
         const unsigned cg = (c4 >> (8 * q)) & 0xffu;
 #pragma unroll
-        for (int l = 0; l < OZ2_LIMBS16; ++l) acc[q][l] += __umul24(cg, crt.W16[g][l]);
+        for (int l = 0; l < LW; ++l) acc[q][l] += __umul24(cg, crt.W16[g][l]);
       }
     }
 #pragma unroll
@@ -342,9 +353,34 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
       const int64_t t = t0 + q;
       if (t < total) {
         const int64_t j = t / M, i = t - j * M;
-        oz2_finish(acc[q], crt, ea[i], fb[j], C + i + j * ldc, alpha, beta);
+        oz2_finish<LW>(acc[q], crt, ea[i], fb[j], C + i + j * ldc, alpha, beta);
       }
     }
+  }
+}
+
+template <int LW>
+hipError_t launch_recon_lw(const uint8_t *R, const Oz2Crt &crt, const int *ea, const int *fb,
+                           const gemmx_problem *p, double beta, unsigned blocks,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(ozaki2_recon_kernel<LW>, dim3(blocks), dim3(256), 0, st, R, crt, ea, fb,
+                     (double *)p->c, p->ldc, p->m, p->n, p->alpha, beta);
+  return hipGetLastError();
+}
+
+/* The reconstruction with as many 16-bit limbs as M needs (M < 2^(L+1)). */
+hipError_t launch_recon(const uint8_t *R, const Oz2Crt &crt, const int *ea, const int *fb,
+                        const gemmx_problem *p, double beta, unsigned blocks, hipStream_t st) {
+  switch ((crt.L + 16) / 16) {
+  case 4: return launch_recon_lw<4>(R, crt, ea, fb, p, beta, blocks, st);
+  case 5: return launch_recon_lw<5>(R, crt, ea, fb, p, beta, blocks, st);
+  case 6: return launch_recon_lw<6>(R, crt, ea, fb, p, beta, blocks, st);
+  case 7: return launch_recon_lw<7>(R, crt, ea, fb, p, beta, blocks, st);
+  case 8: return launch_recon_lw<8>(R, crt, ea, fb, p, beta, blocks, st);
+  case 9: return launch_recon_lw<9>(R, crt, ea, fb, p, beta, blocks, st);
+  case 10: return launch_recon_lw<10>(R, crt, ea, fb, p, beta, blocks, st);
+  case 11: return launch_recon_lw<11>(R, crt, ea, fb, p, beta, blocks, st);
+  default: return hipErrorInvalidValue;
   }
 }
 
@@ -415,10 +451,7 @@ extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *w
     if (e != hipSuccess) return e;
     if (ev && kb0 + OZ2_KCHUNK >= kp) (void)hipEventRecord(ev[2], st);
     /* chunks after the first add onto C: beta = 1 leaves C as it is */
-    hipLaunchKernelGGL(ozaki2_recon_kernel, dim3((unsigned)rblocks), dim3(256), 0, st, R, crt,
-                       ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha,
-                       kb0 == 0 ? p->beta : 1.0);
-    e = hipGetLastError();
+    e = launch_recon(R, crt, ea, fb, p, kb0 == 0 ? p->beta : 1.0, (unsigned)rblocks, st);
     if (e != hipSuccess) return e;
   }
   if (ev) (void)hipEventRecord(ev[3], st);

@@ -134,6 +134,28 @@ def test_ozaki2_bit_exact(cuda, ta, tb, nmod):
         np.testing.assert_array_equal(G, R)
 
 
+@pytest.mark.parametrize("nmod", list(range(8, 23)))
+def test_ozaki2_bit_exact_every_modulus_count(cuda, nmod):
+    """Every modulus count (so every limb count of the reconstruction,
+    4..11 sixteen-bit limbs) against the oracle, bit for bit: heavy-tailed
+    entries, and rows and columns of equal-sign entries at the top of their
+    binade, whose dot products come closest to the CRT range bound."""
+    import torch
+    rng = np.random.default_rng(300 + nmod)
+    m, n, k = 40, 24, 300
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)) * np.exp(4 * rng.standard_normal((m, k))))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)) * np.exp(2 * rng.standard_normal((k, n))))
+    top = np.nextafter(1.0, 0.0)
+    A[:4] = top * 2.0 ** 5
+    A[4:8] = -top
+    B[:, :3] = top * 2.0 ** -3
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    R = C.copy(order="F")
+    oracle.ozaki2_gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.5, R, m, nmod)
+    G = _ozaki2_gpu(torch, "N", "N", m, n, k, 1.0, A, m, B, k, 0.5, C, m, nmod)
+    np.testing.assert_array_equal(G, R)
+
+
 def test_ozaki2_chunks_subnormals_and_semantics(cuda):
     import torch
     rng = np.random.default_rng(9)
