@@ -496,6 +496,33 @@ def run_strided(torch, gx, stream, steps, warmup):
     return out
 
 
+def run_complex(torch, gx, stream, steps, warmup):
+    """zgemm_/cgemm_ (four real products) and the 3m forms (three) at 4096^3
+    NN on device tensors, alpha 1, beta 0; rate = 8mnk / t (complex flops).
+    The Fortran-ABI call synchronises, so HIP events bracket whole calls."""
+    m = n = k = 4096
+    out = {}
+    for prec, forms in (("z", (("zgemm", gx.zgemm), ("zgemm3m", gx.zgemm3m))),
+                        ("c", (("cgemm", gx.cgemm), ("cgemm3m", gx.cgemm3m)))):
+        tdt = torch.complex128 if prec == "z" else torch.complex64
+        rdt = torch.float64 if prec == "z" else torch.float32
+        bufs = []
+        for j, (r, c) in enumerate(((m, k), (k, n), (m, n))):
+            t = torch.empty(2 * r * c, dtype=rdt, device="cuda")
+            gx.fill(t, 2 * r, c, 2 * r, SEED, 10 + j, stream=stream)
+            bufs.append(t.view(tdt))
+        A, B, C = bufs
+        for name, f in forms:
+            def call():
+                f("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+            _, per = time_launches(torch, call, steps, warmup)
+            out[name] = {"tflops": round(8.0 * m * n * k / median(per) / 1e9, 2),
+                         "ms": round(median(per), 3)}
+        del A, B, C, bufs
+    out["shape"] = "4096^3 NN, device operands, alpha 1, beta 0"
+    return out
+
+
 def run_host_pointer(torch, gx):
     """dgemm_ on host (pageable numpy) arrays at the headline shape: the wall
     clock of the whole call, staging copies to and from the GPU included."""
@@ -581,6 +608,8 @@ def main():
                         ("batch", lambda: run_batch(torch, gx, stream, max(3, args.steps // 2), 1)),
                         ("strided_batch", lambda: run_strided(torch, gx, stream,
                                                               max(3, args.steps // 2), 1)),
+                        ("complex_gemm", lambda: run_complex(torch, gx, stream,
+                                                             max(3, args.steps // 2), 1)),
                         ("dgemm_host_pointers", lambda: run_host_pointer(torch, gx))):
             result[key] = fn()
             torch.cuda.empty_cache()

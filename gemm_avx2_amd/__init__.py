@@ -23,7 +23,8 @@ import ctypes
 import os
 
 __all__ = [
-    "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm64", "sgemm64", "dgemm_dev",
+    "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm64", "sgemm64", "zgemm", "cgemm",
+    "zgemm3m", "cgemm3m", "dgemm_dev",
     "sgemm_dev", "dgemm_strided_batched_dev", "sgemm_strided_batched_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
@@ -61,6 +62,12 @@ def _declare(lib, dtype):
     f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p, flp,
                   ctypes.c_void_p, _c_int_p, ctypes.c_void_p, _c_int_p, flp, ctypes.c_void_p,
                   _c_int_p]
+    for name in (("zgemm_", "zgemm3m_") if dtype == "d" else ("cgemm_", "cgemm3m_")):
+        f = getattr(lib, name)
+        f.restype = None
+        f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p, flp,
+                      ctypes.c_void_p, _c_int_p, ctypes.c_void_p, _c_int_p, flp,
+                      ctypes.c_void_p, _c_int_p]
     f = getattr(lib, f"{p}gemm_64_")
     f.restype = None
     f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_i64_p, _c_i64_p, _c_i64_p, flp,
@@ -246,6 +253,43 @@ def dgemm64(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=
 def sgemm64(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
     """sgemm_64_: sgemm_ with 64-bit integer arguments (ILP64)."""
     _fortran64("s", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def _complex(name, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check):
+    dtype = "d" if name[0] == "z" else "s"
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    fl = ctypes.c_double if dtype == "d" else ctypes.c_float
+    ta = transa.encode() if isinstance(transa, str) else bytes(transa)
+    tb = transb.encode() if isinstance(transb, str) else bytes(transb)
+    al, be = complex(alpha), complex(beta)
+    I = ctypes.c_int
+    getattr(lib, name)(ta, tb, ctypes.byref(I(m)), ctypes.byref(I(n)), ctypes.byref(I(k)),
+                       (fl * 2)(al.real, al.imag), _addr(a), ctypes.byref(I(lda)), _addr(b),
+                       ctypes.byref(I(ldb)), (fl * 2)(be.real, be.imag), _addr(c),
+                       ctypes.byref(I(ldc)))
+    if check:
+        _check(dtype)
+
+
+def zgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    """zgemm_: double complex C = alpha op(A) op(B) + beta C ('C' conjugates),
+    four real MFMA products; operands complex128 (host arrays or device
+    tensors), sizes in complex elements."""
+    _complex("zgemm_", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def zgemm3m(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    """zgemm3m_: as zgemm with three real products."""
+    _complex("zgemm3m_", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def cgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    """cgemm_: single complex (complex64) form of zgemm."""
+    _complex("cgemm_", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def cgemm3m(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
+    _complex("cgemm3m_", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
 
 
 def _ch(t):

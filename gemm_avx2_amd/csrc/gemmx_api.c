@@ -399,6 +399,105 @@ EXPORT void sgemm_(const char *transa, const char *transb, const int *m,
   fortran_call(&x, g_smode, 0);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Complex GEMM: zgemm_/cgemm_ (four real products) and zgemm3m_/cgemm3m_   */
+/* (three), the level-3 complex siblings kept outside the reference's build */
+/* (OpenBLAS-like_implementation/zgemm_kernel_4x2_haswell.c,                 */
+/* zgemm3m_kernel_4x4_haswell.c and their c* forms).  BLAS argument order,   */
+/* complex scalars and elements as (re, im) pairs, the transpose rule of     */
+/* dgemm_ ('N'/'n' none, anything else transposes) with 'C'/'c' also         */
+/* conjugating, and dgemm_'s beta rule (beta != 1 scales C first, also 0).   */
+/* ------------------------------------------------------------------------ */
+static void complex_call(int es, char ta_c, char tb_c, int m, int n, int k, const double *alpha,
+                         const void *a, int lda, const void *b, int ldb, const double *beta,
+                         void *c, int ldc, int three) {
+  clear_error();
+  call_t x = {es, ta_c, tb_c, m, n, k, lda, ldb, ldc, alpha[0], beta[0], a, b, c};
+  /* the scalar check of validate reads alpha != 0: either part counts */
+  if (alpha[1] != 0.0) x.alpha = 1.0;
+  if (validate(&x) != GEMMX_OK) return;
+  if (m == 0 || n == 0) return;
+  int dev;
+  if (current_device(&dev) != GEMMX_OK) return;
+  const int product = (alpha[0] != 0.0 || alpha[1] != 0.0) && k > 0;
+  const int ta = !notrans(ta_c), tb = !notrans(tb_c);
+  const int ra = ta ? k : m, ca = ta ? m : k, rb = tb ? n : k, cb = tb ? k : n;
+  const size_t ce = 2 * (size_t)es; /* bytes per complex element */
+  const int host_a = product && !is_device_ptr(a), host_b = product && !is_device_ptr(b);
+  const int host_c = !is_device_ptr(c);
+  const size_t lda2 = host_a ? (size_t)ra : (size_t)lda, ldb2 = host_b ? (size_t)rb : (size_t)ldb;
+  const size_t ldc2 = host_c ? (size_t)m : (size_t)ldc;
+  const size_t bytes_a = host_a ? round_up(lda2 * ca * ce, 256) : 0;
+  const size_t bytes_b = host_b ? round_up(ldb2 * cb * ce, 256) : 0;
+  const size_t bytes_c = host_c ? round_up(ldc2 * (size_t)n * ce, 256) : 0;
+  const size_t bytes_w = product ? round_up(gemmx_complex_workspace(es, ta, tb, m, n, k, three), 256) : 0;
+  const size_t total = bytes_a + bytes_b + bytes_c + bytes_w;
+  gemmx_cproblem p = {.ta = ta, .tb = tb, .conja = ta_c == 'C' || ta_c == 'c',
+                      .conjb = tb_c == 'C' || tb_c == 'c', .m = m, .n = n, .k = k,
+                      .lda = (int64_t)lda2, .ldb = (int64_t)ldb2, .ldc = (int64_t)ldc2,
+                      .alpha = {alpha[0], alpha[1]}, .beta = {beta[0], beta[1]},
+                      .a = a, .b = b, .c = c};
+  hipStream_t s = NULL;
+  devstate_t *d = &g_dev[dev];
+  pthread_mutex_lock(&d->lock);
+  char *base = NULL;
+  if (total > 0 && get_scratch(dev, total, (void **)&base) != GEMMX_OK) goto out;
+  char *pa = base, *pb = pa + bytes_a, *pc = pb + bytes_b, *pw = pc + bytes_c;
+  hipError_t e;
+  if (host_a) {
+    if (gemmx_stage_h2d(dev, pa, lda2 * ce, a, (size_t)lda * ce, (size_t)ra * ce, (size_t)ca,
+                        s) != GEMMX_OK)
+      goto out;
+    p.a = pa;
+  }
+  if (host_b) {
+    if (gemmx_stage_h2d(dev, pb, ldb2 * ce, b, (size_t)ldb * ce, (size_t)rb * ce, (size_t)cb,
+                        s) != GEMMX_OK)
+      goto out;
+    p.b = pb;
+  }
+  if (host_c) {
+    if (gemmx_stage_h2d(dev, pc, ldc2 * ce, c, (size_t)ldc * ce, (size_t)m * ce, (size_t)n,
+                        s) != GEMMX_OK)
+      goto out;
+    p.c = pc;
+  }
+  e = gemmx_complex(es, &p, three, pw, s);
+  if (e != hipSuccess) {
+    hip_fail(e, "complex GEMM launch");
+    (void)hipStreamSynchronize(s);
+    goto out;
+  }
+  if (host_c && gemmx_stage_d2h(dev, c, (size_t)ldc * ce, pc, ldc2 * ce, (size_t)m * ce,
+                                 (size_t)n, s) != GEMMX_OK) {
+    (void)hipStreamSynchronize(s);
+    goto out;
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) hip_fail(e, "complex GEMM execution");
+out:
+  pthread_mutex_unlock(&d->lock);
+}
+
+#define GEMMX_COMPLEX_ENTRY(NAME, T, ES, THREE)                                                   \
+  EXPORT void NAME(const char *transa, const char *transb, const int *m, const int *n,          \
+                   const int *k, const T *alpha, const T *a, const int *lda, const T *b,        \
+                   const int *ldb, const T *beta, T *c, const int *ldc) {                       \
+    if (!transa || !transb || !m || !n || !k || !alpha || !lda || !ldb || !beta || !ldc) {     \
+      clear_error();                                                                            \
+      set_error(GEMMX_ERR_ARG, #NAME ": NULL scalar argument");                                 \
+      return;                                                                                   \
+    }                                                                                           \
+    const double al[2] = {alpha[0], alpha[1]}, be[2] = {beta[0], beta[1]};                      \
+    pthread_once(&g_once, init_once);                                                           \
+    complex_call(ES, *transa, *transb, *m, *n, *k, al, a, *lda, b, *ldb, be, c, *ldc, THREE);   \
+  }
+GEMMX_COMPLEX_ENTRY(zgemm_, double, 8, 0)
+GEMMX_COMPLEX_ENTRY(zgemm3m_, double, 8, 1)
+GEMMX_COMPLEX_ENTRY(cgemm_, float, 4, 0)
+GEMMX_COMPLEX_ENTRY(cgemm3m_, float, 4, 1)
+#undef GEMMX_COMPLEX_ENTRY
+
 /* ILP64 forms: the same calls with 64-bit Fortran integers (callers built
  * with -fdefault-integer-8 and the "_64_" suffix of ILP64 BLAS builds).
  * Leading dimensions are used at full width; m, n and k must stay below

@@ -38,6 +38,23 @@ typedef struct gemmx_problem {
   int64_t sa, sb, sc;
 } gemmx_problem;
 
+/* A complex GEMM (interleaved re, im pairs; sizes and leading dimensions in
+ * complex elements): ta/tb transpose, conja/conjb conjugate (transa 'C'). */
+typedef struct gemmx_cproblem {
+  int ta, tb, conja, conjb;
+  int m, n, k;
+  int64_t lda, ldb, ldc;
+  double alpha[2], beta[2];
+  const void *a;
+  const void *b;
+  void *c;
+} gemmx_cproblem;
+/* Device workspace of gemmx_complex (real planes of the operands and of the
+ * products; three = the three-product form), and the complex GEMM itself
+ * (gemm_complex.hip; es 8 = double complex, 4 = float complex). */
+size_t gemmx_complex_workspace(int es, int ta, int tb, int m, int n, int k, int three);
+hipError_t gemmx_complex(int es, const gemmx_cproblem *p, int three, void *work, hipStream_t s);
+
 /* Kernel configurations of the native MFMA tile kernels.  The tuning table
  * (gemmx_tune.c) picks one per (dtype, trans, shape bucket). */
 enum {

@@ -50,6 +50,37 @@ void sgemm_(const char *transa, const char *transb, const int *m, const int *n,
             const float *b, const int *ldb, const float *beta, float *c,
             const int *ldc);
 
+/* Complex GEMM, the level-3 complex siblings that the reference keeps outside
+ * its build as OpenBLAS kernels (OpenBLAS-like_implementation/
+ * zgemm_kernel_4x2_haswell.c, cgemm_kernel_8x2_haswell.c; three-product
+ * forms zgemm3m_kernel_4x4_haswell.c, cgemm3m_kernel_8x4_haswell.c), with
+ * the BLAS argument order: C = alpha op(A) op(B) + beta C, complex elements
+ * and scalars stored as (re, im) pairs, sizes and leading dimensions in
+ * complex elements.  'N'/'n' = no transpose, 'C'/'c' = conjugate transpose,
+ * any other character = transpose; the rest of dgemm_'s semantics as above
+ * (beta != 1 scales C first, also beta = 0; alpha = 0 skips the product;
+ * rows m..ldc-1 untouched; synchronous; host or device operands).
+ * zgemm_/cgemm_ use four real MFMA products (Re = ArBr - AiBi,
+ * Im = ArBi + AiBr); zgemm3m_/cgemm3m_ three ((Ar + Ai)(Br + Bi) - ArBr -
+ * AiBi for Im), faster, with a larger error when real and imaginary parts
+ * cancel. */
+void zgemm_(const char *transa, const char *transb, const int *m, const int *n,
+            const int *k, const double *alpha, const double *a, const int *lda,
+            const double *b, const int *ldb, const double *beta, double *c,
+            const int *ldc);
+void zgemm3m_(const char *transa, const char *transb, const int *m, const int *n,
+              const int *k, const double *alpha, const double *a, const int *lda,
+              const double *b, const int *ldb, const double *beta, double *c,
+              const int *ldc);
+void cgemm_(const char *transa, const char *transb, const int *m, const int *n,
+            const int *k, const float *alpha, const float *a, const int *lda,
+            const float *b, const int *ldb, const float *beta, float *c,
+            const int *ldc);
+void cgemm3m_(const char *transa, const char *transb, const int *m, const int *n,
+              const int *k, const float *alpha, const float *a, const int *lda,
+              const float *b, const int *ldb, const float *beta, float *c,
+              const int *ldc);
+
 /* ILP64 forms of the same entry points (no counterpart in the reference,
  * which is LP64 only: src/gemm_driver.c:118 takes int): every integer
  * argument is a 64-bit integer, as for Fortran callers built with 64-bit

@@ -104,6 +104,12 @@ def lib():
         ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    for fn, fl in (("oracle_zgemm", ctypes.c_double), ("oracle_cgemm", ctypes.c_float)):
+        f = getattr(L, fn)
+        f.restype = None
+        f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      ctypes.POINTER(fl), ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                      ctypes.c_int, ctypes.POINTER(fl), ctypes.c_void_p, ctypes.c_int]
     L.oracle_bf16_rne.restype = ctypes.c_uint16
     L.oracle_bf16_rne.argtypes = [ctypes.c_float]
     L.oracle_bf16_split.restype = None
@@ -193,6 +199,17 @@ def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, sli
 def ozaki2_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, moduli):
     lib().oracle_ozaki2_dgemm(transa.encode()[:1], transb.encode()[:1], m, n, k, alpha,
                               _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc, moduli)
+
+
+def complex_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """zgemm (complex128 arrays) or cgemm (complex64), Fortran-ordered, in place
+    on c; alpha and beta Python complex numbers."""
+    single = c.dtype == np.complex64
+    fl = ctypes.c_float if single else ctypes.c_double
+    al, be = (fl * 2)(alpha.real, alpha.imag), (fl * 2)(beta.real, beta.imag)
+    fn = lib().oracle_cgemm if single else lib().oracle_zgemm
+    fn(transa.encode()[:1], transb.encode()[:1], m, n, k, al, _ptr(a), lda, _ptr(b), ldb, be,
+       _ptr(c), ldc)
 
 
 def ozaki2_modulus(j):

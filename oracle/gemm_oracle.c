@@ -483,6 +483,98 @@ void oracle_bf16x3_sgemm(char transa, char transb, int m, int n, int k,
 }
 
 /* ---------------------------------------------------------------------- */
+/* Complex GEMM (zgemm_/cgemm_ and the 3m forms of the library).  The      */
+/* reference keeps its complex kernels outside its build                   */
+/* (OpenBLAS-like_implementation/zgemm_kernel_4x2_haswell.c,                */
+/* cgemm_kernel_8x2_haswell.c); this restates the BLAS operation with the   */
+/* library's conventions: (re, im) pairs, 'N'/'n' no transpose, 'C'/'c'     */
+/* conjugate transpose, other characters transpose, and dgemm_'s beta rule  */
+/* (beta != 1 scales C first, also beta = 0).  Re and Im of op(A) op(B) are */
+/* Dot2-accumulated (2k products each) and rounded once; then              */
+/* C = beta C + alpha P in double.                                           */
+/* ---------------------------------------------------------------------- */
+static void complex_gemm(char transa, char transb, int m, int n, int k, const double alpha[2],
+                         const void *a, int lda, const void *b, int ldb, const double beta[2],
+                         void *c, int ldc, int single) {
+  const int na = is_notrans(transa), nb = is_notrans(transb);
+  const double sa = (transa == 'C' || transa == 'c') ? -1.0 : 1.0;
+  const double sb = (transb == 'C' || transb == 'c') ? -1.0 : 1.0;
+  if (m <= 0 || n <= 0) return;
+  const int product = (alpha[0] != 0.0 || alpha[1] != 0.0) && k > 0;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j < n; j++) {
+    for (int i = 0; i < m; i++) {
+      double pr = 0.0, pi = 0.0;
+      if (product) {
+        dd_t re = {0.0, 0.0}, im = {0.0, 0.0};
+        for (int p = 0; p < k; p++) {
+          const int64_t ia = na ? i + (int64_t)p * lda : p + (int64_t)i * lda;
+          const int64_t ib = nb ? p + (int64_t)j * ldb : j + (int64_t)p * ldb;
+          double ar, ai, br, bi;
+          if (single) {
+            ar = ((const float *)a)[2 * ia];
+            ai = ((const float *)a)[2 * ia + 1];
+            br = ((const float *)b)[2 * ib];
+            bi = ((const float *)b)[2 * ib + 1];
+          } else {
+            ar = ((const double *)a)[2 * ia];
+            ai = ((const double *)a)[2 * ia + 1];
+            br = ((const double *)b)[2 * ib];
+            bi = ((const double *)b)[2 * ib + 1];
+          }
+          ai *= sa;
+          bi *= sb;
+          dot2_step(&re, ar, br);
+          dot2_step(&re, -ai, bi);
+          dot2_step(&im, ar, bi);
+          dot2_step(&im, ai, br);
+        }
+        pr = dd_round(re);
+        pi = dd_round(im);
+      }
+      const int64_t ic = i + (int64_t)j * ldc;
+      double cr, ci;
+      if (single) {
+        cr = ((float *)c)[2 * ic];
+        ci = ((float *)c)[2 * ic + 1];
+      } else {
+        cr = ((double *)c)[2 * ic];
+        ci = ((double *)c)[2 * ic + 1];
+      }
+      if (beta[0] != 1.0 || beta[1] != 0.0) {
+        const double nr = beta[0] * cr - beta[1] * ci, ni = beta[0] * ci + beta[1] * cr;
+        cr = nr;
+        ci = ni;
+      }
+      if (product) {
+        cr += alpha[0] * pr - alpha[1] * pi;
+        ci += alpha[0] * pi + alpha[1] * pr;
+      }
+      if (single) {
+        ((float *)c)[2 * ic] = (float)cr;
+        ((float *)c)[2 * ic + 1] = (float)ci;
+      } else {
+        ((double *)c)[2 * ic] = cr;
+        ((double *)c)[2 * ic + 1] = ci;
+      }
+    }
+  }
+}
+
+void oracle_zgemm(char transa, char transb, int m, int n, int k, const double alpha[2],
+                  const double *a, int lda, const double *b, int ldb, const double beta[2],
+                  double *c, int ldc) {
+  complex_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, 0);
+}
+
+void oracle_cgemm(char transa, char transb, int m, int n, int k, const float alpha[2],
+                  const float *a, int lda, const float *b, int ldb, const float beta[2],
+                  float *c, int ldc) {
+  const double al[2] = {alpha[0], alpha[1]}, be[2] = {beta[0], beta[1]};
+  complex_gemm(transa, transb, m, n, k, al, a, lda, b, ldb, be, c, ldc, 1);
+}
+
+/* ---------------------------------------------------------------------- */
 /* Ozaki-II (CRT).  No reference counterpart: restates the emulated DGEMM  */
 /* of the library's modular form.  Per row i of op(A) the exponent e_i      */
 /* (oracle_ozaki_exponent), the bit count P_i (oracle_ozaki2_vector_bits)   */

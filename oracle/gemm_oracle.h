@@ -106,6 +106,18 @@ void oracle_ozaki_dgemm(char transa, char transb, int m, int n, int k,
                         const double *b, int ldb, double beta, double *c,
                         int ldc, int slices);
 
+/* ---- complex GEMM (zgemm_/cgemm_ restatement) -------------------------- */
+/* C = beta C (beta != 1, also 0) + alpha op(A) op(B), (re, im) pairs, sizes
+ * and leading dimensions in complex elements; 'N'/'n' no transpose, 'C'/'c'
+ * conjugate transpose, other characters transpose.  Re and Im of the product
+ * are Dot2-accumulated and rounded once. */
+void oracle_zgemm(char transa, char transb, int m, int n, int k, const double alpha[2],
+                  const double *a, int lda, const double *b, int ldb, const double beta[2],
+                  double *c, int ldc);
+void oracle_cgemm(char transa, char transb, int m, int n, int k, const float alpha[2],
+                  const float *a, int lda, const float *b, int ldb, const float beta[2],
+                  float *c, int ldc);
+
 /* ---- Ozaki-II, CRT over INT8 residues (emulated DGEMM restatement) ---- */
 #define OZAKI2_MIN_MODULI 8
 #define OZAKI2_MAX_MODULI 22

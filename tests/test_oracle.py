@@ -539,3 +539,125 @@ def test_bf16_split_and_bf16x3_gemm():
     R = op(A, "N", m, k).astype(np.float64) @ B.astype(np.float64)
     bound = np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)
     assert max_comp(C3, R, bound) <= 3 * 2.0 ** -16
+
+
+# --------------------------------------------------------------- complex GEMM
+CTRANS = ["N", "T", "C"]
+
+
+def _cplx(rng, shape, dtype=np.complex128):
+    return np.asfortranarray((rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape))
+                             .astype(dtype))
+
+
+def _cop(X, t):
+    """op(X) of a stored complex matrix: 'N', 'T', or 'C' (conjugate transpose)."""
+    return X if t in "Nn" else (X.conj().T if t in "Cc" else X.T)
+
+
+@pytest.mark.parametrize("ta", CTRANS)
+@pytest.mark.parametrize("tb", CTRANS)
+def test_complex_gemm_of_real_matrices_is_dgemm(ta, tb):
+    """With zero imaginary parts (alpha = 1, beta = 0) the real part is
+    oracle_dgemm's result bit for bit (the same Dot2 sums, the extra terms
+    are exact zeros) and the imaginary part is zero; conjugation changes
+    nothing."""
+    rng = np.random.default_rng(600)
+    m, n, k = 7, 5, 33
+    ra, ca = stored_shape(ta, m, k)
+    rb, cb = stored_shape(tb, k, n)
+    Ar = np.asfortranarray(rng.uniform(-1, 1, (ra, ca)))
+    Br = np.asfortranarray(rng.uniform(-1, 1, (rb, cb)))
+    Cz = np.zeros((m, n), dtype=np.complex128, order="F")
+    oracle.complex_gemm(ta, tb, m, n, k, 1 + 0j, np.asfortranarray(Ar + 0j), ra,
+                        np.asfortranarray(Br + 0j), rb, 0j, Cz, m)
+    R = np.zeros((m, n), order="F")
+    oracle.gemm("N" if ta == "N" else "T", "N" if tb == "N" else "T", m, n, k, 1.0, Ar, ra, Br,
+                rb, 0.0, R, m)
+    np.testing.assert_array_equal(Cz.real, R)
+    assert np.all(Cz.imag == 0.0)
+
+
+@pytest.mark.parametrize("t", ["C", "c"])
+def test_complex_gemm_conjugate_transpose_is_transpose_of_the_conjugate(t):
+    """'C' on A gives exactly 'T' on conj(A) as stored (the imaginary plane
+    negated); likewise for B; 't' and any other character act as 'T'."""
+    rng = np.random.default_rng(601)
+    m, n, k = 6, 4, 9
+    A, B, C = _cplx(rng, (k, m)), _cplx(rng, (n, k)), _cplx(rng, (m, n))
+    for ta, tb, A2, B2, ta2, tb2 in ((t, "T", A, B, "T", "T"), ("T", t, A, B, "T", "T")):
+        R1, R2 = C.copy(order="F"), C.copy(order="F")
+        oracle.complex_gemm(ta, tb, m, n, k, 1.5 - 0.5j, A, k, B, n, 0.25j, R1, m)
+        Ac = np.asfortranarray(A.conj()) if ta in "Cc" else A
+        Bc = np.asfortranarray(B.conj()) if tb in "Cc" else B
+        oracle.complex_gemm(ta2, tb2, m, n, k, 1.5 - 0.5j, Ac, k, Bc, n, 0.25j, R2, m)
+        np.testing.assert_array_equal(R1, R2)
+    Bt = np.asfortranarray(B.T)
+    for other in ("t", "X"):
+        R1, R2 = C.copy(order="F"), C.copy(order="F")
+        oracle.complex_gemm(other, "N", m, n, k, 1 + 0j, A, k, Bt, k, 0j, R1, m)
+        oracle.complex_gemm("T", "N", m, n, k, 1 + 0j, A, k, Bt, k, 0j, R2, m)
+        np.testing.assert_array_equal(R1, R2)
+
+
+@pytest.mark.parametrize("ta", CTRANS)
+@pytest.mark.parametrize("tb", CTRANS)
+def test_complex_gemm_against_exact_rationals(ta, tb):
+    """Tiny shapes against the exact product (Python Fractions): the product
+    is rounded once per part, then alpha P + beta C costs a few roundings."""
+    from fractions import Fraction
+    rng = np.random.default_rng(602)
+    m, n, k = 3, 4, 5
+    ra, ca = stored_shape(ta, m, k)
+    rb, cb = stored_shape(tb, k, n)
+    A, B, C = _cplx(rng, (ra, ca)), _cplx(rng, (rb, cb)), _cplx(rng, (m, n))
+    alpha, beta = 0.75 - 1.25j, -0.5 + 0.125j
+    R = C.copy(order="F")
+    oracle.complex_gemm(ta, tb, m, n, k, alpha, A, ra, B, rb, beta, R, m)
+    oa, ob = _cop(A, ta), _cop(B, tb)
+    F = Fraction
+    for i in range(m):
+        for j in range(n):
+            pr = sum(F(oa[i, p].real) * F(ob[p, j].real) - F(oa[i, p].imag) * F(ob[p, j].imag)
+                     for p in range(k))
+            pi = sum(F(oa[i, p].real) * F(ob[p, j].imag) + F(oa[i, p].imag) * F(ob[p, j].real)
+                     for p in range(k))
+            cr, ci = F(C[i, j].real), F(C[i, j].imag)
+            wr = F(beta.real) * cr - F(beta.imag) * ci + F(alpha.real) * pr - F(alpha.imag) * pi
+            wi = F(beta.real) * ci + F(beta.imag) * cr + F(alpha.real) * pi + F(alpha.imag) * pr
+            scale = abs(alpha) * float(sum(abs(oa[i, p]) * abs(ob[p, j]) for p in range(k))) \
+                + abs(beta) * abs(C[i, j])
+            assert abs(R[i, j].real - float(wr)) <= 8 * U64 * scale, (i, j)
+            assert abs(R[i, j].imag - float(wi)) <= 8 * U64 * scale, (i, j)
+
+
+def test_complex_gemm_semantics():
+    """beta != 1 scales C first, also beta = 0 (NaN stays NaN); alpha = 0 or
+    k = 0 leaves beta C; beta = 1 with alpha = 0 leaves C untouched; the
+    single-precision form rounds the same Dot2 sums once to float."""
+    rng = np.random.default_rng(603)
+    m, n, k = 4, 3, 6
+    A, B, C = _cplx(rng, (m, k)), _cplx(rng, (k, n)), _cplx(rng, (m, n))
+    C[1, 2] = complex(np.nan, 0.0)
+    R = C.copy(order="F")
+    oracle.complex_gemm("N", "N", m, n, k, 1 + 0j, A, m, B, k, 0j, R, m)
+    assert np.isnan(R[1, 2].real) and np.isnan(R[1, 2].imag)
+    mask = np.ones((m, n), bool)
+    mask[1, 2] = False
+    np.testing.assert_allclose(R[mask], (A @ B)[mask], rtol=0, atol=1e-14)
+    for alpha, kk, beta in ((0j, k, 2 - 1j), (1 + 1j, 0, 0.5j), (0j, k, 1 + 0j)):
+        R = C.copy(order="F")
+        oracle.complex_gemm("N", "N", m, n, kk, alpha, A, m, B, k, beta, R, m)
+        want = C if beta == 1 else (beta.real * C.real - beta.imag * C.imag) + 1j * (
+            beta.real * C.imag + beta.imag * C.real)
+        np.testing.assert_array_equal(R, want)
+    A32, B32 = A.astype(np.complex64), B.astype(np.complex64)
+    C32 = np.asfortranarray(_cplx(rng, (m, n)).astype(np.complex64))
+    R32 = C32.copy(order="F")
+    oracle.complex_gemm("N", "C", m, n, k, 1 - 2j, A32, m, np.asfortranarray(B32.T), n, 0.5 + 0j,
+                        R32, m)
+    R64 = C32.astype(np.complex128)
+    oracle.complex_gemm("N", "C", m, n, k, 1 - 2j, A32.astype(np.complex128), m,
+                        np.asfortranarray(B32.T.astype(np.complex128)), n, 0.5 + 0j,
+                        np.asfortranarray(R64), m)
+    np.testing.assert_allclose(R32, R64, rtol=0, atol=4e-6 * np.max(np.abs(R64)))
