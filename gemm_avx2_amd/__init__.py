@@ -24,7 +24,7 @@ import os
 
 __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm64", "sgemm64", "zgemm", "cgemm",
-    "zgemm3m", "cgemm3m", "dgemm_dev",
+    "zgemm3m", "cgemm3m", "dtrmm", "dtrsm", "strmm", "strsm", "dgemm_dev",
     "sgemm_dev", "dgemm_strided_batched_dev", "sgemm_strided_batched_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
@@ -62,6 +62,11 @@ def _declare(lib, dtype):
     f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p, flp,
                   ctypes.c_void_p, _c_int_p, ctypes.c_void_p, _c_int_p, flp, ctypes.c_void_p,
                   _c_int_p]
+    for name in (("dtrmm_", "dtrsm_") if dtype == "d" else ("strmm_", "strsm_")):
+        f = getattr(lib, name)
+        f.restype = None
+        f.argtypes = [ctypes.c_char_p] * 4 + [_c_int_p, _c_int_p, flp, ctypes.c_void_p, _c_int_p,
+                                              ctypes.c_void_p, _c_int_p]
     for name in (("zgemm_", "zgemm3m_") if dtype == "d" else ("cgemm_", "cgemm3m_")):
         f = getattr(lib, name)
         f.restype = None
@@ -290,6 +295,36 @@ def cgemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=Tr
 
 def cgemm3m(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check=True):
     _complex("cgemm3m_", transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, check)
+
+
+def _tri(name, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check):
+    dtype = name[0]
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    fl = ctypes.c_double if dtype == "d" else ctypes.c_float
+    ch = [(c.encode() if isinstance(c, str) else bytes(c)) for c in (side, uplo, transa, diag)]
+    I = ctypes.c_int
+    getattr(lib, name)(*ch, ctypes.byref(I(m)), ctypes.byref(I(n)), ctypes.byref(fl(alpha)),
+                       _addr(a), ctypes.byref(I(lda)), _addr(b), ctypes.byref(I(ldb)))
+    if check:
+        _check(dtype)
+
+
+def dtrmm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check=True):
+    """dtrmm_: B := alpha op(A) B (side 'L') or alpha B op(A), A triangular."""
+    _tri("dtrmm_", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check)
+
+
+def dtrsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check=True):
+    """dtrsm_: solves op(A) X = alpha B (side 'L') or X op(A) = alpha B, X over B."""
+    _tri("dtrsm_", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check)
+
+
+def strmm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check=True):
+    _tri("strmm_", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check)
+
+
+def strsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check=True):
+    _tri("strsm_", side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, check)
 
 
 def _ch(t):

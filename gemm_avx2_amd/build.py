@@ -1,9 +1,10 @@
 """Builds the gfx950 libraries of this package in-tree.
 
 Outputs (gemm_avx2_amd/lib/):
-  DGEMM.so         dgemm_, its ILP64 form dgemm_64_, the dgemm_x_* extensions and
-                   the double complex zgemm_ / zgemm3m_
-  SGEMM.so         sgemm_, sgemm_64_, the sgemm_x_* extensions, cgemm_ / cgemm3m_
+  DGEMM.so         dgemm_, its ILP64 form dgemm_64_, the dgemm_x_* extensions, the
+                   double complex zgemm_ / zgemm3m_ and dtrmm_ / dtrsm_
+  SGEMM.so         sgemm_, sgemm_64_, the sgemm_x_* extensions, cgemm_ / cgemm3m_,
+                   strmm_ / strsm_
   libgemmx_gen.so  device-side input generator used by bench.py and the tests
   gemm_mi355x_tune.txt  the shipped gfx950 tuning table (tuning/), if present
 
@@ -31,7 +32,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["gemm_native.hip", "gemm_split.hip", "gemm_products.hip", "gemm_emulated.hip",
-               "gemm_batch.hip", "gemm_complex.hip"]
+               "gemm_batch.hip", "gemm_complex.hip", "gemm_tri.hip"]
 C_SOURCES = ["gemmx_api.c", "gemmx_tune.c", "gemmx_batch.c", "gemmx_stage.c"]
 GEN_SOURCES = ["gemmx_gen.hip"]
 # tuning table measured on MI355X by gemm_avx2_amd.autotune (--set full)
@@ -48,8 +49,8 @@ CC_FLAGS = [
     "-Wno-unused-parameter", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
 ]
 EXPORTS = {
-    "DGEMM.so": ["dgemm_", "dgemm_64_", "dgemm_x_*", "zgemm_", "zgemm3m_"],
-    "SGEMM.so": ["sgemm_", "sgemm_64_", "sgemm_x_*", "cgemm_", "cgemm3m_"],
+    "DGEMM.so": ["dgemm_", "dgemm_64_", "dgemm_x_*", "zgemm_", "zgemm3m_", "dtrmm_", "dtrsm_"],
+    "SGEMM.so": ["sgemm_", "sgemm_64_", "sgemm_x_*", "cgemm_", "cgemm3m_", "strmm_", "strsm_"],
 }
 
 

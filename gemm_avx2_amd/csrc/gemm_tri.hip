@@ -1,0 +1,240 @@
+/*
+ * gemm_tri.hip -- triangular multiply and solve (dtrmm_/strmm_,
+ * dtrsm_/strsm_) on the native real MFMA GEMM.
+ *
+ * The reference keeps these level-3 siblings outside its build as OpenBLAS
+ * kernels (wjc404/GEMM_AVX2 OpenBLAS-like_implementation/
+ * trmm_kernel_4x8_simplified.c, strsm_kernel_8x4_haswell_{LN,LT,RN,RT}.c,
+ * TRSM_KERNEL_{LN,LT,RN,RT}_simplified.c).  Here:
+ *
+ *   TRMM  B := alpha op(A) B (side L) or alpha B op(A) (side R): the stored
+ *         triangle of A (unit diagonal if diag 'U') is copied into a dense
+ *         matrix with zeros elsewhere and one GEMM writes the product into a
+ *         workspace, which is copied back into B.  Twice the triangular
+ *         flops, all of them on the matrix cores.
+ *   TRSM  op(A) X = alpha B (side L) or X op(A) = alpha B (side R), X over
+ *         B: blocked by TRI_NB rows (columns for side R).  The diagonal
+ *         blocks of op(A) are inverted by substitution (one thread per
+ *         column of the inverse, the block in LDS); then, block by block in
+ *         the direction op(A)'s triangle dictates, Y_i = inv(op(A)_ii) B_i
+ *         (GEMM into a workspace Y) and the blocks still to solve lose
+ *         op(A)_ji Y_i (GEMM with beta = 1 in place on B); finally
+ *         B = alpha Y.  The inverted blocks are only as accurate as their
+ *         conditioning allows (the usual trade of GPU TRSMs).
+ */
+#include "gemmx_device.h"
+#include "gemmx_internal.h"
+#include "gemmx_tune.h"
+
+namespace {
+
+constexpr int TRI_NB = 64;
+
+/* Dense k x k copy (leading dimension k) of the stored triangle of A:
+ * upper (row <= col) or lower, unit diagonal when `unit`, zeros elsewhere. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+tri_dense_kernel(const T *__restrict__ A, int64_t lda, int k, int upper, int unit,
+                 T *__restrict__ D) {
+  const int64_t total = (int64_t)k * k;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / k, r = t - c * k;
+    T v = (T)0;
+    if (r == c) v = unit ? (T)1 : A[r + c * lda];
+    else if (upper ? r < c : r > c) v = A[r + c * lda];
+    D[t] = v;
+  }
+}
+
+/* Inverses of the diagonal blocks of op(A) (k x k, blocks of TRI_NB, the
+ * last possibly smaller): block b's inverse, dense TRI_NB x TRI_NB with
+ * leading dimension TRI_NB, at Dinv + b TRI_NB^2.  op(A)(r, c) is A(r, c), or
+ * A(c, r) when `trans`; `lower` tells which triangle op(A) has.  One thread
+ * per column of the inverse: forward substitution on e_c for a lower block,
+ * backward for an upper one. */
+template <typename T>
+__global__ void __launch_bounds__(TRI_NB)
+tri_inv_kernel(const T *__restrict__ A, int64_t lda, int k, int trans, int lower, int unit,
+               T *__restrict__ Dinv) {
+  __shared__ T blk[TRI_NB][TRI_NB + 1]; /* blk[c][r] = op(A)(i0 + r, i0 + c) */
+  __shared__ T xs[TRI_NB][TRI_NB + 1];  /* xs[c][r]: column c of the inverse */
+  const int b = blockIdx.x, i0 = b * TRI_NB;
+  const int kb = k - i0 < TRI_NB ? k - i0 : TRI_NB;
+  const int t = threadIdx.x;
+  for (int c = 0; c < TRI_NB; ++c) {
+    T v = (T)0;
+    if (t < kb && c < kb) {
+      const int64_t gr = i0 + t, gc = i0 + c;
+      v = trans ? A[gc + gr * lda] : A[gr + gc * lda];
+    }
+    blk[c][t] = v;
+    xs[c][t] = (T)0;
+  }
+  __syncthreads();
+  const int c = t;
+  if (c < kb) {
+    T *x = xs[c];
+    if (lower) {
+      for (int r = c; r < kb; ++r) {
+        T s = r == c ? (T)1 : (T)0;
+        for (int q = c; q < r; ++q) s -= blk[q][r] * x[q];
+        x[r] = unit ? s : s / blk[r][r];
+      }
+    } else {
+      for (int r = c; r >= 0; --r) {
+        T s = r == c ? (T)1 : (T)0;
+        for (int q = r + 1; q <= c; ++q) s -= blk[q][r] * x[q];
+        x[r] = unit ? s : s / blk[r][r];
+      }
+    }
+  }
+  __syncthreads();
+  T *out = Dinv + (int64_t)b * TRI_NB * TRI_NB;
+  for (int cc = 0; cc < TRI_NB; ++cc) out[t + (int64_t)cc * TRI_NB] = xs[cc][t];
+}
+
+/* B = alpha Y (m x n), or B = 0 when `zero`. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+scale_copy_kernel(const T *__restrict__ Y, int64_t ldy, T *__restrict__ B, int64_t ldb, int m,
+                  int n, T alpha, int zero) {
+  const int64_t total = (int64_t)m * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = t / m, i = t - j * m;
+    B[i + j * ldb] = zero ? (T)0 : alpha * Y[i + j * ldy];
+  }
+}
+
+unsigned grid_for(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 8192) b = 8192;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+size_t rup(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+/* C (ldc) = alpha op(A) op(B) + beta C on the native kernels. */
+template <typename T>
+hipError_t gemm(int ta, int tb, int m, int n, int k, double alpha, const T *a, int64_t lda,
+                const T *b, int64_t ldb, double beta, T *c, int64_t ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  gemmx_problem q = {};
+  q.ta = ta;
+  q.tb = tb;
+  q.m = m;
+  q.n = n;
+  q.k = k;
+  q.lda = lda;
+  q.ldb = ldb;
+  q.ldc = ldc;
+  q.alpha = alpha;
+  q.beta = beta;
+  q.a = a;
+  q.b = b;
+  q.c = c;
+  constexpr int es = sizeof(T);
+  const int cfg = gemmx_tune_pick(es, ta, tb, m, n, k);
+  return es == 8 ? gemmx_native_f64(&q, cfg, s) : gemmx_native_f32(&q, cfg, s);
+}
+
+template <typename T>
+hipError_t trmm(const gemmx_tproblem *p, void *work, hipStream_t s) {
+  const int m = p->m, n = p->n, k = p->left ? m : n;
+  T *B = (T *)p->b;
+  if (p->alpha == 0.0) {
+    hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
+                       (const T *)nullptr, (int64_t)0, B, p->ldb, m, n, (T)0, 1);
+    return hipGetLastError();
+  }
+  T *D = (T *)work;
+  T *W = (T *)((char *)work + rup((size_t)k * k * sizeof(T), 256));
+  hipLaunchKernelGGL((tri_dense_kernel<T>), dim3(grid_for((int64_t)k * k)), dim3(256), 0, s,
+                     (const T *)p->a, p->lda, k, p->upper, p->unit, D);
+  hipError_t e = hipMemsetAsync(W, 0, (size_t)m * n * sizeof(T), s); /* beta = 0 reads W */
+  if (e != hipSuccess) return e;
+  e = p->left ? gemm<T>(p->trans, 0, m, n, m, p->alpha, D, k, B, p->ldb, 0.0, W, m, s)
+              : gemm<T>(0, p->trans, m, n, n, p->alpha, B, p->ldb, D, k, 0.0, W, m, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
+                     (const T *)W, (int64_t)m, B, p->ldb, m, n, (T)1, 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t trsm(const gemmx_tproblem *p, void *work, hipStream_t s) {
+  const int m = p->m, n = p->n, k = p->left ? m : n;
+  T *B = (T *)p->b;
+  const T *A = (const T *)p->a;
+  if (p->alpha == 0.0) {
+    hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
+                       (const T *)nullptr, (int64_t)0, B, p->ldb, m, n, (T)0, 1);
+    return hipGetLastError();
+  }
+  const int nblk = (k + TRI_NB - 1) / TRI_NB;
+  T *Dinv = (T *)work;
+  T *Y = (T *)((char *)work + rup((size_t)nblk * TRI_NB * TRI_NB * sizeof(T), 256));
+  /* op(A) is lower when (lower stored, no transpose) or (upper, transposed) */
+  const int lower = p->upper ? p->trans : !p->trans;
+  hipLaunchKernelGGL((tri_inv_kernel<T>), dim3(nblk), dim3(TRI_NB), 0, s, A, p->lda, k, p->trans,
+                     lower, p->unit, Dinv);
+  hipError_t e = hipMemsetAsync(Y, 0, (size_t)m * n * sizeof(T), s); /* beta = 0 reads Y */
+  if (e != hipSuccess) return e;
+  /* op(A)(r0.., c0..) as a GEMM operand: stored A at (r0, c0), or (c0, r0)
+   * read transposed */
+  auto opA = [&](int r0, int c0) {
+    return p->trans ? A + c0 + (int64_t)r0 * p->lda : A + r0 + (int64_t)c0 * p->lda;
+  };
+  /* Left: rows of B in blocks, forward when op(A) is lower.  Right: columns
+   * of B, forward when op(A) is upper. */
+  const bool forward = p->left ? lower : !lower;
+  for (int step = 0; step < nblk; ++step) {
+    const int bi = forward ? step : nblk - 1 - step;
+    const int i0 = bi * TRI_NB, kb = k - i0 < TRI_NB ? k - i0 : TRI_NB;
+    const T *Di = Dinv + (int64_t)bi * TRI_NB * TRI_NB;
+    if (p->left) {
+      /* Y_i = inv(op(A)_ii) B_i */
+      e = gemm<T>(0, 0, kb, n, kb, 1.0, Di, TRI_NB, B + i0, p->ldb, 0.0, Y + i0, m, s);
+      if (e != hipSuccess) return e;
+      /* rows still to solve: B_j -= op(A)_ji Y_i */
+      const int r0 = forward ? i0 + kb : 0, rn = forward ? k - (i0 + kb) : i0;
+      if (rn > 0) {
+        e = gemm<T>(p->trans, 0, rn, n, kb, -1.0, opA(r0, i0), p->lda, Y + i0, m, 1.0, B + r0,
+                    p->ldb, s);
+        if (e != hipSuccess) return e;
+      }
+    } else {
+      /* Y_i = B_i inv(op(A)_ii) */
+      e = gemm<T>(0, 0, m, kb, kb, 1.0, B + (int64_t)i0 * p->ldb, p->ldb, Di, TRI_NB, 0.0,
+                  Y + (int64_t)i0 * m, m, s);
+      if (e != hipSuccess) return e;
+      /* columns still to solve: B_j -= Y_i op(A)_ij */
+      const int c0 = forward ? i0 + kb : 0, cn = forward ? k - (i0 + kb) : i0;
+      if (cn > 0) {
+        e = gemm<T>(0, p->trans, m, cn, kb, -1.0, Y + (int64_t)i0 * m, m, opA(i0, c0), p->lda,
+                    1.0, B + (int64_t)c0 * p->ldb, p->ldb, s);
+        if (e != hipSuccess) return e;
+      }
+    }
+  }
+  hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
+                     (const T *)Y, (int64_t)m, B, p->ldb, m, n, (T)p->alpha, 0);
+  return hipGetLastError();
+}
+
+} // namespace
+
+extern "C" size_t gemmx_tri_workspace(int es, int solve, int left, int m, int n) {
+  if (m <= 0 || n <= 0) return 0;
+  const size_t k = left ? m : n;
+  const size_t first = solve ? ((k + TRI_NB - 1) / TRI_NB) * TRI_NB * TRI_NB : k * k;
+  return rup(first * es, 256) + rup((size_t)m * n * es, 256);
+}
+
+extern "C" hipError_t gemmx_tri(int es, int solve, const gemmx_tproblem *p, void *work,
+                                hipStream_t s) {
+  if (p->m <= 0 || p->n <= 0) return hipSuccess;
+  if (solve) return es == 8 ? trsm<double>(p, work, s) : trsm<float>(p, work, s);
+  return es == 8 ? trmm<double>(p, work, s) : trmm<float>(p, work, s);
+}

@@ -498,6 +498,111 @@ GEMMX_COMPLEX_ENTRY(cgemm_, float, 4, 0)
 GEMMX_COMPLEX_ENTRY(cgemm3m_, float, 4, 1)
 #undef GEMMX_COMPLEX_ENTRY
 
+/* ------------------------------------------------------------------------ */
+/* Triangular multiply and solve: dtrmm_/dtrsm_ and strmm_/strsm_, the      */
+/* level-3 siblings kept outside the reference's build                      */
+/* (OpenBLAS-like_implementation/trmm_kernel_4x8_simplified.c,               */
+/* strsm_kernel_8x4_haswell_{LN,LT,RN,RT}.c).  BLAS argument order; side    */
+/* 'L'/'l' = op(A) on the left, else right; uplo 'U'/'u' = upper stored     */
+/* triangle, else lower; transa as for dgemm_; diag 'U'/'u' = unit.  alpha  */
+/* = 0 sets B to zero without reading A or B, as BLAS specifies.            */
+/* ------------------------------------------------------------------------ */
+static void tri_call(int es, int solve, char side, char uplo, char transa, char diag, int m,
+                     int n, double alpha, const void *a, int lda, void *b, int ldb) {
+  clear_error();
+  const int left = side == 'L' || side == 'l';
+  const int k = left ? m : n;
+  if (m < 0 || n < 0) {
+    set_error(GEMMX_ERR_ARG, "negative dimension (m=%d n=%d)", m, n);
+    return;
+  }
+  if (m == 0 || n == 0) return;
+  if (ldb < (m > 1 ? m : 1)) {
+    set_error(GEMMX_ERR_ARG, "ldb=%d < max(1,m=%d)", ldb, m);
+    return;
+  }
+  if (!b) {
+    set_error(GEMMX_ERR_ARG, "B is NULL");
+    return;
+  }
+  if (alpha != 0.0) {
+    if (lda < (k > 1 ? k : 1)) {
+      set_error(GEMMX_ERR_ARG, "lda=%d < max(1,%d)", lda, k);
+      return;
+    }
+    if (!a) {
+      set_error(GEMMX_ERR_ARG, "A is NULL");
+      return;
+    }
+  }
+  int dev;
+  if (current_device(&dev) != GEMMX_OK) return;
+  const int host_a = alpha != 0.0 && !is_device_ptr(a), host_b = !is_device_ptr(b);
+  const size_t e8 = (size_t)es;
+  const size_t lda2 = host_a ? (size_t)k : (size_t)lda, ldb2 = host_b ? (size_t)m : (size_t)ldb;
+  const size_t bytes_a = host_a ? round_up(lda2 * k * e8, 256) : 0;
+  const size_t bytes_b = host_b ? round_up(ldb2 * (size_t)n * e8, 256) : 0;
+  const size_t bytes_w = alpha != 0.0 ? round_up(gemmx_tri_workspace(es, solve, left, m, n), 256) : 0;
+  const size_t total = bytes_a + bytes_b + bytes_w;
+  gemmx_tproblem p = {.left = left, .upper = uplo == 'U' || uplo == 'u', .trans = !notrans(transa),
+                      .unit = diag == 'U' || diag == 'u', .m = m, .n = n,
+                      .lda = (int64_t)lda2, .ldb = (int64_t)ldb2, .alpha = alpha, .a = a,
+                      .b = b};
+  hipStream_t s = NULL;
+  devstate_t *d = &g_dev[dev];
+  pthread_mutex_lock(&d->lock);
+  char *base = NULL;
+  if (total > 0 && get_scratch(dev, total, (void **)&base) != GEMMX_OK) goto out;
+  char *pa = base, *pb = pa + bytes_a, *pw = pb + bytes_b;
+  hipError_t e;
+  if (host_a) {
+    if (gemmx_stage_h2d(dev, pa, lda2 * e8, a, (size_t)lda * e8, (size_t)k * e8, (size_t)k,
+                        s) != GEMMX_OK)
+      goto out;
+    p.a = pa;
+  }
+  if (host_b) {
+    if (gemmx_stage_h2d(dev, pb, ldb2 * e8, b, (size_t)ldb * e8, (size_t)m * e8, (size_t)n,
+                        s) != GEMMX_OK)
+      goto out;
+    p.b = pb;
+  }
+  e = gemmx_tri(es, solve, &p, pw, s);
+  if (e != hipSuccess) {
+    hip_fail(e, solve ? "triangular solve launch" : "triangular multiply launch");
+    (void)hipStreamSynchronize(s);
+    goto out;
+  }
+  if (host_b && gemmx_stage_d2h(dev, b, (size_t)ldb * e8, pb, ldb2 * e8, (size_t)m * e8,
+                                 (size_t)n, s) != GEMMX_OK) {
+    (void)hipStreamSynchronize(s);
+    goto out;
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) hip_fail(e, solve ? "triangular solve" : "triangular multiply");
+out:
+  pthread_mutex_unlock(&d->lock);
+}
+
+#define GEMMX_TRI_ENTRY(NAME, T, ES, SOLVE)                                                      \
+  EXPORT void NAME(const char *side, const char *uplo, const char *transa, const char *diag,   \
+                   const int *m, const int *n, const T *alpha, const T *a, const int *lda,     \
+                   T *b, const int *ldb) {                                                     \
+    if (!side || !uplo || !transa || !diag || !m || !n || !alpha || !lda || !ldb) {            \
+      clear_error();                                                                           \
+      set_error(GEMMX_ERR_ARG, #NAME ": NULL scalar argument");                                \
+      return;                                                                                  \
+    }                                                                                          \
+    pthread_once(&g_once, init_once);                                                          \
+    tri_call(ES, SOLVE, *side, *uplo, *transa, *diag, *m, *n, (double)*alpha, a, *lda, b,      \
+             *ldb);                                                                            \
+  }
+GEMMX_TRI_ENTRY(dtrmm_, double, 8, 0)
+GEMMX_TRI_ENTRY(dtrsm_, double, 8, 1)
+GEMMX_TRI_ENTRY(strmm_, float, 4, 0)
+GEMMX_TRI_ENTRY(strsm_, float, 4, 1)
+#undef GEMMX_TRI_ENTRY
+
 /* ILP64 forms: the same calls with 64-bit Fortran integers (callers built
  * with -fdefault-integer-8 and the "_64_" suffix of ILP64 BLAS builds).
  * Leading dimensions are used at full width; m, n and k must stay below

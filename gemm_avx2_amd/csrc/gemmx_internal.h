@@ -55,6 +55,22 @@ typedef struct gemmx_cproblem {
 size_t gemmx_complex_workspace(int es, int ta, int tb, int m, int n, int k, int three);
 hipError_t gemmx_complex(int es, const gemmx_cproblem *p, int three, void *work, hipStream_t s);
 
+/* A triangular multiply or solve (dtrmm_/dtrsm_ and the s forms): side
+ * left (op(A) B / op(A) X = alpha B) or right, A's stored triangle upper or
+ * lower, op(A) = A or A^T, unit diagonal or not. */
+typedef struct gemmx_tproblem {
+  int left, upper, trans, unit;
+  int m, n;
+  int64_t lda, ldb;
+  double alpha;
+  const void *a;
+  void *b;
+} gemmx_tproblem;
+/* Device workspace of gemmx_tri, and the operation itself (gemm_tri.hip;
+ * solve 1 = TRSM, 0 = TRMM; es 8 or 4). */
+size_t gemmx_tri_workspace(int es, int solve, int left, int m, int n);
+hipError_t gemmx_tri(int es, int solve, const gemmx_tproblem *p, void *work, hipStream_t s);
+
 /* Kernel configurations of the native MFMA tile kernels.  The tuning table
  * (gemmx_tune.c) picks one per (dtype, trans, shape bucket). */
 enum {

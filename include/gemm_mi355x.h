@@ -81,6 +81,31 @@ void cgemm3m_(const char *transa, const char *transb, const int *m, const int *n
               const float *b, const int *ldb, const float *beta, float *c,
               const int *ldc);
 
+/* Triangular multiply (TRMM: B := alpha op(A) B or alpha B op(A)) and solve
+ * (TRSM: op(A) X = alpha B or X op(A) = alpha B, X overwriting B), the
+ * triangular level-3 siblings the reference keeps outside its build as
+ * OpenBLAS kernels (OpenBLAS-like_implementation/trmm_kernel_4x8_simplified.c,
+ * strsm_kernel_8x4_haswell_{LN,LT,RN,RT}.c), with the BLAS argument order.
+ * side 'L'/'l' = op(A) on the left (A is m x m), anything else = right (A is
+ * n x n); uplo 'U'/'u' = A's upper triangle is stored, else the lower; the
+ * other triangle is never used; transa as for dgemm_; diag 'U'/'u' = unit
+ * diagonal (not read); alpha = 0 sets B to zero without reading A or B (as
+ * BLAS specifies).  Synchronous; host or device operands; errors as for
+ * dgemm_.  TRMM runs one MFMA GEMM on a dense copy of the triangle; TRSM
+ * inverts 64 x 64 diagonal blocks and updates the rest with MFMA GEMMs. */
+void dtrmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const double *alpha, const double *a, const int *lda,
+            double *b, const int *ldb);
+void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const double *alpha, const double *a, const int *lda,
+            double *b, const int *ldb);
+void strmm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const float *alpha, const float *a, const int *lda,
+            float *b, const int *ldb);
+void strsm_(const char *side, const char *uplo, const char *transa, const char *diag,
+            const int *m, const int *n, const float *alpha, const float *a, const int *lda,
+            float *b, const int *ldb);
+
 /* ILP64 forms of the same entry points (no counterpart in the reference,
  * which is LP64 only: src/gemm_driver.c:118 takes int): every integer
  * argument is a 64-bit integer, as for Fortran callers built with 64-bit

@@ -110,6 +110,13 @@ def lib():
         f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                       ctypes.POINTER(fl), ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                       ctypes.c_int, ctypes.POINTER(fl), ctypes.c_void_p, ctypes.c_int]
+    for fn, fl in (("oracle_trmm", ctypes.c_double), ("oracle_trsm", ctypes.c_double),
+                   ("oracle_trmm_f32", ctypes.c_float), ("oracle_trsm_f32", ctypes.c_float)):
+        f = getattr(L, fn)
+        f.restype = None
+        f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_char, ctypes.c_int,
+                      ctypes.c_int, fl, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                      ctypes.c_int]
     L.oracle_bf16_rne.restype = ctypes.c_uint16
     L.oracle_bf16_rne.argtypes = [ctypes.c_float]
     L.oracle_bf16_split.restype = None
@@ -210,6 +217,14 @@ def complex_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     fn = lib().oracle_cgemm if single else lib().oracle_zgemm
     fn(transa.encode()[:1], transb.encode()[:1], m, n, k, al, _ptr(a), lda, _ptr(b), ldb, be,
        _ptr(c), ldc)
+
+
+def tri(solve, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
+    """TRSM (solve) or TRMM on float64 or float32 Fortran arrays, in place on b."""
+    single = b.dtype == np.float32
+    name = ("oracle_trsm" if solve else "oracle_trmm") + ("_f32" if single else "")
+    ch = [c.encode()[:1] for c in (side, uplo, transa, diag)]
+    getattr(lib(), name)(*ch, m, n, alpha, _ptr(a), lda, _ptr(b), ldb)
 
 
 def ozaki2_modulus(j):

@@ -575,6 +575,136 @@ void oracle_cgemm(char transa, char transb, int m, int n, int k, const float alp
 }
 
 /* ---------------------------------------------------------------------- */
+/* Triangular multiply and solve (dtrmm_/dtrsm_, strmm_/strsm_ of the      */
+/* library; the reference's own kernels for them live outside its build:   */
+/* OpenBLAS-like_implementation/trmm_kernel_4x8_simplified.c,               */
+/* TRSM_KERNEL_{LN,LT,RN,RT}_simplified.c).  BLAS operations with the       */
+/* library's character rules (side 'L'/'l' left else right, uplo 'U'/'u'   */
+/* upper else lower, transa as dgemm_, diag 'U'/'u' unit); alpha = 0 sets B */
+/* to zero.  Sums are Dot2-accumulated; the solve is substitution.          */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  const void *a;
+  int64_t lda;
+  int upper, trans, unit, single;
+} tri_t;
+
+/* op(A)(r, c) within op(A)'s triangle (0 outside it; 1 on a unit diagonal) */
+static double tri_at(const tri_t *t, int r, int c) {
+  const int sr = t->trans ? c : r, sc = t->trans ? r : c; /* stored indices */
+  if (sr == sc && t->unit) return 1.0;
+  if (t->upper ? sr > sc : sr < sc) return 0.0;
+  const int64_t i = sr + (int64_t)sc * t->lda;
+  return t->single ? (double)((const float *)t->a)[i] : ((const double *)t->a)[i];
+}
+
+static double bget(const void *b, int single, int64_t i) {
+  return single ? (double)((const float *)b)[i] : ((const double *)b)[i];
+}
+static void bset(void *b, int single, int64_t i, double v) {
+  if (single) ((float *)b)[i] = (float)v;
+  else ((double *)b)[i] = v;
+}
+
+static void tri_op(int solve, char side, char uplo, char transa, char diag, int m, int n,
+                   double alpha, const void *a, int lda, void *b, int ldb, int single) {
+  if (m <= 0 || n <= 0) return;
+  const int left = side == 'L' || side == 'l';
+  const tri_t t = {a, lda, uplo == 'U' || uplo == 'u', !is_notrans(transa),
+                   diag == 'U' || diag == 'u', single};
+  if (alpha == 0.0) {
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < m; i++) bset(b, single, i + (int64_t)j * ldb, 0.0);
+    return;
+  }
+  const int k = left ? m : n;
+  /* op(A) lower: (lower stored, no transpose) or (upper, transposed) */
+  const int lower = t.upper ? t.trans : !t.trans;
+  if (left) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < n; j++) {
+      double *col = (double *)malloc(sizeof(double) * k);
+      double *out = (double *)malloc(sizeof(double) * k);
+      for (int i = 0; i < k; i++) col[i] = bget(b, single, i + (int64_t)j * ldb);
+      if (!solve) {
+        for (int i = 0; i < k; i++) {
+          dd_t s = {0.0, 0.0};
+          for (int q = 0; q < k; q++) {
+            const double v = tri_at(&t, i, q);
+            if (v != 0.0) dot2_step(&s, v, col[q]);
+          }
+          out[i] = alpha * dd_round(s);
+        }
+      } else {
+        for (int step = 0; step < k; step++) {
+          const int i = lower ? step : k - 1 - step;
+          dd_t s = {alpha * col[i], 0.0};
+          for (int q = 0; q < k; q++) {
+            if (lower ? q >= i : q <= i) continue;
+            const double v = tri_at(&t, i, q);
+            if (v != 0.0) dot2_step(&s, -v, out[q]);
+          }
+          out[i] = t.unit ? dd_round(s) : dd_round(s) / tri_at(&t, i, i);
+        }
+      }
+      for (int i = 0; i < k; i++) bset(b, single, i + (int64_t)j * ldb, out[i]);
+      free(col);
+      free(out);
+    }
+  } else {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < m; i++) {
+      double *row = (double *)malloc(sizeof(double) * k);
+      double *out = (double *)malloc(sizeof(double) * k);
+      for (int j = 0; j < k; j++) row[j] = bget(b, single, i + (int64_t)j * ldb);
+      if (!solve) {
+        for (int j = 0; j < k; j++) {
+          dd_t s = {0.0, 0.0};
+          for (int q = 0; q < k; q++) {
+            const double v = tri_at(&t, q, j);
+            if (v != 0.0) dot2_step(&s, row[q], v);
+          }
+          out[j] = alpha * dd_round(s);
+        }
+      } else {
+        /* x op(A) = alpha b: column j of op(A) has its entries at rows
+         * q <= j (upper) or q >= j (lower) */
+        for (int step = 0; step < k; step++) {
+          const int j = lower ? k - 1 - step : step;
+          dd_t s = {alpha * row[j], 0.0};
+          for (int q = 0; q < k; q++) {
+            if (lower ? q <= j : q >= j) continue;
+            const double v = tri_at(&t, q, j);
+            if (v != 0.0) dot2_step(&s, -out[q], v);
+          }
+          out[j] = t.unit ? dd_round(s) : dd_round(s) / tri_at(&t, j, j);
+        }
+      }
+      for (int j = 0; j < k; j++) bset(b, single, i + (int64_t)j * ldb, out[j]);
+      free(row);
+      free(out);
+    }
+  }
+}
+
+void oracle_trmm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                 const double *a, int lda, double *b, int ldb) {
+  tri_op(0, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, 0);
+}
+void oracle_trsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                 const double *a, int lda, double *b, int ldb) {
+  tri_op(1, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, 0);
+}
+void oracle_trmm_f32(char side, char uplo, char transa, char diag, int m, int n, float alpha,
+                     const float *a, int lda, float *b, int ldb) {
+  tri_op(0, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, 1);
+}
+void oracle_trsm_f32(char side, char uplo, char transa, char diag, int m, int n, float alpha,
+                     const float *a, int lda, float *b, int ldb) {
+  tri_op(1, side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb, 1);
+}
+
+/* ---------------------------------------------------------------------- */
 /* Ozaki-II (CRT).  No reference counterpart: restates the emulated DGEMM  */
 /* of the library's modular form.  Per row i of op(A) the exponent e_i      */
 /* (oracle_ozaki_exponent), the bit count P_i (oracle_ozaki2_vector_bits)   */

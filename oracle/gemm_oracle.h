@@ -118,6 +118,22 @@ void oracle_cgemm(char transa, char transb, int m, int n, int k, const float alp
                   const float *a, int lda, const float *b, int ldb, const float beta[2],
                   float *c, int ldc);
 
+/* ---- triangular multiply / solve (dtrmm_/dtrsm_ restatement) ---------- */
+/* TRMM: B := alpha op(A) B (side 'L'/'l') or alpha B op(A) (else); TRSM:
+ * op(A) X = alpha B or X op(A) = alpha B, X over B.  uplo 'U'/'u' upper
+ * stored triangle else lower (the other triangle is never read), transa as
+ * for dgemm, diag 'U'/'u' unit (diagonal not read); alpha = 0 sets B to
+ * zero.  Dot2-accumulated sums; substitution for the solve.  The _f32 forms
+ * read and write float and compute the same way. */
+void oracle_trmm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                 const double *a, int lda, double *b, int ldb);
+void oracle_trsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                 const double *a, int lda, double *b, int ldb);
+void oracle_trmm_f32(char side, char uplo, char transa, char diag, int m, int n, float alpha,
+                     const float *a, int lda, float *b, int ldb);
+void oracle_trsm_f32(char side, char uplo, char transa, char diag, int m, int n, float alpha,
+                     const float *a, int lda, float *b, int ldb);
+
 /* ---- Ozaki-II, CRT over INT8 residues (emulated DGEMM restatement) ---- */
 #define OZAKI2_MIN_MODULI 8
 #define OZAKI2_MAX_MODULI 22

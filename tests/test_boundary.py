@@ -40,8 +40,8 @@ def test_exports_match_header():
     decl = _declared()
     d, s = _exports("DGEMM.so"), _exports("SGEMM.so")
     assert "dgemm_" in d and "sgemm_" in s
-    assert d == {f for f in decl if f.startswith(("dgemm_", "zgemm"))}
-    assert s == {f for f in decl if f.startswith(("sgemm_", "cgemm"))}
+    assert d == {f for f in decl if f.startswith(("dgemm_", "zgemm", "dtr"))}
+    assert s == {f for f in decl if f.startswith(("sgemm_", "cgemm", "str"))}
     assert not (d & s)
 
 

@@ -661,3 +661,123 @@ def test_complex_gemm_semantics():
                         np.asfortranarray(B32.T.astype(np.complex128)), n, 0.5 + 0j,
                         np.asfortranarray(R64), m)
     np.testing.assert_allclose(R32, R64, rtol=0, atol=4e-6 * np.max(np.abs(R64)))
+
+
+# --------------------------------------------------------------- triangular
+TRI_CASES = [(s, u, t, d) for s in "LR" for u in "UL" for t in "NT" for d in "NU"]
+
+
+def _tri_dense(A, uplo, diag):
+    T = np.triu(A) if uplo == "U" else np.tril(A)
+    if diag == "U":
+        np.fill_diagonal(T, 1.0)
+    return T
+
+
+@pytest.mark.parametrize("side,uplo,trans,diag", TRI_CASES)
+def test_triangular_ops_never_read_the_other_triangle_or_a_unit_diagonal(side, uplo, trans,
+                                                                          diag):
+    """NaN in the unused triangle (and on a unit diagonal) changes nothing:
+    the results equal those with zeros there, bit for bit."""
+    rng = np.random.default_rng(800)
+    m, n = 9, 6
+    k = m if side == "L" else n
+    A = np.asfortranarray(rng.uniform(-1, 1, (k, k)) / k + np.diag(2 + rng.uniform(0, 1, k)))
+    clean = np.asfortranarray(_tri_dense(A, uplo, "N"))
+    dirty = clean.copy(order="F")
+    other = np.tril(np.ones((k, k)), -1) if uplo == "U" else np.triu(np.ones((k, k)), 1)
+    dirty[other == 1] = np.nan
+    if diag == "U":
+        np.fill_diagonal(dirty, np.nan)
+    B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    for solve in (0, 1):
+        X1, X2 = B.copy(order="F"), B.copy(order="F")
+        oracle.tri(solve, side, uplo, trans, diag, m, n, 1.25, clean, k, X1, m)
+        oracle.tri(solve, side, uplo, trans, diag, m, n, 1.25, dirty, k, X2, m)
+        np.testing.assert_array_equal(X1, X2)
+
+
+@pytest.mark.parametrize("side,uplo,trans,diag", TRI_CASES)
+def test_triangular_ops_on_exact_integer_systems(side, uplo, trans, diag):
+    """Integer triangles with unit or power-of-two diagonals and integer X:
+    B = op(A) X (or X op(A)) is exact in FP64, TRMM must return it exactly
+    and TRSM must recover X exactly (every substitution step is exact)."""
+    rng = np.random.default_rng(801)
+    m, n = 8, 5
+    k = m if side == "L" else n
+    A = np.asfortranarray(rng.integers(-3, 4, (k, k)).astype(np.float64))
+    np.fill_diagonal(A, 2.0 ** rng.integers(0, 3, k) * rng.choice([-1.0, 1.0], k))
+    T = _tri_dense(A, uplo, diag)
+    opT = T.T if trans == "T" else T
+    X = np.asfortranarray(rng.integers(-5, 6, (m, n)).astype(np.float64))
+    B = np.asfortranarray(opT @ X if side == "L" else X @ opT)
+    got = X.copy(order="F")
+    oracle.tri(0, side, uplo, trans, diag, m, n, 1.0, A, k, got, m)
+    np.testing.assert_array_equal(got, B)
+    sol = B.copy(order="F")
+    oracle.tri(1, side, uplo, trans, diag, m, n, 1.0, A, k, sol, m)
+    np.testing.assert_array_equal(sol, X)
+
+
+@pytest.mark.parametrize("side,uplo,trans,diag", TRI_CASES[::3])
+def test_triangular_solve_against_exact_rationals(side, uplo, trans, diag):
+    """A random well-conditioned system solved exactly with Fractions: the
+    oracle's X is within a few units in the last place of it."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(802)
+    m, n = 5, 4
+    k = m if side == "L" else n
+    A = rng.uniform(-1, 1, (k, k)) / k + np.diag(2 + rng.uniform(0, 1, k))
+    A = np.asfortranarray(A)
+    T = _tri_dense(A, uplo, diag)
+    opT = [[F(v) for v in row] for row in (T.T if trans == "T" else T)]
+    B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    sol = B.copy(order="F")
+    oracle.tri(1, side, uplo, trans, diag, m, n, 0.75, A, k, sol, m)
+    # exact solve by Gaussian elimination on the (triangular) system
+    def solve(M, rhs):
+        N = len(M)
+        aug = [list(M[i]) + [rhs[i]] for i in range(N)]
+        for c in range(N):
+            p = next(r for r in range(c, N) if aug[r][c] != 0)
+            aug[c], aug[p] = aug[p], aug[c]
+            for r in range(N):
+                if r != c and aug[r][c] != 0:
+                    f = aug[r][c] / aug[c][c]
+                    aug[r] = [x - f * y for x, y in zip(aug[r], aug[c])]
+        return [aug[i][N] / aug[i][i] for i in range(N)]
+    if side == "L":
+        for j in range(n):
+            x = solve(opT, [F(0.75) * F(B[i, j]) for i in range(m)])
+            for i in range(m):
+                assert abs(sol[i, j] - float(x[i])) <= 16 * U64 * max(1.0, abs(float(x[i]))), (i, j)
+    else:
+        opTt = [list(col) for col in zip(*opT)]  # x op(A) = b  <=>  op(A)^T x^T = b^T
+        for i in range(m):
+            x = solve(opTt, [F(0.75) * F(B[i, j]) for j in range(n)])
+            for j in range(n):
+                assert abs(sol[i, j] - float(x[j])) <= 16 * U64 * max(1.0, abs(float(x[j]))), (i, j)
+
+
+def test_triangular_alpha_zero_and_single_precision():
+    """alpha = 0 writes zeros without reading A or B (NaN in both); the
+    float forms equal the double computation rounded to float within the
+    float rounding of the sums."""
+    rng = np.random.default_rng(803)
+    m, n = 6, 4
+    A = np.full((m, m), np.nan, order="F")
+    B = np.full((m, n), np.nan, order="F")
+    for solve in (0, 1):
+        X = B.copy(order="F")
+        oracle.tri(solve, "L", "U", "N", "N", m, n, 0.0, A, m, X, m)
+        assert np.all(X == 0.0)
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, m)) / m + np.diag(2 + rng.uniform(0, 1, m)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    for solve in (0, 1):
+        X32 = B.astype(np.float32, order="F")
+        oracle.tri(solve, "L", "L", "T", "N", m, n, 1.5, A.astype(np.float32, order="F"), m,
+                   X32, m)
+        X64 = B.astype(np.float32).astype(np.float64, order="F")
+        oracle.tri(solve, "L", "L", "T", "N", m, n, 1.5,
+                   A.astype(np.float32).astype(np.float64, order="F"), m, X64, m)
+        np.testing.assert_allclose(X32, X64, rtol=4 * U32 * m, atol=0)
