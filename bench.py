@@ -497,9 +497,11 @@ def run_strided(torch, gx, stream, steps, warmup):
 
 
 def run_complex(torch, gx, stream, steps, warmup):
-    """zgemm_/cgemm_ (four real products) and the 3m forms (three) at 4096^3
-    NN on device tensors, alpha 1, beta 0; rate = 8mnk / t (complex flops).
-    The Fortran-ABI call synchronises, so HIP events bracket whole calls."""
+    """The level-3 siblings: zgemm_/cgemm_ (four real products) and the 3m
+    forms (three) at 4096^3 NN on device tensors, alpha 1, beta 0, rate
+    8mnk / t (complex flops); dtrmm_/dtrsm_ and the s forms at 8192 (see
+    below).  The Fortran-ABI calls synchronise, so HIP events bracket whole
+    calls."""
     m = n = k = 4096
     out = {}
     for prec, forms in (("z", (("zgemm", gx.zgemm), ("zgemm3m", gx.zgemm3m))),
@@ -520,6 +522,25 @@ def run_complex(torch, gx, stream, steps, warmup):
                          "ms": round(median(per), 3)}
         del A, B, C, bufs
     out["shape"] = "4096^3 NN, device operands, alpha 1, beta 0"
+    # triangular: B (8192 x 8192) with A 8192 x 8192 lower, left, no transpose,
+    # well-conditioned; rate = m^2 n / t (the triangular flop count)
+    m = n = 8192
+    for prec, mul, solve in (("d", gx.dtrmm, gx.dtrsm), ("s", gx.strmm, gx.strsm)):
+        tdt = torch.float64 if prec == "d" else torch.float32
+        A = torch.empty(m * m, dtype=tdt, device="cuda")
+        B = torch.empty(m * n, dtype=tdt, device="cuda")
+        gx.fill(A, m, m, m, SEED, 20, stream=stream)
+        gx.fill(B, m, n, m, SEED, 21, stream=stream)
+        A.view(m, m).diagonal().add_(float(m))  # diagonally dominant
+        torch.cuda.synchronize()
+        for name, f in ((prec + "trmm", mul), (prec + "trsm", solve)):
+            def call():
+                f("L", "L", "N", "N", m, n, 1.0, A, m, B, m)
+            _, per = time_launches(torch, call, max(2, steps // 2), 1)
+            out[name] = {"tflops": round(1.0 * m * m * n / median(per) / 1e9, 2),
+                         "ms": round(median(per), 3)}
+        del A, B
+    out["triangular_shape"] = "m = n = 8192, side L, lower, no transpose, device operands"
     return out
 
 
@@ -608,8 +629,8 @@ def main():
                         ("batch", lambda: run_batch(torch, gx, stream, max(3, args.steps // 2), 1)),
                         ("strided_batch", lambda: run_strided(torch, gx, stream,
                                                               max(3, args.steps // 2), 1)),
-                        ("complex_gemm", lambda: run_complex(torch, gx, stream,
-                                                             max(3, args.steps // 2), 1)),
+                        ("level3_siblings", lambda: run_complex(torch, gx, stream,
+                                                                max(3, args.steps // 2), 1)),
                         ("dgemm_host_pointers", lambda: run_host_pointer(torch, gx))):
             result[key] = fn()
             torch.cuda.empty_cache()

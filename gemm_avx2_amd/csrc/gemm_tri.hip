@@ -13,14 +13,14 @@
  *         workspace, which is copied back into B.  Twice the triangular
  *         flops, all of them on the matrix cores.
  *   TRSM  op(A) X = alpha B (side L) or X op(A) = alpha B (side R), X over
- *         B: blocked by TRI_NB rows (columns for side R).  The diagonal
- *         blocks of op(A) are inverted by substitution (one thread per
- *         column of the inverse, the block in LDS); then, block by block in
- *         the direction op(A)'s triangle dictates, Y_i = inv(op(A)_ii) B_i
- *         (GEMM into a workspace Y) and the blocks still to solve lose
- *         op(A)_ji Y_i (GEMM with beta = 1 in place on B); finally
- *         B = alpha Y.  The inverted blocks are only as accurate as their
- *         conditioning allows (the usual trade of GPU TRSMs).
+ *         B.  The TRI_NB x TRI_NB diagonal blocks of op(A) are inverted by
+ *         substitution (one thread per column of the inverse, the block in
+ *         LDS); the solve recurses on halves (trsm_rec): a block is
+ *         Y_i = inv(op(A)_ii) B_i (a GEMM into the workspace Y), and between
+ *         the two halves of a range one GEMM with beta = 1 subtracts the
+ *         solved half from the other in place on B; finally B = alpha Y.
+ *         The inverted blocks are only as accurate as their conditioning
+ *         allows (the usual trade of GPU TRSMs).
  */
 #include "gemmx_device.h"
 #include "gemmx_internal.h"
@@ -162,6 +162,47 @@ hipError_t trmm(const gemmx_tproblem *p, void *work, hipStream_t s) {
   return hipGetLastError();
 }
 
+/* Solves the rows (side L) or columns (side R) i0 .. i0 + kk of op(A) Y = B
+ * (Y op(A) = B) into Y, B holding the right-hand sides less the parts already
+ * solved: one TRI_NB block is Y_i = inv(op(A)_ii) B_i (or B_i inv(op(A)_ii));
+ * a larger range splits at a multiple of TRI_NB near its middle, solves the
+ * half op(A)'s triangle puts first, subtracts its contribution from the
+ * other half with one GEMM of depth = the solved half, and solves that
+ * half.  Most flops thus land in GEMMs of large depth. */
+template <typename T>
+hipError_t trsm_rec(const gemmx_tproblem *p, const T *Dinv, T *Y, int lower, int i0, int kk,
+                    hipStream_t s) {
+  const int m = p->m, n = p->n;
+  T *B = (T *)p->b;
+  const T *A = (const T *)p->a;
+  if (kk <= TRI_NB) {
+    const T *Di = Dinv + (int64_t)(i0 / TRI_NB) * TRI_NB * TRI_NB;
+    return p->left
+               ? gemm<T>(0, 0, kk, n, kk, 1.0, Di, TRI_NB, B + i0, p->ldb, 0.0, Y + i0, m, s)
+               : gemm<T>(0, 0, m, kk, kk, 1.0, B + (int64_t)i0 * p->ldb, p->ldb, Di, TRI_NB, 0.0,
+                         Y + (int64_t)i0 * m, m, s);
+  }
+  const int h = ((kk / 2 + TRI_NB - 1) / TRI_NB) * TRI_NB; /* 0 < h < kk */
+  /* left: rows in order when op(A) is lower; right: columns in order when
+   * op(A) is upper */
+  const bool forward = p->left ? lower : !lower;
+  const int s1 = forward ? i0 : i0 + h, k1 = forward ? h : kk - h; /* solved first */
+  const int s2 = forward ? i0 + h : i0, k2 = kk - k1;              /* then */
+  hipError_t e = trsm_rec<T>(p, Dinv, Y, lower, s1, k1, s);
+  if (e != hipSuccess) return e;
+  /* op(A)(r0.., c0..) as a GEMM operand: stored A at (r0, c0), or at (c0, r0)
+   * read transposed */
+  auto opA = [&](int r0, int c0) {
+    return p->trans ? A + c0 + (int64_t)r0 * p->lda : A + r0 + (int64_t)c0 * p->lda;
+  };
+  e = p->left ? gemm<T>(p->trans, 0, k2, n, k1, -1.0, opA(s2, s1), p->lda, Y + s1, m, 1.0,
+                        B + s2, p->ldb, s)
+              : gemm<T>(0, p->trans, m, k2, k1, -1.0, Y + (int64_t)s1 * m, m, opA(s1, s2),
+                        p->lda, 1.0, B + (int64_t)s2 * p->ldb, p->ldb, s);
+  if (e != hipSuccess) return e;
+  return trsm_rec<T>(p, Dinv, Y, lower, s2, k2, s);
+}
+
 template <typename T>
 hipError_t trsm(const gemmx_tproblem *p, void *work, hipStream_t s) {
   const int m = p->m, n = p->n, k = p->left ? m : n;
@@ -181,43 +222,8 @@ hipError_t trsm(const gemmx_tproblem *p, void *work, hipStream_t s) {
                      lower, p->unit, Dinv);
   hipError_t e = hipMemsetAsync(Y, 0, (size_t)m * n * sizeof(T), s); /* beta = 0 reads Y */
   if (e != hipSuccess) return e;
-  /* op(A)(r0.., c0..) as a GEMM operand: stored A at (r0, c0), or (c0, r0)
-   * read transposed */
-  auto opA = [&](int r0, int c0) {
-    return p->trans ? A + c0 + (int64_t)r0 * p->lda : A + r0 + (int64_t)c0 * p->lda;
-  };
-  /* Left: rows of B in blocks, forward when op(A) is lower.  Right: columns
-   * of B, forward when op(A) is upper. */
-  const bool forward = p->left ? lower : !lower;
-  for (int step = 0; step < nblk; ++step) {
-    const int bi = forward ? step : nblk - 1 - step;
-    const int i0 = bi * TRI_NB, kb = k - i0 < TRI_NB ? k - i0 : TRI_NB;
-    const T *Di = Dinv + (int64_t)bi * TRI_NB * TRI_NB;
-    if (p->left) {
-      /* Y_i = inv(op(A)_ii) B_i */
-      e = gemm<T>(0, 0, kb, n, kb, 1.0, Di, TRI_NB, B + i0, p->ldb, 0.0, Y + i0, m, s);
-      if (e != hipSuccess) return e;
-      /* rows still to solve: B_j -= op(A)_ji Y_i */
-      const int r0 = forward ? i0 + kb : 0, rn = forward ? k - (i0 + kb) : i0;
-      if (rn > 0) {
-        e = gemm<T>(p->trans, 0, rn, n, kb, -1.0, opA(r0, i0), p->lda, Y + i0, m, 1.0, B + r0,
-                    p->ldb, s);
-        if (e != hipSuccess) return e;
-      }
-    } else {
-      /* Y_i = B_i inv(op(A)_ii) */
-      e = gemm<T>(0, 0, m, kb, kb, 1.0, B + (int64_t)i0 * p->ldb, p->ldb, Di, TRI_NB, 0.0,
-                  Y + (int64_t)i0 * m, m, s);
-      if (e != hipSuccess) return e;
-      /* columns still to solve: B_j -= Y_i op(A)_ij */
-      const int c0 = forward ? i0 + kb : 0, cn = forward ? k - (i0 + kb) : i0;
-      if (cn > 0) {
-        e = gemm<T>(0, p->trans, m, cn, kb, -1.0, Y + (int64_t)i0 * m, m, opA(i0, c0), p->lda,
-                    1.0, B + (int64_t)c0 * p->ldb, p->ldb, s);
-        if (e != hipSuccess) return e;
-      }
-    }
-  }
+  e = trsm_rec<T>(p, Dinv, Y, lower, 0, k, s);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
                      (const T *)Y, (int64_t)m, B, p->ldb, m, n, (T)p->alpha, 0);
   return hipGetLastError();
