@@ -426,6 +426,163 @@ slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict
                            n0 + wn * WTN, li, kg);
 }
 
+/* Direct-to-LDS form with 128-byte K stages in two LDS buffers (64 KB
+ * each): one barrier per two MFMA k-steps.  A 128-byte row holds its eight
+ * 16-byte chunks with chunk j at position j ^ ((row >> 1) & 7).  Iteration q:
+ * the fragments of (stage q, k-step 0) are in registers; those of k-step 1
+ * are read during k-step 0's MFMAs; after this wave's loads of stage q + 1
+ * and its reads are complete, a barrier frees stage q's buffer for stage
+ * q + 2, whose loads are issued; (stage q + 1, k-step 0) is read during
+ * k-step 1's MFMAs. */
+__device__ __forceinline__ int dl2_swz(int r) { return (r >> 1) & 7; }
+
+template <int KIND>
+__global__ void __launch_bounds__(512)
+slice_products_dl2_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
+                          int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
+                          SliceGroups G, int tiles_m, int tiles_n, void *__restrict__ S,
+                          float *__restrict__ C, int64_t ldc, float alpha, float beta) {
+  constexpr int BM = 256, BN = 256, BKB = 128, WM = 2, WN = 4, NW = WM * WN;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  constexpr int STAGE = (BM + BN) * BKB;
+  constexpr int LPA = BM * BKB / 1024 / NW, LPB = BN * BKB / 1024 / NW;
+  constexpr bool BF16 = KIND == SP_BF16;
+  typedef SliceMfma<BF16> MM;
+  typedef typename MM::acc_t acc_t;
+  extern __shared__ __align__(16) int smem[];
+  char *const lds = reinterpret_cast<char *>(smem);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int li = lane & 15, kg = lane >> 4;
+  const int ntiles = tiles_m * tiles_n;
+  const int g = blockIdx.x / ntiles;
+  int tm, tn;
+  tile_coords(blockIdx.x - g * ntiles, ntiles, tiles_m, tiles_n, &tm, &tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int pc = G.gcnt[g];
+  const unsigned tpack = G.tpack[g], upack = G.upack[g];
+  const int kt_begin = kb_begin / BKB;
+  const int nkt = (kb_end - kb_begin) / BKB;
+  const int iters = pc * nkt;
+
+  /* loads: LDS chunk c = (LP wave + u) 64 + lane, row c / 8, position c % 8
+   * holding chunk (c % 8) ^ dl2_swz(row) (rows past the matrix clamped) */
+  int64_t a_off[LPA], b_off[LPB];
+#pragma unroll
+  for (int u = 0; u < LPA; ++u) {
+    const int c = (LPA * wave + u) * 64 + lane, r = c >> 3, j = (c & 7) ^ dl2_swz(r);
+    a_off[u] = (r < M - m0 ? r : M - m0 - 1) * kpad_bytes + 16 * j;
+  }
+#pragma unroll
+  for (int u = 0; u < LPB; ++u) {
+    const int c = (LPB * wave + u) * 64 + lane, r = c >> 3, j = (c & 7) ^ dl2_swz(r);
+    b_off[u] = (r < N - n0 ? r : N - n0 - 1) * kpad_bytes + 16 * j;
+  }
+  const int8_t *a_tile = As + (int64_t)m0 * kpad_bytes;
+  const int8_t *b_tile = Bs + (int64_t)n0 * kpad_bytes;
+  auto issue = [&](int q, int buf) {
+    const int pl = q / nkt, kt = kt_begin + q - pl * nkt;
+    const int t = KIND == SP_I8_MOD ? g : (tpack >> (4 * pl)) & 15;
+    const int uu = KIND == SP_I8_MOD ? g : (upack >> (4 * pl)) & 15;
+    const int8_t *ga = a_tile + (int64_t)t * M * kpad_bytes + kt * BKB;
+    const int8_t *gb = b_tile + (int64_t)uu * N * kpad_bytes + kt * BKB;
+    char *la = lds + buf * STAGE, *lb = la + BM * BKB;
+#pragma unroll
+    for (int u = 0; u < LPA; ++u)
+      __builtin_amdgcn_global_load_lds((const void *)(ga + a_off[u]),
+                                       (__attribute__((address_space(3))) void *)(la + (LPA * wave + u) * 1024),
+                                       16, 0, 0);
+#pragma unroll
+    for (int u = 0; u < LPB; ++u)
+      __builtin_amdgcn_global_load_lds((const void *)(gb + b_off[u]),
+                                       (__attribute__((address_space(3))) void *)(lb + (LPB * wave + u) * 1024),
+                                       16, 0, 0);
+  };
+  struct Frag {
+    i4_t a[FM], b[FN];
+  };
+  auto read = [&](int buf, int ks, Frag &f) {
+    const char *sa = lds + buf * STAGE, *sb = sa + BM * BKB;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int r = wm * WTM + i * 16 + li;
+      f.a[i] = *reinterpret_cast<const i4_t *>(sa + r * BKB + 16 * ((ks * 4 + kg) ^ dl2_swz(r)));
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int r = wn * WTN + j * 16 + li;
+      f.b[j] = *reinterpret_cast<const i4_t *>(sb + r * BKB + 16 * ((ks * 4 + kg) ^ dl2_swz(r)));
+    }
+  };
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  constexpr int NR = FM + FN, NMF = FM * FN, PER = NMF / NR;
+  auto mmas = [&](const Frag &f) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = MM::mma(f.a[i], f.b[j], acc[i][j]);
+  };
+
+  issue(0, 0);
+  if (iters > 1) issue(1, 1);
+  if (iters > 1) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(LPA + LPB) : "memory");
+  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  asm volatile("s_barrier" ::: "memory");
+  Frag f0, f1;
+  read(0, 0, f0);
+  for (int q = 0; q < iters; ++q) {
+    const int buf = q & 1;
+    /* k-step 1's fragments during k-step 0's MFMAs */
+    read(buf, 1, f1);
+    mmas(f0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, NMF - PER * NR, 0);
+    /* stage q + 1 landed (this wave's loads), stage q fully read */
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
+    asm volatile("s_barrier" ::: "memory");
+    if (q + 2 < iters) issue(q + 2, buf);
+    if (q + 1 < iters) read(buf ^ 1, 0, f0);
+    mmas(f1);
+    __builtin_amdgcn_sched_group_barrier(0x020, LPA + LPB, 0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, NMF - PER * NR, 0);
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+
+  store_tile<KIND, FM, FN>(acc, g, G, S, C, ldc, alpha, beta, M, N, m0 + wm * WTM,
+                           n0 + wn * WTN, li, kg);
+}
+
+template <int KIND>
+hipError_t launch_dl2(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad, int kb0,
+                      int kb1, const SliceGroups &G, void *S, float *C, int64_t ldc,
+                      float alpha, float beta, hipStream_t st) {
+  auto kern = slice_products_dl2_kernel<KIND>;
+  const size_t lds = (size_t)2 * 512 * 128;
+  hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  const int tiles_m = (m + 255) / 256, tiles_n = (n + 255) / 256;
+  hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n * G.ngroups), dim3(512), lds, st, as, bs, m,
+                     n, kpad, kb0, kb1, G, tiles_m, tiles_n, S, C, ldc, alpha, beta);
+  return hipGetLastError();
+}
+
 template <int KIND, int NST, int WM, int WN, int BN = 256>
 hipError_t launch_dl(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad, int kb0,
                      int kb1, const SliceGroups &G, void *S, float *C, int64_t ldc,
@@ -461,25 +618,27 @@ hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
  * stages, MFMAs interleaved with the fragment reads and LDS writes, lines
  * padded by 16 bytes; 1 = 256x256 with 4 waves of 128x128 and 64-byte K
  * stages; 2 = as 0 with the compiler's own order; 3 = as 0 with 32 bytes of
- * padding (3% faster than 0); 4 = direct-to-LDS loads, 8 waves of 128x64,
- * 64-byte stages, 3 stages ahead; 5 = as 4 with 2 stages ahead.  Defaults,
- * measured at 8192^3: 4 for residue products (1.5% faster than 3), 3 for the
- * others (4 is even for slice products and 2% slower for BF16).  Variants
- * measured slower and dropped: 4 waves of 128x128 with 128-byte stages, and
- * direct-to-LDS with 4 waves of 128x128 (both spill), 256x128 and 128x256
- * tiles (one wave per SIMD), 64-byte stages with 8 waves and register
- * staging, the compiler's built-in GEMM interleaving, and a bank-conflict-free
- * fragment-ordered direct-to-LDS layout (32-byte global runs: slower).
- * GEMMX_SLICE_CFG overrides for every kind. */
+ * padding; 4 = direct-to-LDS loads, 8 waves of 128x64, 64-byte stages, 3
+ * stages ahead; 5 = as 4 with 2 stages ahead; 6 = direct-to-LDS, 128-byte
+ * stages in two buffers (one barrier per two k-steps), the default for every
+ * kind: measured at 8192^3 it is 5-6% faster than 3 (and 4.6% than 4) for
+ * residue, slice and BF16 products alike.  Variants measured slower and
+ * dropped: 4 waves of 128x128 with 128-byte stages, and direct-to-LDS with 4
+ * waves of 128x128 (both spill), 256x128 and 128x256 tiles (one wave per
+ * SIMD), two 256x128 blocks per CU, four direct-to-LDS stages ahead, 64-byte
+ * stages with 8 waves and register staging, the compiler's built-in GEMM
+ * interleaving, and a bank-conflict-free fragment-ordered direct-to-LDS
+ * layout (32-byte global runs: slower).  GEMMX_SLICE_CFG overrides for every
+ * kind. */
 int slice_cfg(int kind) {
   static int cfg = -2;
   if (cfg == -2) {
     const char *e = getenv("GEMMX_SLICE_CFG");
     cfg = (e && *e) ? atoi(e) : -1;
-    if (cfg < -1 || cfg > 5) cfg = -1;
+    if (cfg < -1 || cfg > 6) cfg = -1;
   }
-  if (cfg >= 0) return cfg;
-  return kind == SP_I8_MOD ? 4 : 3;
+  (void)kind;
+  return cfg >= 0 ? cfg : 6;
 }
 
 template <int KIND>
@@ -500,6 +659,8 @@ hipError_t products(const int8_t *as, const int8_t *bs, int m, int n, int64_t kp
     return launch_dl<KIND, 4, 2, 4>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc, alpha, beta, st);
   case 5:
     return launch_dl<KIND, 3, 2, 4>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc, alpha, beta, st);
+  case 6:
+    return launch_dl2<KIND>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc, alpha, beta, st);
   default:
     return launch_cfg<KIND, 256, 256, 2, 4, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
                                                        ldc, alpha, beta, st);
