@@ -670,9 +670,9 @@ def main():
                             "frac": round(ops / pm / 1e9 / PEAK["i8"], 4),
                             "traffic": profiled_traffic(
                                 f"dgemm_{info['scheme']}_{cnt}_nn_8192.md"),
-                            "kernel": ("slice_products_dl_kernel (INT8 MFMA, direct-to-LDS)"
+                            "kernel": ("slice_products_dl2_kernel (INT8 MFMA, direct-to-LDS)"
                                        if info["scheme"] == "ozaki2" else
-                                       "slice_products_kernel (INT8 MFMA)")
+                                       "slice_products_dl2_kernel (INT8 MFMA, direct-to-LDS)")
                                       + ", product phase timed with HIP events in the library",
                             "kernel_ms": round(pm, 4)}
     line.update(result)
