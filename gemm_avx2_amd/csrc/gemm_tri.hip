@@ -7,11 +7,12 @@
  * trmm_kernel_4x8_simplified.c, strsm_kernel_8x4_haswell_{LN,LT,RN,RT}.c,
  * TRSM_KERNEL_{LN,LT,RN,RT}_simplified.c).  Here:
  *
- *   TRMM  B := alpha op(A) B (side L) or alpha B op(A) (side R): the stored
- *         triangle of A (unit diagonal if diag 'U') is copied into a dense
- *         matrix with zeros elsewhere and one GEMM writes the product into a
- *         workspace, which is copied back into B.  Twice the triangular
- *         flops, all of them on the matrix cores.
+ *   TRMM  B := alpha op(A) B (side L) or alpha B op(A) (side R), in place,
+ *         recursive on halves (trmm_rec): the half no other half feeds is
+ *         updated first, then the other receives the off-diagonal product
+ *         (one deep GEMM), then is updated; leaves of up to TRMM_LEAF are
+ *         one GEMM on a dense copy of the diagonal block (unit diagonal
+ *         written, zeros outside the stored triangle) through a workspace.
  *   TRSM  op(A) X = alpha B (side L) or X op(A) = alpha B (side R), X over
  *         B.  The TRI_NB x TRI_NB diagonal blocks of op(A) are inverted by
  *         substitution (one thread per column of the inverse, the block in
@@ -139,6 +140,70 @@ hipError_t gemm(int ta, int tb, int m, int n, int k, double alpha, const T *a, i
   return es == 8 ? gemmx_native_f64(&q, cfg, s) : gemmx_native_f32(&q, cfg, s);
 }
 
+constexpr int TRMM_LEAF = 256;
+
+/* op(A)(r0.., c0..) as a GEMM operand: stored A at (r0, c0), or at (c0, r0)
+ * read transposed. */
+template <typename T>
+const T *op_at(const gemmx_tproblem *p, int r0, int c0) {
+  const T *A = (const T *)p->a;
+  return p->trans ? A + c0 + (int64_t)r0 * p->lda : A + r0 + (int64_t)c0 * p->lda;
+}
+
+/* B's rows (side L) or columns (side R) i0 .. i0 + kk := alpha op(A)_range
+ * times them.  A range of at most TRMM_LEAF is one GEMM on a dense copy D of
+ * its diagonal block of A (zeros outside the stored triangle) into W, copied
+ * back.  A larger one splits at a multiple of TRI_NB near its middle: the
+ * half that no other half feeds is updated first (recursively), then the
+ * other half receives its off-diagonal product (one GEMM of depth = that
+ * half, beta = 1, reading the not-yet-updated half), then that half is
+ * updated.  W and D are workspace for the largest leaf. */
+template <typename T>
+hipError_t trmm_rec(const gemmx_tproblem *p, int lower, int i0, int kk, T *D, T *W,
+                    hipStream_t s) {
+  const int m = p->m, n = p->n;
+  T *B = (T *)p->b;
+  if (kk <= TRMM_LEAF) {
+    const T *Ablk = (const T *)p->a + i0 + (int64_t)i0 * p->lda;
+    hipLaunchKernelGGL((tri_dense_kernel<T>), dim3(grid_for((int64_t)kk * kk)), dim3(256), 0, s,
+                       Ablk, p->lda, kk, p->upper, p->unit, D);
+    const int64_t wsz = p->left ? (int64_t)kk * n : (int64_t)m * kk;
+    hipError_t e = hipMemsetAsync(W, 0, (size_t)wsz * sizeof(T), s); /* beta = 0 reads W */
+    if (e != hipSuccess) return e;
+    if (p->left) {
+      e = gemm<T>(p->trans, 0, kk, n, kk, p->alpha, D, kk, B + i0, p->ldb, 0.0, W, kk, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for(wsz)), dim3(256), 0, s,
+                         (const T *)W, (int64_t)kk, B + i0, p->ldb, kk, n, (T)1, 0);
+    } else {
+      e = gemm<T>(0, p->trans, m, kk, kk, p->alpha, B + (int64_t)i0 * p->ldb, p->ldb, D, kk,
+                  0.0, W, m, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for(wsz)), dim3(256), 0, s,
+                         (const T *)W, (int64_t)m, B + (int64_t)i0 * p->ldb, p->ldb, m, kk, (T)1,
+                         0);
+    }
+    return hipGetLastError();
+  }
+  const int h = ((kk / 2 + TRI_NB - 1) / TRI_NB) * TRI_NB; /* 0 < h < kk */
+  /* Left, op(A) lower: new B2 = L21 B1 + L22 B2 needs the old B1, so B2 goes
+   * first; upper: new B1 = U11 B1 + U12 B2 needs the old B2, so B1 first.
+   * Right: upper -> new B2 = B1 U12 + B2 U22, B2 first; lower -> B1 first. */
+  const bool second_first = p->left ? lower : !lower;
+  const int a0 = i0, a1 = i0 + h;           /* the two halves' starts */
+  const int ka = h, kb2 = kk - h;
+  const int u0 = second_first ? a1 : a0, ku = second_first ? kb2 : ka; /* updated first */
+  const int v0 = second_first ? a0 : a1, kv = second_first ? ka : kb2; /* the feeding half */
+  hipError_t e = trmm_rec<T>(p, lower, u0, ku, D, W, s);
+  if (e != hipSuccess) return e;
+  e = p->left ? gemm<T>(p->trans, 0, ku, n, kv, p->alpha, op_at<T>(p, u0, v0), p->lda, B + v0,
+                        p->ldb, 1.0, B + u0, p->ldb, s)
+              : gemm<T>(0, p->trans, m, ku, kv, p->alpha, B + (int64_t)v0 * p->ldb, p->ldb,
+                        op_at<T>(p, v0, u0), p->lda, 1.0, B + (int64_t)u0 * p->ldb, p->ldb, s);
+  if (e != hipSuccess) return e;
+  return trmm_rec<T>(p, lower, v0, kv, D, W, s);
+}
+
 template <typename T>
 hipError_t trmm(const gemmx_tproblem *p, void *work, hipStream_t s) {
   const int m = p->m, n = p->n, k = p->left ? m : n;
@@ -148,18 +213,11 @@ hipError_t trmm(const gemmx_tproblem *p, void *work, hipStream_t s) {
                        (const T *)nullptr, (int64_t)0, B, p->ldb, m, n, (T)0, 1);
     return hipGetLastError();
   }
+  const int leaf = k < TRMM_LEAF ? k : TRMM_LEAF;
   T *D = (T *)work;
-  T *W = (T *)((char *)work + rup((size_t)k * k * sizeof(T), 256));
-  hipLaunchKernelGGL((tri_dense_kernel<T>), dim3(grid_for((int64_t)k * k)), dim3(256), 0, s,
-                     (const T *)p->a, p->lda, k, p->upper, p->unit, D);
-  hipError_t e = hipMemsetAsync(W, 0, (size_t)m * n * sizeof(T), s); /* beta = 0 reads W */
-  if (e != hipSuccess) return e;
-  e = p->left ? gemm<T>(p->trans, 0, m, n, m, p->alpha, D, k, B, p->ldb, 0.0, W, m, s)
-              : gemm<T>(0, p->trans, m, n, n, p->alpha, B, p->ldb, D, k, 0.0, W, m, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((scale_copy_kernel<T>), dim3(grid_for((int64_t)m * n)), dim3(256), 0, s,
-                     (const T *)W, (int64_t)m, B, p->ldb, m, n, (T)1, 0);
-  return hipGetLastError();
+  T *W = (T *)((char *)work + rup((size_t)leaf * leaf * sizeof(T), 256));
+  const int lower = p->upper ? p->trans : !p->trans;
+  return trmm_rec<T>(p, lower, 0, k, D, W, s);
 }
 
 /* Solves the rows (side L) or columns (side R) i0 .. i0 + kk of op(A) Y = B
@@ -234,8 +292,12 @@ hipError_t trsm(const gemmx_tproblem *p, void *work, hipStream_t s) {
 extern "C" size_t gemmx_tri_workspace(int es, int solve, int left, int m, int n) {
   if (m <= 0 || n <= 0) return 0;
   const size_t k = left ? m : n;
-  const size_t first = solve ? ((k + TRI_NB - 1) / TRI_NB) * TRI_NB * TRI_NB : k * k;
-  return rup(first * es, 256) + rup((size_t)m * n * es, 256);
+  if (solve) /* diagonal-block inverses, Y */
+    return rup(((k + TRI_NB - 1) / TRI_NB) * TRI_NB * TRI_NB * es, 256) +
+           rup((size_t)m * n * es, 256);
+  /* the largest leaf's dense triangle and product */
+  const size_t leaf = k < (size_t)TRMM_LEAF ? k : (size_t)TRMM_LEAF;
+  return rup(leaf * leaf * es, 256) + rup(leaf * (left ? (size_t)n : (size_t)m) * es, 256);
 }
 
 extern "C" hipError_t gemmx_tri(int es, int solve, const gemmx_tproblem *p, void *work,
