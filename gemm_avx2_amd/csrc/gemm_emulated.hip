@@ -308,10 +308,11 @@ __device__ __forceinline__ void oz2_finish(const unsigned (&acc)[LW], const Oz2C
   *cp = __builtin_fma(alpha, scaled, beta * *cp);
 }
 
-/* Four consecutive elements (column-major index t) per thread: the modulus
- * loop reads each plane's bytes of the four with one load and each M_j limb
- * once for the four. */
-template <int LW>
+/* Four consecutive elements (column-major index t) per thread: each plane's
+ * bytes of the four come with one load (VEC: m n a multiple of 4) or four
+ * clamped byte loads, all planes' loads issued before the first is used, and
+ * each W_j limb is read once for the four. */
+template <int LW, bool VEC>
 __global__ void __launch_bounds__(256)
 ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
                     const int *__restrict__ ea, const int *__restrict__ fb,
@@ -320,9 +321,10 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
   constexpr int E = 4;
   const int64_t total = (int64_t)M * N;
   const int64_t quads = (total + E - 1) / E;
-  const bool vec = (total & (E - 1)) == 0;
-  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < quads;
-       u += (int64_t)gridDim.x * blockDim.x) {
+  /* one quad per thread (no grid-stride loop, whose invariant W_j loads the
+   * compiler would keep in registers across iterations) */
+  const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (u < quads) {
     const int64_t t0 = u * E;
     /* X'_e = sum_g c_g W_g < nmod 2^8 M */
     unsigned acc[E][LW];
@@ -330,22 +332,34 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
     for (int q = 0; q < E; ++q)
 #pragma unroll
       for (int l = 0; l < LW; ++l) acc[q][l] = 0u;
-#pragma unroll 1
-    for (int g = 0; g < crt.nmod; ++g) {
-      const uint8_t *rp = R + (int64_t)g * total + t0;
-      unsigned c4 = 0;
-      if (vec) {
-        c4 = *reinterpret_cast<const unsigned *>(rp);
-      } else {
+    /* every plane's bytes are loaded before the first is used, so the
+     * loads overlap */
+    unsigned c4[OZ2_MAX_MODULI];
 #pragma unroll
-        for (int q = 0; q < E; ++q)
-          if (t0 + q < total) c4 |= (unsigned)rp[q] << (8 * q);
+    for (int g = 0; g < OZ2_MAX_MODULI; ++g) {
+      c4[g] = 0;
+      if (g < crt.nmod) {
+        const uint8_t *rp = R + (int64_t)g * total;
+        if constexpr (VEC) {
+          c4[g] = *reinterpret_cast<const unsigned *>(rp + t0);
+        } else { /* clamped reads, bytes past the end taken as 0 */
+#pragma unroll
+          for (int q = 0; q < E; ++q) {
+            const unsigned b = rp[t0 + q < total ? t0 + q : total - 1];
+            c4[g] |= (t0 + q < total ? b : 0u) << (8 * q);
+          }
+        }
       }
+    }
 #pragma unroll
-      for (int q = 0; q < E; ++q) {
-        const unsigned cg = (c4 >> (8 * q)) & 0xffu;
+    for (int g = 0; g < OZ2_MAX_MODULI; ++g) {
+      if (g < crt.nmod) {
 #pragma unroll
-        for (int l = 0; l < LW; ++l) acc[q][l] += __umul24(cg, crt.W16[g][l]);
+        for (int q = 0; q < E; ++q) {
+          const unsigned cg = (c4[g] >> (8 * q)) & 0xffu;
+#pragma unroll
+          for (int l = 0; l < LW; ++l) acc[q][l] += __umul24(cg, crt.W16[g][l]);
+        }
       }
     }
 #pragma unroll
@@ -363,8 +377,12 @@ template <int LW>
 hipError_t launch_recon_lw(const uint8_t *R, const Oz2Crt &crt, const int *ea, const int *fb,
                            const gemmx_problem *p, double beta, unsigned blocks,
                            hipStream_t st) {
-  hipLaunchKernelGGL(ozaki2_recon_kernel<LW>, dim3(blocks), dim3(256), 0, st, R, crt, ea, fb,
-                     (double *)p->c, p->ldc, p->m, p->n, p->alpha, beta);
+  if (((int64_t)p->m * p->n) % 4 == 0)
+    hipLaunchKernelGGL((ozaki2_recon_kernel<LW, true>), dim3(blocks), dim3(256), 0, st, R, crt,
+                       ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha, beta);
+  else
+    hipLaunchKernelGGL((ozaki2_recon_kernel<LW, false>), dim3(blocks), dim3(256), 0, st, R, crt,
+                       ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha, beta);
   return hipGetLastError();
 }
 
@@ -442,8 +460,7 @@ extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *w
   }
   const int kp = (int)L.kpad;
   const int64_t total = (int64_t)p->m * p->n;
-  int64_t rblocks = (total + 4 * 256 - 1) / (4 * 256); /* four elements per thread */
-  if (rblocks > 65536) rblocks = 65536;
+  const int64_t rblocks = (total + 4 * 256 - 1) / (4 * 256); /* four elements per thread */
   for (int kb0 = 0; kb0 < kp; kb0 += OZ2_KCHUNK) {
     const int kb1 = kb0 + OZ2_KCHUNK < kp ? kb0 + OZ2_KCHUNK : kp;
     e = slice_products(SP_I8_MOD, as, bs, p->m, p->n, L.kpad, kb0, kb1, G, R, NULL, 0, 0.f,

@@ -251,55 +251,37 @@ ozaki_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
 }
 
 /* ---- Ozaki-II residues ------------------------------------------------
- * x = (-1)^neg * mant * 2^q exactly (mant < 2^53); the scaled integer is
- * rint(x * 2^(P - e)) = rint(mant * 2^sh), sh = q + D, D = P - e, and
- * sh <= P - 53 <= 24 since |x| < 2^e.
- * Its magnitude (mant << sh, or rint(mant / 2^-sh) ties to even; < 2^78) is
- * split as h * 2^39 + l with h, l < 2^39, both exact in FP64, so that for
- * each modulus m
- *   |a| mod m = (h * (2^39 mod m) + l) mod m,
- * with v = h * (2^39 mod m) + l < 2^48 exact in FP64, v - rint(v / m) * m
- * exact (one fma), brought into [-(m-1)/2, (m-1)/2] by one correction each
- * way, and negated for a negative a.  m = 256 takes the low byte.  The
- * residue is stored as a signed byte. */
+ * The scaled integer a = rint(x * 2^D), D = P - e (|a| < 2^78; x * 2^D is
+ * exact except below 2^-1022, where rint gives 0 either way; ties to even),
+ * is split as a = h * 2^39 + l with h = trunc(a / 2^39): h and l carry a's
+ * sign, |h|, |l| < 2^39, all exact in FP64.  For an odd modulus m,
+ *   v = h * (2^39 mod m) + l  (|v| < 2^47, exact, v = a mod m)
+ * and q = rint(v / m) is the exact nearest integer from one rounded product
+ * v * (1/m): its error |v/m| 2^-52 stays below 1/(2m), the least distance of
+ * v/m from a half-integer (2v - (2k+1)m is odd), so r = v - q m is the
+ * centred residue in [-(m-1)/2, (m-1)/2].  Only r's low byte is stored, so
+ * it comes from integer arithmetic modulo 2^8: q from the low word of
+ * fma(v, 1/m, 1.5 * 2^52), v's low bits from the low words of h and l (each
+ * read the same way), and r = v + q (2^24 - m) with 24-bit multiply-adds.
+ * m = 256 takes l's low byte (256 divides 2^39). */
 struct Oz2Val {
-  double h, l;   /* |a| = h 2^39 + l */
-  unsigned low;  /* |a| mod 2^32 */
-  int neg;
+  double h, l;      /* a = h 2^39 + l */
+  unsigned hlo, llo; /* h, l modulo 2^32 */
 };
+
+constexpr double OZ2_MAGIC = 0x1.8p52; /* x + 1.5 * 2^52: low word = rint(x) mod 2^32 */
+
+__device__ __forceinline__ unsigned oz2_low_word(double x) {
+  return (unsigned)__double_as_longlong(x + OZ2_MAGIC);
+}
 
 __device__ __forceinline__ Oz2Val oz2_val(double x, int D) {
   Oz2Val r;
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-  r.neg = (int)(bits >> 63);
-  const int be = (int)((bits >> 52) & 0x7ff);
-  unsigned long long mant = bits & 0xfffffffffffffull;
-  int q = -1074;
-  if (be != 0) {
-    mant |= 1ull << 52;
-    q = be - 1075;
-  }
-  const int sh = q + D;
-  constexpr unsigned long long M39 = (1ull << 39) - 1;
-  unsigned long long hi, lo;
-  if (sh >= 0) { /* sh <= 24 */
-    hi = mant >> (39 - sh);
-    lo = (mant << sh) & M39;
-    r.low = (unsigned)(mant << sh);
-  } else {
-    const int s = -sh;
-    unsigned long long v = 0;
-    if (s <= 63) {
-      v = mant >> s;
-      const unsigned long long rem = mant & ((1ull << s) - 1), half = 1ull << (s - 1);
-      v += (rem > half || (rem == half && (v & 1))) ? 1 : 0;
-    }
-    hi = v >> 39;
-    lo = v & M39;
-    r.low = (unsigned)v;
-  }
-  r.h = (double)hi;
-  r.l = (double)lo;
+  const double a = __builtin_rint(ldexp(x, D));
+  r.h = __builtin_trunc(a * 0x1p-39);
+  r.l = __builtin_fma(-r.h, 0x1p39, a);
+  r.hlo = oz2_low_word(r.h);
+  r.llo = oz2_low_word(r.l);
   return r;
 }
 
@@ -309,23 +291,36 @@ constexpr unsigned pow2_mod(int e, int m) {
   return r;
 }
 
+/* a * b + c over the low 24 bits of a and b (v_mad_u32_u24, b scalar). */
+__device__ __forceinline__ unsigned oz2_mad24(unsigned a, unsigned b, unsigned c) {
+  unsigned r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
+
+/* Residue of a modulo OZ2_MODULI[J] in the low byte (higher bits unused). */
 template <int J>
 __device__ __forceinline__ unsigned oz2_residue_byte(const Oz2Val &a) {
   constexpr int m = OZ2_MODULI[J];
-  int r;
-  if (m == 256) {
-    r = (int)(a.low & 255u);
-    r = a.neg ? -r : r;
+  if constexpr (m == 256) {
+    return a.llo;
   } else {
-    constexpr double c39 = (double)pow2_mod(39, m), rcp = 1.0 / (double)m;
-    const double v = __builtin_fma(a.h, c39, a.l);
-    const double qd = __builtin_rint(v * rcp);
-    r = (int)__builtin_fma(-qd, (double)m, v);
-    r -= r > (m - 1) / 2 ? m : 0;
-    r += r < -(m - 1) / 2 ? m : 0;
-    r = a.neg ? -r : r;
+    /* The constants sit in scalar registers, so that both products and both
+     * multiply-adds are single three-operand instructions. */
+    double c39d = (double)pow2_mod(39, m), rcp = 1.0 / (double)m;
+    unsigned c39 = pow2_mod(39, m), mneg = (1u << 24) - (unsigned)m;
+    asm("" : "+s"(c39d), "+s"(rcp), "+s"(c39), "+s"(mneg));
+    const double v = __builtin_fma(a.h, c39d, a.l);
+    const unsigned q = (unsigned)__double_as_longlong(__builtin_fma(v, rcp, OZ2_MAGIC));
+    return oz2_mad24(q, mneg, oz2_mad24(a.hlo, c39, a.llo));
   }
-  return (unsigned)r & 0xffu;
+}
+
+/* Low bytes of b0..b3 -> one word, b0 lowest. */
+__device__ __forceinline__ int oz2_pack4(unsigned b0, unsigned b1, unsigned b2, unsigned b3) {
+  const unsigned lo = __builtin_amdgcn_perm(b1, b0, 0x0c0c0400u);
+  const unsigned hi = __builtin_amdgcn_perm(b3, b2, 0x0c0c0400u);
+  return (int)__builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
 /* The residue bytes of four consecutive entries, packed into word W of each
@@ -334,8 +329,8 @@ template <int J, int W>
 __device__ __forceinline__ void oz2_encode4(const Oz2Val (&a)[4], int nmod,
                                             i4_t (&res)[OZ2_MAX_MODULI]) {
   if (J < nmod)
-    res[J][W] = (int)(oz2_residue_byte<J>(a[0]) | (oz2_residue_byte<J>(a[1]) << 8) |
-                      (oz2_residue_byte<J>(a[2]) << 16) | (oz2_residue_byte<J>(a[3]) << 24));
+    res[J][W] = oz2_pack4(oz2_residue_byte<J>(a[0]), oz2_residue_byte<J>(a[1]),
+                          oz2_residue_byte<J>(a[2]), oz2_residue_byte<J>(a[3]));
   if constexpr (J + 1 < OZ2_MAX_MODULI) oz2_encode4<J + 1, W>(a, nmod, res);
 }
 
@@ -367,10 +362,11 @@ ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int 
   GEMMX_OZ2_WORD(2)
   GEMMX_OZ2_WORD(3)
 #undef GEMMX_OZ2_WORD
+  int8_t *o = out + v * kpad + blk * 16;
+  const int64_t plane = (int64_t)nvec * kpad;
 #pragma unroll
-  for (int j = 0; j < OZ2_MAX_MODULI; ++j)
-    if (j < nmod)
-      *reinterpret_cast<i4_t *>(out + ((int64_t)j * nvec + v) * kpad + blk * 16) = res[j];
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j, o += plane)
+    if (j < nmod) *reinterpret_cast<i4_t *>(o) = res[j];
 }
 
 /* The same for vectors adjacent in memory, through the LDS transpose of
@@ -411,10 +407,11 @@ ozaki2_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec,
   GEMMX_OZ2_WORD(2)
   GEMMX_OZ2_WORD(3)
 #undef GEMMX_OZ2_WORD
+  int8_t *o = out + (int64_t)v * kpad + p0 + 16 * q;
+  const int64_t plane = (int64_t)nvec * kpad;
 #pragma unroll
-  for (int j = 0; j < OZ2_MAX_MODULI; ++j)
-    if (j < nmod)
-      *reinterpret_cast<i4_t *>(out + ((int64_t)j * nvec + v) * kpad + p0 + 16 * q) = res[j];
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j, o += plane)
+    if (j < nmod) *reinterpret_cast<i4_t *>(o) = res[j];
 }
 
 __device__ __forceinline__ unsigned bf16_rne_bits(float x) {
