@@ -91,41 +91,77 @@ exp_from_bits_kernel(const unsigned long long *__restrict__ maxbits, int nvec,
   if (v < nvec) out[v] = exponent_of_absbits(maxbits[v]);
 }
 
-/* Ozaki-II bound term of one entry: ceil(|x| 2^(10-e)) squared (exact: the
- * scaling by a power of two and the ceil are exact, t <= 1024). */
-__device__ __forceinline__ unsigned long long oz2_term(double x, int e) {
-  const unsigned long long t = (unsigned long long)ceil(ldexp(fabs(x), 10 - e));
-  return t * t;
-}
 
 __device__ __forceinline__ int oz2_shift(int L, int e, unsigned long long S) {
   return e == OZ_NONFINITE ? OZ_NONFINITE : ozaki2_bits_for_sum(L, S) - e;
 }
 
-/* Ozaki-II shift D = P - e per vector with elements contiguous: a wave per
- * vector, the exponent pass of exp_contig_kernel, then the bound sum over
- * the same (cached) vector. */
+/* Ozaki-II shift D = P - e per vector with elements contiguous: a block of
+ * 256 threads per vector.  Vectors of up to 256 * OZ2_SHIFT_REGS entries are
+ * read once, into registers (every load issued before the first is used),
+ * and both the exponent and the bound sum come from there; longer vectors
+ * are read twice.  The bound terms are summed in FP64, exactly (integers
+ * below 2^53), then in 64-bit integers. */
+constexpr int OZ2_SHIFT_REGS = 32;
+
+/* Ozaki-II bound term of one entry: ceil(|x| 2^(10-e)) squared, exact in
+ * FP64 (the scaling by a power of two and the ceil are exact, t <= 1024). */
+__device__ __forceinline__ double oz2_term_d(double x, int e) {
+  const double t = ceil(ldexp(fabs(x), 10 - e));
+  return t * t;
+}
+
+__device__ __forceinline__ unsigned long long max_u64(unsigned long long a,
+                                                      unsigned long long b) {
+  return a > b ? a : b;
+}
+
 __global__ void __launch_bounds__(256)
 oz2_shift_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len, int L,
                         int *__restrict__ out) {
-  const int v = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (v >= nvec) return;
+  constexpr int R = OZ2_SHIFT_REGS;
+  __shared__ unsigned long long red[4];
+  __shared__ int eshared;
+  const int v = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double *p = x + v * vld;
+  const bool in_regs = len <= 256 * R;
+  double r[R];
   unsigned long long mx = 0;
-  for (int i = lane; i < len; i += 64) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(p[i]) & 0x7fffffffffffffffull;
-    mx = b > mx ? b : mx;
+  if (in_regs) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int j = tid + 256 * i;
+      r[i] = j < len ? p[j] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      mx = max_u64(mx, (unsigned long long)__double_as_longlong(r[i]) & 0x7fffffffffffffffull);
+  } else {
+    for (int j = tid; j < len; j += 256)
+      mx = max_u64(mx, (unsigned long long)__double_as_longlong(p[j]) & 0x7fffffffffffffffull);
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(mx, off, 64);
-    mx = o > mx ? o : mx;
+  for (int off = 32; off > 0; off >>= 1) mx = max_u64(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  if (tid == 0)
+    eshared = exponent_of_absbits(max_u64(max_u64(red[0], red[1]), max_u64(red[2], red[3])));
+  __syncthreads();
+  const int e = eshared;
+  double sd = 0.0;
+  if (e != OZ_NONFINITE) {
+    if (in_regs) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) sd += oz2_term_d(r[i], e);
+    } else {
+      for (int j = tid; j < len; j += 256) sd += oz2_term_d(p[j], e);
+    }
   }
-  const int e = exponent_of_absbits(mx);
-  unsigned long long S = 0;
-  if (e != OZ_NONFINITE)
-    for (int i = lane; i < len; i += 64) S += oz2_term(p[i], e);
+  unsigned long long S = (unsigned long long)sd;
   for (int off = 32; off > 0; off >>= 1) S += __shfl_xor(S, off, 64);
-  if (lane == 0) out[v] = oz2_shift(L, e, S);
+  __syncthreads(); /* red[] reused */
+  if (lane == 0) red[wave] = S;
+  __syncthreads();
+  if (tid == 0) out[v] = oz2_shift(L, e, red[0] + red[1] + red[2] + red[3]);
 }
 
 /* Bound sums per vector with vectors adjacent (x[v + p*eld]), in the blocks
@@ -138,14 +174,15 @@ oz2_sums_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec, int
   const int vl = threadIdx.x & 63, kq = threadIdx.x >> 6;
   const int v = blockIdx.x * 64 + vl;
   const int p0 = blockIdx.y * 256 + kq * 64;
-  unsigned long long S = 0;
+  double sd = 0.0; /* at most 64 terms of at most 2^20: exact */
   if (v < nvec) {
     const int e = ex[v];
     if (e != OZ_NONFINITE) {
       const int p1 = p0 + 64 < len ? p0 + 64 : len;
-      for (int p = p0; p < p1; ++p) S += oz2_term(x[v + p * eld], e);
+      for (int p = p0; p < p1; ++p) sd += oz2_term_d(x[v + p * eld], e);
     }
   }
+  unsigned long long S = (unsigned long long)sd;
   part[kq][vl] = S;
   __syncthreads();
   if (kq == 0 && v < nvec) {
@@ -325,93 +362,134 @@ __device__ __forceinline__ int oz2_pack4(unsigned b0, unsigned b1, unsigned b2, 
 
 /* The residue bytes of four consecutive entries, packed into word W of each
  * of the nmod plane vectors (all indices compile-time constants). */
-template <int J, int W>
+template <int J, int W, typename V>
 __device__ __forceinline__ void oz2_encode4(const Oz2Val (&a)[4], int nmod,
-                                            i4_t (&res)[OZ2_MAX_MODULI]) {
+                                            V (&res)[OZ2_MAX_MODULI]) {
   if (J < nmod)
     res[J][W] = oz2_pack4(oz2_residue_byte<J>(a[0]), oz2_residue_byte<J>(a[1]),
                           oz2_residue_byte<J>(a[2]), oz2_residue_byte<J>(a[3]));
   if constexpr (J + 1 < OZ2_MAX_MODULI) oz2_encode4<J + 1, W>(a, nmod, res);
 }
 
-/* nmod residue planes of 16 consecutive K entries of one vector per thread
- * (as ozaki_split_kernel). */
+/* nmod residue planes of 4 NW consecutive K entries of one vector per thread
+ * (as ozaki_split_kernel).  With eld = 1 a thread whose entries are all in
+ * range and 16-byte aligned reads them with 16-byte loads. */
+template <int NW>
 __global__ void __launch_bounds__(256)
 ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int nvec,
                     int len, int64_t kpad, const int *__restrict__ ex, int v_fast,
                     int nmod, int8_t *__restrict__ out) {
-  const int64_t nblk = kpad / 16;
+  constexpr int EPT = 4 * NW;
+  typedef int vec_t __attribute__((ext_vector_type(NW)));
+  const int64_t nblk = kpad / EPT;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (tid >= nblk * nvec) return;
   int64_t v, blk;
   if (v_fast) { v = tid % nvec; blk = tid / nvec; }
   else        { v = tid / nblk; blk = tid % nblk; }
   const int d = ex[v]; /* shift P - e */
-  i4_t res[OZ2_MAX_MODULI];
+  const double *xp = x + v * vld + blk * EPT * eld;
+  double xv[EPT];
+  if (eld == 1 && blk * EPT + EPT <= len && d != OZ_NONFINITE &&
+      (reinterpret_cast<uintptr_t>(xp) & 15) == 0) {
 #pragma unroll
-  for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
+    for (int i = 0; i < EPT / 2; ++i) {
+      const d2_t t = reinterpret_cast<const d2_t *>(xp)[i];
+      xv[2 * i] = t[0];
+      xv[2 * i + 1] = t[1];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < EPT; ++i)
+      xv[i] = (blk * EPT + i < len && d != OZ_NONFINITE) ? xp[i * eld] : 0.0;
+  }
+  vec_t res[OZ2_MAX_MODULI];
+#pragma unroll
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = vec_t{};
   Oz2Val a[4];
 #define GEMMX_OZ2_WORD(W)                                                                \
-  for (int q = 0; q < 4; ++q) {                                                          \
-    const int64_t p = blk * 16 + 4 * (W) + q;                                            \
-    a[q] = oz2_val((p < len && d != OZ_NONFINITE) ? x[v * vld + p * eld] : 0.0, d);     \
-  }                                                                                      \
-  oz2_encode4<0, W>(a, nmod, res);
+  if constexpr ((W) < NW) {                                                              \
+    for (int q = 0; q < 4; ++q) a[q] = oz2_val(xv[4 * (W) + q], d);                      \
+    oz2_encode4<0, W>(a, nmod, res);                                                     \
+  }
   GEMMX_OZ2_WORD(0)
   GEMMX_OZ2_WORD(1)
   GEMMX_OZ2_WORD(2)
   GEMMX_OZ2_WORD(3)
 #undef GEMMX_OZ2_WORD
-  int8_t *o = out + v * kpad + blk * 16;
+  int8_t *o = out + v * kpad + blk * EPT;
   const int64_t plane = (int64_t)nvec * kpad;
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j, o += plane)
-    if (j < nmod) *reinterpret_cast<i4_t *>(o) = res[j];
+    if (j < nmod) *reinterpret_cast<vec_t *>(o) = res[j];
 }
 
-/* The same for vectors adjacent in memory, through the LDS transpose of
- * ozaki_split_strided_kernel. */
-__global__ void __launch_bounds__(256)
+/* The same for vectors adjacent in memory (x[v + p*eld]) through an LDS
+ * tile of 32 vectors x 128 K entries: entry p of a vector sits in column
+ * (p % EPT) TPV + p / EPT of its padded row, so that the TPV threads of a
+ * vector, thread q taking entries q EPT .. q EPT + EPT - 1, read adjacent
+ * columns.  The tile is loaded two vectors at a time (16-byte loads when x
+ * and eld allow). */
+template <int NW>
+__global__ void __launch_bounds__(1024 / NW)
 ozaki2_split_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec, int len,
                             int64_t kpad, const int *__restrict__ ex, int nmod,
                             int8_t *__restrict__ out) {
-  constexpr int ROW = 129;
+  constexpr int EPT = 4 * NW, TPV = 128 / EPT, NT = 32 * TPV, ROW = 129;
+  typedef int vec_t __attribute__((ext_vector_type(NW)));
   __shared__ double tile[32 * ROW];
+  const int tid = threadIdx.x;
   const int v0 = blockIdx.x * 32;
   const int64_t p0 = (int64_t)blockIdx.y * 128;
-  {
-    const int vl = threadIdx.x & 31, pr = threadIdx.x >> 5;
-    const int v = v0 + vl;
+  const bool aligned = (reinterpret_cast<uintptr_t>(x) & 15) == 0 && (eld & 1) == 0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int p = pr + 8 * j;
-      const double val = (v < nvec && p0 + p < len) ? x[v + (p0 + p) * eld] : 0.0;
-      tile[vl * ROW + (p % 16) * 8 + p / 16] = val;
+  for (int j = 0; j < 2048 / NT; ++j) {
+    const int idx = tid + NT * j, vp = idx & 15, pr = idx >> 4;
+    const int v = v0 + 2 * vp;
+    const int64_t p = p0 + pr;
+    double a0 = 0.0, a1 = 0.0;
+    if (p < len) {
+      const double *xp = x + v + p * eld;
+      if (aligned && v + 1 < nvec) {
+        const d2_t t = *reinterpret_cast<const d2_t *>(xp);
+        a0 = t[0];
+        a1 = t[1];
+      } else {
+        if (v < nvec) a0 = xp[0];
+        if (v + 1 < nvec) a1 = xp[1];
+      }
     }
+    const int col = (pr % EPT) * TPV + pr / EPT;
+    tile[(2 * vp) * ROW + col] = a0;
+    tile[(2 * vp + 1) * ROW + col] = a1;
   }
   __syncthreads();
-  const int vl = threadIdx.x >> 3, q = threadIdx.x & 7;
+  const int vl = tid / TPV, q = tid % TPV;
   const int v = v0 + vl;
   if (v >= nvec) return;
   const int d = ex[v]; /* shift P - e */
-  i4_t res[OZ2_MAX_MODULI];
+  double xv[EPT];
 #pragma unroll
-  for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = i4_t{0, 0, 0, 0};
+  for (int i = 0; i < EPT; ++i) xv[i] = d != OZ_NONFINITE ? tile[vl * ROW + i * TPV + q] : 0.0;
+  vec_t res[OZ2_MAX_MODULI];
+#pragma unroll
+  for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = vec_t{};
   Oz2Val a[4];
 #define GEMMX_OZ2_WORD(W)                                                                \
-  for (int i = 0; i < 4; ++i)                                                            \
-    a[i] = oz2_val(d != OZ_NONFINITE ? tile[vl * ROW + (4 * (W) + i) * 8 + q] : 0.0, d);    \
-  oz2_encode4<0, W>(a, nmod, res);
+  if constexpr ((W) < NW) {                                                              \
+    for (int i = 0; i < 4; ++i) a[i] = oz2_val(xv[4 * (W) + i], d);                      \
+    oz2_encode4<0, W>(a, nmod, res);                                                     \
+  }
   GEMMX_OZ2_WORD(0)
   GEMMX_OZ2_WORD(1)
   GEMMX_OZ2_WORD(2)
   GEMMX_OZ2_WORD(3)
 #undef GEMMX_OZ2_WORD
-  int8_t *o = out + (int64_t)v * kpad + p0 + 16 * q;
+  int8_t *o = out + (int64_t)v * kpad + p0 + EPT * q;
   const int64_t plane = (int64_t)nvec * kpad;
 #pragma unroll
   for (int j = 0; j < OZ2_MAX_MODULI; ++j, o += plane)
-    if (j < nmod) *reinterpret_cast<i4_t *>(o) = res[j];
+    if (j < nmod) *reinterpret_cast<vec_t *>(o) = res[j];
 }
 
 __device__ __forceinline__ unsigned bf16_rne_bits(float x) {
@@ -524,11 +602,14 @@ hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
 hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec, int k,
                           int64_t kpad, int nmod, int L, int *ex, unsigned long long *mbits,
                           unsigned long long *sums, int8_t *res, hipStream_t st) {
+  /* 8 entries per thread (two words per plane): fewer registers, so 5 waves
+   * per SIMD, measured faster than 16 for both forms */
+  constexpr int NW = 2;
   if (contiguous) {
-    hipLaunchKernelGGL(oz2_shift_contig_kernel, dim3((nvec + 3) / 4), dim3(256), 0, st, x,
-                       ld, nvec, k, L, ex);
-    const int64_t threads = (kpad / 16) * nvec;
-    hipLaunchKernelGGL(ozaki2_split_kernel, dim3((unsigned)((threads + 255) / 256)),
+    hipLaunchKernelGGL(oz2_shift_contig_kernel, dim3(nvec), dim3(256), 0, st, x, ld, nvec, k,
+                       L, ex);
+    const int64_t threads = (kpad / (4 * NW)) * nvec;
+    hipLaunchKernelGGL(ozaki2_split_kernel<NW>, dim3((unsigned)((threads + 255) / 256)),
                        dim3(256), 0, st, x, ld, (int64_t)1, nvec, k, kpad, ex, 0, nmod, res);
     return hipGetLastError();
   }
@@ -542,8 +623,9 @@ hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec
   hipLaunchKernelGGL(oz2_sums_strided_kernel, red, dim3(256), 0, st, x, ld, nvec, k, ex, sums);
   hipLaunchKernelGGL(oz2_shift_from_sums_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
                      sums, nvec, L, ex);
-  hipLaunchKernelGGL(ozaki2_split_strided_kernel, dim3((nvec + 31) / 32, (unsigned)(kpad / 128)),
-                     dim3(256), 0, st, x, ld, nvec, k, kpad, ex, nmod, res);
+  hipLaunchKernelGGL(ozaki2_split_strided_kernel<NW>,
+                     dim3((nvec + 31) / 32, (unsigned)(kpad / 128)), dim3(1024 / NW), 0, st, x,
+                     ld, nvec, k, kpad, ex, nmod, res);
   return hipGetLastError();
 }
 
