@@ -89,40 +89,48 @@ SliceGroups ozaki_groups(int s) {
 /* ---- Ozaki-II reconstruction ------------------------------------------
  * The residues c_j = X mod m_j of the exact integer X = sum_p a_ip b_pj give
  * X by the Chinese remainder theorem: with M = prod m_j, M_j = M / m_j,
- * w_j = M_j^-1 mod m_j and W_j = (w_j M_j) mod M, X = (sum_j c_j W_j) mod M,
- * taken in (-M/2, M/2) (|X| < M/2 by the vectors' bit counts).  The sum
- * (< nmod 2^8 M < 2^185) is formed exactly with 24-bit multiply-adds of the
- * byte c_j into 16-bit limbs of W_j (limb sums stay below 2^29), carried
- * into 192-bit integers (32-bit limbs), reduced modulo M with an FP64
- * quotient and one correction, rounded to 53 bits (ties to even) and scaled
- * by 2^-(D_i + D_j), D = P - e of row i and column j, with one ldexp. */
+ * w_j = M_j^-1 mod m_j and W_j = (w_j M_j) mod M, X = X' - q M with
+ * X' = sum_j c_j W_j and q = rint(X' / M), since |X| <= 2^(L-1) < M/2 (the
+ * vectors' bit counts).  X' (< nmod 2^8 M) is formed exactly with 24-bit
+ * multiply-adds of the byte c_j into LW 16-bit limbs of W_j (limb sums below
+ * 2^29).  q comes exactly from the top four limb sums in FP64: X'/M lies at
+ * least (M - 2^L) / (2M) >= 2^-6.2 from a half-integer for every modulus
+ * count (checked on the host), far above the estimate's error (< 2^-18).
+ * Adding q (2^(16 LW) - M) limb by limb gives X in two's complement over
+ * 16 LW bits; |X| is rounded to 53 bits (ties to even) by one conversion of
+ * its normalised top 64 bits with the lower bits folded into a sticky bit,
+ * and scaled by 2^-(D_i + D_j), D = P - e of row i and column j, with one
+ * ldexp. */
 constexpr int OZ2_LIMBS = 6;    /* 32-bit limbs: M < 2^171 */
 constexpr int OZ2_LIMBS16 = 11; /* 16-bit limbs of M_j */
 
 struct Oz2Crt {
   int nmod, L; /* L = floor(log2 M) */
   unsigned W16[OZ2_MAX_MODULI][OZ2_LIMBS16]; /* W_j = (w_j M_j) mod M, 16-bit limbs */
-  unsigned M[OZ2_LIMBS], H[OZ2_LIMBS];        /* M and floor(M / 2) */
-  double rMd;                                 /* 1 / M, rounded */
+  unsigned Mc16[OZ2_LIMBS16];                /* 2^(16 LW) - M, 16-bit limbs */
+  double rMs;                                /* 2^(16 (LW - 4)) / M, rounded */
 };
+
+/* 16-bit limbs of W_j and of M for L = floor(log2 M). */
+inline int oz2_lw(int L) { return (L + 16) / 16; }
 
 /* Host: the constants of the first nmod moduli. */
 bool oz2_constants(int nmod, Oz2Crt *c) {
   if (nmod < OZ2_MIN_MODULI || nmod > OZ2_MAX_MODULI) return false;
   memset(c, 0, sizeof *c);
   c->nmod = nmod;
-  c->M[0] = 1;
+  unsigned M[OZ2_LIMBS] = {1u}; /* M, 32-bit limbs */
   for (int j = 0; j < nmod; ++j) {
     unsigned long long carry = 0;
     for (int l = 0; l < OZ2_LIMBS; ++l) {
-      const unsigned long long v = (unsigned long long)c->M[l] * OZ2_MODULI[j] + carry;
-      c->M[l] = (unsigned)v;
+      const unsigned long long v = (unsigned long long)M[l] * OZ2_MODULI[j] + carry;
+      M[l] = (unsigned)v;
       carry = v >> 32;
     }
   }
   for (int l = OZ2_LIMBS - 1; l >= 0; --l)
-    if (c->M[l]) {
-      c->L = 32 * l + 31 - __builtin_clz(c->M[l]);
+    if (M[l]) {
+      c->L = 32 * l + 31 - __builtin_clz(M[l]);
       break;
     }
   for (int j = 0; j < nmod; ++j) {
@@ -130,7 +138,7 @@ bool oz2_constants(int nmod, Oz2Crt *c) {
     unsigned mj[OZ2_LIMBS];
     unsigned long long rem = 0;
     for (int l = OZ2_LIMBS - 1; l >= 0; --l) { /* M_j = M / m, exact */
-      const unsigned long long cur = (rem << 32) | c->M[l];
+      const unsigned long long cur = (rem << 32) | M[l];
       mj[l] = (unsigned)(cur / m);
       rem = cur % m;
     }
@@ -155,154 +163,90 @@ bool oz2_constants(int nmod, Oz2Crt *c) {
     for (int l = 0; l < OZ2_LIMBS16; ++l)
       c->W16[j][l] = (wj[l / 2] >> (16 * (l % 2))) & 0xffffu;
   }
-  for (int l = 0; l < OZ2_LIMBS; ++l)
-    c->H[l] = (c->M[l] >> 1) | (l + 1 < OZ2_LIMBS ? c->M[l + 1] << 31 : 0u);
+  const int lw = oz2_lw(c->L);
+  /* 2^(16 lw) - M in 16-bit limbs: the complement of M's limbs plus one */
+  unsigned carry16 = 1;
+  for (int l = 0; l < lw; ++l) {
+    const unsigned m16 = (M[l / 2] >> (16 * (l % 2))) & 0xffffu;
+    const unsigned v = (~m16 & 0xffffu) + carry16;
+    c->Mc16[l] = v & 0xffffu;
+    carry16 = v >> 16;
+  }
   double md = 0.0;
-  for (int l = OZ2_LIMBS - 1; l >= 0; --l) md = md * 4294967296.0 + (double)c->M[l];
-  c->rMd = 1.0 / md;
+  for (int l = OZ2_LIMBS - 1; l >= 0; --l) md = md * 4294967296.0 + (double)M[l];
+  c->rMs = ldexp(1.0 / md, 16 * (lw - 4));
+  /* the rounding of X'/M needs its distance from a half-integer, at least
+   * (M - 2^L) / (2M), well above the estimate's error */
+  if ((md - ldexp(1.0, c->L)) / (2.0 * md) < 0x1p-10) return false;
   return true;
 }
 
-/* Bits [s, s+64) of an NL-limb number. */
-template <int NL>
-__device__ __forceinline__ unsigned long long bits64(const unsigned (&x)[NL], int s) {
-  unsigned long long r = 0;
-#pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    const int pos = 32 * l - s; /* where limb l lands in the result */
-    if (pos >= 64 || pos <= -32) continue;
-    const unsigned long long v = x[l];
-    r |= pos >= 0 ? v << pos : v >> -pos;
-  }
-  return r;
-}
-
-/* X from the carried 16-bit limb sums, reduced modulo M, rounded and scaled;
- * then the alpha/beta update of C(i, j).  LW: 16-bit limbs of the W_j (M <
- * 2^(16 LW)); the sum X' = sum_j c_j W_j < 22 * 2^8 * M needs NL = LW / 2 + 1
- * 32-bit limbs. */
+/* X from the limb sums acc (X' = sum_l acc[l] 2^(16 l), each below 2^29),
+ * rounded and scaled; then the alpha/beta update of C(i, j).  LW: 16-bit
+ * limbs of the W_j (M < 2^(16 LW)), at least 4. */
 template <int LW>
-__device__ __forceinline__ void oz2_finish(const unsigned (&acc)[LW], const Oz2Crt &crt,
-                                           int da, int db, double *cp, double alpha,
-                                           double beta) {
-  constexpr int NL = LW / 2 + 1;
-  unsigned x[NL];
-  {
-    /* 16-bit digits of X': the limb sums (< 2^29) carried upward; the final
-     * carry (< 2^14) is digit LW */
-    unsigned d[2 * NL];
-    unsigned carry = 0;
+__device__ __forceinline__ void oz2_finish(unsigned (&acc)[LW], const Oz2Crt &crt, int da,
+                                           int db, double *cp, double alpha, double beta) {
+  constexpr int NW = (LW + 1) / 2; /* 32-bit words of X */
+  /* q = rint(X' / M) from the top four limb sums */
+  double xd = (double)acc[LW - 1];
 #pragma unroll
-    for (int l = 0; l < LW; ++l) {
-      const unsigned v = acc[l] + carry;
-      d[l] = v & 0xffffu;
-      carry = v >> 16;
-    }
+  for (int l = LW - 2; l >= LW - 4; --l) xd = __builtin_fma(xd, 65536.0, (double)acc[l]);
+  const unsigned q = (unsigned)__builtin_rint(xd * crt.rMs);
+  /* X = X' + q (2^(16 LW) - M) modulo 2^(16 LW), carried into 32-bit words */
 #pragma unroll
-    for (int l = LW; l < 2 * NL; ++l) {
-      d[l] = carry & 0xffffu;
-      carry >>= 16;
-    }
+  for (int l = 0; l < LW; ++l) acc[l] += __umul24(q, crt.Mc16[l]);
+  unsigned w[NW];
+  unsigned carry = 0;
 #pragma unroll
-    for (int l = 0; l < NL; ++l) x[l] = d[2 * l] | (d[2 * l + 1] << 16);
+  for (int j = 0; j < NW; ++j) {
+    const unsigned lo = acc[2 * j] + carry;
+    const unsigned hi = (2 * j + 1 < LW ? acc[2 * j + 1] : 0u) + (lo >> 16);
+    w[j] = (lo & 0xffffu) | (hi << 16);
+    carry = hi >> 16;
   }
-  /* X = X' mod M: subtract q M with q < nmod 2^8 from FP64 (off by at most
-   * one), then correct */
-  double xd = 0.0;
+  constexpr int TOP = (16 * LW - 1) & 31; /* sign bit within w[NW - 1] */
+  const bool neg = (w[NW - 1] >> TOP) & 1u;
+  if constexpr (TOP != 31) w[NW - 1] &= (2u << TOP) - 1u;
+  if (neg) { /* |X| = 2^(16 LW) - w */
+    unsigned c = 1;
 #pragma unroll
-  for (int l = NL - 1; l >= 0; --l) xd = xd * 4294967296.0 + (double)x[l];
-  const long long q = (long long)(xd * crt.rMd);
-  long long borrow = 0;
-#pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    const long long d = (long long)x[l] - (long long)((unsigned long long)q * crt.M[l]) + borrow;
-    x[l] = (unsigned)d;
-    borrow = d >> 32; /* arithmetic: floor(d / 2^32) */
-  }
-  if (borrow < 0) { /* X < 0: add M */
-    unsigned long long c2 = 0;
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const unsigned long long v = (unsigned long long)x[l] + crt.M[l] + c2;
-      x[l] = (unsigned)v;
-      c2 = v >> 32;
+    for (int j = 0; j < NW; ++j) {
+      const unsigned nv = ~w[j] + c;
+      c = (c && nv == 0u) ? 1u : 0u;
+      w[j] = nv;
     }
+    if constexpr (TOP != 31) w[NW - 1] &= (2u << TOP) - 1u;
   }
-  bool ge = true; /* X >= M: subtract M */
+  /* top nonzero word k and the two below it */
+  unsigned t0 = 0, t1 = 0, t2 = 0;
+  int k = -1;
 #pragma unroll
-  for (int l = NL - 1; l >= 0; --l) {
-    if (x[l] != crt.M[l]) {
-      ge = x[l] > crt.M[l];
-      break;
+  for (int j = 0; j < NW; ++j) {
+    if (w[j] != 0u) {
+      k = j;
+      t0 = w[j];
+      t1 = j >= 1 ? w[j - 1] : 0u;
+      t2 = j >= 2 ? w[j - 2] : 0u;
     }
-  }
-  if (ge) {
-    long long b2 = 0;
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const long long d = (long long)x[l] - (long long)crt.M[l] + b2;
-      x[l] = (unsigned)d;
-      b2 = d >> 32;
-    }
-  }
-  /* symmetric range: X > M/2 stands for X - M */
-  bool gt = false;
-#pragma unroll
-  for (int l = NL - 1; l >= 0; --l) {
-    if (x[l] != crt.H[l]) {
-      gt = x[l] > crt.H[l];
-      break;
-    }
-  }
-  const bool neg = gt;
-  if (neg) {
-    long long b3 = 0;
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      const long long d = (long long)crt.M[l] - (long long)x[l] + b3;
-      x[l] = (unsigned)d;
-      b3 = d >> 32;
-    }
-  }
-  /* round |X| to 53 bits, ties to even */
-  int top = -1;
-#pragma unroll
-  for (int l = NL - 1; l >= 0; --l)
-    if (top < 0 && x[l]) top = 32 * l + 31 - __builtin_clz(x[l]);
-  double mag = 0.0;
-  int shift = 0;
-  if (top >= 0) {
-    unsigned long long mant;
-    if (top <= 52) {
-      mant = (unsigned long long)x[0] | ((unsigned long long)x[1] << 32);
-    } else {
-      shift = top - 52;
-      mant = bits64(x, shift) & ((1ull << 53) - 1);
-      const int hb = shift - 1;
-      const bool half = (bits64(x, hb) & 1ull) != 0;
-      bool sticky = false;
-#pragma unroll
-      for (int l = 0; l < NL; ++l) {
-        const int lo = 32 * l;
-        if (lo >= hb) continue;
-        const unsigned mask = hb - lo >= 32 ? 0xffffffffu : ((1u << (hb - lo)) - 1u);
-        sticky |= (x[l] & mask) != 0;
-      }
-      if (half && (sticky || (mant & 1))) {
-        ++mant;
-        if (mant == (1ull << 53)) {
-          mant >>= 1;
-          ++shift;
-        }
-      }
-    }
-    mag = (double)mant;
   }
   double scaled;
   if (da == OZ_NONFINITE || db == OZ_NONFINITE) {
     scaled = __builtin_nan("");
+  } else if (k < 0) {
+    scaled = 0.0;
   } else {
-    scaled = ldexp(mag, shift - da - db);
+    /* |X| = (N 2^32 + rest) 2^(32 (k - 2) - z) + (words below k - 2) */
+    unsigned under = 0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) under |= j + 3 <= k ? w[j] : 0u;
+    const int z = __builtin_clz(t0);
+    const unsigned long long top = ((unsigned long long)t0 << 32) | t1;
+    unsigned long long N = z ? (top << z) | (t2 >> (32 - z)) : top;
+    const unsigned rest = z ? t2 << z : t2;
+    N |= (rest | under) != 0u ? 1ull : 0ull; /* sticky: below the rounding bit */
+    const double mag = (double)N; /* one rounding, to nearest even */
+    scaled = ldexp(mag, 32 * (k - 1) - z - da - db);
     scaled = neg ? -scaled : scaled;
   }
   *cp = __builtin_fma(alpha, scaled, beta * *cp);
@@ -362,12 +306,14 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
         }
       }
     }
+    /* (i, j) of the first element by one division, then stepped */
+    int64_t j = t0 / M, i = t0 - j * M;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      const int64_t t = t0 + q;
-      if (t < total) {
-        const int64_t j = t / M, i = t - j * M;
-        oz2_finish<LW>(acc[q], crt, ea[i], fb[j], C + i + j * ldc, alpha, beta);
+      if (t0 + q < total) oz2_finish<LW>(acc[q], crt, ea[i], fb[j], C + i + j * ldc, alpha, beta);
+      if (++i == M) {
+        i = 0;
+        ++j;
       }
     }
   }

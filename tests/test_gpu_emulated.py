@@ -156,6 +156,38 @@ def test_ozaki2_bit_exact_every_modulus_count(cuda, nmod):
     np.testing.assert_array_equal(G, R)
 
 
+@pytest.mark.parametrize("nmod", [8, 12, 16, 19, 22])
+def test_ozaki2_rounding_ties_and_word_positions(cuda, nmod):
+    """Integer products that need rounding to 53 bits: odd 54-bit products are
+    exact ties (to even), (2^54 - 1) rounds up across a binade, and the
+    other pairs give 55-bit and short results, of both signs, plus zeros.
+    The modulus counts place the reconstructed integer in different 32-bit
+    words.  The oracle is pinned to Python's correctly rounded int -> float
+    conversion where the split keeps the integers exact (P >= 53 with
+    k = 1, nmod >= 16), and the device must match the oracle bit for bit."""
+    import torch
+    a = [1, 3, 5, 7, -3, 9, 2**26 + 1, -(2**25 + 3)]
+    b = [(2**53 + 1) // 3, (2**54 - 1) // 3, 2**52 + 1, 2**53 - 1, -(2**52 + 3),
+         4503599627370497, 1, 0]
+    A = np.asfortranarray(np.array(a, dtype=np.float64).reshape(8, 1))
+    B = np.asfortranarray(np.array(b, dtype=np.float64).reshape(1, 8))
+    C = np.zeros((8, 8), order="F")
+    R = C.copy(order="F")
+    oracle.ozaki2_gemm("N", "N", 8, 8, 1, 1.0, A, 8, B, 1, 0.0, R, 8, nmod)
+    if nmod >= 16:
+        want = np.array([[float(x * y) for y in b] for x in a])
+        np.testing.assert_array_equal(R, want)
+    G = _ozaki2_gpu(torch, "N", "N", 8, 8, 1, 1.0, A, 8, B, 1, 0.0, C, 8, nmod)
+    np.testing.assert_array_equal(G, R)
+    # the same pairs as two-term sums with a cancelling partner
+    A2 = np.asfortranarray(np.hstack([A, -A]))
+    B2 = np.asfortranarray(np.vstack([B, B - 2.0]))
+    R2 = np.zeros((8, 8), order="F")
+    oracle.ozaki2_gemm("N", "N", 8, 8, 2, 1.0, A2, 8, B2, 2, 0.0, R2, 8, nmod)
+    G2 = _ozaki2_gpu(torch, "N", "N", 8, 8, 2, 1.0, A2, 8, B2, 2, 0.0, np.zeros((8, 8), order="F"), 8, nmod)
+    np.testing.assert_array_equal(G2, R2)
+
+
 def test_ozaki2_chunks_subnormals_and_semantics(cuda):
     import torch
     rng = np.random.default_rng(9)

@@ -416,6 +416,24 @@ def test_ozaki2_scaled_integers_are_rounded_to_nearest_even():
             assert abs(want) <= 2 ** P
 
 
+def test_ozaki2_gemm_rounds_tie_products_to_even():
+    """k = 1 products of odd integers: 54-bit odd products are exact ties,
+    (2^54 - 1) rounds up across a binade; with 16 or more moduli the split
+    keeps the integers exact, so the result is Python's correctly rounded
+    int -> float conversion of the exact product."""
+    a = [1, 3, 5, 7, -3, 9, 2**26 + 1, -(2**25 + 3)]
+    b = [(2**53 + 1) // 3, (2**54 - 1) // 3, 2**52 + 1, 2**53 - 1, -(2**52 + 3),
+         4503599627370497, 1, 0]
+    A = np.asfortranarray(np.array(a, dtype=np.float64).reshape(8, 1))
+    B = np.asfortranarray(np.array(b, dtype=np.float64).reshape(1, 8))
+    want = np.array([[float(x * y) for y in b] for x in a])
+    assert sum(float(x * y) != x * y for x in a for y in b) >= 30  # most need rounding
+    for nmod in (16, 19, 22):
+        R = np.zeros((8, 8), order="F")
+        oracle.ozaki2_gemm("N", "N", 8, 8, 1, 1.0, A, 8, B, 1, 0.0, R, 8, nmod)
+        np.testing.assert_array_equal(R, want)
+
+
 def test_ozaki2_round_is_rne_then_one_ldexp():
     from fractions import Fraction
     rng = np.random.default_rng(22)
