@@ -28,7 +28,7 @@ __all__ = [
     "sgemm_dev", "dgemm_strided_batched_dev", "sgemm_strided_batched_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
-    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
+    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "set_product_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
     "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2", "MODE_AUTO",
 ]
 
@@ -93,6 +93,9 @@ def _declare(lib, dtype):
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.POINTER(ctypes.c_char_p)]
     f = getattr(lib, f"{p}gemm_x_set_tile_config")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_int]
+    f = getattr(lib, f"{p}gemm_x_set_product_config")
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_int]
     f = getattr(lib, f"{p}gemm_x_load_tuning")
@@ -515,6 +518,16 @@ def set_tile_config(dtype: str, cfg: int):
     -1 returns to the tuning table."""
     lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
     rc = getattr(lib, f"{dtype}gemm_x_set_tile_config")(cfg)
+    if rc:
+        _check(dtype, rc)
+
+
+def set_product_config(dtype: str, cfg: int):
+    """Product kernel of the emulated forms in the library of `dtype` ('d':
+    Ozaki-I and Ozaki-II, 's': BF16x3): 1 direct-to-LDS (default), 0
+    register-staged, -1 back to GEMMX_SLICE_CFG / the default."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    rc = getattr(lib, f"{dtype}gemm_x_set_product_config")(cfg)
     if rc:
         _check(dtype, rc)
 

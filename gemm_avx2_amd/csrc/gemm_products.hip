@@ -17,6 +17,8 @@
  */
 #include <stdlib.h>
 
+#include <atomic>
+
 #include "gemmx_device.h"
 #include "gemmx_slices.h"
 
@@ -148,7 +150,7 @@ __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, co
     }
 }
 
-template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE, int PAD>
+template <int KIND, int BM, int BN, int WM, int WN, int BKB, int PAD>
 __global__ void __launch_bounds__(64 * WM * WN)
 slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
                       int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
@@ -235,7 +237,7 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
     }
     ca.template store<LS>(sA(cur ^ 1), tid);
     cb.template store<LS>(sB(cur ^ 1), tid);
-    if (PIPE) {
+    {
       /* Issue order: the stage's global loads, the first K step's fragment
        * reads, then every later K step's fragment reads spread over the
        * previous step's MFMAs, and the LDS writes of the next stage spread
@@ -262,165 +264,6 @@ slice_products_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ 
     }
     __syncthreads();
   }
-
-  store_tile<KIND, FM, FN>(acc, g, G, S, C, ldc, alpha, beta, M, N, m0 + wm * WTM,
-                           n0 + wn * WTN, li, kg);
-}
-
-/* The same product with global->LDS direct loads (global_load_lds_dwordx4:
- * lane L of an instruction writes 16 bytes at LDS base + 16 L): 256x256 tile,
- * WM x WN waves, 64-byte K stages in NST LDS buffers, loads issued NST - 1
- * stages ahead and no register staging.  A stage holds 64-byte rows, four
- * lanes of a load reading one row; the row's 16-byte chunk j sits at position
- * j ^ dl_swz(row). */
-
-/* Position swizzle of a 64-byte row: bits 2 and 3 of the row, swapped, so
- * that the 16 rows of a fragment read at one chunk take each position once
- * per row % 4.  (Measured on gfx950: ds_read_b128 over 64-byte rows shows a
- * 2-way bank conflict with any in-row or in-block chunk permutation; a
- * conflict-free fragment-ordered layout, which needs 32-byte instead of
- * 64-byte global runs per load, ran slower, so the rows stay whole.) */
-__device__ __forceinline__ int dl_swz(int r) { return ((r >> 1) & 2) | ((r >> 3) & 1); }
-
-template <int KIND, int NST, int WM, int WN, int BN>
-__global__ void __launch_bounds__(64 * WM * WN, 8 / (WM * WN))
-slice_products_dl_kernel(const int8_t *__restrict__ As, const int8_t *__restrict__ Bs,
-                         int M, int N, int64_t kpad_bytes, int kb_begin, int kb_end,
-                         SliceGroups G, int tiles_m, int tiles_n, void *__restrict__ S,
-                         float *__restrict__ C, int64_t ldc, float alpha, float beta) {
-  constexpr int BM = 256, BKB = 64, NW = WM * WN;
-  constexpr int WTM = BM / WM, WTN = BN / WN;
-  constexpr int FM = WTM / 16, FN = WTN / 16;
-  constexpr int STAGE = (BM + BN) * BKB; /* bytes per LDS stage */
-  constexpr int LPA = BM * BKB / 1024 / NW; /* 1 KB loads per wave and stage: A */
-  constexpr int LPB = BN * BKB / 1024 / NW; /* and B */
-  constexpr int WAITN = (NST - 2) * (LPA + LPB); /* loads of NST - 2 stages */
-  static_assert(LPA * NW * 1024 == BM * BKB && LPB * NW * 1024 == BN * BKB, "load split");
-  constexpr bool BF16 = KIND == SP_BF16;
-  typedef SliceMfma<BF16> MM;
-  typedef typename MM::acc_t acc_t;
-  extern __shared__ __align__(16) int smem[];
-  char *const lds = reinterpret_cast<char *>(smem);
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WM, wn = wave / WM;
-  const int li = lane & 15, kg = lane >> 4;
-  const int ntiles = tiles_m * tiles_n;
-  const int g = blockIdx.x / ntiles;
-  int tm, tn;
-  tile_coords(blockIdx.x - g * ntiles, ntiles, tiles_m, tiles_n, &tm, &tn);
-  const int m0 = tm * BM, n0 = tn * BN;
-  const int pc = G.gcnt[g];
-  const unsigned tpack = G.tpack[g], upack = G.upack[g];
-  const int kt_begin = kb_begin / BKB;
-  const int nkt = (kb_end - kb_begin) / BKB;
-  const int iters = pc * nkt;
-
-  /* This wave's loads per operand and stage: LDS chunks
-   * c = (LPA wave + u) 64 + lane (LPB for B), row c / 4, position c % 4, holding chunk
-   * (c % 4) ^ dl_swz(row) of that row (rows past the matrix are clamped to
-   * its last row; their products are never stored). */
-  int64_t a_off[LPA], b_off[LPB];
-#pragma unroll
-  for (int u = 0; u < LPA; ++u) {
-    const int c = (LPA * wave + u) * 64 + lane, r = c >> 2, j = (c & 3) ^ dl_swz(r);
-    a_off[u] = (r < M - m0 ? r : M - m0 - 1) * kpad_bytes + 16 * j;
-  }
-#pragma unroll
-  for (int u = 0; u < LPB; ++u) {
-    const int c = (LPB * wave + u) * 64 + lane, r = c >> 2, j = (c & 3) ^ dl_swz(r);
-    b_off[u] = (r < N - n0 ? r : N - n0 - 1) * kpad_bytes + 16 * j;
-  }
-  const int8_t *a_tile = As + (int64_t)m0 * kpad_bytes;
-  const int8_t *b_tile = Bs + (int64_t)n0 * kpad_bytes;
-  auto issue = [&](int q, int buf) {
-    const int pl = q / nkt, kt = kt_begin + q - pl * nkt;
-    const int t = KIND == SP_I8_MOD ? g : (tpack >> (4 * pl)) & 15;
-    const int uu = KIND == SP_I8_MOD ? g : (upack >> (4 * pl)) & 15;
-    const int8_t *ga = a_tile + (int64_t)t * M * kpad_bytes + kt * BKB;
-    const int8_t *gb = b_tile + (int64_t)uu * N * kpad_bytes + kt * BKB;
-    char *la = lds + buf * STAGE, *lb = la + BM * BKB;
-#pragma unroll
-    for (int u = 0; u < LPA; ++u)
-      __builtin_amdgcn_global_load_lds((const void *)(ga + a_off[u]),
-                                       (__attribute__((address_space(3))) void *)(la + (LPA * wave + u) * 1024),
-                                       16, 0, 0);
-#pragma unroll
-    for (int u = 0; u < LPB; ++u)
-      __builtin_amdgcn_global_load_lds((const void *)(gb + b_off[u]),
-                                       (__attribute__((address_space(3))) void *)(lb + (LPB * wave + u) * 1024),
-                                       16, 0, 0);
-  };
-
-  struct Frag {
-    i4_t a[FM], b[FN];
-  };
-  auto read = [&](int buf, Frag &f) {
-    const char *sa = lds + buf * STAGE, *sb = sa + BM * BKB;
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-      const int r = wm * WTM + i * 16 + li;
-      f.a[i] = *reinterpret_cast<const i4_t *>(sa + r * BKB + 16 * (kg ^ dl_swz(r)));
-    }
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int r = wn * WTN + j * 16 + li;
-      f.b[j] = *reinterpret_cast<const i4_t *>(sb + r * BKB + 16 * (kg ^ dl_swz(r)));
-    }
-  };
-
-  acc_t acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
-  /* Iteration q multiplies the fragments of stage q, already in registers,
-   * while it reads those of stage q + 1 from LDS; stages q + 2 .. q + NST are
-   * in flight.  Its barrier follows this wave's wait for its own loads of
-   * stage q + 1 (at most the loads of the NST - 2 stages after it still
-   * outstanding) and for its fragment reads of stage q, so after it stage
-   * q + 1 is in LDS and the buffer of stage q is free for stage q + NST.  Past
-   * the last stage the loads repeat it (into free buffers), which keeps the
-   * number of loads in flight, and so the wait count, the same in every
-   * iteration. */
-  auto step = [&](int q, const Frag &cur, Frag &nxt) {
-    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(WAITN) : "memory");
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    /* A plain barrier: __syncthreads' release fence would also wait for the
-     * loads still in flight to the later stages. */
-    asm volatile("s_barrier" ::: "memory");
-    issue(q + NST < iters ? q + NST : iters - 1, q % NST);
-    read((q + 1) % NST, nxt);
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-      for (int j = 0; j < FN; ++j) acc[i][j] = MM::mma(cur.a[i], cur.b[j], acc[i][j]);
-    /* the stage's loads, then the next fragments' reads spread over the
-     * MFMAs */
-    constexpr int NR = FM + FN, NMF = FM * FN, PER = NMF / NR;
-    __builtin_amdgcn_sched_group_barrier(0x020, LPA + LPB, 0);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      __builtin_amdgcn_sched_group_barrier(0x008, PER, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-    }
-    __builtin_amdgcn_sched_group_barrier(0x008, NMF - PER * NR, 0);
-  };
-
-#pragma unroll
-  for (int s = 0; s < NST; ++s) issue(s < iters ? s : iters - 1, s);
-  asm volatile("s_waitcnt vmcnt(%0)" ::"n"((NST - 1) * (LPA + LPB)) : "memory");
-  asm volatile("s_barrier" ::: "memory");
-  Frag f0, f1;
-  read(0, f0);
-  int q = 0;
-  for (; q + 1 < iters; q += 2) {
-    step(q, f0, f1);
-    step(q + 1, f1, f0);
-  }
-  if (q < iters) step(q, f0, f1);
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); /* no LDS write in flight at exit */
 
   store_tile<KIND, FM, FN>(acc, g, G, S, C, ldc, alpha, beta, M, N, m0 + wm * WTM,
                            n0 + wn * WTN, li, kg);
@@ -583,26 +426,11 @@ hipError_t launch_dl2(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
   return hipGetLastError();
 }
 
-template <int KIND, int NST, int WM, int WN, int BN = 256>
-hipError_t launch_dl(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad, int kb0,
-                     int kb1, const SliceGroups &G, void *S, float *C, int64_t ldc,
-                     float alpha, float beta, hipStream_t st) {
-  auto kern = slice_products_dl_kernel<KIND, NST, WM, WN, BN>;
-  const size_t lds = (size_t)NST * (256 + BN) * 64;
-  hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  const int tiles_m = (m + 255) / 256, tiles_n = (n + BN - 1) / BN;
-  hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n * G.ngroups), dim3(64 * WM * WN), lds, st,
-                     as, bs, m, n, kpad, kb0, kb1, G, tiles_m, tiles_n, S, C, ldc, alpha, beta);
-  return hipGetLastError();
-}
-
-template <int KIND, int BM, int BN, int WM, int WN, int BKB, bool PIPE, int PAD = 4>
+template <int KIND, int BM, int BN, int WM, int WN, int BKB, int PAD = 4>
 hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
                       int kb0, int kb1, const SliceGroups &G, void *S, float *C,
                       int64_t ldc, float alpha, float beta, hipStream_t st) {
-  auto kern = slice_products_kernel<KIND, BM, BN, WM, WN, BKB, PIPE, PAD>;
+  auto kern = slice_products_kernel<KIND, BM, BN, WM, WN, BKB, PAD>;
   const size_t lds = (size_t)2 * (BM + BN) * (BKB / 4 + PAD) * 4;
   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -614,57 +442,43 @@ hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
   return hipGetLastError();
 }
 
-/* Tile configuration: 0 = 256x256 with 8 waves of 128x64 and 128-byte K
- * stages, MFMAs interleaved with the fragment reads and LDS writes, lines
- * padded by 16 bytes; 1 = 256x256 with 4 waves of 128x128 and 64-byte K
- * stages; 2 = as 0 with the compiler's own order; 3 = as 0 with 32 bytes of
- * padding; 4 = direct-to-LDS loads, 8 waves of 128x64, 64-byte stages, 3
- * stages ahead; 5 = as 4 with 2 stages ahead; 6 = direct-to-LDS, 128-byte
- * stages in two buffers (one barrier per two k-steps), the default for every
- * kind: measured at 8192^3 it is 5-6% faster than 3 (and 4.6% than 4) for
+/* Product kernel: 1 (the default for every kind) = direct-to-LDS loads,
+ * 128-byte K stages in two buffers, one barrier per two k-steps; 0 =
+ * register-staged 128-byte stages with MFMAs interleaved with the fragment
+ * reads and LDS writes, lines padded by 16 bytes.  Both: 256x256 tiles, 8
+ * waves of 128x64.  Measured at 8192^3, 1 is 5-6% faster than 0 for
  * residue, slice and BF16 products alike.  Variants measured slower and
- * dropped: 4 waves of 128x128 with 128-byte stages, and direct-to-LDS with 4
- * waves of 128x128 (both spill), 256x128 and 128x256 tiles (one wave per
- * SIMD), two 256x128 blocks per CU, four direct-to-LDS stages ahead, 64-byte
- * stages with 8 waves and register staging, the compiler's built-in GEMM
- * interleaving, and a bank-conflict-free fragment-ordered direct-to-LDS
- * layout (32-byte global runs: slower).  GEMMX_SLICE_CFG overrides for every
- * kind. */
-int slice_cfg(int kind) {
-  static int cfg = -2;
+ * dropped: direct-to-LDS with 64-byte stages in four buffers, three stages
+ * ahead (4.6% slower than 1) or two ahead, 4 waves of 128x128 with 128-byte
+ * stages and direct-to-LDS with 4 waves of 128x128 (both spill), 256x128 and
+ * 128x256 tiles (one wave per SIMD), two 256x128 blocks per CU, 32 bytes of
+ * line padding, 64-byte stages with 8 waves and register staging, the
+ * compiler's built-in GEMM interleaving, and a bank-conflict-free
+ * fragment-ordered direct-to-LDS layout (32-byte global runs: slower).
+ * gemmx_set_product_config (else GEMMX_SLICE_CFG, read at first use)
+ * selects one for every kind. */
+std::atomic<int> g_product_cfg{-2};
+
+int product_cfg() {
+  int cfg = g_product_cfg.load(std::memory_order_relaxed);
   if (cfg == -2) {
     const char *e = getenv("GEMMX_SLICE_CFG");
-    cfg = (e && *e) ? atoi(e) : -1;
-    if (cfg < -1 || cfg > 6) cfg = -1;
+    const int v = (e && *e) ? atoi(e) : -1;
+    cfg = (v == 0 || v == 1) ? v : -1;
+    int expected = -2;
+    if (!g_product_cfg.compare_exchange_strong(expected, cfg)) cfg = expected;
   }
-  (void)kind;
-  return cfg >= 0 ? cfg : 6;
+  return cfg >= 0 ? cfg : 1;
 }
 
 template <int KIND>
 hipError_t products(const int8_t *as, const int8_t *bs, int m, int n, int64_t kpad,
                     int kb0, int kb1, const SliceGroups &G, void *S, float *C,
                     int64_t ldc, float alpha, float beta, hipStream_t st) {
-  switch (slice_cfg(KIND)) {
-  case 1:
-    return launch_cfg<KIND, 256, 256, 2, 2, 64, false>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
-                                                       ldc, alpha, beta, st);
-  case 2:
-    return launch_cfg<KIND, 256, 256, 2, 4, 128, false>(as, bs, m, n, kpad, kb0, kb1, G, S,
-                                                        C, ldc, alpha, beta, st);
-  case 3:
-    return launch_cfg<KIND, 256, 256, 2, 4, 128, true, 8>(as, bs, m, n, kpad, kb0, kb1, G, S,
-                                                          C, ldc, alpha, beta, st);
-  case 4:
-    return launch_dl<KIND, 4, 2, 4>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc, alpha, beta, st);
-  case 5:
-    return launch_dl<KIND, 3, 2, 4>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc, alpha, beta, st);
-  case 6:
-    return launch_dl2<KIND>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc, alpha, beta, st);
-  default:
-    return launch_cfg<KIND, 256, 256, 2, 4, 128, true>(as, bs, m, n, kpad, kb0, kb1, G, S, C,
-                                                       ldc, alpha, beta, st);
-  }
+  if (product_cfg() == 0)
+    return launch_cfg<KIND, 256, 256, 2, 4, 128>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc,
+                                                 alpha, beta, st);
+  return launch_dl2<KIND>(as, bs, m, n, kpad, kb0, kb1, G, S, C, ldc, alpha, beta, st);
 }
 
 } // namespace
@@ -687,3 +501,9 @@ hipError_t slice_products(int kind, const int8_t *as, const int8_t *bs, int m, i
 }
 
 } // namespace gemmx
+
+extern "C" int gemmx_set_product_config(int cfg) {
+  if (cfg < -1 || cfg > 1) return -1;
+  gemmx::g_product_cfg.store(cfg, std::memory_order_relaxed);
+  return 0;
+}

@@ -689,6 +689,18 @@ EXPORT int sgemm_x_set_tile_config(int cfg) {
   if (gemmx_tune_force(4, cfg) != 0) return set_error(GEMMX_ERR_ARG, "tile config %d", cfg);
   return GEMMX_OK;
 }
+/* Product kernel of the emulated forms: 1 direct-to-LDS (default), 0
+ * register-staged, -1 back to GEMMX_SLICE_CFG / the default. */
+EXPORT int dgemm_x_set_product_config(int cfg) {
+  if (gemmx_set_product_config(cfg) != 0)
+    return set_error(GEMMX_ERR_ARG, "product config %d", cfg);
+  return GEMMX_OK;
+}
+EXPORT int sgemm_x_set_product_config(int cfg) {
+  if (gemmx_set_product_config(cfg) != 0)
+    return set_error(GEMMX_ERR_ARG, "product config %d", cfg);
+  return GEMMX_OK;
+}
 EXPORT int dgemm_x_load_tuning(const char *path) {
   pthread_once(&g_once, init_once);
   return gemmx_tune_load(path);

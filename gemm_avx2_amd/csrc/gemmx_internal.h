@@ -170,6 +170,11 @@ hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int moduli, void *work,
 size_t gemmx_bf16x3_workspace(int m, int n, int k);
 hipError_t gemmx_bf16x3_f32(const gemmx_problem *p, void *work, hipStream_t s);
 
+/* Product kernel of the emulated forms (gemm_products.hip): 1 direct-to-LDS
+ * (default), 0 register-staged, -1 back to GEMMX_SLICE_CFG / the default.
+ * Returns 0, or -1 for another value. */
+int gemmx_set_product_config(int cfg);
+
 #ifdef __cplusplus
 }
 #endif

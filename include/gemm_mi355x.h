@@ -167,6 +167,13 @@ int sgemm_x_set_mode(int mode);
 #define GEMMX_TILE_CONFIGS 7
 int dgemm_x_set_tile_config(int cfg);
 int sgemm_x_set_tile_config(int cfg);
+/* Product kernel of the emulated forms (Ozaki-I, Ozaki-II, BF16x3), for every
+ * later call: 1 = direct-to-LDS loads with 128-byte K stages (the default),
+ * 0 = register-staged; -1 returns to GEMMX_SLICE_CFG (read at first use) or
+ * the default.  Both give identical integer products.  No counterpart in
+ * the reference (its blocking is fixed at compile time, dgemm_tune.h). */
+int dgemm_x_set_product_config(int cfg);
+int sgemm_x_set_product_config(int cfg);
 int dgemm_x_load_tuning(const char *path);
 int sgemm_x_load_tuning(const char *path);
 int dgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);

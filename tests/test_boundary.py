@@ -136,6 +136,19 @@ def test_mode_switches():
 
 
 @pytest.mark.parametrize("dtype", ["d", "s"])
+def test_product_config_accepts_only_its_two_kernels(dtype):
+    lib = gx.load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    f = getattr(lib, f"{dtype}gemm_x_set_product_config")
+    for bad in (2, 6, -2):
+        assert f(bad) == gx.ERR_ARG
+    for good in (0, 1, -1):
+        assert f(good) == gx.OK
+    with pytest.raises(gx.GemmError):
+        gx.set_product_config(dtype, 3)
+    gx.set_product_config(dtype, -1)
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
 def test_batch_checks_arguments_before_the_device(dtype):
     """Grouped batch (no reference counterpart): per-problem argument checks
     name the problem; a valid batch on a machine without a device reports

@@ -156,6 +156,53 @@ def test_ozaki2_bit_exact_every_modulus_count(cuda, nmod):
     np.testing.assert_array_equal(G, R)
 
 
+def test_both_product_kernels(cuda):
+    """The register-staged (0) and direct-to-LDS (1, default) product kernels:
+    Ozaki-I and Ozaki-II bit-identical to the oracle, BF16x3 within FP32
+    accumulation error of it and bit-identical between the two kernels (the
+    same MFMA sequence per element), on ragged shapes that leave partial
+    tiles and K stages."""
+    import torch
+    rng = np.random.default_rng(77)
+    try:
+        bf = {}
+        for c in (0, 1):
+            gx.set_product_config("d", c)
+            gx.set_product_config("s", c)
+            for (m, n, k) in [(300, 270, 333), (17, 520, 129)]:
+                A = np.asfortranarray(rng.uniform(-1, 1, (m, k)) * np.exp(2 * rng.standard_normal((m, k))))
+                B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+                C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+                for s in (4, 7):
+                    R = C.copy(order="F")
+                    oracle.ozaki_gemm("N", "N", m, n, k, 1.25, A, m, B, k, 0.5, R, m, s)
+                    G = _ozaki_gpu(torch, "N", "N", m, n, k, 1.25, A, m, B, k, 0.5, C, m, s)
+                    np.testing.assert_array_equal(G, R)
+                for nmod in (12, 16):
+                    R = C.copy(order="F")
+                    oracle.ozaki2_gemm("N", "N", m, n, k, 1.25, A, m, B, k, 0.5, R, m, nmod)
+                    G = _ozaki2_gpu(torch, "N", "N", m, n, k, 1.25, A, m, B, k, 0.5, C, m, nmod)
+                    np.testing.assert_array_equal(G, R)
+            m, n, k = 260, 200, 300
+            r2 = np.random.default_rng(78)
+            A = np.asfortranarray(r2.uniform(-1, 1, (k, m)).astype(np.float32))
+            B = np.asfortranarray(r2.uniform(-1, 1, (k, n)).astype(np.float32))
+            C = np.asfortranarray(r2.uniform(-1, 1, (m, n)).astype(np.float32))
+            R3 = C.copy(order="F")
+            oracle.bf16x3_gemm("T", "N", m, n, k, 1.5, A, k, B, k, -0.5, R3, m)
+            dA, dB, dC = to_device(A, torch), to_device(B, torch), to_device(C, torch)
+            gx.sgemm_bf16x3_dev("T", "N", m, n, k, 1.5, dA, k, dB, k, -0.5, dC, m)
+            torch.cuda.synchronize()
+            G = from_device(dC, C.shape)
+            bound = abs_bound(A, B, C, "T", "N", m, n, k, 1.5, -0.5)
+            assert max_comp(G, R3, bound) <= 64 * k * U32
+            bf[c] = G
+        np.testing.assert_array_equal(bf[0], bf[1])
+    finally:
+        gx.set_product_config("d", -1)
+        gx.set_product_config("s", -1)
+
+
 @pytest.mark.parametrize("nmod", [8, 12, 16, 19, 22])
 def test_ozaki2_rounding_ties_and_word_positions(cuda, nmod):
     """Integer products that need rounding to 53 bits: odd 54-bit products are
