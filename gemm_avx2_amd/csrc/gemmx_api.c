@@ -288,6 +288,95 @@ static size_t emu_ws(const call_t *x, int mode, int slices) {
   return 0;
 }
 
+/* Column panels of C for a native call with host B or C: about eight
+ * panels, multiples of 256 columns; 0 (no panels) for emulated arithmetic,
+ * narrow C or less than 64 MB of B and C to move. */
+#define GEMMX_MAX_PANELS 16
+static int panel_cols(const call_t *x, int mode, int need_ab, int host_b, int host_c) {
+  if (mode != GEMMX_MODE_NATIVE || !need_ab || !(host_b || host_c) || x->n < 512) return 0;
+  const size_t moved = ((host_b ? (size_t)x->k : 0) + (host_c ? (size_t)x->m : 0)) *
+                       (size_t)x->n * (size_t)x->es;
+  if (moved < ((size_t)64 << 20)) return 0;
+  int np = (x->n + 7) / 8;
+  np = (np + 255) / 256 * 256;
+  return np < x->n ? np : 0;
+}
+
+/* A native call with host operands, pipelined over column panels of C: A is
+ * staged first; then for each panel j the panel of op(B) and of C are staged
+ * and its GEMM is queued, and the previous panel's C is copied back after an
+ * event recorded behind that panel's GEMM, so the transfers of panel j + 1
+ * and j - 1 run while the GPU computes panel j.  p holds the device
+ * operands (staged buffers for host operands: A of leading dimension p->lda,
+ * op(B) columns j0.. at column j0 of B (transb N) or row j0 (transb T)). */
+static int panelled_call(const call_t *x, const gemmx_problem *p, int dev, int np, int host_a,
+                         int host_b, int host_c, hipStream_t s) {
+  const size_t es = (size_t)x->es;
+  const int np_count = (x->n + np - 1) / np;
+  if (np_count > GEMMX_MAX_PANELS) return set_error(GEMMX_ERR_UNSUPPORTED, "too many panels");
+  hipEvent_t start = NULL, done[GEMMX_MAX_PANELS] = {0};
+  int rc = GEMMX_OK, made = 0;
+  hipError_t e = hipEventCreateWithFlags(&start, hipEventDisableTiming);
+  for (; e == hipSuccess && made < np_count; made++)
+    e = hipEventCreateWithFlags(&done[made], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(start, s);
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "panel events");
+    goto end;
+  }
+  const int ra = notrans(x->ta) ? x->m : x->k, ca = notrans(x->ta) ? x->k : x->m;
+  if (host_a && (rc = gemmx_stage_h2d(dev, (void *)p->a, (size_t)p->lda * es, x->a,
+                                      (size_t)x->lda * es, (size_t)ra * es, (size_t)ca, s,
+                                      start)) != GEMMX_OK)
+    goto end;
+  for (int j = 0; j <= np_count && rc == GEMMX_OK; j++) {
+    if (j < np_count) {
+      const int j0 = j * np, nj = x->n - j0 < np ? x->n - j0 : np;
+      /* op(B) columns j0 .. j0 + nj - 1: columns of B, or rows of B^T */
+      const size_t boff_d = p->tb ? (size_t)j0 * es : (size_t)j0 * (size_t)p->ldb * es;
+      const size_t boff_h = p->tb ? (size_t)j0 * es : (size_t)j0 * (size_t)x->ldb * es;
+      if (host_b &&
+          (rc = p->tb ? gemmx_stage_h2d(dev, (char *)p->b + boff_d, (size_t)p->ldb * es,
+                                        (const char *)x->b + boff_h, (size_t)x->ldb * es,
+                                        (size_t)nj * es, (size_t)x->k, s, start)
+                      : gemmx_stage_h2d(dev, (char *)p->b + boff_d, (size_t)p->ldb * es,
+                                        (const char *)x->b + boff_h, (size_t)x->ldb * es,
+                                        (size_t)x->k * es, (size_t)nj, s, start)) != GEMMX_OK)
+        break;
+      if (host_c &&
+          (rc = gemmx_stage_h2d(dev, (char *)p->c + (size_t)j0 * (size_t)p->ldc * es,
+                                (size_t)p->ldc * es,
+                                (const char *)x->c + (size_t)j0 * (size_t)x->ldc * es,
+                                (size_t)x->ldc * es, (size_t)x->m * es, (size_t)nj, s,
+                                start)) != GEMMX_OK)
+        break;
+      gemmx_problem q = *p;
+      q.n = nj;
+      q.b = (const char *)p->b + boff_d;
+      q.c = (char *)p->c + (size_t)j0 * (size_t)p->ldc * es;
+      if ((rc = launch_device(&q, x->es, GEMMX_MODE_NATIVE, 0, NULL, s)) != GEMMX_OK) break;
+      if ((e = hipEventRecord(done[j], s)) != hipSuccess) {
+        rc = hip_fail(e, "panel event");
+        break;
+      }
+    }
+    if (j >= 1 && host_c) { /* panel j - 1 is complete: copy it back */
+      const int i0 = (j - 1) * np, ni = x->n - i0 < np ? x->n - i0 : np;
+      rc = gemmx_stage_d2h(dev, (char *)x->c + (size_t)i0 * (size_t)x->ldc * es,
+                           (size_t)x->ldc * es,
+                           (const char *)p->c + (size_t)i0 * (size_t)p->ldc * es,
+                           (size_t)p->ldc * es, (size_t)x->m * es, (size_t)ni, s, done[j - 1]);
+    }
+  }
+end:
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess && rc == GEMMX_OK) rc = hip_fail(e, "GEMM execution");
+  for (int i = 0; i < made; i++)
+    if (done[i]) (void)hipEventDestroy(done[i]);
+  if (start) (void)hipEventDestroy(start);
+  return rc;
+}
+
 /* The Fortran-ABI path: synchronous, host or device operands. */
 static void fortran_call(call_t *x, int mode, int slices) {
   clear_error();
@@ -333,23 +422,31 @@ static void fortran_call(call_t *x, int mode, int slices) {
   if (total > 0 && get_scratch(dev, total, (void **)&base) != GEMMX_OK) goto out;
   char *pa = base, *pb = pa + bytes_a, *pc = pb + bytes_b, *pw = pc + bytes_c;
   hipError_t e;
+  const int np = panel_cols(x, mode, need_ab, host_b, host_c);
+  if (np > 0) {
+    if (host_a) p.a = pa;
+    if (host_b) p.b = pb;
+    if (host_c) p.c = pc;
+    (void)panelled_call(x, &p, dev, np, host_a, host_b, host_c, s);
+    goto out;
+  }
   /* Host operands go through the pinned staging ring (gemmx_stage.c). */
   if (host_a) {
     if (gemmx_stage_h2d(dev, pa, lda2 * es, x->a, (size_t)x->lda * es, (size_t)ra * es,
-                        (size_t)ca, s) != GEMMX_OK)
+                        (size_t)ca, s, NULL) != GEMMX_OK)
       goto out;
     p.a = pa;
   }
   if (host_b) {
     if (gemmx_stage_h2d(dev, pb, ldb2 * es, x->b, (size_t)x->ldb * es, (size_t)rb * es,
-                        (size_t)cb, s) != GEMMX_OK)
+                        (size_t)cb, s, NULL) != GEMMX_OK)
       goto out;
     p.b = pb;
   }
   if (host_c) {
     /* C is always read: beta == 0 still multiplies (NaN propagates). */
     if (gemmx_stage_h2d(dev, pc, ldc2 * es, x->c, (size_t)x->ldc * es, (size_t)x->m * es,
-                        (size_t)x->n, s) != GEMMX_OK)
+                        (size_t)x->n, s, NULL) != GEMMX_OK)
       goto out;
     p.c = pc;
   }
@@ -358,7 +455,7 @@ static void fortran_call(call_t *x, int mode, int slices) {
     goto out;
   }
   if (host_c && gemmx_stage_d2h(dev, x->c, (size_t)x->ldc * es, pc, ldc2 * es,
-                                 (size_t)x->m * es, (size_t)x->n, s) != GEMMX_OK) {
+                                 (size_t)x->m * es, (size_t)x->n, s, NULL) != GEMMX_OK) {
     (void)hipStreamSynchronize(s);
     goto out;
   }
@@ -446,19 +543,19 @@ static void complex_call(int es, char ta_c, char tb_c, int m, int n, int k, cons
   hipError_t e;
   if (host_a) {
     if (gemmx_stage_h2d(dev, pa, lda2 * ce, a, (size_t)lda * ce, (size_t)ra * ce, (size_t)ca,
-                        s) != GEMMX_OK)
+                        s, NULL) != GEMMX_OK)
       goto out;
     p.a = pa;
   }
   if (host_b) {
     if (gemmx_stage_h2d(dev, pb, ldb2 * ce, b, (size_t)ldb * ce, (size_t)rb * ce, (size_t)cb,
-                        s) != GEMMX_OK)
+                        s, NULL) != GEMMX_OK)
       goto out;
     p.b = pb;
   }
   if (host_c) {
     if (gemmx_stage_h2d(dev, pc, ldc2 * ce, c, (size_t)ldc * ce, (size_t)m * ce, (size_t)n,
-                        s) != GEMMX_OK)
+                        s, NULL) != GEMMX_OK)
       goto out;
     p.c = pc;
   }
@@ -469,7 +566,7 @@ static void complex_call(int es, char ta_c, char tb_c, int m, int n, int k, cons
     goto out;
   }
   if (host_c && gemmx_stage_d2h(dev, c, (size_t)ldc * ce, pc, ldc2 * ce, (size_t)m * ce,
-                                 (size_t)n, s) != GEMMX_OK) {
+                                 (size_t)n, s, NULL) != GEMMX_OK) {
     (void)hipStreamSynchronize(s);
     goto out;
   }
@@ -557,13 +654,13 @@ static void tri_call(int es, int solve, char side, char uplo, char transa, char 
   hipError_t e;
   if (host_a) {
     if (gemmx_stage_h2d(dev, pa, lda2 * e8, a, (size_t)lda * e8, (size_t)k * e8, (size_t)k,
-                        s) != GEMMX_OK)
+                        s, NULL) != GEMMX_OK)
       goto out;
     p.a = pa;
   }
   if (host_b) {
     if (gemmx_stage_h2d(dev, pb, ldb2 * e8, b, (size_t)ldb * e8, (size_t)m * e8, (size_t)n,
-                        s) != GEMMX_OK)
+                        s, NULL) != GEMMX_OK)
       goto out;
     p.b = pb;
   }
@@ -574,7 +671,7 @@ static void tri_call(int es, int solve, char side, char uplo, char transa, char 
     goto out;
   }
   if (host_b && gemmx_stage_d2h(dev, b, (size_t)ldb * e8, pb, ldb2 * e8, (size_t)m * e8,
-                                 (size_t)n, s) != GEMMX_OK) {
+                                 (size_t)n, s, NULL) != GEMMX_OK) {
     (void)hipStreamSynchronize(s);
     goto out;
   }

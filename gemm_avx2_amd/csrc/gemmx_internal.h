@@ -91,15 +91,17 @@ int gemmx_last_code(void);
 int gemmx_current_device(int *dev);
 
 /* Staging of host operands through pinned buffers (gemmx_stage.c): cols
- * columns of row_bytes bytes, leading dimensions in bytes; h2d makes
- * `compute` wait for the copies, d2h starts after the work queued on
- * `compute` and returns when the host block is complete.  The caller holds
- * the device lock.  GEMMX_STAGE_CHUNK_MB (default 32) sets the chunk,
- * GEMMX_COPY_THREADS (default 8) the host copy threads. */
+ * columns of row_bytes bytes, leading dimensions in bytes.  The copies start
+ * after event `after`, or, when it is NULL, after the work queued on
+ * `compute` at the call; h2d makes `compute` wait for its copies, d2h returns
+ * when the host block is complete.  (Blocks below 4 MB go through one runtime
+ * copy on `compute` instead.)  The caller holds the device lock.
+ * GEMMX_STAGE_CHUNK_MB (default 32) sets the chunk, GEMMX_COPY_THREADS
+ * (default 8) the host copy threads. */
 int gemmx_stage_h2d(int dev, void *dst, size_t dld, const void *src, size_t sld,
-                    size_t row_bytes, size_t cols, hipStream_t compute);
+                    size_t row_bytes, size_t cols, hipStream_t compute, hipEvent_t after);
 int gemmx_stage_d2h(int dev, void *dst, size_t dld, const void *src, size_t sld,
-                    size_t row_bytes, size_t cols, hipStream_t compute);
+                    size_t row_bytes, size_t cols, hipStream_t compute, hipEvent_t after);
 size_t gemmx_stage_chunk_bytes(void);
 
 /* Native tile kernels (gemm_native.hip).  alpha != 0 and k > 0 here; the

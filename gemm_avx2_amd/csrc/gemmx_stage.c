@@ -178,7 +178,7 @@ static size_t cols_per_chunk(size_t row_bytes, size_t chunk) {
 }
 
 int gemmx_stage_h2d(int dev, void *dst, size_t dld, const void *src, size_t sld,
-                    size_t row_bytes, size_t cols, hipStream_t compute) {
+                    size_t row_bytes, size_t cols, hipStream_t compute, hipEvent_t after) {
   const size_t chunk = gemmx_stage_chunk_bytes();
   const size_t cpc = cols_per_chunk(dld, chunk);
   hipError_t e;
@@ -192,10 +192,12 @@ int gemmx_stage_h2d(int dev, void *dst, size_t dld, const void *src, size_t sld,
   ring_t *r;
   int rc = ring_get(dev, chunk, &r);
   if (rc != GEMMX_OK) return rc;
-  /* the copy stream starts after work already queued on the compute stream
-   * (the destination may still be in use) */
-  if ((e = hipEventRecord(r->start, compute)) != hipSuccess ||
-      (e = hipStreamWaitEvent(r->copy, r->start, 0)) != hipSuccess)
+  /* the copy stream starts after `after`, or else after the work already
+   * queued on the compute stream (the destination may still be in use) */
+  if (!after && (e = hipEventRecord(r->start, compute)) != hipSuccess)
+    return gemmx_report(GEMMX_ERR_LAUNCH, "staging: stream ordering failed: %s",
+                        hipGetErrorString(e));
+  if ((e = hipStreamWaitEvent(r->copy, after ? after : r->start, 0)) != hipSuccess)
     return gemmx_report(GEMMX_ERR_LAUNCH, "staging: stream ordering failed: %s",
                         hipGetErrorString(e));
   size_t i = 0;
@@ -205,8 +207,14 @@ int gemmx_stage_h2d(int dev, void *dst, size_t dld, const void *src, size_t sld,
     /* buf[b] may still feed a DMA of this or an earlier block */
     if ((e = hipEventSynchronize(r->ev[b])) != hipSuccess) break;
     parallel_copy((char *)r->buf[b], dld, (const char *)src + c0 * sld, sld, row_bytes, nc);
-    if ((e = hipMemcpyAsync((char *)dst + c0 * dld, r->buf[b], nc * dld, hipMemcpyHostToDevice,
-                            r->copy)) != hipSuccess ||
+    /* a block narrower than its leading dimension (a row panel) must leave
+     * the device rows outside it alone: copy row_bytes per column */
+    e = row_bytes < dld
+            ? hipMemcpy2DAsync((char *)dst + c0 * dld, dld, r->buf[b], dld, row_bytes, nc,
+                               hipMemcpyHostToDevice, r->copy)
+            : hipMemcpyAsync((char *)dst + c0 * dld, r->buf[b], nc * dld, hipMemcpyHostToDevice,
+                             r->copy);
+    if (e != hipSuccess ||
         (e = hipEventRecord(r->ev[b], r->copy)) != hipSuccess)
       break;
   }
@@ -224,7 +232,7 @@ int gemmx_stage_h2d(int dev, void *dst, size_t dld, const void *src, size_t sld,
 }
 
 int gemmx_stage_d2h(int dev, void *dst, size_t dld, const void *src, size_t sld,
-                    size_t row_bytes, size_t cols, hipStream_t compute) {
+                    size_t row_bytes, size_t cols, hipStream_t compute, hipEvent_t after) {
   const size_t chunk = gemmx_stage_chunk_bytes();
   const size_t cpc = cols_per_chunk(sld, chunk);
   hipError_t e;
@@ -239,10 +247,13 @@ int gemmx_stage_d2h(int dev, void *dst, size_t dld, const void *src, size_t sld,
   int rc = ring_get(dev, chunk, &r);
   if (rc != GEMMX_OK) return rc;
   const size_t nchunks = (cols + cpc - 1) / cpc;
-  /* the copies start after the computation on the compute stream; the copy
-   * stream is in order, so they also follow every earlier DMA of the ring */
-  if ((e = hipEventRecord(r->start, compute)) != hipSuccess ||
-      (e = hipStreamWaitEvent(r->copy, r->start, 0)) != hipSuccess)
+  /* the copies start after `after`, or else after the computation queued on
+   * the compute stream; the copy stream is in order, so they also follow
+   * every earlier DMA of the ring */
+  if (!after && (e = hipEventRecord(r->start, compute)) != hipSuccess)
+    return gemmx_report(GEMMX_ERR_LAUNCH, "staging: stream ordering failed: %s",
+                        hipGetErrorString(e));
+  if ((e = hipStreamWaitEvent(r->copy, after ? after : r->start, 0)) != hipSuccess)
     return gemmx_report(GEMMX_ERR_LAUNCH, "staging: stream ordering failed: %s",
                         hipGetErrorString(e));
   /* queue the first NBUF chunks, then for each chunk: wait, copy out, queue

@@ -294,3 +294,38 @@ def test_host_pointer_staging_ring(cuda, dt, monkeypatch):
         bound = abs_bound(A, B, C0, ta, tb, m, n, k, 0.5, 2.0)
         assert max_comp(C[:m], R[:m], bound) <= _tol(dt, k), (ta, tb)
         assert np.all(C[m:] == 321.0)
+
+
+@pytest.mark.parametrize("chunk_mb", ["32", "1"])
+def test_host_pointer_column_panels(cuda, chunk_mb, monkeypatch):
+    """Native calls moving 64 MB or more of host B and C run in column panels
+    of C (multiples of 256 columns, about eight): the panel's op(B) columns
+    and C columns are staged while the previous panel computes, and each C
+    panel is copied back behind its own GEMM.  Columns at every panel edge
+    (and the ragged last panel) against the oracle, for NN, NT (op(B)
+    columns are rows of B) and TN in single precision; rows m..ldc-1 of C
+    untouched.  With 1 MB chunks the staging ring's buffers alternate between
+    uploads and downloads of different panels."""
+    monkeypatch.setenv("GEMMX_STAGE_CHUNK_MB", chunk_mb)
+    rng = np.random.default_rng(41)
+    for (dt, ta, tb, m, n, k, pad) in [(np.float64, "N", "N", 300, 2300, 4000, 3),
+                                       (np.float64, "N", "T", 400, 2100, 4200, 1),
+                                       (np.float32, "T", "N", 700, 2600, 7000, 2)]:
+        ra, ca = stored_shape(ta, m, k)
+        rb, cb = stored_shape(tb, k, n)
+        A = np.asfortranarray(rng.uniform(-1, 1, (ra + pad, ca)).astype(dt))
+        B = np.asfortranarray(rng.uniform(-1, 1, (rb + pad, cb)).astype(dt))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + pad, n)).astype(dt))
+        C[m:] = 321.0
+        C0 = C.copy(order="F")
+        _gemm_host(dt, ta, tb, m, n, k, 0.5, A, ra + pad, B, rb + pad, -1.5, C, m + pad)
+        assert np.all(C[m:] == 321.0)
+        np_cols = -(-((n + 7) // 8) // 256) * 256
+        cols = sorted({c for j in range(0, n, np_cols) for c in (j - 1, j, j + 1) if 0 <= c < n}
+                      | {n - 1} | set(rng.integers(0, n, 6).tolist()))
+        Bs = np.asfortranarray(B[:k, cols] if tb == "N" else B[cols, :k])
+        Cs = np.asfortranarray(C0[:m, cols])
+        R = Cs.copy(order="F")
+        oracle.gemm(ta, tb, m, len(cols), k, 0.5, A, ra + pad, Bs, Bs.shape[0], -1.5, R, m)
+        bound = abs_bound(A, Bs, Cs, ta, tb, m, len(cols), k, 0.5, -1.5)
+        assert max_comp(C[:m, cols], R, bound) <= _tol(dt, k), (dt, ta, tb)
