@@ -9,8 +9,10 @@
  * pool of copy threads while the DMA engine moves earlier chunks, so host
  * copies and PCIe transfers overlap.  Device->host runs the same ring the
  * other way: the DMA of chunk i + NBUF - 1 is queued before the host copies
- * chunk i out.  Pinned buffers are allocated once per device on first use
- * (hipHostMalloc) and kept until the process ends.
+ * chunk i out.  Uploads and downloads have rings of their own (buffers, copy
+ * stream, events), so the two directions overlap.  Pinned buffers are
+ * allocated once per device and direction on first use (hipHostMalloc) and
+ * kept until the process ends.
  */
 #define _GNU_SOURCE
 #include <pthread.h>
@@ -121,7 +123,10 @@ typedef struct {
   int ready;
 } ring_t;
 
-static ring_t g_ring[MAX_DEV];
+/* One ring per direction and device (0 uploads, 1 downloads): each has its
+ * own copy stream, so a download can run while uploads are in flight (PCIe
+ * carries both directions at once). */
+static ring_t g_ring[2][MAX_DEV];
 
 size_t gemmx_stage_chunk_bytes(void) {
   const char *e = getenv("GEMMX_STAGE_CHUNK_MB");
@@ -133,8 +138,8 @@ size_t gemmx_stage_chunk_bytes(void) {
 
 /* The device's ring with buffers of at least `bytes`; caller holds the
  * device lock. */
-static int ring_get(int dev, size_t bytes, ring_t **out) {
-  ring_t *r = &g_ring[dev];
+static int ring_get(int dev, int dir, size_t bytes, ring_t **out) {
+  ring_t *r = &g_ring[dir][dev];
   if (!r->ready) {
     if (hipStreamCreateWithFlags(&r->copy, hipStreamNonBlocking) != hipSuccess)
       return gemmx_report(GEMMX_ERR_LAUNCH, "staging: stream creation failed");
@@ -190,7 +195,7 @@ int gemmx_stage_h2d(int dev, void *dst, size_t dld, const void *src, size_t sld,
                                           hipGetErrorString(e));
   }
   ring_t *r;
-  int rc = ring_get(dev, chunk, &r);
+  int rc = ring_get(dev, 0, chunk, &r);
   if (rc != GEMMX_OK) return rc;
   /* the copy stream starts after `after`, or else after the work already
    * queued on the compute stream (the destination may still be in use) */
@@ -244,7 +249,7 @@ int gemmx_stage_d2h(int dev, void *dst, size_t dld, const void *src, size_t sld,
                                           hipGetErrorString(e));
   }
   ring_t *r;
-  int rc = ring_get(dev, chunk, &r);
+  int rc = ring_get(dev, 1, chunk, &r);
   if (rc != GEMMX_OK) return rc;
   const size_t nchunks = (cols + cpc - 1) / cpc;
   /* the copies start after `after`, or else after the computation queued on
