@@ -329,3 +329,36 @@ def test_host_pointer_column_panels(cuda, chunk_mb, monkeypatch):
         oracle.gemm(ta, tb, m, len(cols), k, 0.5, A, ra + pad, Bs, Bs.shape[0], -1.5, R, m)
         bound = abs_bound(A, Bs, Cs, ta, tb, m, len(cols), k, 0.5, -1.5)
         assert max_comp(C[:m, cols], R, bound) <= _tol(dt, k), (dt, ta, tb)
+
+
+def test_host_pointer_column_panels_mixed_operands(cuda):
+    """Column panels with some operands already on the device: A on the device
+    with B and C host arrays (NT), and A host, B on the device, C host (NN,
+    only C moves enough to take panels).  Sampled columns at every panel edge
+    against the oracle; the device operands are left unchanged."""
+    import torch
+    rng = np.random.default_rng(43)
+    for (ta, tb, m, n, k, dev_a) in [("N", "T", 300, 2300, 4000, True),
+                                     ("N", "N", 2000, 4200, 300, False)]:
+        ra, ca = stored_shape(ta, m, k)
+        rb, cb = stored_shape(tb, k, n)
+        A = np.asfortranarray(rng.uniform(-1, 1, (ra, ca)))
+        B = np.asfortranarray(rng.uniform(-1, 1, (rb, cb)))
+        C = np.asfortranarray(rng.uniform(-1, 1, (m + 1, n)))
+        C[m:] = 321.0
+        C0 = C.copy(order="F")
+        a_arg = to_device(A, torch) if dev_a else A
+        b_arg = B if dev_a else to_device(B, torch)
+        gx.dgemm(ta, tb, m, n, k, 1.25, a_arg, ra, b_arg, rb, 0.5, C, m + 1)
+        torch.cuda.synchronize()
+        dev_op, host_op = (a_arg, A) if dev_a else (b_arg, B)
+        np.testing.assert_array_equal(from_device(dev_op, host_op.shape), host_op)
+        assert np.all(C[m:] == 321.0)
+        np_cols = -(-((n + 7) // 8) // 256) * 256
+        cols = sorted({c for j in range(0, n, np_cols) for c in (j - 1, j) if 0 <= c < n} | {n - 1})
+        Bs = np.asfortranarray(B[:k, cols] if tb == "N" else B[cols, :k])
+        Cs = np.asfortranarray(C0[:m, cols])
+        R = Cs.copy(order="F")
+        oracle.gemm(ta, tb, m, len(cols), k, 1.25, A, ra, Bs, Bs.shape[0], 0.5, R, m)
+        bound = abs_bound(A, Bs, Cs, ta, tb, m, len(cols), k, 1.25, 0.5)
+        assert max_comp(C[:m, cols], R, bound) <= _tol(np.float64, k), (ta, tb)
