@@ -29,7 +29,7 @@ __all__ = [
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "set_product_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
-    "BatchPlan", "set_batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2", "MODE_AUTO",
+    "BatchPlan", "set_batch_config", "batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2", "MODE_AUTO",
 ]
 
 LIB_DIR = os.environ.get("GEMMX_LIB_DIR") or os.path.join(
@@ -151,6 +151,9 @@ def _declare_batch(lib, dtype):
     f = getattr(lib, f"{dtype}gemm_x_batch_set_config")
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_int]
+    f = getattr(lib, f"{dtype}gemm_x_batch_config")
+    f.restype = ctypes.c_int
+    f.argtypes = []
 
 
 def load(name: str):
@@ -499,6 +502,13 @@ class BatchPlan:
 
 
 BATCH_CONFIGS = 4
+
+
+def batch_config(dtype: str) -> int:
+    """The batch kernel configuration a plan of `dtype` created now would use
+    (the one set, else GEMMX_BATCH_CFG, else the tuning table, else 0)."""
+    return int(getattr(load("DGEMM.so" if dtype == "d" else "SGEMM.so"),
+                       f"{dtype}gemm_x_batch_config")())
 
 
 def set_batch_config(dtype: str, cfg: int):

@@ -18,7 +18,9 @@ inputs (|difference| within 64 k u (|alpha| k + |beta|), the bound for
 U[-1,1) inputs).  For DGEMM shapes with min(m, n, k) >= 1024 it also picks
 the arithmetic of dgemm_'s automatic mode: Ozaki-II with the smallest
 modulus count whose error matches native FP64's on phi in {0, 1, 4} inputs,
-when that is faster than native (an `emu` entry).  With --resume, keys already
+when that is faster than native (an `emu` entry), and the grouped-batch
+kernel configuration per element type on a mixed batch (a `batch` entry).
+With --resume, keys already
 in the output table are kept and not measured again.  The table is written in
 sorted key order with the measured times as comments.
 """
@@ -63,6 +65,8 @@ _ENTRY = re.compile(r"^\s*(f64|f32)\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\
 _EMU = re.compile(r"^\s*emu\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(native|ozaki|ozaki2)"
                   r"\s+(\d+)\s*(#.*)?$")
 _EMU_RANGE = {"native": (0, 0), "ozaki": (3, 8), "ozaki2": (8, 22)}
+_BATCH = re.compile(r"^\s*batch\s+(f64|f32)\s+(\d+)\s*(#.*)?$")
+BATCH_CONFIGS = 4  # compiled grouped-batch kernel configurations (GEMMX_BATCH_CONFIGS)
 
 
 def emu_key(trans, m, n, k):
@@ -82,6 +86,13 @@ def read_table(path):
                 raise ValueError(f"{path}: table format {body[1]}, expected {FORMAT}")
             if len(body) == 2 and body[0] == "arch":
                 arch = body[1]
+                continue
+            m = _BATCH.match(line)
+            if m:
+                key = ("batch", m.group(1))
+                if 0 <= int(m.group(2)) < BATCH_CONFIGS:
+                    entries[key] = int(m.group(2))
+                    notes[key] = (m.group(3) or "").lstrip("# ").rstrip()
                 continue
             m = _EMU.match(line)
             if m:
@@ -113,14 +124,19 @@ def write_table(path, entries, notes, arch, header=()):
     if arch:
         lines += [f"arch {arch}"]
     lines += ["# dtype trans mbucket nbucket kbucket cfg",
-              "# emu trans mbucket nbucket kbucket scheme count   (DGEMM arithmetic, auto mode)"]
+              "# emu trans mbucket nbucket kbucket scheme count   (DGEMM arithmetic, auto mode)",
+              "# batch dtype cfg   (grouped-batch kernel)"]
     order = {"f64": 0, "f32": 1, "emu": 2}
-    for key in sorted(entries, key=lambda k: (order[k[0]], TRANS.index(k[1]), k[2], k[3], k[4])):
+    tiles = [k for k in entries if k[0] != "batch"]
+    for key in sorted(tiles, key=lambda k: (order[k[0]], TRANS.index(k[1]), k[2], k[3], k[4])):
         note = notes.get(key, "")
         val = entries[key]
         val = f"{val[0]} {val[1]}" if key[0] == "emu" else str(val)
         lines.append(f"{key[0]} {key[1]} {key[2]} {key[3]} {key[4]} {val}"
                      + (f"  # {note}" if note else ""))
+    for key in sorted(k for k in entries if k[0] == "batch"):
+        note = notes.get(key, "")
+        lines.append(f"batch {key[1]} {entries[key]}" + (f"  # {note}" if note else ""))
     tmp = path + ".tmp"
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(tmp, "w") as f:
@@ -193,6 +209,74 @@ def measure(torch, gx, dtype, trans, m, n, k, reps):
             times.append(_median(ev))
     finally:
         gx.set_tile_config(dtype, -1)
+    best = min((t, c) for c, t in enumerate(times) if ok[c])[1]
+    return times, ok, best
+
+
+# ------------------------------------------------------------------ grouped batch
+BATCH_MIX = (4096, 16, 1024)  # problems; m, n, k log-uniform integers in [lo, hi]
+
+
+def measure_batch(torch, gx, dtype, reps, mix=BATCH_MIX, seed=11):
+    """Median event time (ms) of a grouped-batch plan per kernel
+    configuration on a mixed batch (m, n, k log-uniform, transposes uniform
+    over {N, T}, alpha ~ U[0.5, 2], beta ~ U[-1, 1], ld = stored rows), and
+    the fastest configuration whose C agrees with configuration 0's (each
+    result from the same C, within 64 k u (|alpha| |A||B| + |beta C|) for
+    U[-1,1) operands)."""
+    import math
+    import numpy as np
+    count, lo, hi = mix
+    rng = np.random.default_rng(seed)
+    tdt = torch.float64 if dtype == "d" else torch.float32
+    u = 2.0 ** -53 if dtype == "d" else 2.0 ** -24
+    shapes, offs = [], [0, 0, 0]
+    for _ in range(count):
+        m, n, k = (int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+                   for _ in range(3))
+        ta, tb = "NT"[rng.integers(2)], "NT"[rng.integers(2)]
+        shapes.append((ta, tb, m, n, k, float(rng.uniform(0.5, 2.0)),
+                       float(rng.uniform(-1.0, 1.0)), tuple(offs)))
+        for j, size in enumerate((m * k, k * n, m * n)):
+            offs[j] += (size + 1) // 2 * 2  # 16-byte aligned operands
+    bufs = [torch.empty(max(o, 1), dtype=tdt, device="cuda") for o in offs]
+    for j, b in enumerate(bufs):
+        gx.fill(b, b.numel(), 1, b.numel(), seed, j)
+    c0 = bufs[2].clone()
+    es = bufs[0].element_size()
+    specs = [(ta, tb, m, n, k, alpha, bufs[0].data_ptr() + oa * es, m if ta == "N" else k,
+              bufs[1].data_ptr() + ob * es, k if tb == "N" else n, beta,
+              bufs[2].data_ptr() + oc * es, m)
+             for ta, tb, m, n, k, alpha, beta, (oa, ob, oc) in shapes]
+    kmax = max(s[4] for s in specs)
+    bound = 64 * kmax * u * (2.0 * kmax + 1.0)
+    stream = torch.cuda.current_stream()
+    times, ok, ref = [], [], None
+    try:
+        for cfg in range(BATCH_CONFIGS):
+            gx.set_batch_config(dtype, cfg)
+            plan = gx.BatchPlan(dtype, specs)
+            bufs[2].copy_(c0)
+            plan.run(stream)
+            torch.cuda.synchronize()
+            if ref is None:
+                ref = bufs[2].clone()
+                ok.append(True)
+            else:
+                ok.append(bool((bufs[2] - ref).abs().max().item() <= bound))
+            ev = []
+            for _ in range(reps):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                plan.run(stream)
+                e1.record(stream)
+                e1.synchronize()
+                ev.append(e0.elapsed_time(e1))
+            times.append(_median(ev))
+            plan.destroy()
+    finally:
+        gx.set_batch_config(dtype, -1)
     best = min((t, c) for c, t in enumerate(times) if ok[c])[1]
     return times, ok, best
 
@@ -308,6 +392,9 @@ def main(argv=None):
     ap.add_argument("--emulated", choices=["on", "off"], default="on",
                     help="also choose the DGEMM arithmetic (native or Ozaki-II) per bucket for "
                          "shapes with min(m, n, k) >= 1024")
+    ap.add_argument("--batch", choices=["on", "off"], default="on",
+                    help="also choose the grouped-batch kernel configuration per element type "
+                         "on a mix of %d problems with m, n, k log-uniform in [%d, %d]" % BATCH_MIX)
     args = ap.parse_args(argv)
 
     import torch
@@ -357,6 +444,18 @@ def main(argv=None):
             notes[key] = note
             write_table(args.out, entries, notes, arch, header)
             print(f"[emu {i + 1}/{len(etodo)}] {t} {m}x{n}x{k}: {scheme} {cnt} ({note})",
+                  flush=True)
+    if args.batch == "on":
+        for d in args.dtypes.split(","):
+            key = ("batch", "f64" if d == "d" else "f32")
+            if args.resume and key in entries:
+                continue
+            times, ok, best = measure_batch(torch, gx, d, args.reps)
+            entries[key] = best
+            notes[key] = ("mix of %d problems, m, n, k in [%d, %d] ms: " % BATCH_MIX) + " ".join(
+                f"c{c}={tm:.3f}" + ("" if ok[c] else "!") for c, tm in enumerate(times))
+            write_table(args.out, entries, notes, arch, header)
+            print(f"[batch] {key[1]}: cfg {best} ({times[best]:.3f} ms; cfg0 {times[0]:.3f} ms)",
                   flush=True)
     print(f"autotune: {len(entries)} entries in {args.out}")
     return 0

@@ -141,6 +141,7 @@ static int current_device(int *dev) {
 }
 
 int gemmx_current_device(int *dev) { return current_device(dev); }
+void gemmx_init(void) { pthread_once(&g_once, init_once); }
 
 /* Grows the device's scratch to `bytes`; caller holds g_dev[dev].lock and
  * has no work in flight on the old scratch. */

@@ -16,6 +16,7 @@
 
 #include "../../include/gemm_mi355x.h"
 #include "gemmx_internal.h"
+#include "gemmx_tune.h"
 
 #define EXPORT __attribute__((visibility("default")))
 
@@ -54,8 +55,10 @@ static void destroy(struct gemmx_batch *p) {
 
 static int notrans(char c) { return c == 'N' || c == 'n'; }
 
-/* Batch kernel configuration per element size (index 0: FP64, 1: FP32);
- * -1 = GEMMX_BATCH_CFG from the environment, else the default. */
+/* Batch kernel configuration per element size (index 0: FP64, 1: FP32): the
+ * one set by *_x_batch_set_config, else GEMMX_BATCH_CFG from the
+ * environment, else the tuning table's `batch` entry (measured by the
+ * autotuner), else the default. */
 static int g_batch_cfg[2] = {-1, -1};
 static const int k_batch_default[2] = {0, 0};
 
@@ -67,7 +70,8 @@ static int batch_cfg(int es) {
     const int c = atoi(e);
     if (c >= 0 && c < GEMMX_BATCH_CONFIGS) return c;
   }
-  return k_batch_default[i];
+  const int t = gemmx_tune_pick_batch(es);
+  return t >= 0 ? t : k_batch_default[i];
 }
 
 static int set_batch_cfg(int es, int cfg) {
@@ -279,6 +283,15 @@ EXPORT double sgemm_x_batch_flops(const void *plan) {
 
 EXPORT int dgemm_x_batch_set_config(int cfg) { return set_batch_cfg(8, cfg); }
 EXPORT int sgemm_x_batch_set_config(int cfg) { return set_batch_cfg(4, cfg); }
+/* The configuration a plan created now would use (no device needed). */
+EXPORT int dgemm_x_batch_config(void) {
+  gemmx_init();
+  return batch_cfg(8);
+}
+EXPORT int sgemm_x_batch_config(void) {
+  gemmx_init();
+  return batch_cfg(4);
+}
 
 EXPORT int dgemm_x_batch_destroy(void *plan) {
   destroy((struct gemmx_batch *)plan);

@@ -89,6 +89,8 @@ int gemmx_report(int code, const char *fmt, ...);
 void gemmx_clear_error(void);
 int gemmx_last_code(void);
 int gemmx_current_device(int *dev);
+/* One-time library set-up (modes from the environment, tuning table). */
+void gemmx_init(void);
 
 /* Staging of host operands through pinned buffers (gemmx_stage.c): cols
  * columns of row_bytes bytes, leading dimensions in bytes.  The copies start

@@ -22,6 +22,8 @@
 static signed char g_table[2][4][NB][NB][NB]; /* [f64/f32][trans][m][n][k] */
 /* DGEMM arithmetic per bucket: 0 = no entry, else (mode + 1) << 8 | count */
 static short g_emu[4][NB][NB][NB];
+/* grouped-batch kernel configuration per element size: 0 = no entry, else cfg + 1 */
+static signed char g_batch[2];
 static int g_entries;
 static int g_force[2] = {-1, -1}; /* [f64/f32]: one configuration for every call */
 static char g_path[1024];
@@ -49,6 +51,7 @@ static int trans_index(const char *t) {
 static void clear_table(void) {
   memset(g_table, 0, sizeof g_table);
   memset(g_emu, 0, sizeof g_emu);
+  memset(g_batch, 0, sizeof g_batch);
   g_entries = 0;
   g_path[0] = 0;
 }
@@ -59,6 +62,7 @@ static int load_table(const char *path) {
   if (!f) return -1;
   static signed char staged[2][4][NB][NB][NB];
   static short staged_emu[4][NB][NB][NB];
+  signed char staged_batch[2] = {0, 0};
   memset(staged, 0, sizeof staged);
   memset(staged_emu, 0, sizeof staged_emu);
   char line[256];
@@ -80,6 +84,17 @@ static int load_table(const char *path) {
     }
     char dt[8], tr[8], sc[16];
     int mb, nb, kb, cfg;
+    if (sscanf(line, "batch %7s %d", dt, &cfg) == 2) {
+      if (format != GEMMX_TUNE_FORMAT) {
+        ok = 0;
+        break;
+      }
+      const int d = strcmp(dt, "f64") == 0 ? 0 : strcmp(dt, "f32") == 0 ? 1 : -1;
+      if (d < 0 || cfg < 0 || cfg >= GEMMX_BATCH_CONFIGS) continue;
+      staged_batch[d] = (signed char)(cfg + 1);
+      entries++;
+      continue;
+    }
     if (sscanf(line, "emu %7s %d %d %d %15s %d", tr, &mb, &nb, &kb, sc, &cfg) == 6) {
       if (format != GEMMX_TUNE_FORMAT) {
         ok = 0;
@@ -115,6 +130,7 @@ static int load_table(const char *path) {
   if (!ok || format != GEMMX_TUNE_FORMAT) return -1;
   memcpy(g_table, staged, sizeof g_table);
   memcpy(g_emu, staged_emu, sizeof g_emu);
+  memcpy(g_batch, staged_batch, sizeof g_batch);
   g_entries = entries;
   snprintf(g_path, sizeof g_path, "%s", path);
   return entries;
@@ -189,3 +205,5 @@ int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {
   const long tiles128 = (long)((m + 127) / 128) * ((n + 127) / 128);
   return tiles128 >= 512 ? GEMMX_NCFG_128x128 : GEMMX_NCFG_64x64;
 }
+
+int gemmx_tune_pick_batch(int es) { return g_batch[es == 8 ? 0 : 1] - 1; }

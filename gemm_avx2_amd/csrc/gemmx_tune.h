@@ -16,7 +16,9 @@
  * where a bucket is floor(log2(dim)) clamped to 4..15 and cfg is a
  * configuration id, or, for the arithmetic of DGEMM in the automatic mode,
  *     emu <trans> <mbucket> <nbucket> <kbucket> <native|ozaki|ozaki2> <count>
- * (count: slices 3..8 for ozaki, moduli 8..22 for ozaki2, 0 for native).
+ * (count: slices 3..8 for ozaki, moduli 8..22 for ozaki2, 0 for native), or,
+ * for the grouped-batch kernel of one element type,
+ *     batch <dtype:f64|f32> <cfg>
  * A later entry for the same key replaces an earlier one.  Missing entries
  * fall back to a size heuristic (tile) or native arithmetic.
  */
@@ -37,6 +39,9 @@ int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k);
  * _OZAKI2, with the slice or modulus count in *count. */
 int gemmx_tune_pick_emulated(int ta, int tb, int m, int n, int k, int *count);
 int gemmx_tune_bucket(int dim);
+/* Grouped-batch kernel configuration of the table for element size es (8 or
+ * 4), or -1 when it has none. */
+int gemmx_tune_pick_batch(int es);
 /* Number of entries loaded from the table file (0 when none). */
 int gemmx_tune_entries(void);
 /* Forces one configuration for every call of the given element size (8 or

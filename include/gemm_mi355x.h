@@ -306,10 +306,15 @@ double sgemm_x_batch_flops(const void *plan);
 int sgemm_x_batch_destroy(void *plan);
 /* Tile configuration of the batch kernel for plans created afterwards
  * (0 64x64x16, 1 64x64x32, 2 128x64x16, 3 32x32x32; -1 = the environment's
- * GEMMX_BATCH_CFG, else the default).  No reference counterpart. */
+ * GEMMX_BATCH_CFG, else the tuning table's `batch` entry written by the
+ * autotuner, else the default).  *_x_batch_config returns the configuration
+ * a plan created now would use.  No reference counterpart (the reference's
+ * blocking is fixed per build, dgemm_tune.h / sgemm_tune.h). */
 #define GEMMX_BATCH_CONFIGS 4
 int dgemm_x_batch_set_config(int cfg);
 int sgemm_x_batch_set_config(int cfg);
+int dgemm_x_batch_config(void);
+int sgemm_x_batch_config(void);
 
 #ifdef __cplusplus
 }

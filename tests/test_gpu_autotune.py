@@ -17,12 +17,14 @@ def test_autotune_table_is_used_and_correct(cuda, tmp_path, capsys):
     import torch
     out = str(tmp_path / "tune.txt")
     shapes = "256,256,256;640,200,300"
-    assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--reps", "2"]) == 0
+    assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--reps", "2",
+                    "--batch", "off"]) == 0
     entries, notes, arch = at.read_table(out)
     assert len(entries) == 8 and arch.startswith("gfx")
     capsys.readouterr()
     # resumed: nothing left to measure, same entries
-    assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--resume"]) == 0
+    assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--resume",
+                    "--batch", "off"]) == 0
     assert "[1/" not in capsys.readouterr().out
     assert at.read_table(out)[0] == entries
     rng = np.random.default_rng(5)
@@ -89,3 +91,17 @@ def test_autotune_emulated_choice_drives_auto_mode(cuda, tmp_path):
     finally:
         gx.set_dgemm_mode(gx.MODE_NATIVE)
         gx.load_tuning(at.DEFAULT_OUT if os.path.exists(at.DEFAULT_OUT) else None, "d")
+
+
+@pytest.mark.parametrize("d", ["d", "s"])
+def test_autotune_batch_measurement(cuda, d):
+    """The grouped-batch measurement times every kernel configuration on a
+    mixed batch (here 96 problems up to 256), checks each configuration's C
+    against configuration 0's and returns the fastest agreeing one; the
+    forced configuration is released afterwards."""
+    import torch
+    before = gx.batch_config(d)
+    times, ok, best = at.measure_batch(torch, gx, d, 2, mix=(96, 16, 256))
+    assert len(times) == at.BATCH_CONFIGS and all(ok) and 0 <= best < at.BATCH_CONFIGS
+    assert times[best] == min(times)
+    assert gx.batch_config(d) == before  # no forced value left behind

@@ -156,3 +156,42 @@ def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
     assert gx.load_tuning(str(bad), "d") == 1
     assert gx.tune_lookup_emulated("N", "N", 8192, 8192, 8192) == (gx.MODE_OZAKI2, 20)
     assert at.read_table(str(bad))[0] == {("emu", "NN", 13, 13, 13): ("ozaki2", 20)}
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_batch_entry_drives_the_batch_configuration(tmp_path, dtype, restore_tables,
+                                                    monkeypatch):
+    """A `batch <dtype> <cfg>` entry (written and read by the autotuner, read
+    by the libraries) gives the grouped-batch kernel configuration; a forced
+    configuration and GEMMX_BATCH_CFG take precedence; without an entry the
+    default (0) is used."""
+    monkeypatch.delenv("GEMMX_BATCH_CFG", raising=False)
+    tag = "f64" if dtype == "d" else "f32"
+    e = dict(_entries())
+    e[("batch", tag)] = 2
+    e[("batch", "f32" if dtype == "d" else "f64")] = 3
+    path = str(tmp_path / "t.txt")
+    at.write_table(path, e, {("batch", tag): "mix"}, None)
+    got, notes, _ = at.read_table(path)
+    assert got == e and notes[("batch", tag)] == "mix"
+    try:
+        assert gx.load_tuning(path, dtype) == len(e)
+        assert gx.batch_config(dtype) == 2
+        gx.set_batch_config(dtype, 1)
+        assert gx.batch_config(dtype) == 1
+        gx.set_batch_config(dtype, -1)
+        monkeypatch.setenv("GEMMX_BATCH_CFG", "3")
+        assert gx.batch_config(dtype) == 3
+        monkeypatch.delenv("GEMMX_BATCH_CFG")
+        assert gx.batch_config(dtype) == 2
+        at.write_table(path, _entries(), {}, None)
+        gx.load_tuning(path, dtype)
+        assert gx.batch_config(dtype) == 0
+        # out-of-range configurations are skipped by both readers
+        with open(path, "a") as f:
+            f.write(f"batch {tag} {at.BATCH_CONFIGS}\n")
+        assert ("batch", tag) not in at.read_table(path)[0]
+        gx.load_tuning(path, dtype)
+        assert gx.batch_config(dtype) == 0
+    finally:
+        gx.set_batch_config(dtype, -1)
