@@ -362,3 +362,54 @@ def test_host_pointer_column_panels_mixed_operands(cuda):
         oracle.gemm(ta, tb, m, len(cols), k, 1.25, A, ra, Bs, Bs.shape[0], 0.5, R, m)
         bound = abs_bound(A, Bs, Cs, ta, tb, m, len(cols), k, 1.25, 0.5)
         assert max_comp(C[:m, cols], R, bound) <= _tol(np.float64, k), (ta, tb)
+
+
+def test_concurrent_calls_from_host_threads(cuda):
+    """dgemm_/sgemm_ from four host threads at once (ctypes releases the GIL):
+    column-panel host calls, small host calls and device-pointer calls
+    interleave on one device; every result matches the oracle (sampled
+    columns for the large ones) and no call reports an error."""
+    import threading
+    import torch
+    rng = np.random.default_rng(47)
+    jobs = []
+    for t in range(4):
+        for (dt, m, n, k) in [(np.float64, 300, 2300, 4000), (np.float32, 130, 70, 90),
+                              (np.float64, 97, 65, 130)]:
+            A = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(dt))
+            B = np.asfortranarray(rng.uniform(-1, 1, (k, n)).astype(dt))
+            C = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(dt))
+            jobs.append((t, dt, m, n, k, A, B, C, C.copy(order="F"), (m * n) % 2 == 0))
+    errors = []
+
+    def worker(tid):
+        try:
+            for (t, dt, m, n, k, A, B, C, C0, on_dev) in jobs:
+                if t != tid:
+                    continue
+                if on_dev and m * n < 10**5:
+                    dA, dB, dC = to_device(A, torch), to_device(B, torch), to_device(C0, torch)
+                    _gemm_host(dt, "N", "N", m, n, k, 0.75, dA, m, dB, k, 0.5, dC, m)
+                    torch.cuda.synchronize()
+                    C[:] = from_device(dC, C.shape)
+                else:
+                    _gemm_host(dt, "N", "N", m, n, k, 0.75, A, m, B, k, 0.5, C, m)
+                code, msg = gx.last_error("d" if dt == np.float64 else "s")
+                if code != gx.OK:
+                    errors.append(msg)
+        except Exception as exc:  # reported below, in the main thread
+            errors.append(repr(exc))
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not any(th.is_alive() for th in threads) and not errors, errors
+    for (t, dt, m, n, k, A, B, C, C0, on_dev) in jobs:
+        cols = sorted(set(rng.integers(0, n, 5).tolist()) | {0, n - 1})
+        Bs, Cs = np.asfortranarray(B[:, cols]), np.asfortranarray(C0[:, cols])
+        R = Cs.copy(order="F")
+        oracle.gemm("N", "N", m, len(cols), k, 0.75, A, m, Bs, k, 0.5, R, m)
+        bound = abs_bound(A, Bs, Cs, "N", "N", m, len(cols), k, 0.75, 0.5)
+        assert max_comp(C[:, cols], R, bound) <= _tol(dt, k), (t, dt, m, n, k)
