@@ -200,28 +200,40 @@ oz2_shift_from_sums_kernel(const unsigned long long *__restrict__ sums, int nvec
   if (v < nvec) ex[v] = oz2_shift(L, ex[v], sums[v]);
 }
 
-/* 16 consecutive K entries of one vector per thread -> 16 bytes per slice.
- * v_fast: consecutive threads take consecutive vectors (strided input). */
-template <int S>
+/* 4 NW consecutive K entries of one vector per thread -> 4 NW bytes per
+ * slice (vectors contiguous: x[v*vld + p]); a thread whose entries are all in
+ * range and 16-byte aligned reads them with 16-byte loads.  Eight entries
+ * (NW = 2) keep many waves per SIMD and the loads wide. */
+template <int S, int NW>
 __global__ void __launch_bounds__(256)
-ozaki_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld,
-                   int nvec, int len, int64_t kpad, const int *__restrict__ ex,
-                   int v_fast, int8_t *__restrict__ out) {
-  const int64_t nblk = kpad / 16;
+ozaki_split_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len, int64_t kpad,
+                   const int *__restrict__ ex, int8_t *__restrict__ out) {
+  constexpr int EPT = 4 * NW;
+  typedef int vec_t __attribute__((ext_vector_type(NW)));
+  const int64_t nblk = kpad / EPT;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (tid >= nblk * nvec) return;
-  int64_t v, blk;
-  if (v_fast) { v = tid % nvec; blk = tid / nvec; }
-  else        { v = tid / nblk; blk = tid % nblk; }
+  const int64_t v = tid / nblk, blk = tid % nblk;
   const int e = ex[v];
-  i4_t dig[S];
+  const double *xp = x + v * vld + blk * EPT;
+  double xv[EPT];
+  if (blk * EPT + EPT <= len && (reinterpret_cast<uintptr_t>(xp) & 15) == 0) {
 #pragma unroll
-  for (int t = 0; t < S; ++t) dig[t] = i4_t{0, 0, 0, 0};
+    for (int i = 0; i < EPT / 2; ++i) {
+      const d2_t t = reinterpret_cast<const d2_t *>(xp)[i];
+      xv[2 * i] = t[0];
+      xv[2 * i + 1] = t[1];
+    }
+  } else {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int64_t p = blk * 16 + q;
-    const double val = (p < len) ? x[v * vld + p * eld] : 0.0;
-    double r = (e == OZ_NONFINITE) ? 0.0 : ldexp(val, -e);
+    for (int i = 0; i < EPT; ++i) xv[i] = blk * EPT + i < len ? xp[i] : 0.0;
+  }
+  vec_t dig[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t) dig[t] = vec_t{};
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    double r = (e == OZ_NONFINITE) ? 0.0 : ldexp(xv[q], -e);
 #pragma unroll
     for (int t = 0; t < S; ++t) {
       const double y = r * 128.0;
@@ -231,9 +243,10 @@ ozaki_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld,
       dig[t][q >> 2] |= (int)(byte << (8 * (q & 3)));
     }
   }
+  int8_t *o = out + v * kpad + blk * EPT;
+  const int64_t plane = (int64_t)nvec * kpad;
 #pragma unroll
-  for (int t = 0; t < S; ++t)
-    *reinterpret_cast<i4_t *>(out + ((int64_t)t * nvec + v) * kpad + blk * 16) = dig[t];
+  for (int t = 0; t < S; ++t, o += plane) *reinterpret_cast<vec_t *>(o) = dig[t];
 }
 
 /* Same digits for vectors adjacent in memory (x[v + p*eld]), through an LDS
@@ -372,26 +385,22 @@ __device__ __forceinline__ void oz2_encode4(const Oz2Val (&a)[4], int nmod,
 }
 
 /* nmod residue planes of 4 NW consecutive K entries of one vector per thread
- * (as ozaki_split_kernel).  With eld = 1 a thread whose entries are all in
- * range and 16-byte aligned reads them with 16-byte loads. */
+ * (vectors contiguous: x[v*vld + p]); a thread whose entries are all in range
+ * and 16-byte aligned reads them with 16-byte loads. */
 template <int NW>
 __global__ void __launch_bounds__(256)
-ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int nvec,
-                    int len, int64_t kpad, const int *__restrict__ ex, int v_fast,
-                    int nmod, int8_t *__restrict__ out) {
+ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len, int64_t kpad,
+                    const int *__restrict__ ex, int nmod, int8_t *__restrict__ out) {
   constexpr int EPT = 4 * NW;
   typedef int vec_t __attribute__((ext_vector_type(NW)));
   const int64_t nblk = kpad / EPT;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (tid >= nblk * nvec) return;
-  int64_t v, blk;
-  if (v_fast) { v = tid % nvec; blk = tid / nvec; }
-  else        { v = tid / nblk; blk = tid % nblk; }
+  const int64_t v = tid / nblk, blk = tid % nblk;
   const int d = ex[v]; /* shift P - e */
-  const double *xp = x + v * vld + blk * EPT * eld;
+  const double *xp = x + v * vld + blk * EPT;
   double xv[EPT];
-  if (eld == 1 && blk * EPT + EPT <= len && d != OZ_NONFINITE &&
-      (reinterpret_cast<uintptr_t>(xp) & 15) == 0) {
+  if (blk * EPT + EPT <= len && d != OZ_NONFINITE && (reinterpret_cast<uintptr_t>(xp) & 15) == 0) {
 #pragma unroll
     for (int i = 0; i < EPT / 2; ++i) {
       const d2_t t = reinterpret_cast<const d2_t *>(xp)[i];
@@ -401,7 +410,7 @@ ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int64_t eld, int 
   } else {
 #pragma unroll
     for (int i = 0; i < EPT; ++i)
-      xv[i] = (blk * EPT + i < len && d != OZ_NONFINITE) ? xp[i * eld] : 0.0;
+      xv[i] = (blk * EPT + i < len && d != OZ_NONFINITE) ? xp[i] : 0.0;
   }
   vec_t res[OZ2_MAX_MODULI];
 #pragma unroll
@@ -499,26 +508,37 @@ __device__ __forceinline__ unsigned bf16_rne_bits(float x) {
   return u >> 16;
 }
 
-/* 8 consecutive K entries of one vector per thread -> 16 bytes of hi and of
- * lo.  out is [2][nvec][kpad] bf16 (hi slice first). */
+/* 8 consecutive K entries of one vector per thread (vectors contiguous:
+ * x[v*vld + p]) -> 16 bytes of hi and of lo.  out is [2][nvec][kpad] bf16 (hi
+ * slice first). */
 __global__ void __launch_bounds__(256)
-bf16_split_kernel(const float *__restrict__ x, int64_t vld, int64_t eld,
-                  int nvec, int len, int64_t kpad, int v_fast,
+bf16_split_kernel(const float *__restrict__ x, int64_t vld, int nvec, int len, int64_t kpad,
                   unsigned short *__restrict__ out) {
   const int64_t nblk = kpad / 8;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (tid >= nblk * nvec) return;
-  int64_t v, blk;
-  if (v_fast) { v = tid % nvec; blk = tid / nvec; }
-  else        { v = tid / nblk; blk = tid % nblk; }
+  const int64_t v = tid / nblk, blk = tid % nblk;
+  float xv[8];
+  const float *xp = x + v * vld + blk * 8;
+  if (blk * 8 + 8 <= len && (reinterpret_cast<uintptr_t>(xp) & 15) == 0) {
+    /* in range and aligned: two 16-byte loads */
+    const f4_t a = reinterpret_cast<const f4_t *>(xp)[0], b = reinterpret_cast<const f4_t *>(xp)[1];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      xv[i] = a[i];
+      xv[4 + i] = b[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) xv[i] = blk * 8 + i < len ? xp[i] : 0.0f;
+  }
   i4_t hi, lo;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     unsigned h2[2], l2[2];
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      const int64_t p = blk * 8 + 2 * q + e;
-      const float val = (p < len) ? x[v * vld + p * eld] : 0.0f;
+      const float val = xv[2 * q + e];
       h2[e] = bf16_rne_bits(val);
       l2[e] = bf16_rne_bits(val - __uint_as_float(h2[e] << 16));
     }
@@ -529,13 +549,55 @@ bf16_split_kernel(const float *__restrict__ x, int64_t vld, int64_t eld,
   *reinterpret_cast<i4_t *>(out + ((int64_t)nvec + v) * kpad + blk * 8) = lo;
 }
 
+/* The same for vectors adjacent in memory (x[v + p*eld]) through an LDS tile
+ * of 32 vectors x 64 K entries: the loads read 32 adjacent vectors per entry,
+ * and the 8 threads of a vector (thread q taking entries 8q .. 8q + 7, kept in
+ * LDS column (p % 8) * 8 + p / 8 of the vector's padded row) write 128
+ * contiguous bytes of hi and of lo. */
+__global__ void __launch_bounds__(256)
+bf16_split_strided_kernel(const float *__restrict__ x, int64_t eld, int nvec, int len,
+                          int64_t kpad, unsigned short *__restrict__ out) {
+  constexpr int ROW = 65;
+  __shared__ float tile[32 * ROW];
+  const int tid = threadIdx.x;
+  const int v0 = blockIdx.x * 32;
+  const int64_t p0 = (int64_t)blockIdx.y * 64;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int idx = tid + 256 * j, vl = idx & 31, pr = idx >> 5;
+    const int v = v0 + vl;
+    const int64_t p = p0 + pr;
+    const float val = (v < nvec && p < len) ? x[v + p * eld] : 0.0f;
+    tile[vl * ROW + (pr % 8) * 8 + pr / 8] = val;
+  }
+  __syncthreads();
+  const int vl = tid >> 3, q = tid & 7;
+  const int v = v0 + vl;
+  if (v >= nvec) return;
+  i4_t hi, lo;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    unsigned h2[2], l2[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const float val = tile[vl * ROW + (2 * w + e) * 8 + q];
+      h2[e] = bf16_rne_bits(val);
+      l2[e] = bf16_rne_bits(val - __uint_as_float(h2[e] << 16));
+    }
+    hi[w] = (int)(h2[0] | (h2[1] << 16));
+    lo[w] = (int)(l2[0] | (l2[1] << 16));
+  }
+  *reinterpret_cast<i4_t *>(out + (int64_t)v * kpad + p0 + 8 * q) = hi;
+  *reinterpret_cast<i4_t *>(out + ((int64_t)nvec + v) * kpad + p0 + 8 * q) = lo;
+}
+
 template <int S>
-hipError_t launch_ozaki_split(const double *x, int64_t vld, int64_t eld, int nvec,
-                              int len, int64_t kpad, const int *ex, int v_fast,
-                              int8_t *out, hipStream_t st) {
-  const int64_t threads = (kpad / 16) * nvec;
-  hipLaunchKernelGGL((ozaki_split_kernel<S>), dim3((unsigned)((threads + 255) / 256)),
-                     dim3(256), 0, st, x, vld, eld, nvec, len, kpad, ex, v_fast, out);
+hipError_t launch_ozaki_split(const double *x, int64_t vld, int nvec, int len, int64_t kpad,
+                              const int *ex, int8_t *out, hipStream_t st) {
+  constexpr int NW = 2;
+  const int64_t threads = (kpad / (4 * NW)) * nvec;
+  hipLaunchKernelGGL((ozaki_split_kernel<S, NW>), dim3((unsigned)((threads + 255) / 256)),
+                     dim3(256), 0, st, x, vld, nvec, len, kpad, ex, out);
   return hipGetLastError();
 }
 
@@ -559,16 +621,16 @@ hipError_t ozaki_split_strided(int s, const double *x, int64_t eld, int nvec, in
   }
 }
 
-hipError_t ozaki_split(int s, const double *x, int64_t vld, int64_t eld, int nvec,
-                       int len, int64_t kpad, const int *ex, int v_fast, int8_t *out,
-                       hipStream_t st) {
+/* Contiguous vectors (x[v*vld + p]). */
+hipError_t ozaki_split(int s, const double *x, int64_t vld, int nvec, int len, int64_t kpad,
+                       const int *ex, int8_t *out, hipStream_t st) {
   switch (s) {
-  case 3: return launch_ozaki_split<3>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
-  case 4: return launch_ozaki_split<4>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
-  case 5: return launch_ozaki_split<5>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
-  case 6: return launch_ozaki_split<6>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
-  case 7: return launch_ozaki_split<7>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
-  default: return launch_ozaki_split<8>(x, vld, eld, nvec, len, kpad, ex, v_fast, out, st);
+  case 3: return launch_ozaki_split<3>(x, vld, nvec, len, kpad, ex, out, st);
+  case 4: return launch_ozaki_split<4>(x, vld, nvec, len, kpad, ex, out, st);
+  case 5: return launch_ozaki_split<5>(x, vld, nvec, len, kpad, ex, out, st);
+  case 6: return launch_ozaki_split<6>(x, vld, nvec, len, kpad, ex, out, st);
+  case 7: return launch_ozaki_split<7>(x, vld, nvec, len, kpad, ex, out, st);
+  default: return launch_ozaki_split<8>(x, vld, nvec, len, kpad, ex, out, st);
   }
 }
 
@@ -586,7 +648,7 @@ hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
                        ld, nvec, k, ex);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return ozaki_split(s, x, ld, 1, nvec, k, kpad, ex, 0, slices, st);
+    return ozaki_split(s, x, ld, nvec, k, kpad, ex, slices, st);
   }
   hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, mbits,
                      nvec);
@@ -610,7 +672,7 @@ hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec
                        L, ex);
     const int64_t threads = (kpad / (4 * NW)) * nvec;
     hipLaunchKernelGGL(ozaki2_split_kernel<NW>, dim3((unsigned)((threads + 255) / 256)),
-                       dim3(256), 0, st, x, ld, (int64_t)1, nvec, k, kpad, ex, 0, nmod, res);
+                       dim3(256), 0, st, x, ld, nvec, k, kpad, ex, nmod, res);
     return hipGetLastError();
   }
   const dim3 red((nvec + 63) / 64, (k + 255) / 256);
@@ -634,11 +696,10 @@ hipError_t bf16_prepare(const float *x, int64_t ld, bool contiguous, int nvec,
   const int64_t threads = (kpad / 8) * nvec;
   const dim3 grid((unsigned)((threads + 255) / 256));
   if (contiguous)
-    hipLaunchKernelGGL(bf16_split_kernel, grid, dim3(256), 0, st, x, ld, (int64_t)1,
-                       nvec, k, kpad, 0, out);
-  else
-    hipLaunchKernelGGL(bf16_split_kernel, grid, dim3(256), 0, st, x, (int64_t)1, ld,
-                       nvec, k, kpad, 1, out);
+    hipLaunchKernelGGL(bf16_split_kernel, grid, dim3(256), 0, st, x, ld, nvec, k, kpad, out);
+  else /* kpad is a multiple of 64 */
+    hipLaunchKernelGGL(bf16_split_strided_kernel, dim3((nvec + 31) / 32, (unsigned)(kpad / 64)),
+                       dim3(256), 0, st, x, ld, nvec, k, kpad, out);
   return hipGetLastError();
 }
 
