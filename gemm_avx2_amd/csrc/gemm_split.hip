@@ -433,6 +433,98 @@ ozaki2_split_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len
     if (j < nmod) *reinterpret_cast<vec_t *>(o) = res[j];
 }
 
+/* Bound pass and residue split in one kernel for contiguous vectors of at
+ * most 256 * 8 * OZ2_FUSED_CHUNKS entries: a block of 256 threads per vector
+ * reads it once, as chunks of 8 entries (chunk c = thread + 256 i, 16-byte
+ * loads when aligned), derives the shift D = P - e exactly as
+ * oz2_shift_contig_kernel does, and writes every chunk's residue words
+ * (8 bytes per plane) from the registers, so the operand is read from HBM
+ * once instead of twice. */
+constexpr int OZ2_FUSED_CHUNKS = 4;
+
+__global__ void __launch_bounds__(256)
+ozaki2_shift_split_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len,
+                                 int64_t kpad, int L, int nmod, int *__restrict__ ex,
+                                 int8_t *__restrict__ out) {
+  constexpr int R = OZ2_FUSED_CHUNKS, EPT = 8;
+  typedef int vec_t __attribute__((ext_vector_type(2)));
+  __shared__ unsigned long long red[4];
+  __shared__ int eshared, dshared;
+  const int v = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double *p = x + (int64_t)v * vld;
+  const int nchunks = (int)(kpad / EPT);
+  double xv[R][EPT];
+  unsigned long long mx = 0;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int c = tid + 256 * i;
+    const double *cp = p + (int64_t)c * EPT;
+    if (c * EPT + EPT <= len && (reinterpret_cast<uintptr_t>(cp) & 15) == 0) {
+#pragma unroll
+      for (int h = 0; h < EPT / 2; ++h) {
+        const d2_t t = reinterpret_cast<const d2_t *>(cp)[h];
+        xv[i][2 * h] = t[0];
+        xv[i][2 * h + 1] = t[1];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) xv[i][q] = c * EPT + q < len ? cp[q] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int q = 0; q < EPT; ++q)
+      mx = max_u64(mx, (unsigned long long)__double_as_longlong(xv[i][q]) & 0x7fffffffffffffffull);
+  for (int off = 32; off > 0; off >>= 1) mx = max_u64(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  if (tid == 0)
+    eshared = exponent_of_absbits(max_u64(max_u64(red[0], red[1]), max_u64(red[2], red[3])));
+  __syncthreads();
+  const int e = eshared;
+  double sd = 0.0;
+  if (e != OZ_NONFINITE) {
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) sd += oz2_term_d(xv[i][q], e);
+  }
+  unsigned long long S = (unsigned long long)sd;
+  for (int off = 32; off > 0; off >>= 1) S += __shfl_xor(S, off, 64);
+  __syncthreads(); /* red[] reused */
+  if (lane == 0) red[wave] = S;
+  __syncthreads();
+  if (tid == 0) {
+    const int d = oz2_shift(L, e, red[0] + red[1] + red[2] + red[3]);
+    ex[v] = d;
+    dshared = d;
+  }
+  __syncthreads();
+  const int d = dshared; /* shift P - e */
+  const int64_t plane = (int64_t)nvec * kpad;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int c = tid + 256 * i;
+    if (c >= nchunks) break;
+    vec_t res[OZ2_MAX_MODULI];
+#pragma unroll
+    for (int j = 0; j < OZ2_MAX_MODULI; ++j) res[j] = vec_t{};
+    Oz2Val a[4];
+#define GEMMX_OZ2_WORD(W)                                                                \
+    for (int q = 0; q < 4; ++q)                                                          \
+      a[q] = oz2_val(d != OZ_NONFINITE ? xv[i][4 * (W) + q] : 0.0, d);                   \
+    oz2_encode4<0, W>(a, nmod, res);
+    GEMMX_OZ2_WORD(0)
+    GEMMX_OZ2_WORD(1)
+#undef GEMMX_OZ2_WORD
+    int8_t *o = out + (int64_t)v * kpad + (int64_t)c * EPT;
+#pragma unroll
+    for (int j = 0; j < OZ2_MAX_MODULI; ++j, o += plane)
+      if (j < nmod) *reinterpret_cast<vec_t *>(o) = res[j];
+  }
+}
+
 /* The same for vectors adjacent in memory (x[v + p*eld]) through an LDS
  * tile of 32 vectors x 128 K entries: entry p of a vector sits in column
  * (p % EPT) TPV + p / EPT of its padded row, so that the TPV threads of a
@@ -667,6 +759,11 @@ hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec
   /* 8 entries per thread (two words per plane): fewer registers, so 5 waves
    * per SIMD, measured faster than 16 for both forms */
   constexpr int NW = 2;
+  if (contiguous && kpad <= 256 * 8 * OZ2_FUSED_CHUNKS) {
+    hipLaunchKernelGGL(ozaki2_shift_split_contig_kernel, dim3(nvec), dim3(256), 0, st, x, ld,
+                       nvec, k, kpad, L, nmod, ex, res);
+    return hipGetLastError();
+  }
   if (contiguous) {
     hipLaunchKernelGGL(oz2_shift_contig_kernel, dim3(nvec), dim3(256), 0, st, x, ld, nvec, k,
                        L, ex);
