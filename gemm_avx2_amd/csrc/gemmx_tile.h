@@ -67,7 +67,6 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
   constexpr int B_NL = TB ? BK : BN, B_LEN = TB ? BN : BK, B_LS = B_LEN + VE;
   static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0 && KS % KV == 0, "tile shape");
   typedef typename Mfma16<T>::acc_t acc_t;
-  typedef typename Vec16<T>::type vec_t;
   typedef T kvec_t __attribute__((ext_vector_type(KV)));
 
   // Stage pointers by arithmetic on `lds` (an array of pointers indexed at
