@@ -5,7 +5,7 @@ It takes the place of the reference's hand-edited compile-time tuning
 headers (wjc404/GEMM_AVX2 dgemm_tune.h:6-10, sgemm_tune.h:7-11, README.md:13-15:
 edit UNROLL_N / block sizes, rebuild, time with an external benchmark).  Here
 the choice is a run-time table keyed by (element type, transposes, shape
-bucket), bucket = floor(log2(dim)) clamped to 4..15, so one build serves every
+bucket), bucket = round(log2(dim)) clamped to 4..15, so one build serves every
 shape and retuning needs no rebuild.
 
     python -m gemm_avx2_amd.autotune [--out PATH] [--dtypes d,s]
@@ -51,8 +51,10 @@ SHAPES = {
 
 
 def bucket(dim: int) -> int:
-    """floor(log2(dim)) clamped to 4..15, as the libraries compute it."""
-    b = max(int(dim), 1).bit_length() - 1
+    """round(log2(dim)) clamped to 4..15, as the libraries compute it: the
+    power of two nearest dim on a log scale, floor(log2(2*dim^2) / 2)."""
+    d = max(int(dim), 1)
+    b = ((2 * d * d).bit_length() - 1) >> 1
     return min(max(b, 4), 15)
 
 

@@ -47,36 +47,46 @@ using namespace gemmx;
 
 namespace {
 
-/* FAST: grid over the interior tiles (tiles_m x tiles_n of them, all inside
- * C, operands 16-byte loadable).  !FAST: grid over the edge tiles -- the
+/* FAST (operands 16-byte loadable): tiles_m x tiles_n are the interior tiles,
+ * all inside C.  The first `lead` workgroups take the edge tiles -- the
  * partial tile-row tm = tiles_m (when M % BM != 0, tiles_n + (N % BN != 0)
- * tiles) then the partial tile-column tn = tiles_n for tm < tiles_m -- or,
- * with edge_all, over every tile of a tiles_m x tiles_n grid. */
+ * tiles) then the partial tile-column tn = tiles_n for tm < tiles_m -- on the
+ * predicated path (workgroups past the last edge tile exit); the rest take
+ * the interior tiles on the branch-free path.  Dispatched first, the edge
+ * tiles run beside the interior ones instead of in a trailing wave of their
+ * own.  `lead` is a multiple of 8, so the interior workgroups keep their
+ * XCD-to-tile mapping.  !FAST: every tile of tiles_m x tiles_n predicated. */
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW,
           int KVMAX, bool FAST>
 __global__ void __launch_bounds__(64 * WM * WN, MINW)
 gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
                    T *__restrict__ C, int M, int N, int K, int64_t lda,
                    int64_t ldb, int64_t ldc, T alpha, T beta, int tiles_m,
-                   int tiles_n, int vecA, int vecB, int edge_all, int64_t sa, int64_t sb,
+                   int tiles_n, int vecA, int vecB, int lead, int64_t sa, int64_t sb,
                    int64_t sc) {
   __shared__ __align__(16) T lds[tile_lds_elems<T, TA, TB, BM, BN, BK>()];
   A += blockIdx.y * sa;
   B += blockIdx.y * sb;
   C += blockIdx.y * sc;
   int tm, tn;
-  if (FAST || edge_all) {
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, &tm, &tn);
-  } else {
+  if (FAST && (int)blockIdx.x < lead) {
     const int row_tiles = (M % BM) ? tiles_n + ((N % BN) ? 1 : 0) : 0;
-    if ((int)blockIdx.x < row_tiles) {
+    const int col_tiles = (N % BN) ? tiles_m : 0;
+    const int e = blockIdx.x;
+    if (e >= row_tiles + col_tiles) return; /* padding of `lead` */
+    if (e < row_tiles) {
       tm = tiles_m;
-      tn = blockIdx.x;
+      tn = e;
     } else {
-      tm = blockIdx.x - row_tiles;
+      tm = e - row_tiles;
       tn = tiles_n;
     }
+    gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, false>(A, B, C, M, N, K, lda, ldb, ldc,
+                                                           alpha, beta, tm * BM, tn * BN,
+                                                           vecA, vecB, lds);
+    return;
   }
+  tile_coords(blockIdx.x - lead, gridDim.x - lead, tiles_m, tiles_n, &tm, &tn);
   gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, FAST>(A, B, C, M, N, K, lda, ldb, ldc,
                                                         alpha, beta, tm * BM, tn * BN, vecA,
                                                         vecB, lds);
@@ -94,9 +104,10 @@ scale_kernel(T *__restrict__ C, int64_t ldc, int M, int N, T beta, int64_t sc) {
   }
 }
 
-/* Interior tiles go to the FAST kernel, the rest to the predicated one; both
- * launches are on stream s and touch disjoint tiles of C.  A strided batch
- * goes in grid rows of at most GEMMX_GRID_Y problems per launch. */
+/* With 16-byte loadable operands one launch covers C: its leading
+ * workgroups the edge tiles (predicated), the rest the interior tiles
+ * (branch-free).  Otherwise every tile is predicated.  A strided batch goes
+ * in grid rows of at most GEMMX_GRID_Y problems per launch. */
 constexpr int GEMMX_GRID_Y = 65535;
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
@@ -116,20 +127,16 @@ hipError_t launch_trans(const gemmx_problem *p, hipStream_t s) {
     const T *A = (const T *)p->a + b0 * sa, *B = (const T *)p->b + b0 * sb;
     T *C = (T *)p->c + b0 * sc;
     if (vecA && vecB && full_m > 0 && full_n > 0) {
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
-                         dim3(full_m * full_n, nb), block, 0, s, A, B, C, p->m, p->n, p->k,
-                         p->lda, p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0,
-                         sa, sb, sc);
       const int edges = tiles_m * tiles_n - full_m * full_n;
-      if (edges > 0)
-        hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
-                           dim3(edges, nb), block, 0, s, A, B, C, p->m, p->n, p->k, p->lda,
-                           p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB, 0, sa, sb,
-                           sc);
+      const int lead = (edges + 7) / 8 * 8;
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
+                         dim3(lead + full_m * full_n, nb), block, 0, s, A, B, C, p->m, p->n,
+                         p->k, p->lda, p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB,
+                         lead, sa, sb, sc);
     } else {
       hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
                          dim3(tiles_m * tiles_n, nb), block, 0, s, A, B, C, p->m, p->n, p->k,
-                         p->lda, p->ldb, p->ldc, alpha, beta, tiles_m, tiles_n, vecA, vecB, 1,
+                         p->lda, p->ldb, p->ldc, alpha, beta, tiles_m, tiles_n, vecA, vecB, 0,
                          sa, sb, sc);
     }
     const hipError_t e = hipGetLastError();
@@ -152,7 +159,9 @@ hipError_t launch_native(const gemmx_problem *p, int cfg, hipStream_t s) {
   switch (cfg) {
   case 1: return launch_cfg<T, 64, 64, 16, 2, 2, 2, 1>(p, s);
   case 2: return launch_cfg<T, 256, 128, 16, 4, 2, 2, 1>(p, s);
-  case 3: return launch_cfg<T, 128, 128, 32, 2, 2, 1, 4>(p, s);
+  /* FP32 128x128 with K-vector reads fits two workgroups per CU's register
+   * file; FP64 (twice the accumulator registers) takes one */
+  case 3: return launch_cfg<T, 128, 128, 32, 2, 2, std::is_same<T, float>::value ? 2 : 1, 4>(p, s);
   case 4: return launch_cfg<T, 128, 128, 16, 2, 2, 1, 4>(p, s);
   case 5: return launch_cfg<T, 128, 128, 16, 4, 2, 2, 1>(p, s);
   case 6: return launch_cfg<T, 256, 128, 16, 4, 2, 1, 4>(p, s);

@@ -29,12 +29,17 @@ static int g_force[2] = {-1, -1}; /* [f64/f32]: one configuration for every call
 static char g_path[1024];
 static char g_arch[64];           /* current device arch, "" = unknown */
 
+/* The table is measured at powers of two, so a dimension takes the nearest
+ * one on a log scale: bucket b covers [2^b/sqrt(2), 2^b*sqrt(2)), i.e.
+ * b = floor(log2(2*dim^2) / 2). */
 int gemmx_tune_bucket(int dim) {
-  int b = 0;
-  while (dim > 1 && b < 31) {
-    dim >>= 1;
-    b++;
+  uint64_t v = dim > 0 ? 2 * (uint64_t)dim * (uint64_t)dim : 1;
+  int f = -1;
+  while (v) {
+    v >>= 1;
+    f++;
   }
+  int b = f >> 1;
   if (b < 4) b = 4;
   if (b > 15) b = 15;
   return b;

@@ -22,6 +22,15 @@ def restore_tables():
         gx.load_tuning(DEFAULT if os.path.exists(DEFAULT) else None, d)
 
 
+def _bucket_edge(b, scale):
+    """Lowest (scale 0), central (1) and highest (2) dimension of bucket b:
+    bucket b holds [2^b/sqrt(2), 2^b*sqrt(2)), so its lowest member is
+    ceil(2^(b-1/2)) = isqrt(2^(2b-1) - 1) + 1."""
+    import math
+    lowest = lambda c: math.isqrt((1 << (2 * c - 1)) - 1) + 1
+    return (lowest(b), 1 << b, lowest(b + 1) - 1)[scale]
+
+
 def _entries():
     return {("f64", "NN", 13, 13, 13): 6, ("f64", "TN", 8, 8, 8): 1, ("f64", "NT", 10, 12, 9): 3,
             ("f32", "NN", 13, 13, 13): 2, ("f32", "TT", 4, 15, 15): 5}
@@ -49,10 +58,10 @@ def test_library_lookup_follows_the_table(tmp_path, dtype, restore_tables):
     for (dt, tr, mb, nb, kb), cfg in e.items():
         if dt != want:
             continue
-        for scale in (0, 1):  # lowest and highest size in each bucket
-            m, n, k = ((1 << b) if not scale else (1 << (b + 1)) - 1 for b in (mb, nb, kb))
+        for scale in (0, 1, 2):  # lowest, middle and highest size in each bucket
+            m, n, k = (_bucket_edge(b, scale) for b in (mb, nb, kb))
             if mb == 4:
-                m = 3  # buckets clamp: anything below 16 is bucket 4
+                m = 3  # buckets clamp: anything below 23 is bucket 4
             assert gx.tune_lookup(dtype, tr[0], tr[1], m, n, k) == cfg
             # only the first character counts, and lower case is the same
             assert gx.tune_lookup(dtype, tr[0].lower(), "C" if tr[1] == "T" else "n",
@@ -113,11 +122,13 @@ def test_resume_skips_measured_keys():
     wanted = [("d", "NN", 8192, 8192, 8192), ("d", "NN", 9000, 9000, 9000),
               ("d", "TN", 256, 256, 256), ("s", "NN", 8192, 8192, 8192)]
     have = {at.key_of("d", "NN", 8192, 8192, 8192): 0}
-    # one shape per key (9000 falls in the 8192 bucket)
+    # one shape per key (9000 falls in the 8192 bucket: 2^13 nearest on a log scale)
     assert at.pending(wanted, {}, False) == [wanted[0], wanted[2], wanted[3]]
     assert at.pending(wanted, have, True) == [wanted[2], wanted[3]]
-    assert at.bucket(1) == 4 and at.bucket(16) == 4 and at.bucket(31) == 4
-    assert at.bucket(32) == 5 and at.bucket(8191) == 12 and at.bucket(1 << 20) == 15
+    assert at.bucket(1) == 4 and at.bucket(16) == 4 and at.bucket(22) == 4
+    assert at.bucket(23) == 5 and at.bucket(5792) == 12 and at.bucket(5793) == 13
+    assert at.bucket(8191) == 13 and at.bucket(11585) == 13 and at.bucket(11586) == 14
+    assert at.bucket(1 << 20) == 15 and at.bucket(0) == 4
 
 
 def test_configuration_counts_agree():
@@ -143,8 +154,8 @@ def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
     got, _, _ = at.read_table(path)
     assert got == e
     assert gx.load_tuning(path, "d") == len(e)
-    assert gx.tune_lookup_emulated("N", "N", 8192, 9000, 16000) == (gx.MODE_OZAKI2, 18)
-    assert gx.tune_lookup_emulated("n", "n", 8192, 9000, 16000) == (gx.MODE_OZAKI2, 18)
+    assert gx.tune_lookup_emulated("N", "N", 8192, 9000, 11000) == (gx.MODE_OZAKI2, 18)
+    assert gx.tune_lookup_emulated("n", "n", 8192, 9000, 11000) == (gx.MODE_OZAKI2, 18)
     assert gx.tune_lookup_emulated("T", "N", 4096, 4096, 4096) == (gx.MODE_NATIVE, 0)
     assert gx.tune_lookup_emulated("N", "T", 16384, 16384, 1024) == (gx.MODE_OZAKI, 6)
     assert gx.tune_lookup_emulated("T", "T", 8192, 8192, 8192) == (gx.MODE_NATIVE, 0)
