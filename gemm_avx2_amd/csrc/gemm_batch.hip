@@ -12,8 +12,9 @@
  * heaviest problem first (the host sorts), which with dynamic tickets
  * approximates longest-processing-time scheduling.  A tile is computed by
  * the same gemm_tile as the single-GEMM kernels: the branch-free loop for
- * interior tiles of 16-byte loadable operands, the bounds-checked loop for
- * edge tiles, instantiated for the four transpose pairs and chosen per tile.
+ * every tile of 16-byte loadable operands (an edge tile's loads clamped back
+ * into the operand), the bounds-checked loop for the others, instantiated
+ * for the four transpose pairs and chosen per tile.
  */
 #include "gemmx_device.h"
 #include "gemmx_internal.h"
@@ -76,11 +77,12 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
       }
     } else {
       const int va = (f & GEMMX_BF_VEC_A) != 0, vb = (f & GEMMX_BF_VEC_B) != 0;
-      /* interior tiles with 16-byte loadable operands take the branch-free
-       * loop; edge tiles the bounds-checked one */
-      const int fast = va && vb && m0 + G::BM <= q.m && n0 + G::BN <= q.n;
+      /* tiles of 16-byte loadable operands take the branch-free loop, an
+       * edge tile's loads past m or n clamped back into the operand; the
+       * others the bounds-checked loop */
+      const int fast = va && vb;
 #define GEMMX_TILE(TA, TB, FAST)                                                        \
-  gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, FAST>(                     \
+  gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, FAST, FAST>(               \
       A, B, C, q.m, q.n, q.k, q.lda, q.ldb, q.ldc, alpha, beta, m0, n0, va, vb, lds)
       switch ((f & (GEMMX_BF_TA | GEMMX_BF_TB)) | (fast ? 4 : 0)) {
       case 0: GEMMX_TILE(false, false, false); break;

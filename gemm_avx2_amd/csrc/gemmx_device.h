@@ -111,6 +111,24 @@ struct TileCopy {
     }
   }
 
+  /* A full-K slice of an edge tile with aligned operands: lines at or past
+   * nl_valid re-read line nl_valid-1, and chunks at or past len_valid re-read
+   * the last chunk holding a valid element, so every chunk is still one
+   * 16-byte load.  Each load is the aligned 16-byte block around a valid
+   * element (an aligned block never crosses a page), and what lands in a
+   * position past nl_valid/len_valid feeds only rows or columns of C that the
+   * epilogue does not store. */
+  __device__ __forceinline__ void load_clamp(const T *__restrict__ g, int64_t ld, int nl_valid,
+                                             int len_valid, int tid) {
+    const int last_off = ((len_valid - 1) / VE) * VE;
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const int ch = tid + c * NT;
+      const int line = min(ch / CPL, nl_valid - 1), off = min((ch % CPL) * VE, last_off);
+      r[c] = *reinterpret_cast<const vec_t *>(g + line * ld + off);
+    }
+  }
+
   template <int LS>
   __device__ __forceinline__ void store(T *s, int tid) const {
 #pragma unroll

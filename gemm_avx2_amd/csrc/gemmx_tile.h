@@ -21,7 +21,10 @@
  * the full K slices with an explicit issue order -- the next slice's global
  * loads, each step's fragment reads spread over the previous step's MFMAs,
  * the next slice's LDS writes spread over the last step -- and one peeled
- * predicated slice for the K remainder.  !FAST: every slice predicated.
+ * predicated slice for the K remainder.  FAST with CLAMP: the same loop for
+ * an edge tile of 16-byte loadable operands, its loads past M or N clamped
+ * back into the operand (TileCopy::load_clamp).  !FAST: every slice
+ * predicated.
  * The epilogue writes C = fma(alpha, acc, beta*C) for rows < M, columns < N
  * only (the reference's masked stores, src/kernel/gemm_kernel_unroll6.S:
  * 145-155); alpha moves from the reference's packed B (src/gemm_copy.c:65)
@@ -50,7 +53,7 @@ constexpr int tile_lds_max() {
 }
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KVMAX,
-          bool FAST>
+          bool FAST, bool CLAMP = false>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           T *__restrict__ C, int M, int N, int K,
                                           int64_t lda, int64_t ldb, int64_t ldc, T alpha,
@@ -89,8 +92,15 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     else     cb.load(bbase(k0), ldb, K - k0, N - n0, vecB, tid);
   };
   auto load_fast = [&](int k0) {
-    ca.load_vec(abase(k0), lda, tid);
-    cb.load_vec(bbase(k0), ldb, tid);
+    if (CLAMP) {
+      if (!TA) ca.load_clamp(abase(k0), lda, A_NL, M - m0, tid);
+      else     ca.load_clamp(abase(k0), lda, M - m0, A_LEN, tid);
+      if (!TB) cb.load_clamp(bbase(k0), ldb, N - n0, B_LEN, tid);
+      else     cb.load_clamp(bbase(k0), ldb, B_NL, N - n0, tid);
+    } else {
+      ca.load_vec(abase(k0), lda, tid);
+      cb.load_vec(bbase(k0), ldb, tid);
+    }
   };
   auto store = [&](int buf) {
     ca.template store<A_LS>(sA(buf), tid);

@@ -26,7 +26,7 @@ def _tol(dt, k):
 def _problems(rng, count, lo, hi, dt):
     """Random problems: log-uniform sizes in [lo, hi], every trans spelling,
     odd leading dimensions, alpha == 0 / k == 0 / beta == 0 / beta == 1 cases,
-    NaN in C where beta == 0."""
+    NaN in C where beta == 0 and in the ld padding of A and B."""
     out = []
     for i in range(count):
         m, n, k = (int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(3))
@@ -42,6 +42,8 @@ def _problems(rng, count, lo, hi, dt):
         lda, ldb, ldc = max(ra, 1) + pad, max(rb, 1) + pad, m + pad
         A = np.asfortranarray(rng.uniform(-1, 1, (lda, ca)).astype(dt))
         B = np.asfortranarray(rng.uniform(-1, 1, (ldb, cb)).astype(dt))
+        A[ra:] = np.nan  # ld padding: a value read from it must not reach C
+        B[rb:] = np.nan
         C = np.asfortranarray(rng.uniform(-1, 1, (ldc, n)).astype(dt))
         C[m:] = 777.0
         if beta == 0.0 and m > 1 and n > 1:

@@ -273,6 +273,46 @@ def test_every_tile_config(cuda, dt):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_edge_tiles_with_aligned_operands(cuda, dt, ta, tb):
+    """Edge tiles of 16-byte-loadable operands (aligned base, ld a multiple of
+    four elements) run the branch-free loop with their loads past m or n
+    clamped back into the operand; m and n one past, one short of and far
+    from every tile edge, odd sizes whose last 16-byte chunk straddles the
+    end of a column, for every tile configuration.  The ld padding of A and B
+    holds NaN, so a value read from it that reached a stored element would
+    show; rows m..ldc-1 of C stay untouched."""
+    import torch
+    d = "d" if dt == np.float64 else "s"
+    rng = np.random.default_rng(29)
+    try:
+        for cfg in range(gx.TILE_CONFIGS):
+            gx.set_tile_config(d, cfg)
+            for (m, n, k) in [(131, 70, 77), (257, 383, 40), (1, 129, 33), (129, 1, 16),
+                              (300, 260, 19), (63, 127, 35)]:
+                ra, ca = stored_shape(ta, m, k)
+                rb, cb = stored_shape(tb, k, n)
+                lda, ldb, ldc = (ra + 3) // 4 * 4 + 4, (rb + 3) // 4 * 4, m + 3
+                A = np.full((lda, ca), np.nan, dtype=dt, order="F")
+                B = np.full((ldb, cb), np.nan, dtype=dt, order="F")
+                A[:ra] = rng.uniform(-1, 1, (ra, ca))
+                B[:rb] = rng.uniform(-1, 1, (rb, cb))
+                C = np.full((ldc, n), 777.0, dtype=dt, order="F")
+                C[:m] = rng.uniform(-1, 1, (m, n))
+                R = C.copy(order="F")
+                oracle.gemm(ta, tb, m, n, k, 1.5, A, lda, B, ldb, -0.5, R, ldc)
+                dA, dB, dC = to_device(A, torch), to_device(B, torch), to_device(C, torch)
+                _gemm_dev(dt, ta, tb, m, n, k, 1.5, dA, lda, dB, ldb, -0.5, dC, ldc)
+                torch.cuda.synchronize()
+                G = from_device(dC, C.shape)
+                bound = abs_bound(A[:ra], B[:rb], C[:m], ta, tb, m, n, k, 1.5, -0.5)
+                assert max_comp(G[:m], R[:m], bound) <= _tol(dt, k), (cfg, m, n, k)
+                assert np.all(G[m:] == 777.0), (cfg, m, n, k)
+    finally:
+        gx.set_tile_config(d, -1)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_host_pointer_staging_ring(cuda, dt, monkeypatch):
     """Host operands above the staging threshold go through the ring of
     pinned chunks (1 MB chunks here, so A, B and C each take many chunks and
