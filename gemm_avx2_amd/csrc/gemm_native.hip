@@ -30,10 +30,11 @@
  * row blockIdx.y takes problem (batch base + blockIdx.y), its operands offset
  * by the per-problem strides.
  *
- * With operands that allow 16-byte loads every workgroup runs a branch-free
- * loop over the full K slices (the K remainder, if any, is one peeled
- * predicated slice); an edge tile's loads past M or N are clamped back into
- * the operand and its stores masked.  Otherwise a predicated loop.  In the
+ * Every workgroup runs a branch-free loop over the full K slices (the K
+ * remainder, if any, is one peeled predicated slice); an edge tile's loads
+ * past M or N are clamped back into the operand and its stores masked.
+ * Operands that allow 16-byte loads are copied in 16-byte chunks, others one
+ * element per load (the LDS layout is the same).  In the
  * branch-free loop the issue order is set explicitly: the next slice's global
  * loads first, each MFMA step's fragment reads spread over the previous
  * step's MFMAs, and the next slice's LDS writes spread over the last step.
@@ -48,20 +49,20 @@ using namespace gemmx;
 
 namespace {
 
-/* FAST (operands 16-byte loadable): tiles_m x tiles_n are the interior tiles,
- * all inside C.  The first `lead` workgroups take the edge tiles -- the
- * partial tile-row tm = tiles_m (when M % BM != 0, tiles_n + (N % BN != 0)
- * tiles) then the partial tile-column tn = tiles_n for tm < tiles_m -- on the
- * branch-free path with loads clamped into the operands; the rest take the
- * interior tiles.  Dispatched first, the edge tiles run beside the interior
+/* tiles_m x tiles_n are the interior tiles, all inside C.  The first `lead`
+ * workgroups take the edge tiles -- the partial tile-row tm = tiles_m (when
+ * M % BM != 0, tiles_n + (N % BN != 0) tiles) then the partial tile-column
+ * tn = tiles_n for tm < tiles_m -- on the branch-free path with loads clamped
+ * into the operands; the rest take the interior tiles.  NARROW: operands
+ * that do not allow 16-byte loads, loaded one element per load.  Dispatched first, the edge tiles run beside the interior
  * ones instead of in a trailing wave of their own.  Interior tile ids start
  * at `lead`, which shifts every workgroup's XCD (blockIdx.x % 8) by the same
  * amount, so the workgroups of one XCD still share tile rows and columns.
  * The grid holds exactly one workgroup per tile: a padding workgroup would
  * give one XCD an extra tile and, when the tiles fill whole waves, a wave of
- * its own.  !FAST: every tile of tiles_m x tiles_n predicated. */
+ * its own. */
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW,
-          int KVMAX, bool FAST>
+          int KVMAX, bool NARROW>
 __global__ void __launch_bounds__(64 * WM * WN, MINW)
 gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
                    T *__restrict__ C, int M, int N, int K, int64_t lda,
@@ -73,7 +74,7 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
   B += blockIdx.y * sb;
   C += blockIdx.y * sc;
   int tm, tn;
-  if (FAST && (int)blockIdx.x < lead) {
+  if ((int)blockIdx.x < lead) {
     const int row_tiles = (M % BM) ? tiles_n + ((N % BN) ? 1 : 0) : 0;
     const int col_tiles = (N % BN) ? tiles_m : 0;
     const int e = blockIdx.x;
@@ -85,15 +86,13 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
       tm = e - row_tiles;
       tn = tiles_n;
     }
-    gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, true>(A, B, C, M, N, K, lda, ldb,
-                                                                ldc, alpha, beta, tm * BM,
-                                                                tn * BN, vecA, vecB, lds);
+    gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, true, NARROW>(
+        A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN, vecA, vecB, lds);
     return;
   }
   tile_coords(blockIdx.x - lead, gridDim.x - lead, tiles_m, tiles_n, &tm, &tn);
-  gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, FAST>(A, B, C, M, N, K, lda, ldb, ldc,
-                                                        alpha, beta, tm * BM, tn * BN, vecA,
-                                                        vecB, lds);
+  gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, false, NARROW>(
+      A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN, vecA, vecB, lds);
 }
 
 template <typename T>
@@ -108,10 +107,10 @@ scale_kernel(T *__restrict__ C, int64_t ldc, int M, int N, T beta, int64_t sc) {
   }
 }
 
-/* With 16-byte loadable operands one launch covers C: its leading
- * workgroups the edge tiles (clamped loads), the rest the interior tiles.
- * Otherwise every tile is predicated.  A strided batch goes
- * in grid rows of at most GEMMX_GRID_Y problems per launch. */
+/* One launch covers C: its leading workgroups the edge tiles (clamped
+ * loads), the rest the interior tiles; 16-byte loads where both operands
+ * allow them, else one element per load.  A strided batch goes in grid rows
+ * of at most GEMMX_GRID_Y problems per launch. */
 constexpr int GEMMX_GRID_Y = 65535;
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
@@ -130,19 +129,16 @@ hipError_t launch_trans(const gemmx_problem *p, hipStream_t s) {
     const unsigned nb = (unsigned)(batch - b0 < GEMMX_GRID_Y ? batch - b0 : GEMMX_GRID_Y);
     const T *A = (const T *)p->a + b0 * sa, *B = (const T *)p->b + b0 * sb;
     T *C = (T *)p->c + b0 * sc;
-    if (vecA && vecB) {
-      const int edges = tiles_m * tiles_n - full_m * full_n;
-      const int lead = edges;
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
-                         dim3(lead + full_m * full_n, nb), block, 0, s, A, B, C, p->m, p->n,
-                         p->k, p->lda, p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB,
-                         lead, sa, sb, sc);
-    } else {
+    const int lead = tiles_m * tiles_n - full_m * full_n; /* edge tiles */
+    const dim3 grid(lead + full_m * full_n, nb);
+    if (vecA && vecB)
       hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
-                         dim3(tiles_m * tiles_n, nb), block, 0, s, A, B, C, p->m, p->n, p->k,
-                         p->lda, p->ldb, p->ldc, alpha, beta, tiles_m, tiles_n, vecA, vecB, 0,
-                         sa, sb, sc);
-    }
+                         grid, block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb, p->ldc,
+                         alpha, beta, full_m, full_n, vecA, vecB, lead, sa, sb, sc);
+    else
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
+                         grid, block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb, p->ldc,
+                         alpha, beta, full_m, full_n, vecA, vecB, lead, sa, sb, sc);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -129,6 +129,26 @@ struct TileCopy {
     }
   }
 
+  /* A full-K slice of operands that do not allow 16-byte loads: one load per
+   * element (consecutive lanes still read consecutive elements of a line).
+   * With CLAMP (edge tile) a line or element index past nl_valid/len_valid is
+   * clamped to the last valid one, with the same effect as in load_clamp. */
+  template <bool CLAMP>
+  __device__ __forceinline__ void load_narrow(const T *__restrict__ g, int64_t ld, int nl_valid,
+                                              int len_valid, int tid) {
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const int ch = tid + c * NT;
+      const int line = CLAMP ? min(ch / CPL, nl_valid - 1) : ch / CPL;
+      const int off = (ch % CPL) * VE;
+      const T *p = g + line * ld;
+      vec_t v;
+#pragma unroll
+      for (int e = 0; e < VE; ++e) v[e] = p[CLAMP ? min(off + e, len_valid - 1) : off + e];
+      r[c] = v;
+    }
+  }
+
   template <int LS>
   __device__ __forceinline__ void store(T *s, int tid) const {
 #pragma unroll

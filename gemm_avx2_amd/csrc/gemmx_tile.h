@@ -22,9 +22,10 @@
  * loads, each step's fragment reads spread over the previous step's MFMAs,
  * the next slice's LDS writes spread over the last step -- and one peeled
  * predicated slice for the K remainder.  FAST with CLAMP: the same loop for
- * an edge tile of 16-byte loadable operands, its loads past M or N clamped
- * back into the operand (TileCopy::load_clamp).  !FAST: every slice
- * predicated.
+ * an edge tile, its loads past M or N clamped back into the operand
+ * (TileCopy::load_clamp).  FAST with NARROW: the same loop for operands that
+ * do not allow 16-byte loads, one load per element (TileCopy::load_narrow).
+ * !FAST: every slice predicated.
  * The epilogue writes C = fma(alpha, acc, beta*C) for rows < M, columns < N
  * only (the reference's masked stores, src/kernel/gemm_kernel_unroll6.S:
  * 145-155); alpha moves from the reference's packed B (src/gemm_copy.c:65)
@@ -53,7 +54,7 @@ constexpr int tile_lds_max() {
 }
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KVMAX,
-          bool FAST, bool CLAMP = false>
+          bool FAST, bool CLAMP = false, bool NARROW = false>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           T *__restrict__ C, int M, int N, int K,
                                           int64_t lda, int64_t ldb, int64_t ldc, T alpha,
@@ -92,7 +93,12 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     else     cb.load(bbase(k0), ldb, K - k0, N - n0, vecB, tid);
   };
   auto load_fast = [&](int k0) {
-    if (CLAMP) {
+    if (NARROW) {
+      if (!TA) ca.template load_narrow<CLAMP>(abase(k0), lda, A_NL, M - m0, tid);
+      else     ca.template load_narrow<CLAMP>(abase(k0), lda, M - m0, A_LEN, tid);
+      if (!TB) cb.template load_narrow<CLAMP>(bbase(k0), ldb, N - n0, B_LEN, tid);
+      else     cb.template load_narrow<CLAMP>(bbase(k0), ldb, B_NL, N - n0, tid);
+    } else if (CLAMP) {
       if (!TA) ca.load_clamp(abase(k0), lda, A_NL, M - m0, tid);
       else     ca.load_clamp(abase(k0), lda, M - m0, A_LEN, tid);
       if (!TB) cb.load_clamp(bbase(k0), ldb, N - n0, B_LEN, tid);
@@ -168,9 +174,10 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
       store((kt + 1) & 1);
       {
         constexpr int NLD = TileCopy<T, A_NL, A_LEN, NT>::PER + TileCopy<T, B_NL, B_LEN, NT>::PER;
+        constexpr int NGL = NLD * (NARROW ? VE : 1); /* global loads per slice */
         constexpr int NR = (TA ? FM : FM * KV) + (TB ? FN * KV : FN); /* LDS reads per KV steps */
         constexpr int NMF = KV * FM * FN;                             /* MFMAs per KV steps */
-        __builtin_amdgcn_sched_group_barrier(0x020, NLD, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, NGL, 0);
         __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
 #pragma unroll
         for (int s = KV; s < KS; s += KV) {

@@ -274,25 +274,32 @@ def test_every_tile_config(cuda, dt):
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("ta,tb", TRANS)
-def test_edge_tiles_with_aligned_operands(cuda, dt, ta, tb):
-    """Edge tiles of 16-byte-loadable operands (aligned base, ld a multiple of
-    four elements) run the branch-free loop with their loads past m or n
-    clamped back into the operand; m and n one past, one short of and far
-    from every tile edge, odd sizes whose last 16-byte chunk straddles the
-    end of a column, for every tile configuration.  The ld padding of A and B
-    holds NaN, so a value read from it that reached a stored element would
-    show; rows m..ldc-1 of C stay untouched."""
+def test_edge_tiles_with_clamped_loads(cuda, dt, ta, tb):
+    """Every tile runs the branch-free loop, an edge tile's loads past m or n
+    clamped back into the operand: with 16-byte loads (aligned base, ld a
+    multiple of four elements; odd sizes whose last 16-byte chunk straddles
+    the end of a column) and with one load per element (odd ld); m and n one
+    past, one short of and far from every tile edge, for every tile
+    configuration.  The ld padding of A and B holds NaN, so a value read from
+    it that reached a stored element would show; rows m..ldc-1 of C stay
+    untouched."""
     import torch
     d = "d" if dt == np.float64 else "s"
     rng = np.random.default_rng(29)
     try:
         for cfg in range(gx.TILE_CONFIGS):
             gx.set_tile_config(d, cfg)
-            for (m, n, k) in [(131, 70, 77), (257, 383, 40), (1, 129, 33), (129, 1, 16),
-                              (300, 260, 19), (63, 127, 35)]:
+            for (m, n, k, wide) in [(131, 70, 77, 1), (257, 383, 40, 1), (1, 129, 33, 1),
+                                    (129, 1, 16, 1), (300, 260, 19, 1), (63, 127, 35, 1),
+                                    (131, 70, 77, 0), (257, 383, 40, 0), (128, 129, 33, 0),
+                                    (64, 64, 16, 0)]:
                 ra, ca = stored_shape(ta, m, k)
                 rb, cb = stored_shape(tb, k, n)
-                lda, ldb, ldc = (ra + 3) // 4 * 4 + 4, (rb + 3) // 4 * 4, m + 3
+                if wide:  # 16-byte loadable
+                    lda, ldb = (ra + 3) // 4 * 4 + 4, (rb + 3) // 4 * 4
+                else:  # odd leading dimensions
+                    lda, ldb = ra + 1 + (ra & 1), rb + 1 + (rb & 1)
+                ldc = m + 3
                 A = np.full((lda, ca), np.nan, dtype=dt, order="F")
                 B = np.full((ldb, cb), np.nan, dtype=dt, order="F")
                 A[:ra] = rng.uniform(-1, 1, (ra, ca))
@@ -306,8 +313,8 @@ def test_edge_tiles_with_aligned_operands(cuda, dt, ta, tb):
                 torch.cuda.synchronize()
                 G = from_device(dC, C.shape)
                 bound = abs_bound(A[:ra], B[:rb], C[:m], ta, tb, m, n, k, 1.5, -0.5)
-                assert max_comp(G[:m], R[:m], bound) <= _tol(dt, k), (cfg, m, n, k)
-                assert np.all(G[m:] == 777.0), (cfg, m, n, k)
+                assert max_comp(G[:m], R[:m], bound) <= _tol(dt, k), (cfg, m, n, k, lda, ldb)
+                assert np.all(G[m:] == 777.0), (cfg, m, n, k, lda, ldb)
     finally:
         gx.set_tile_config(d, -1)
 
