@@ -309,28 +309,35 @@ def run_headline(torch, gx, stream, args):
 # ---------------------------------------------------------------- other workloads
 def run_trans_and_ragged(torch, gx, stream):
     """Native DGEMM and SGEMM, 8192^3 in all four transpose pairs and the
-    ragged 8191 x 8189 x 8187 NN, ld 8200, alpha 1.5, beta -0.5, U[-1,1)."""
-    LD, al, be = 8200, 1.5, -0.5
+    ragged 8191 x 8189 x 8187 NN, ld 8200, then 8191^3 NN with ld 8191 (a
+    tight odd-sized array: no 16-byte loads); alpha 1.5, beta -0.5, U[-1,1)."""
+    al, be = 1.5, -0.5
     out = {}
     for d, tdt, dev in (("f64", torch.float64, gx.dgemm_dev), ("f32", torch.float32, gx.sgemm_dev)):
-        A = torch.empty(LD * K, dtype=tdt, device="cuda")
-        B = torch.empty(LD * K, dtype=tdt, device="cuda")
-        C = torch.empty(LD * K, dtype=tdt, device="cuda")
-        for t, i in ((A, 0), (B, 1), (C, 2)):
-            gx.fill(t, 8192, 8192, LD, SEED, i, stream=stream)
-        C0 = C.clone()
-        res = {}
-        for (ta, tb, m, n, k) in (("N", "N", M, N, K), ("N", "T", M, N, K), ("T", "N", M, N, K),
-                                  ("T", "T", M, N, K), ("N", "N", 8191, 8189, 8187)):
-            def f(ta=ta, tb=tb, m=m, n=n, k=k):
+        A = torch.empty(8200 * K, dtype=tdt, device="cuda")
+        B = torch.empty(8200 * K, dtype=tdt, device="cuda")
+        C = torch.empty(8200 * K, dtype=tdt, device="cuda")
+        res, filled = {}, None
+        for (ta, tb, m, n, k, LD) in (("N", "N", M, N, K, 8200), ("N", "T", M, N, K, 8200),
+                                      ("T", "N", M, N, K, 8200), ("T", "T", M, N, K, 8200),
+                                      ("N", "N", 8191, 8189, 8187, 8200),
+                                      ("N", "N", 8191, 8191, 8191, 8191)):
+            if filled != LD:
+                rows = min(LD, 8192)
+                for t, i in ((A, 0), (B, 1), (C, 2)):
+                    gx.fill(t, rows, rows, LD, SEED, i, stream=stream)
+                C0, filled = C.clone(), LD
+
+            def f(ta=ta, tb=tb, m=m, n=n, k=k, LD=LD):
                 dev(ta, tb, m, n, k, al, A, LD, B, LD, be, C, LD, stream=stream)
             _, per = time_launches(torch, f, 3, 1)
-            smp = Samples(torch, ta, tb, m, n, k, A, LD, B, LD, C0, LD, seed=3)
+            ne = LD * min(LD, 8192)  # whole columns of the filled matrices
+            smp = Samples(torch, ta, tb, m, n, k, A[:ne], LD, B[:ne], LD, C0[:ne], LD, seed=3)
             C.copy_(C0)
             f()
             torch.cuda.synchronize()
-            err = smp.error(torch, C, al, be)
-            name = ta + tb if m == M else f"{ta}{tb}_{m}x{n}x{k}"
+            err = smp.error(torch, C[:ne], al, be)
+            name = ta + tb if m == M else f"{ta}{tb}_{m}x{n}x{k}" + ("" if LD == 8200 else f"_ld{LD}")
             res[name] = {"tflops": round(2.0 * m * n * k / median(per) / 1e9, 2),
                          "rel_fro": err["rel_fro"], "max_comp": err["max_comp"]}
         out[d] = res

@@ -11,10 +11,10 @@
  * starts, so its latency hides behind the tile's work.  Tiles are numbered
  * heaviest problem first (the host sorts), which with dynamic tickets
  * approximates longest-processing-time scheduling.  A tile is computed by
- * the same gemm_tile as the single-GEMM kernels: the branch-free loop for
- * every tile of 16-byte loadable operands (an edge tile's loads clamped back
- * into the operand), the bounds-checked loop for the others, instantiated
- * for the four transpose pairs and chosen per tile.
+ * the same gemm_tile as the single-GEMM kernels: the branch-free loop with
+ * an edge tile's loads clamped back into the operand, in 16-byte chunks
+ * where both operands allow them and one element per load otherwise,
+ * instantiated for the four transpose pairs and chosen per problem.
  */
 #include "gemmx_device.h"
 #include "gemmx_internal.h"
@@ -77,12 +77,12 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
       }
     } else {
       const int va = (f & GEMMX_BF_VEC_A) != 0, vb = (f & GEMMX_BF_VEC_B) != 0;
-      /* tiles of 16-byte loadable operands take the branch-free loop, an
-       * edge tile's loads past m or n clamped back into the operand; the
-       * others the bounds-checked loop */
+      /* every tile takes the branch-free loop, an edge tile's loads past m
+       * or n clamped back into the operand: 16-byte loads where both operands
+       * allow them, else one element per load */
       const int fast = va && vb;
 #define GEMMX_TILE(TA, TB, FAST)                                                        \
-  gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, FAST, FAST>(               \
+  gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, true, true, !(FAST)>(      \
       A, B, C, q.m, q.n, q.k, q.lda, q.ldb, q.ldc, alpha, beta, m0, n0, va, vb, lds)
       switch ((f & (GEMMX_BF_TA | GEMMX_BF_TB)) | (fast ? 4 : 0)) {
       case 0: GEMMX_TILE(false, false, false); break;
