@@ -252,19 +252,21 @@ __device__ __forceinline__ void oz2_finish(unsigned (&acc)[LW], const Oz2Crt &cr
   *cp = __builtin_fma(alpha, scaled, beta * *cp);
 }
 
-/* Four consecutive elements (column-major index t) per thread: each plane's
- * bytes of the four come with one load (VEC: m n a multiple of 4) or four
- * clamped byte loads, all planes' loads issued before the first is used, and
- * each W_j limb is read once for the four. */
-template <int LW, bool VEC>
+/* Four consecutive rows of one column per thread: the residue planes are
+ * Mp x N bytes with column stride Mp = M rounded up to a multiple of 4, so
+ * each plane's bytes of the four come with one aligned 4-byte load (rows
+ * M..Mp-1 are padding, skipped); all planes' loads are issued before the
+ * first is used, and each W_j limb is read once for the four. */
+template <int LW>
 __global__ void __launch_bounds__(256)
 ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
                     const int *__restrict__ ea, const int *__restrict__ fb,
                     double *__restrict__ C, int64_t ldc, int M, int N, double alpha,
                     double beta) {
   constexpr int E = 4;
-  const int64_t total = (int64_t)M * N;
-  const int64_t quads = (total + E - 1) / E;
+  const int Mp = (M + 3) & ~3;
+  const int64_t total = (int64_t)Mp * N;
+  const int64_t quads = total / E;
   /* one quad per thread (no grid-stride loop, whose invariant W_j loads the
    * compiler would keep in registers across iterations) */
   const int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -283,16 +285,7 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
     for (int g = 0; g < OZ2_MAX_MODULI; ++g) {
       c4[g] = 0;
       if (g < crt.nmod) {
-        const uint8_t *rp = R + (int64_t)g * total;
-        if constexpr (VEC) {
-          c4[g] = *reinterpret_cast<const unsigned *>(rp + t0);
-        } else { /* clamped reads, bytes past the end taken as 0 */
-#pragma unroll
-          for (int q = 0; q < E; ++q) {
-            const unsigned b = rp[t0 + q < total ? t0 + q : total - 1];
-            c4[g] |= (t0 + q < total ? b : 0u) << (8 * q);
-          }
-        }
+        c4[g] = *reinterpret_cast<const unsigned *>(R + (int64_t)g * total + t0);
       }
     }
 #pragma unroll
@@ -306,16 +299,13 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
         }
       }
     }
-    /* (i, j) of the first element by one division, then stepped */
-    int64_t j = t0 / M, i = t0 - j * M;
+    /* (i, j) of the first element; the four share column j */
+    const int64_t j = t0 / Mp;
+    const int i = (int)(t0 - j * Mp);
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-      if (t0 + q < total) oz2_finish<LW>(acc[q], crt, ea[i], fb[j], C + i + j * ldc, alpha, beta);
-      if (++i == M) {
-        i = 0;
-        ++j;
-      }
-    }
+    for (int q = 0; q < E; ++q)
+      if (i + q < M)
+        oz2_finish<LW>(acc[q], crt, ea[i + q], fb[j], C + (i + q) + j * ldc, alpha, beta);
   }
 }
 
@@ -323,12 +313,8 @@ template <int LW>
 hipError_t launch_recon_lw(const uint8_t *R, const Oz2Crt &crt, const int *ea, const int *fb,
                            const gemmx_problem *p, double beta, unsigned blocks,
                            hipStream_t st) {
-  if (((int64_t)p->m * p->n) % 4 == 0)
-    hipLaunchKernelGGL((ozaki2_recon_kernel<LW, true>), dim3(blocks), dim3(256), 0, st, R, crt,
-                       ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha, beta);
-  else
-    hipLaunchKernelGGL((ozaki2_recon_kernel<LW, false>), dim3(blocks), dim3(256), 0, st, R, crt,
-                       ea, fb, (double *)p->c, p->ldc, p->m, p->n, p->alpha, beta);
+  hipLaunchKernelGGL((ozaki2_recon_kernel<LW>), dim3(blocks), dim3(256), 0, st, R, crt, ea, fb,
+                     (double *)p->c, p->ldc, p->m, p->n, p->alpha, beta);
   return hipGetLastError();
 }
 
@@ -362,7 +348,7 @@ Oz2Layout oz2_layout(int m, int n, int k, int nmod) {
   L.as = L.sq + rup((size_t)(m > n ? m : n) * 8, 256);
   L.bs = L.as + rup((size_t)nmod * m * L.kpad, 256);
   L.r8 = L.bs + rup((size_t)nmod * n * L.kpad, 256);
-  L.total = L.r8 + rup((size_t)nmod * m * n, 256);
+  L.total = L.r8 + rup((size_t)nmod * ((m + 3) & ~3) * n, 256); /* Mp x n per modulus */
   return L;
 }
 
@@ -405,8 +391,8 @@ extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *w
     G.add(OZ2_MODULI[g], 1, &zero, &zero);
   }
   const int kp = (int)L.kpad;
-  const int64_t total = (int64_t)p->m * p->n;
-  const int64_t rblocks = (total + 4 * 256 - 1) / (4 * 256); /* four elements per thread */
+  const int64_t total = (int64_t)((p->m + 3) & ~3) * p->n; /* residue plane bytes */
+  const int64_t rblocks = (total + 4 * 256 - 1) / (4 * 256); /* four rows per thread */
   for (int kb0 = 0; kb0 < kp; kb0 += OZ2_KCHUNK) {
     const int kb1 = kb0 + OZ2_KCHUNK < kp ? kb0 + OZ2_KCHUNK : kp;
     e = slice_products(SP_I8_MOD, as, bs, p->m, p->n, L.kpad, kb0, kb1, G, R, NULL, 0, 0.f,

@@ -93,12 +93,15 @@ __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, co
     const int mod = G.gout[g];
     const int c16 = 65536 % mod;
     const float rcp = 1.0f / (float)mod;
-    const bool vec = (M & 3) == 0;
-    uint8_t *const sbase = (uint8_t *)S + (int64_t)g * M * N + r0 + kg * 4 + (int64_t)(c0 + li) * M;
+    /* residue planes are Mp x N bytes, column stride Mp = M rounded up to a
+     * multiple of 4, so each lane's four rows are one aligned 4-byte store;
+     * rows M..Mp-1 are padding that nothing reads */
+    const int Mp = (M + 3) & ~3;
+    uint8_t *const sbase = (uint8_t *)S + (int64_t)g * Mp * N + r0 + kg * 4 + (int64_t)(c0 + li) * Mp;
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       if (c0 + j * 16 + li >= N) continue;
-      uint8_t *const sj = sbase + (int64_t)(j * 16) * M;
+      uint8_t *const sj = sbase + (int64_t)(j * 16) * Mp;
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         const int row = r0 + i * 16 + kg * 4;
@@ -113,14 +116,7 @@ __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, co
           rem -= rem >= mod ? mod : 0;
           bytes |= (unsigned)rem << (8 * r);
         }
-        uint8_t *sp = sj + i * 16;
-        if (vec && row + 3 < M) {
-          *reinterpret_cast<unsigned *>(sp) = bytes;
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (row + r < M) sp[r] = (uint8_t)(bytes >> (8 * r));
-        }
+        if (row < Mp) *reinterpret_cast<unsigned *>(sj + i * 16) = bytes;
       }
     }
     return;
