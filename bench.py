@@ -46,8 +46,9 @@ HBM_PEAK = 8000.0  # GB/s
 M = N = K = 8192
 PHIS = (0.0, 1.0, 4.0)
 SLICES = (3, 4, 5, 6, 7, 8)
-MODULI = (14, 15, 16, 17, 18, 20, 22)
+MODULI = (14, 15, 16, 17, 18, 19, 20, 21, 22)
 SEED = 1
+HEAD_SEED = 3  # the headline inputs: the emu8192 cases of tests/golden
 
 
 def parse():
@@ -57,6 +58,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--profile-only",
                     choices=["native", "ozaki", "ozaki2", "sgemm", "bf16x3", "batch", "copy"],
                     help="run just one form (for rocprofv3)")
@@ -90,33 +92,15 @@ def median(xs):
 
 
 # ---------------------------------------------------------------- exact reference
-def _two_prod(a, b):
-    """Dekker/Veltkamp TwoProduct in numpy: a*b = p + e exactly."""
-    c = 134217729.0
-
-    def split(x):
-        t = c * x
-        hi = t - (t - x)
-        return hi, x - hi
-    p = a * b
-    ah, al = split(a)
-    bh, bl = split(b)
-    e = ((ah * bh - p) + ah * bl + al * bh) + al * bl
-    return p, e
+from gemm_avx2_amd.autotune import dot2_pairs, sample_errors as errors  # noqa: E402
 
 
 def exact_entries(arows, bcols, c0, alpha, beta):
-    """alpha*sum(a*b) + beta*c0 per sampled entry, the sum correctly rounded
-    (math.fsum over the exact products); arows (r, k), bcols (k, c)."""
-    out = np.empty((arows.shape[0], bcols.shape[1]))
-    for i in range(arows.shape[0]):
-        for j in range(bcols.shape[1]):
-            p, e = _two_prod(arows[i], bcols[:, j])
-            s = math.fsum(np.concatenate([p, e]))
-            # alpha*s rounded once more, then beta*c0: error ~1 ulp, far below
-            # the errors measured here
-            out[i, j] = math.fsum([alpha * s, beta * c0[i, j]])
-    return out
+    """alpha*sum(a*b) + beta*c0 on the grid arows (r, k) x bcols (k, c)."""
+    r, c = arows.shape[0], bcols.shape[1]
+    ri, cj = np.repeat(np.arange(r), c), np.tile(np.arange(c), r)
+    v, _ = dot2_pairs(arows, bcols.T, ri, cj, alpha, beta * c0.reshape(-1))
+    return v.reshape(r, c)
 
 
 class Samples:
@@ -156,41 +140,107 @@ class Samples:
 
 
 # ---------------------------------------------------------------- CPU baseline
-def cpu_baseline():
-    """The reference's own dgemm_ (compiled from its sources into oracle/_ref)
-    on this host's cores over the headline workload itself (8192^3 NN, ld
-    8192, alpha 1, beta 0, phi = 0 inputs), best of 2.  Falls back to the
+def cpu_info():
+    """CPU model, logical CPUs and physical cores this process may run on,
+    and the nominal clock (base frequency where the kernel exposes it, else
+    the current 'cpu MHz' of /proc/cpuinfo), with its source."""
+    cpus = sorted(os.sched_getaffinity(0))
+    model, mhz = "unknown", None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("cpu MHz") and mhz is None:
+                    mhz = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    ghz, src = (mhz / 1000.0 if mhz else None), "/proc/cpuinfo cpu MHz (current clock)"
+    for name in ("base_frequency", "cpuinfo_max_freq"):
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{cpus[0]}/cpufreq/{name}") as f:
+                ghz, src = int(f.read()) / 1e6, f"cpufreq/{name}"
+                break
+        except (OSError, ValueError, IndexError):
+            continue
+    return {"model": model, "logical_cpus": len(cpus), "physical_cores": len(cores),
+            "ghz": round(ghz, 3) if ghz else None, "ghz_source": src}
+
+
+def cpu_baseline_child():
+    """Runs in a child process that never touches the GPU (started before the
+    parent initialises it), so the OpenMP environment the parent set applies
+    when libgomp starts: the reference's own dgemm_ (compiled from its
+    sources into oracle/_ref) over the headline workload itself (8192^3 NN,
+    ld 8192, alpha 1, beta 0, phi = 0 inputs), best of 3.  Falls back to the
     oracle's Dot2 port on a 1024 x 128 x 1024 slice when the reference build
-    is absent."""
+    is absent.  Prints one JSON line."""
     import ctypes
+    info = cpu_info()  # before libgomp binds this thread to one CPU
     import oracle
-    cores = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    os.environ.setdefault("OMP_PROC_BIND", "close")
     fn = oracle.ref("d")
     if fn is not None:
         m, n, k, ld = M, N, K, M
         kind = "reference"
-        sample = "dgemm_ NN 8192^3 ld 8192 alpha 1 beta 0, phi 0 (the whole workload), best of 2"
+        sample = ("dgemm_ NN 8192^3 ld 8192 alpha 1 beta 0, phi 0 (the whole workload), "
+                  "best of 3")
     else:
         m, n, k, ld = 1024, 128, 1024, 1024
-        kind, sample = "port", "oracle Dot2 port, NN 1024x128x1024, best of 2"
+        kind, sample = "port", "oracle Dot2 port, NN 1024x128x1024, best of 3"
         fn = oracle.lib().oracle_dgemm
-    A = oracle.fill(m, k, ld, SEED, 0, np.float64, 1, 0.0)
-    B = oracle.fill(k, n, ld, SEED, 1, np.float64, 1, 0.0)
-    C = oracle.fill(m, n, ld, SEED, 2)
+    A = oracle.fill(m, k, ld, HEAD_SEED, 0, np.float64, 1, 0.0)
+    B = oracle.fill(k, n, ld, HEAD_SEED, 1, np.float64, 1, 0.0)
+    C = oracle.fill(m, n, ld, HEAD_SEED, 2)
     I = ctypes.c_int
     best = float("inf")
-    for _ in range(2):
+    for _ in range(3):
         t0 = time.perf_counter()
         fn(b"N", b"N", ctypes.byref(I(m)), ctypes.byref(I(n)), ctypes.byref(I(k)),
            ctypes.byref(ctypes.c_double(1.0)), A.ctypes.data, ctypes.byref(I(ld)),
            B.ctypes.data, ctypes.byref(I(ld)), ctypes.byref(ctypes.c_double(0.0)),
            C.ctypes.data, ctypes.byref(I(ld)))
         best = min(best, time.perf_counter() - t0)
-    return {"value": round(2.0 * m * n * k / best / 1e12, 4), "unit": "TFLOP/s",
-            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)) if kind == "reference" else 1,
-            "kind": kind, "sample": sample, "seconds": round(best, 3)}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1")) if kind == "reference" else 1
+    out = {"value": round(2.0 * m * n * k / best / 1e12, 4), "unit": "TFLOP/s",
+           "cores": threads, "kind": kind, "sample": sample, "seconds": round(best, 3),
+           "cpu_model": info["model"], "physical_cores": info["physical_cores"],
+           "logical_cpus": info["logical_cpus"], "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "omp_places": os.environ.get("OMP_PLACES")}
+    if info["ghz"]:
+        # two 256-bit FMA pipes per core: 16 FP64 flops per cycle with AVX2
+        peak = threads * info["ghz"] * 16 / 1e3
+        out.update({"nominal_ghz": info["ghz"], "ghz_source": info["ghz_source"],
+                    "nominal_avx2_peak_tflops": round(peak, 4),
+                    "frac_of_nominal_peak": round(out["value"] / peak, 3)})
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline():
+    """The CPU baseline in a fresh child process (no GPU use), with
+    OMP_NUM_THREADS = the box's thread allowance (OMP_NUM_THREADS if set,
+    else the physical cores available), OMP_PROC_BIND=close,
+    OMP_PLACES=cores."""
+    import subprocess
+    info = cpu_info()
+    env = dict(os.environ)
+    env.setdefault("OMP_NUM_THREADS", str(max(1, min(info["physical_cores"],
+                                                      info["logical_cpus"]))))
+    env["OMP_PROC_BIND"] = "close"
+    env["OMP_PLACES"] = "cores"
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child"],
+                       env=env, capture_output=True, text=True, timeout=900)
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"value": None, "error": (r.stderr or r.stdout)[-400:]}
+    return json.loads(lines[-1])
 
 
 # ---------------------------------------------------------------- HBM traffic
@@ -212,8 +262,31 @@ def profiled_traffic(summary):
 
 
 # ---------------------------------------------------------------- headline
+def headline_samples():
+    """Per phi: the 2048 sampled entries (i, j) of C and the reference's own
+    dgemm_ values there, from the golden cases of the headline workload
+    (tests/golden, group emu8192, seed 3: made by the reference compiled from
+    its sources); None when the fixtures are absent."""
+    gold = os.path.join(ROOT, "tests", "golden")
+    try:
+        with open(os.path.join(gold, "manifest.json")) as f:
+            cases = json.load(f)
+        vals = np.fromfile(os.path.join(gold, "values.f64"), dtype="<f8")
+        idx = np.fromfile(os.path.join(gold, "index.i32"), dtype="<i4")
+    except OSError:
+        return None
+    out = {}
+    for c in cases:
+        if c["group"] != "emu8192" or c["seed"] != HEAD_SEED:
+            continue
+        ij = idx[c["ioff"]:c["ioff"] + 2 * c["count"]].reshape(-1, 2)
+        out[float(c["param"])] = (ij[:, 0].copy(), ij[:, 1].copy(),
+                                  vals[c["voff"]:c["voff"] + c["count"]].copy())
+    return out if all(p in out for p in PHIS) else None
+
+
 class Headline:
-    """8192^3 NN, ld 8192, alpha 1, beta 0 on phi-distributed inputs."""
+    """8192^3 NN, ld 8192, alpha 1, beta 0 on phi-distributed inputs (seed 3)."""
 
     def __init__(self, torch, gx, stream, slices=max(SLICES), moduli=max(MODULI)):
         self.torch, self.gx, self.stream = torch, gx, stream
@@ -228,10 +301,9 @@ class Headline:
 
     def set_phi(self, phi):
         gx, s = self.gx, self.stream
-        gx.fill(self.A, M, K, M, SEED, 0, 1, phi, stream=s)
-        gx.fill(self.B, K, N, K, SEED, 1, 1, phi, stream=s)
-        gx.fill(self.C, M, N, M, SEED, 2, stream=s)
-        self.C0 = self.C.clone()
+        gx.fill(self.A, M, K, M, HEAD_SEED, 0, 1, phi, stream=s)
+        gx.fill(self.B, K, N, K, HEAD_SEED, 1, 1, phi, stream=s)
+        gx.fill(self.C, M, N, M, HEAD_SEED, 2, stream=s)
         self.torch.cuda.synchronize()
 
     def native(self):
@@ -246,64 +318,103 @@ class Headline:
         self.gx.dgemm_ozaki2_dev("N", "N", M, N, K, 1.0, self.A, M, self.B, K, 0.0, self.C, M,
                                  nmod, self.ws, self.ws.numel(), stream=self.stream)
 
-    def error(self, fn, samples):
-        self.C.copy_(self.C0)
+    def reference(self, is_, js):
+        """Dot2 values and sum |a||b| at the pairs (is_, js), from the rows of
+        op(A) and columns of op(B) the GPU multiplies (beta = 0)."""
+        torch = self.torch
+        ui, ri = np.unique(is_, return_inverse=True)
+        uj, cj = np.unique(js, return_inverse=True)
+        rows = self.A.view(K, M)[:, torch.from_numpy(ui).cuda()].T.contiguous().cpu().numpy()
+        cols = self.B.view(N, K)[torch.from_numpy(uj).cuda()].contiguous().cpu().numpy()
+        self.flat = torch.from_numpy(is_.astype(np.int64) + js.astype(np.int64) * M).cuda()
+        return dot2_pairs(rows, cols, ri, cj)
+
+    def sampled(self, fn):
         fn()
         self.torch.cuda.synchronize()
-        return samples.error(self.torch, self.C, 1.0, 0.0)
+        return self.C[self.flat].cpu().numpy()
 
 
 def run_headline(torch, gx, stream, args):
-    """Native FP64 and the two emulated forms at every slice / modulus count:
-    event times on the phi = 0 inputs, errors at every phi.  The fastest
-    form whose relative Frobenius error is no larger than native's at every
-    phi is the headline; for an emulated winner its INT8 product phase is
-    also timed with the library's phase events (the roofline kernel)."""
+    """Native FP64 and the two emulated forms at every slice / modulus count,
+    each timed with HIP events over `steps` launches after `warmup` on the
+    phi = 0 inputs; errors at every phi on the 2048 sampled entries of the
+    golden cases against Dot2 values of the same device inputs (and against
+    the reference's own values).  A form qualifies when its relative
+    Frobenius error and its largest componentwise error (relative to
+    sum |a||b|) are both no larger than native FP64's at every phi; the
+    fastest of native and the qualifying forms is the headline.  For an
+    emulated winner its INT8 product phase is timed with the library's phase
+    events over the same launches (the roofline kernel)."""
     h = Headline(torch, gx, stream)
     flops = 2.0 * M * N * K
     wall_n, per_n = time_launches(torch, h.native, args.steps, args.warmup)
-    native = {"ms_event": median(per_n), "ms_wall": wall_n / args.steps, "error": {}}
+    native = {"ms_event": median(per_n), "ms_wall": wall_n / args.steps, "error": {},
+              "error_vs_reference": {}}
     forms = {}  # name -> (call, info)
     for s in (SLICES if not args.quick else (8,)):
         forms[f"ozaki_s{s}"] = (lambda s=s: h.ozaki(s), {"scheme": "ozaki1", "count": s})
-    for nm in (MODULI if not args.quick else (15, 16, 17, 18)):
+    for nm in (MODULI if not args.quick else (18, 19, 20)):
         forms[f"ozaki2_n{nm}"] = (lambda nm=nm: h.ozaki2(nm), {"scheme": "ozaki2", "count": nm,
                                                                "bits": gx.dgemm_ozaki2_bits(nm, K)})
     for name, (fn, info) in forms.items():
-        _, per = time_launches(torch, fn, max(3, args.steps // 2), 1)
+        wall, per = time_launches(torch, fn, args.steps, args.warmup)
         info["ms_event"] = median(per)
+        info["ms_wall"] = wall / args.steps
         info["tflops"] = round(flops / info["ms_event"] / 1e9, 3)
-        info["error"] = {}
+        info["error"], info["error_vs_reference"] = {}, {}
+    fixtures = headline_samples()
+    rng = np.random.default_rng(7)
     for phi in PHIS:
         h.set_phi(phi)
-        smp = Samples(torch, "N", "N", M, N, K, h.A, M, h.B, K, h.C, M, seed=int(phi) + 1)
-        native["error"][f"phi{phi:g}"] = h.error(h.native, smp)
+        if fixtures:
+            is_, js, refv = fixtures[phi]
+        else:
+            is_, js, refv = rng.integers(0, M, 2048), rng.integers(0, N, 2048), None
+        exact, ab = h.reference(is_, js)
+        key = f"phi{phi:g}"
+        got = h.sampled(h.native)
+        native["error"][key] = errors(got, exact, ab)
+        if refv is not None:
+            native["error_vs_reference"][key] = errors(got, refv, ab)
         for name, (fn, info) in forms.items():
-            info["error"][f"phi{phi:g}"] = h.error(fn, smp)
+            got = h.sampled(fn)
+            info["error"][key] = errors(got, exact, ab)
+            if refv is not None:
+                info["error_vs_reference"][key] = errors(got, refv, ab)
     h.set_phi(0.0)
     for name, (fn, info) in forms.items():
-        info["qualifies"] = all(info["error"][p]["rel_fro"] <= native["error"][p]["rel_fro"]
-                                for p in native["error"])
-    best = ("native", native["ms_event"])
+        info["qualifies_rel_fro"] = all(info["error"][p]["rel_fro"] <=
+                                        native["error"][p]["rel_fro"] for p in native["error"])
+        info["qualifies_max_comp"] = all(info["error"][p]["max_comp"] <=
+                                         native["error"][p]["max_comp"] for p in native["error"])
+        info["qualifies"] = info["qualifies_rel_fro"] and info["qualifies_max_comp"]
+    best = ("native", native["ms_event"], native["ms_wall"])
     for name, (fn, info) in forms.items():
         if info["qualifies"] and info["ms_event"] < best[1]:
-            best = (name, info["ms_event"])
+            best = (name, info["ms_event"], info["ms_wall"])
     phases = None
     if best[0] != "native":
         fn, info = forms[best[0]]
+        for _ in range(args.warmup):
+            fn()
         gx.set_phase_timing(True)
         times = []
-        for _ in range(max(3, args.steps // 2)):
+        for _ in range(args.steps):
             fn()
             times.append(gx.phase_times())
         gx.set_phase_timing(False)
         phases = {"split_ms": median([t[0] for t in times]),
                   "products_ms": median([t[1] for t in times]),
-                  "recombine_ms": median([t[2] for t in times])}
+                  "recombine_ms": median([t[2] for t in times]), "launches": args.steps}
         info["phases"] = phases
     out = {"native": dict(native, tflops=round(flops / native["ms_event"] / 1e9, 3)),
-           "emulated": {name: info for name, (fn, info) in forms.items()}}
-    return out, best, native["ms_wall"], phases
+           "emulated": {name: info for name, (fn, info) in forms.items()},
+           "samples": ("2048 entries per phi of the golden cases (reference dgemm_ values)"
+                       if fixtures else "2048 random entries per phi (golden cases absent)"),
+           "qualification": "rel_fro and max_comp (|error| / sum|a||b|) both <= native FP64's "
+                            "at every phi, against Dot2 values of the device inputs"}
+    return out, best, phases
 
 
 # ---------------------------------------------------------------- other workloads
@@ -360,6 +471,50 @@ def run_small(torch, gx, stream):
     torch.cuda.synchronize()
     return {"tflops": round(2.0 * n ** 3 / median(per) / 1e9, 2), "ms": round(median(per), 4),
             **smp.error(torch, C, 1.0, 0.0)}
+
+
+def run_few_tiles(torch, gx, stream, steps, warmup):
+    """Native FP64 and FP32 on shapes with few C tiles, where the library
+    splits K (split-K, the number of K ranges it picks is reported):
+    512 x 512 x 65536 in all four transpose pairs and 1024^3 NN, alpha 1.5,
+    beta -0.5, U[-1,1), ld = rows; the one-pass rate beside it."""
+    out = {}
+    for d, tdt, dev in (("f64", torch.float64, gx.dgemm_dev), ("f32", torch.float32, gx.sgemm_dev)):
+        dd = "d" if d == "f64" else "s"
+        res = {}
+        for (ta, tb, m, n, k) in (("N", "N", 512, 512, 65536), ("N", "T", 512, 512, 65536),
+                                  ("T", "N", 512, 512, 65536), ("T", "T", 512, 512, 65536),
+                                  ("N", "N", 1024, 1024, 1024)):
+            ra, ca = (m, k) if ta == "N" else (k, m)
+            rb, cb = (k, n) if tb == "N" else (n, k)
+            A = torch.empty(ra * ca, dtype=tdt, device="cuda")
+            B = torch.empty(rb * cb, dtype=tdt, device="cuda")
+            C = torch.empty(m * n, dtype=tdt, device="cuda")
+            for t, i, r, c in ((A, 0, ra, ca), (B, 1, rb, cb), (C, 2, m, n)):
+                gx.fill(t, r, c, r, SEED, i, stream=stream)
+            C0 = C.clone()
+
+            def f():
+                dev(ta, tb, m, n, k, 1.5, A, ra, B, rb, -0.5, C, m, stream=stream)
+            _, per = time_launches(torch, f, steps, warmup)
+            gx.set_split_config(dd, 1)
+            try:
+                _, per1 = time_launches(torch, f, steps, warmup)
+            finally:
+                gx.set_split_config(dd, -1)
+            smp = Samples(torch, ta, tb, m, n, k, A, ra, B, rb, C0, m, seed=6)
+            C.copy_(C0)
+            f()
+            torch.cuda.synchronize()
+            err = smp.error(torch, C, 1.5, -0.5)
+            res[f"{ta}{tb}_{m}x{n}x{k}"] = {
+                "tflops": round(2.0 * m * n * k / median(per) / 1e9, 2),
+                "ms": round(median(per), 4),
+                "splits": gx.tune_lookup_split(dd, ta, tb, m, n, k)[0],
+                "one_pass_tflops": round(2.0 * m * n * k / median(per1) / 1e9, 2), **err}
+            del A, B, C, C0
+        out[d] = res
+    return out
 
 
 def run_bf16x3(torch, gx, stream):
@@ -553,11 +708,15 @@ def run_complex(torch, gx, stream, steps, warmup):
 
 def run_host_pointer(torch, gx):
     """dgemm_ on host (pageable numpy) arrays at the headline shape: the wall
-    clock of the whole call, staging copies to and from the GPU included."""
-    import oracle
-    A = oracle.fill(M, K, M, SEED, 0, np.float64, 1, 0.0)
-    B = oracle.fill(K, N, K, SEED, 1, np.float64, 1, 0.0)
-    C = oracle.fill(M, N, M, SEED, 2)
+    clock of the whole call, staging copies to and from the GPU included.
+    The inputs are drawn on the device and copied to host arrays first."""
+    host = []
+    for rows, cols, mid, dist in ((M, K, 0, 1), (K, N, 1, 1), (M, N, 2, 0)):
+        t = torch.empty(rows * cols, dtype=torch.float64, device="cuda")
+        gx.fill(t, rows, cols, rows, HEAD_SEED, mid, dist, 0.0)
+        host.append(np.asfortranarray(t.cpu().numpy().reshape((rows, cols), order="F")))
+        del t
+    A, B, C = host
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
@@ -566,7 +725,8 @@ def run_host_pointer(torch, gx):
     t = min(times[1:])
     return {"ms_wall": round(t * 1e3, 2),
             "tflops_pcie_inclusive": round(2.0 * M * N * K / t / 1e12, 2),
-            "note": "pageable host arrays; H2D of A, B, C and D2H of C included"}
+            "note": "pageable host arrays; H2D of A, B, C and D2H of C included, best of "
+                    "the last two of three calls"}
 
 
 # ---------------------------------------------------------------- profiling entry
@@ -611,13 +771,38 @@ def profile_only(torch, gx, stream, args):
     del keep
 
 
+def profiled_kernel_ms(summary):
+    """Average duration of the dominant kernel recorded in a rocprofv3
+    summary under profiles/ (None when absent)."""
+    try:
+        with open(os.path.join(PROFILES, summary)) as f:
+            for line in f:
+                if line.startswith("Dominant kernel") and "average" in line:
+                    return float(line.split("average", 1)[1].split("ms")[0])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 # ---------------------------------------------------------------- main
 def main():
     args = parse()
+    if args.cpu_baseline_child:
+        cpu_baseline_child()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(every box here has one MI355X)", file=sys.stderr)
         sys.exit(2)
+    cpu = None
+    if not args.no_cpu and not args.profile_only:
+        # before this process touches the GPU: the child's OpenMP threads run alone
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # reported, never fatal for the GPU number
+            cpu = {"value": None, "error": str(e)}
+        print(f"bench: cpu baseline done ({cpu.get('value')} TFLOP/s)", file=sys.stderr,
+              flush=True)
     import torch
     import gemm_avx2_amd as gx
 
@@ -627,11 +812,14 @@ def main():
         return
 
     flops = 2.0 * M * N * K
-    head, (best_name, best_ms), native_wall, phases = run_headline(torch, gx, stream, args)
+    head, (best_name, best_ms, best_wall), phases = run_headline(torch, gx, stream, args)
+    print(f"bench: headline done ({best_name}, {best_ms:.3f} ms)", file=sys.stderr, flush=True)
     result = {"dgemm": head}
     if not args.quick:
         for key, fn in (("native_8192_ld8200", lambda: run_trans_and_ragged(torch, gx, stream)),
                         ("dgemm_1024", lambda: run_small(torch, gx, stream)),
+                        ("native_few_tiles", lambda: run_few_tiles(torch, gx, stream,
+                                                                   args.steps, args.warmup)),
                         ("sgemm_bf16x3", lambda: run_bf16x3(torch, gx, stream)),
                         ("batch", lambda: run_batch(torch, gx, stream, max(3, args.steps // 2), 1)),
                         ("strided_batch", lambda: run_strided(torch, gx, stream,
@@ -641,6 +829,7 @@ def main():
                         ("dgemm_host_pointers", lambda: run_host_pointer(torch, gx))):
             result[key] = fn()
             torch.cuda.empty_cache()
+            print(f"bench: {key} done", file=sys.stderr, flush=True)
 
     ev_n = head["native"]["ms_event"]
     line = {
@@ -650,20 +839,29 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(native_wall if best_name == "native" else best_ms, 3),
+        "ms_per_step": round(best_ms, 3),
+        "ms_per_step_wall": round(best_wall, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (counter-based generator, seed 1; A, B (u-0.5)exp(phi g), phi 0 timed)",
+        "data": "synthetic (counter-based generator, seed 3; A, B (u-0.5)exp(phi g), phi 0 "
+                "timed, phi 0/1/4 error-checked)",
         "config": {"workload": f"DGEMM {best_name} NN m=n=k=8192 ld=8192 alpha=1 beta=0",
                    "m": M, "n": N, "k": K, "ld": M, "alpha": 1.0, "beta": 0.0,
-                   "phi": list(PHIS), "mode": best_name},
+                   "phi": list(PHIS), "mode": best_name,
+                   "qualified_by": head["qualification"],
+                   "timing": f"median of {args.steps} HIP-event-timed launches after "
+                             f"{args.warmup} warm-up launches, inputs resident in HBM"},
         "roofline": {"bound": "mfma", "achieved": round(flops / ev_n / 1e9, 3),
                      "peak": PEAK["f64"], "unit": "TFLOP/s",
                      "frac": round(flops / ev_n / 1e9 / PEAK["f64"], 4),
                      "traffic": profiled_traffic("dgemm_native_nn_8192.md"),
-                     "kernel": "gemm_native_kernel<double> (native FP64 MFMA)"},
+                     "kernel": "gemm_native_kernel<double> (native FP64 MFMA)",
+                     "kernel_ms": round(ev_n, 4),
+                     "kernel_ms_source": "HIP events around each launch on the launch "
+                                         "stream, this run (the call is this one kernel)",
+                     "profiled_kernel_ms": profiled_kernel_ms("dgemm_native_nn_8192.md")},
     }
     if best_name != "native":
         info = head["emulated"][best_name]
@@ -672,22 +870,22 @@ def main():
         # or one per modulus (Ozaki-II), each 2mnk operations
         ops = (cnt * (cnt + 1) / 2 if info["scheme"] == "ozaki1" else cnt) * flops
         pm = phases["products_ms"]
+        prof = f"dgemm_{info['scheme']}_{cnt}_nn_8192.md"
         line["roofline"] = {"bound": "mfma", "achieved": round(ops / pm / 1e9, 1),
                             "peak": PEAK["i8"], "unit": "TOP/s",
                             "frac": round(ops / pm / 1e9 / PEAK["i8"], 4),
-                            "traffic": profiled_traffic(
-                                f"dgemm_{info['scheme']}_{cnt}_nn_8192.md"),
-                            "kernel": ("slice_products_dl2_kernel (INT8 MFMA, direct-to-LDS)"
-                                       if info["scheme"] == "ozaki2" else
-                                       "slice_products_dl2_kernel (INT8 MFMA, direct-to-LDS)")
-                                      + ", product phase timed with HIP events in the library",
-                            "kernel_ms": round(pm, 4)}
+                            "traffic": profiled_traffic(prof),
+                            "kernel": "slice_products_dl2_kernel<%d> (INT8 MFMA, direct-to-LDS)"
+                                      % (2 if info["scheme"] == "ozaki2" else 1),
+                            "kernel_ms": round(pm, 4),
+                            "kernel_ms_source": "HIP events the library records around its "
+                                                "product phase (one launch of this kernel) on "
+                                                "the call's stream, median of %d calls, this run"
+                                                % phases["launches"],
+                            "profiled_kernel_ms": profiled_kernel_ms(prof)}
     line.update(result)
-    if not args.no_cpu:
-        try:
-            line["cpu_baseline"] = cpu_baseline()
-        except Exception as e:  # reported, never fatal for the GPU number
-            line["cpu_baseline"] = {"value": None, "error": str(e)}
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 

@@ -28,7 +28,8 @@ __all__ = [
     "sgemm_dev", "dgemm_strided_batched_dev", "sgemm_strided_batched_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
-    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "set_product_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
+    "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "set_split_config",
+    "tune_lookup_split", "build_rocm", "set_product_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
     "BatchPlan", "set_batch_config", "batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2", "MODE_AUTO",
 ]
 
@@ -532,6 +533,36 @@ def set_tile_config(dtype: str, cfg: int):
         _check(dtype, rc)
 
 
+def set_split_config(dtype: str, split: int):
+    """Forces `split` K ranges (split-K, 1 = one pass) for every native call
+    ('d' or 's'); -1 returns to the tuning table / heuristic."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    rc = getattr(lib, f"{dtype}gemm_x_set_split_config")(split)
+    if rc:
+        _check(dtype, rc)
+
+
+def tune_lookup_split(dtype, transa, transb, m, n, k):
+    """(K ranges, partial-tile bytes) of a native call with these arguments
+    (1 range = one pass)."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    f = getattr(lib, f"{dtype}gemm_x_tune_lookup_split")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_size_t)]
+    b = ctypes.c_size_t()
+    return int(f(_ch(transa), _ch(transb), m, n, k, ctypes.byref(b))), int(b.value)
+
+
+def build_rocm(dtype: str = "d") -> str:
+    """"major.minor" of the ROCm release the library was built with (the
+    `rocm` key of the tuning tables it accepts)."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    f = getattr(lib, f"{dtype}gemm_x_build_rocm")
+    f.restype = ctypes.c_char_p
+    return f().decode()
+
+
 def set_product_config(dtype: str, cfg: int):
     """Product kernel of the emulated forms in the library of `dtype` ('d':
     Ozaki-I and Ozaki-II, 's': BF16x3): 1 direct-to-LDS (default), 0
@@ -544,7 +575,8 @@ def set_product_config(dtype: str, cfg: int):
 
 def load_tuning(path, dtype: str = "d") -> int:
     """Loads a tuning table into the library of `dtype`; returns entries read,
-    or -1 when the file is unreadable or of another format version or arch."""
+    or -1 when the file is unreadable, of another format version, arch or
+    ROCm release, or names no ROCm release."""
     lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
     return int(getattr(lib, f"{dtype}gemm_x_load_tuning")(
         None if path is None else str(path).encode()))

@@ -17,8 +17,10 @@ is written, after its result is checked against configuration 0's on the same
 inputs (|difference| within 64 k u (|alpha| k + |beta|), the bound for
 U[-1,1) inputs).  For DGEMM shapes with min(m, n, k) >= 1024 it also picks
 the arithmetic of dgemm_'s automatic mode: Ozaki-II with the smallest
-modulus count whose error matches native FP64's on phi in {0, 1, 4} inputs,
-when that is faster than native (an `emu` entry), and the grouped-batch
+modulus count whose error is no larger than native FP64's on phi in
+{0, 1, 4} inputs (relative Frobenius and largest componentwise error over
+2048 sampled entries, two seeds), when that is faster than native (an `emu`
+entry), and the grouped-batch
 kernel configuration per element type on a mixed batch (a `batch` entry).
 With --resume, keys already
 in the output table are kept and not measured again.  The table is written in
@@ -33,7 +35,7 @@ import re
 import sys
 import time
 
-FORMAT = 1
+FORMAT = 2
 TILE_CONFIGS = 7  # compiled native tile configurations (GEMMX_TILE_CONFIGS)
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
                            "gemm_mi355x_tune.txt")
@@ -63,7 +65,10 @@ def key_of(dtype: str, trans: str, m: int, n: int, k: int):
 
 
 # ------------------------------------------------------------------ table I/O
-_ENTRY = re.compile(r"^\s*(f64|f32)\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)\s*(#.*)?$")
+_ENTRY = re.compile(r"^\s*(f64|f32)\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)"
+                    r"(?:\s+(\d+))?\s*(#.*)?$")
+MAX_SPLIT = 64  # GEMMX_MAX_SPLIT
+SPLIT_CONFIGS = (0, 1, 5)  # native tile configurations with split-K kernels
 _EMU = re.compile(r"^\s*emu\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(native|ozaki|ozaki2)"
                   r"\s+(\d+)\s*(#.*)?$")
 _EMU_RANGE = {"native": (0, 0), "ozaki": (3, 8), "ozaki2": (8, 22)}
@@ -76,11 +81,12 @@ def emu_key(trans, m, n, k):
 
 
 def read_table(path):
-    """(entries {key: cfg}, comments {key: text}, arch or None); later lines
-    replace earlier ones, like the libraries' reader."""
-    entries, notes, arch = {}, {}, None
+    """(entries, comments, arch or None, rocm or None); later lines replace
+    earlier ones, like the libraries' reader.  A tile entry's value is its
+    configuration, or (configuration, split) when the line gives a split."""
+    entries, notes, arch, rocm = {}, {}, None, None
     if not os.path.exists(path):
-        return entries, notes, arch
+        return entries, notes, arch, rocm
     with open(path) as f:
         for line in f:
             body = line.split("#", 1)[0].split()
@@ -88,6 +94,9 @@ def read_table(path):
                 raise ValueError(f"{path}: table format {body[1]}, expected {FORMAT}")
             if len(body) == 2 and body[0] == "arch":
                 arch = body[1]
+                continue
+            if len(body) == 2 and body[0] == "rocm":
+                rocm = body[1]
                 continue
             m = _BATCH.match(line)
             if m:
@@ -109,23 +118,31 @@ def read_table(path):
             if m:
                 key = (m.group(1), m.group(2), int(m.group(3)), int(m.group(4)), int(m.group(5)))
                 cfg = int(m.group(6))
+                split = int(m.group(7)) if m.group(7) is not None else None
                 # the libraries skip entries outside the buckets or configurations
                 if not all(4 <= b <= 15 for b in key[2:]) or not 0 <= cfg < TILE_CONFIGS:
                     continue
-                entries[key] = cfg
-                notes[key] = (m.group(7) or "").lstrip("# ").rstrip()
-    return entries, notes, arch
+                if split is not None and not 1 <= split <= MAX_SPLIT:
+                    continue
+                entries[key] = cfg if split is None else (cfg, split)
+                notes[key] = (m.group(8) or "").lstrip("# ").rstrip()
+    return entries, notes, arch, rocm
 
 
-def write_table(path, entries, notes, arch, header=()):
+def write_table(path, entries, notes, arch, header=(), rocm=None):
     """Writes the table in sorted key order (the same entries always give
-    the same file)."""
+    the same file).  rocm: the release key, by default that of the libraries
+    this package loads."""
+    if rocm is None:
+        import gemm_avx2_amd as gx
+        rocm = gx.build_rocm()
     lines = ["# gemm_mi355x tuning table, written by gemm_avx2_amd.autotune"]
     lines += [f"# {h}" for h in header]
     lines += [f"format {FORMAT}"]
     if arch:
         lines += [f"arch {arch}"]
-    lines += ["# dtype trans mbucket nbucket kbucket cfg",
+    lines += [f"rocm {rocm}"]
+    lines += ["# dtype trans mbucket nbucket kbucket cfg [split]",
               "# emu trans mbucket nbucket kbucket scheme count   (DGEMM arithmetic, auto mode)",
               "# batch dtype cfg   (grouped-batch kernel)"]
     order = {"f64": 0, "f32": 1, "emu": 2}
@@ -133,7 +150,8 @@ def write_table(path, entries, notes, arch, header=()):
     for key in sorted(tiles, key=lambda k: (order[k[0]], TRANS.index(k[1]), k[2], k[3], k[4])):
         note = notes.get(key, "")
         val = entries[key]
-        val = f"{val[0]} {val[1]}" if key[0] == "emu" else str(val)
+        if isinstance(val, tuple):
+            val = f"{val[0]} {val[1]}"
         lines.append(f"{key[0]} {key[1]} {key[2]} {key[3]} {key[4]} {val}"
                      + (f"  # {note}" if note else ""))
     for key in sorted(k for k in entries if k[0] == "batch"):
@@ -166,11 +184,36 @@ def _median(xs):
     return xs[n // 2] if n % 2 else 0.5 * (xs[n // 2 - 1] + xs[n // 2])
 
 
+TILE_DIMS = ((128, 128), (64, 64), (256, 128), (128, 128), (128, 128), (128, 128), (256, 128))
+
+
+def split_candidates(cfg, m, n, k, es):
+    """Split factors worth timing for configuration cfg: powers of two from
+    2 while the tiles and their K ranges give at most 4096 workgroups, each
+    K range at least 64 deep, and the partial tiles fit in 256 MiB; none for
+    a configuration without split-K kernels or with enough tiles already."""
+    if cfg not in SPLIT_CONFIGS:
+        return []
+    bm, bn = TILE_DIMS[cfg]
+    tiles = -(-m // bm) * -(-n // bn)
+    if tiles >= 1024:
+        return []
+    out, sp = [], 2
+    while (sp <= MAX_SPLIT and tiles * sp <= 4096 and k // sp >= 64
+           and sp * tiles * bm * bn * es <= (256 << 20)):
+        out.append(sp)
+        sp *= 2
+    return out
+
+
 def measure(torch, gx, dtype, trans, m, n, k, reps):
-    """Median event time (ms) of every configuration, and the index of the
-    fastest whose result agrees with configuration 0's."""
+    """Median event time (ms) of every configuration in one pass and, for
+    the configurations with split-K kernels, of every candidate split factor
+    (split_candidates), keyed (cfg, split); which results agree with
+    configuration 0's one-pass result; and the fastest agreeing (cfg, split)."""
     tdt = torch.float64 if dtype == "d" else torch.float32
     u = 2.0 ** -53 if dtype == "d" else 2.0 ** -24
+    es = 8 if dtype == "d" else 4
     ra, ca = (m, k) if trans[0] == "N" else (k, m)
     rb, cb = (k, n) if trans[1] == "N" else (n, k)
     A = torch.empty(ra * ca, dtype=tdt, device="cuda")
@@ -186,19 +229,22 @@ def measure(torch, gx, dtype, trans, m, n, k, reps):
     def call():
         dev(trans[0], trans[1], m, n, k, ALPHA, A, ra, B, rb, BETA, C, m, stream=stream)
 
-    times, ref, ok = [], None, []
+    times, ok, ref = {}, {}, None
     bound = 64 * k * u * (abs(ALPHA) * k + abs(BETA))
+    runs = [(c, 1) for c in range(gx.TILE_CONFIGS)]
+    runs += [(c, sp) for c in SPLIT_CONFIGS for sp in split_candidates(c, m, n, k, es)]
     try:
-        for cfg in range(gx.TILE_CONFIGS):
+        for cfg, sp in runs:
             gx.set_tile_config(dtype, cfg)
+            gx.set_split_config(dtype, sp)
             C.copy_(C0)
             call()
             torch.cuda.synchronize()
             if ref is None:
                 ref = C.clone()
-                ok.append(True)
+                ok[(cfg, sp)] = True
             else:
-                ok.append(bool((C - ref).abs().max().item() <= 2 * bound))
+                ok[(cfg, sp)] = bool((C - ref).abs().max().item() <= 2 * bound)
             ev = []
             for _ in range(reps):
                 e0 = torch.cuda.Event(enable_timing=True)
@@ -208,11 +254,18 @@ def measure(torch, gx, dtype, trans, m, n, k, reps):
                 e1.record(stream)
                 e1.synchronize()
                 ev.append(e0.elapsed_time(e1))
-            times.append(_median(ev))
+            times[(cfg, sp)] = _median(ev)
     finally:
         gx.set_tile_config(dtype, -1)
-    best = min((t, c) for c, t in enumerate(times) if ok[c])[1]
+        gx.set_split_config(dtype, -1)
+    best = min((t, key) for key, t in times.items() if ok[key])[1]
     return times, ok, best
+
+
+def _times_note(m, n, k, times, ok):
+    return f"{m}x{n}x{k} ms: " + " ".join(
+        (f"c{c}" if sp == 1 else f"c{c}s{sp}") + f"={tm:.3f}" + ("" if ok[(c, sp)] else "!")
+        for (c, sp), tm in sorted(times.items()))
 
 
 # ------------------------------------------------------------------ grouped batch
@@ -285,7 +338,9 @@ def measure_batch(torch, gx, dtype, reps, mix=BATCH_MIX, seed=11):
 
 # ------------------------------------------------------------------ emulated DGEMM
 PHIS = (0.0, 1.0, 4.0)
-MODULI = (14, 15, 16, 17, 18, 20, 22)
+MODULI = tuple(range(14, 23))
+EMU_SEEDS = (41, 43)
+EMU_SAMPLES = 2048
 
 
 def _two_prod(a, b):
@@ -302,25 +357,44 @@ def _two_prod(a, b):
     return p, ((ah * bh - p) + ah * bl + al * bh) + al * bl
 
 
-def _exact_dots(arows, bcols):
-    """Correctly rounded dot products of every row of arows with every column
-    of bcols (error-free products summed with math.fsum)."""
-    import math
+def dot2_pairs(rows, cols, ri, cj, alpha=1.0, c1=0.0):
+    """alpha * (rows[ri[t]] . cols[cj[t]]) + c1[t] for every pair t with Dot2
+    (TwoProduct and TwoSum: as accurate as twice the working precision, then
+    rounded once), vectorised over the pairs; also sum |a||b| per pair.
+    rows (nr, k) and cols (nc, k) are float64 numpy arrays."""
     import numpy as np
-    out = np.empty((arows.shape[0], bcols.shape[1]))
-    for i in range(arows.shape[0]):
-        for j in range(bcols.shape[1]):
-            p, e = _two_prod(arows[i], bcols[:, j])
-            out[i, j] = math.fsum(np.concatenate([p, e]))
-    return out
+    X = np.ascontiguousarray(rows[ri].T)  # (k, count)
+    Y = np.ascontiguousarray(cols[cj].T)
+    s = np.broadcast_to(np.asarray(c1, dtype=np.float64), (X.shape[1],)).copy()
+    c = np.zeros_like(s)
+    ab = np.zeros_like(s)
+    for p in range(X.shape[0]):
+        h, e = _two_prod(X[p], Y[p] * alpha)
+        t = s + h
+        z = t - s
+        c += ((s - (t - z)) + (h - z)) + e
+        s = t
+        ab += np.abs(X[p] * Y[p])
+    return s + c, ab
 
 
-def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI):
+def sample_errors(got, exact, ab):
+    """Relative Frobenius error and the largest error relative to
+    sum |a||b| (componentwise) over sampled entries."""
+    import numpy as np
+    err = np.abs(np.asarray(got, dtype=np.float64) - exact)
+    return {"rel_fro": float(np.linalg.norm(err) / max(np.linalg.norm(exact), 1e-300)),
+            "max_comp": float(np.max(err / (ab + 1e-300)))}
+
+
+def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_SEEDS,
+                     samples=EMU_SAMPLES):
     """DGEMM arithmetic for one shape: native FP64, or Ozaki-II with the
-    smallest modulus count whose relative error (12 x 12 sampled entries
-    against exactly rounded dot products, alpha = 1, beta = 0) is no larger
-    than native's for entries (u - 0.5) exp(phi g), phi in {0, 1, 4}, if it
-    is faster.  Returns (scheme, count, note)."""
+    smallest modulus count whose error at `samples` random entries (alpha 1,
+    beta 0, against Dot2 values of the same inputs) is no larger than
+    native's -- in relative Frobenius norm and in the largest error relative
+    to sum |a||b| -- for entries (u - 0.5) exp(phi g), phi in {0, 1, 4}, on
+    every seed, if it is faster.  Returns (scheme, count, note)."""
     import numpy as np
     ra, ca = (m, k) if trans[0] == "N" else (k, m)
     rb, cb = (k, n) if trans[1] == "N" else (n, k)
@@ -330,9 +404,6 @@ def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI):
     ws = torch.empty(gx.dgemm_ozaki2_workspace(m, n, k, max(moduli)), dtype=torch.uint8,
                      device="cuda")
     stream = torch.cuda.current_stream()
-    rng = np.random.default_rng(3)
-    rows = torch.tensor(sorted(rng.choice(m, min(12, m), replace=False).tolist()), device="cuda")
-    cols = torch.tensor(sorted(rng.choice(n, min(12, n), replace=False).tolist()), device="cuda")
 
     def native():
         gx.dgemm_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, stream=stream)
@@ -354,26 +425,34 @@ def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI):
             ev.append(e0.elapsed_time(e1))
         return _median(ev)
 
-    def error(fn, ref):
-        fn()
-        torch.cuda.synchronize()
-        got = C.view(-1, m)[cols][:, rows].T.double().cpu().numpy()
-        return float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300))
+    qual = set(moduli)
+    for seed in seeds:
+        rng = np.random.default_rng(seed)
+        is_, js = rng.integers(0, m, samples), rng.integers(0, n, samples)
+        ui, ri = np.unique(is_, return_inverse=True)
+        uj, cj = np.unique(js, return_inverse=True)
+        tui, tuj = torch.from_numpy(ui).cuda(), torch.from_numpy(uj).cuda()
+        flat = torch.from_numpy(is_.astype(np.int64) + js.astype(np.int64) * m).cuda()
+        for phi in PHIS:
+            gx.fill(A, ra, ca, ra, seed, 0, 1, phi)
+            gx.fill(B, rb, cb, rb, seed, 1, 1, phi)
+            torch.cuda.synchronize()
+            # op(A) rows: A(i, :) of an m x k (N) or A(:, i) of a k x m (T) array
+            rows = A.view(k, m)[:, tui].T if trans[0] == "N" else A.view(m, k)[tui]
+            cols = B.view(n, k)[tuj] if trans[1] == "N" else B.view(k, n)[:, tuj].T
+            exact, ab = dot2_pairs(rows.contiguous().cpu().numpy(),
+                                   cols.contiguous().cpu().numpy(), ri, cj)
 
-    errs_nat, errs = [], {nm: [] for nm in moduli}
-    for phi in PHIS:
-        gx.fill(A, ra, ca, ra, 41, 0, 1, phi)
-        gx.fill(B, rb, cb, rb, 41, 1, 1, phi)
-        torch.cuda.synchronize()
-        A2, B2 = A.view(-1, ra), B.view(-1, rb)
-        arows = (A2[:k, rows].T if trans[0] == "N" else A2[rows, :k]).double().cpu().numpy()
-        bcols = (B2[cols, :k].T if trans[1] == "N" else B2[:k, cols]).double().cpu().numpy()
-        ref = _exact_dots(arows, bcols)
-        errs_nat.append(error(native, ref))
-        for nm in moduli:
-            errs[nm].append(error(lambda nm=nm: emulated(nm), ref))
+            def err(fn):
+                fn()
+                torch.cuda.synchronize()
+                return sample_errors(C[flat].cpu().numpy(), exact, ab)
+            en = err(native)
+            for nm in sorted(qual):
+                e = err(lambda nm=nm: emulated(nm))
+                if e["rel_fro"] > en["rel_fro"] or e["max_comp"] > en["max_comp"]:
+                    qual.discard(nm)
     t_nat = timed(native)
-    qual = [nm for nm in moduli if all(e <= en for e, en in zip(errs[nm], errs_nat))]
     if not qual:
         return "native", 0, f"{m}x{n}x{k} native {t_nat:.3f} ms; no modulus count matches its error"
     nm = min(qual)
@@ -408,11 +487,16 @@ def main(argv=None):
     arch = getattr(props, "gcnArchName", "unknown").split(":")[0]
     shapes = ([tuple(int(v) for v in s.split(",")) for s in args.shapes.split(";")]
               if args.shapes else SHAPES[args.set])
-    entries, notes, old_arch = read_table(args.out) if args.resume else ({}, {}, None)
+    entries, notes, old_arch, old_rocm = (read_table(args.out) if args.resume
+                                          else ({}, {}, None, None))
     if old_arch and old_arch != arch:
         raise SystemExit(f"autotune: {args.out} is for {old_arch}, this device is {arch}")
+    if old_rocm and old_rocm != gx.build_rocm():
+        raise SystemExit(f"autotune: {args.out} is for ROCm {old_rocm}, the libraries are built "
+                         f"with {gx.build_rocm()}")
     header = [f"device: {props.name}, {props.multi_processor_count} CUs, arch {arch}",
-              f"hip: {torch.version.hip}; torch: {torch.__version__}; host: {platform.node()}",
+              f"libraries built with ROCm {gx.build_rocm()}; torch {torch.__version__} "
+              f"(HIP {torch.version.hip}); host: {platform.node()}",
               f"written: {time.strftime('%Y-%m-%d %H:%M:%S')}; set: {args.shapes or args.set}; "
               f"reps: {args.reps}"]
     wanted = []
@@ -425,11 +509,10 @@ def main(argv=None):
         times, ok, best = measure(torch, gx, d, t, m, n, k, args.reps)
         key = key_of(d, t, m, n, k)
         entries[key] = best
-        notes[key] = f"{m}x{n}x{k} ms: " + " ".join(
-            f"c{c}={tm:.3f}" + ("" if ok[c] else "!") for c, tm in enumerate(times))
+        notes[key] = _times_note(m, n, k, times, ok)
         write_table(args.out, entries, notes, arch, header)  # after every shape: resumable
-        print(f"[{i + 1}/{len(todo)}] {key[0]} {t} {m}x{n}x{k}: cfg {best} "
-              f"({times[best]:.3f} ms; cfg0 {times[0]:.3f} ms)", flush=True)
+        print(f"[{i + 1}/{len(todo)}] {key[0]} {t} {m}x{n}x{k}: cfg {best[0]} split {best[1]} "
+              f"({times[best]:.3f} ms; cfg0 {times[(0, 1)]:.3f} ms)", flush=True)
     if args.emulated == "on" and "d" in args.dtypes.split(","):
         etodo, seen = [], set()
         for t in args.trans.split(","):

@@ -31,8 +31,10 @@ BUILD = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["gemm_native.hip", "gemm_split.hip", "gemm_products.hip", "gemm_emulated.hip",
-               "gemm_batch.hip", "gemm_complex.hip", "gemm_tri.hip"]
+HIP_SOURCES = ["gemm_native.hip"] + [f"gemm_native_{t}_{tr}.hip" for t in ("f64", "f32")
+                                      for tr in ("nn", "nt", "tn", "tt")] + [
+    "gemm_split.hip", "gemm_products.hip", "gemm_emulated.hip", "gemm_batch.hip",
+    "gemm_complex.hip", "gemm_tri.hip"]
 C_SOURCES = ["gemmx_api.c", "gemmx_tune.c", "gemmx_batch.c", "gemmx_stage.c"]
 GEN_SOURCES = ["gemmx_gen.hip"]
 # tuning table measured on MI355X by gemm_avx2_amd.autotune (--set full)

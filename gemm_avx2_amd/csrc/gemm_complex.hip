@@ -129,7 +129,8 @@ hipError_t real_gemm(int ta, int tb, int m, int n, int k, double alpha, const T 
   q.c = c;
   constexpr int es = sizeof(T);
   const int cfg = gemmx_tune_pick(es, ta, tb, m, n, k);
-  return es == 8 ? gemmx_native_f64(&q, cfg, s) : gemmx_native_f32(&q, cfg, s);
+  return es == 8 ? gemmx_native_f64(&q, cfg, 1, NULL, s)
+                 : gemmx_native_f32(&q, cfg, 1, NULL, s);
 }
 
 template <typename T>

@@ -30,6 +30,18 @@
  * row blockIdx.y takes problem (batch base + blockIdx.y), its operands offset
  * by the per-problem strides.
  *
+ * Split-K (a single problem with few tiles; the reference instead divides
+ * every call's N range over all its threads, src/gemm_driver.c:128-151):
+ * grid row q computes the K range [q kc, (q+1) kc) of every tile (kc whole
+ * BK slices) and stores the raw accumulators, in the registers' fragment
+ * order, to a workspace of partial tiles; splitk_reduce_kernel then sums the
+ * ranges in order q = 0, 1, ... (so the result does not depend on which
+ * workgroup finished first) and writes C = alpha*sum + beta*C.
+ *
+ * The kernels and launchers are in gemm_native_impl.h, instantiated by one
+ * translation unit per element type and transpose pair
+ * (gemm_native_{f64,f32}_{nn,nt,tn,tt}.hip); this file dispatches.
+ *
  * Every workgroup runs a branch-free loop over the full K slices (the K
  * remainder, if any, is one peeled predicated slice); an edge tile's loads
  * past M or N are clamped back into the operand and its stores masked.
@@ -39,61 +51,28 @@
  * loads first, each MFMA step's fragment reads spread over the previous
  * step's MFMAs, and the next slice's LDS writes spread over the last step.
  */
-#include <type_traits>
-
-#include "gemmx_device.h"
-#include "gemmx_internal.h"
-#include "gemmx_tile.h"
+#include "gemm_native_impl.h"
 
 using namespace gemmx;
 
-namespace {
+extern "C" hipError_t gemmx_native_f64_nn(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
+extern "C" hipError_t gemmx_native_f64_nt(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
+extern "C" hipError_t gemmx_native_f64_tn(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
+extern "C" hipError_t gemmx_native_f64_tt(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
+extern "C" hipError_t gemmx_native_f32_nn(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
+extern "C" hipError_t gemmx_native_f32_nt(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
+extern "C" hipError_t gemmx_native_f32_tn(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
+extern "C" hipError_t gemmx_native_f32_tt(const gemmx_problem *p, int cfg, int split, void *work,
+                                      hipStream_t s);
 
-/* tiles_m x tiles_n are the interior tiles, all inside C.  The first `lead`
- * workgroups take the edge tiles -- the partial tile-row tm = tiles_m (when
- * M % BM != 0, tiles_n + (N % BN != 0) tiles) then the partial tile-column
- * tn = tiles_n for tm < tiles_m -- on the branch-free path with loads clamped
- * into the operands; the rest take the interior tiles.  NARROW: operands
- * that do not allow 16-byte loads, loaded one element per load.  Dispatched first, the edge tiles run beside the interior
- * ones instead of in a trailing wave of their own.  Interior tile ids start
- * at `lead`, which shifts every workgroup's XCD (blockIdx.x % 8) by the same
- * amount, so the workgroups of one XCD still share tile rows and columns.
- * The grid holds exactly one workgroup per tile: a padding workgroup would
- * give one XCD an extra tile and, when the tiles fill whole waves, a wave of
- * its own. */
-template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW,
-          int KVMAX, bool NARROW>
-__global__ void __launch_bounds__(64 * WM * WN, MINW)
-gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
-                   T *__restrict__ C, int M, int N, int K, int64_t lda,
-                   int64_t ldb, int64_t ldc, T alpha, T beta, int tiles_m,
-                   int tiles_n, int vecA, int vecB, int lead, int64_t sa, int64_t sb,
-                   int64_t sc) {
-  __shared__ __align__(16) T lds[tile_lds_elems<T, TA, TB, BM, BN, BK>()];
-  A += blockIdx.y * sa;
-  B += blockIdx.y * sb;
-  C += blockIdx.y * sc;
-  int tm, tn;
-  if ((int)blockIdx.x < lead) {
-    const int row_tiles = (M % BM) ? tiles_n + ((N % BN) ? 1 : 0) : 0;
-    const int col_tiles = (N % BN) ? tiles_m : 0;
-    const int e = blockIdx.x;
-    if (e >= row_tiles + col_tiles) return; /* lead == row_tiles + col_tiles */
-    if (e < row_tiles) {
-      tm = tiles_m;
-      tn = e;
-    } else {
-      tm = e - row_tiles;
-      tn = tiles_n;
-    }
-    gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, true, NARROW>(
-        A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN, vecA, vecB, lds);
-    return;
-  }
-  tile_coords(blockIdx.x - lead, gridDim.x - lead, tiles_m, tiles_n, &tm, &tn);
-  gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, false, NARROW>(
-      A, B, C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN, vecA, vecB, lds);
-}
+namespace {
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -104,69 +83,6 @@ scale_kernel(T *__restrict__ C, int64_t ldc, int M, int N, T beta, int64_t sc) {
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t j = t / M, i = t - j * M;
     C[i + j * ldc] *= beta;
-  }
-}
-
-/* One launch covers C: its leading workgroups the edge tiles (clamped
- * loads), the rest the interior tiles; 16-byte loads where both operands
- * allow them, else one element per load.  A strided batch goes in grid rows
- * of at most GEMMX_GRID_Y problems per launch. */
-constexpr int GEMMX_GRID_Y = 65535;
-
-template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
-hipError_t launch_trans(const gemmx_problem *p, hipStream_t s) {
-  constexpr int VE = Vec16<T>::N;
-  const int batch = p->batch > 1 ? p->batch : 1;
-  const int64_t sa = batch > 1 ? p->sa : 0, sb = batch > 1 ? p->sb : 0;
-  const int64_t sc = batch > 1 ? p->sc : 0;
-  const int vecA = ((uintptr_t)p->a % 16 == 0) && (p->lda % VE == 0) && (sa % VE == 0);
-  const int vecB = ((uintptr_t)p->b % 16 == 0) && (p->ldb % VE == 0) && (sb % VE == 0);
-  const T alpha = (T)p->alpha, beta = (T)p->beta;
-  const dim3 block(64 * WM * WN);
-  const int tiles_m = (p->m + BM - 1) / BM, tiles_n = (p->n + BN - 1) / BN;
-  const int full_m = p->m / BM, full_n = p->n / BN;
-  for (int b0 = 0; b0 < batch; b0 += GEMMX_GRID_Y) {
-    const unsigned nb = (unsigned)(batch - b0 < GEMMX_GRID_Y ? batch - b0 : GEMMX_GRID_Y);
-    const T *A = (const T *)p->a + b0 * sa, *B = (const T *)p->b + b0 * sb;
-    T *C = (T *)p->c + b0 * sc;
-    const int lead = tiles_m * tiles_n - full_m * full_n; /* edge tiles */
-    const dim3 grid(lead + full_m * full_n, nb);
-    if (vecA && vecB)
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false>),
-                         grid, block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb, p->ldc,
-                         alpha, beta, full_m, full_n, vecA, vecB, lead, sa, sb, sc);
-    else
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true>),
-                         grid, block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb, p->ldc,
-                         alpha, beta, full_m, full_n, vecA, vecB, lead, sa, sb, sc);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-
-template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX>
-hipError_t launch_cfg(const gemmx_problem *p, hipStream_t s) {
-  if (!p->ta && !p->tb) return launch_trans<T, false, false, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
-  if (!p->ta && p->tb) return launch_trans<T, false, true, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
-  if (p->ta && !p->tb) return launch_trans<T, true, false, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
-  return launch_trans<T, true, true, BM, BN, BK, WM, WN, MINW, KVMAX>(p, s);
-}
-
-template <typename T>
-hipError_t launch_native(const gemmx_problem *p, int cfg, hipStream_t s) {
-  /* BM, BN, BK, waves (M x N), min waves per SIMD, K steps per LDS read */
-  switch (cfg) {
-  case 1: return launch_cfg<T, 64, 64, 16, 2, 2, 2, 1>(p, s);
-  case 2: return launch_cfg<T, 256, 128, 16, 4, 2, 2, 1>(p, s);
-  /* FP32 128x128 with K-vector reads fits two workgroups per CU's register
-   * file; FP64 (twice the accumulator registers) takes one */
-  case 3: return launch_cfg<T, 128, 128, 32, 2, 2, std::is_same<T, float>::value ? 2 : 1, 4>(p, s);
-  case 4: return launch_cfg<T, 128, 128, 16, 2, 2, 1, 4>(p, s);
-  case 5: return launch_cfg<T, 128, 128, 16, 4, 2, 2, 1>(p, s);
-  case 6: return launch_cfg<T, 256, 128, 16, 4, 2, 1, 4>(p, s);
-  case 0:
-  default: return launch_cfg<T, 128, 128, 16, 2, 2, 2, 1>(p, s);
   }
 }
 
@@ -190,11 +106,38 @@ hipError_t launch_scale(T *c, int64_t ldc, int m, int n, T beta, int batch, int6
 
 } // namespace
 
-extern "C" hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, hipStream_t s) {
-  return launch_native<double>(p, cfg, s);
+extern "C" hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, int split, void *work,
+                                       hipStream_t s) {
+  if (!p->ta) return p->tb ? gemmx_native_f64_nt(p, cfg, split, work, s)
+                           : gemmx_native_f64_nn(p, cfg, split, work, s);
+  return p->tb ? gemmx_native_f64_tt(p, cfg, split, work, s)
+               : gemmx_native_f64_tn(p, cfg, split, work, s);
 }
-extern "C" hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, hipStream_t s) {
-  return launch_native<float>(p, cfg, s);
+extern "C" hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, int split, void *work,
+                                       hipStream_t s) {
+  if (!p->ta) return p->tb ? gemmx_native_f32_nt(p, cfg, split, work, s)
+                           : gemmx_native_f32_nn(p, cfg, split, work, s);
+  return p->tb ? gemmx_native_f32_tt(p, cfg, split, work, s)
+               : gemmx_native_f32_tn(p, cfg, split, work, s);
+}
+extern "C" int gemmx_native_tile(int cfg, int *bm, int *bn) {
+  if (cfg < 0 || cfg >= GEMMX_NCFG_COUNT) return -1;
+  *bm = kNativeCfg[cfg].bm;
+  *bn = kNativeCfg[cfg].bn;
+  return kNativeCfg[cfg].split_ok;
+}
+extern "C" int gemmx_native_splits(int cfg, int k, int split) {
+  if (cfg < 0 || cfg >= GEMMX_NCFG_COUNT || !kNativeCfg[cfg].split_ok || split <= 1 || k < 1)
+    return 1;
+  const int kc = split_depth(k, split, kNativeCfg[cfg].bk);
+  return (k + kc - 1) / kc;
+}
+extern "C" size_t gemmx_native_workspace(int es, int cfg, int m, int n, int k, int split) {
+  const int ns = gemmx_native_splits(cfg, k, split);
+  if (ns <= 1 || m <= 0 || n <= 0) return 0;
+  const NativeCfg &c = kNativeCfg[cfg];
+  return (size_t)ns * (size_t)((m + c.bm - 1) / c.bm) * (size_t)((n + c.bn - 1) / c.bn) *
+         (size_t)c.bm * (size_t)c.bn * (size_t)es;
 }
 extern "C" hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n,
                                       double beta, int batch, int64_t sc, hipStream_t s) {

@@ -1,0 +1,7 @@
+/*
+ * gemm_native_f32_tn.hip -- native FP32 tile kernels, transa T, transb N
+ * (every tile configuration, one pass and split-K); see gemm_native.hip.
+ */
+#include "gemm_native_impl.h"
+
+GEMMX_NATIVE_INSTANCE(float, true, false, gemmx_native_f32_tn)

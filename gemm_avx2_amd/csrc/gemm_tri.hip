@@ -137,7 +137,8 @@ hipError_t gemm(int ta, int tb, int m, int n, int k, double alpha, const T *a, i
   q.c = c;
   constexpr int es = sizeof(T);
   const int cfg = gemmx_tune_pick(es, ta, tb, m, n, k);
-  return es == 8 ? gemmx_native_f64(&q, cfg, s) : gemmx_native_f32(&q, cfg, s);
+  return es == 8 ? gemmx_native_f64(&q, cfg, 1, NULL, s)
+                 : gemmx_native_f32(&q, cfg, 1, NULL, s);
 }
 
 constexpr int TRMM_LEAF = 256;

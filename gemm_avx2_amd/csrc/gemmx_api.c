@@ -87,6 +87,8 @@ typedef struct {
   pthread_mutex_t lock; /* held for the whole of a call that uses scratch */
   void *scratch;
   size_t scratch_bytes;
+  int pool_tried;      /* stream-ordered pool of split-K partials (dev_call) */
+  hipMemPool_t pool;
 } devstate_t;
 
 static pthread_once_t g_once = PTHREAD_ONCE_INIT;
@@ -273,13 +275,29 @@ static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "bf16x3 kernel launch");
   }
   const int cfg = gemmx_tune_pick(es, p->ta, p->tb, p->m, p->n, p->k);
-  e = es == 8 ? gemmx_native_f64(p, cfg, s) : gemmx_native_f32(p, cfg, s);
+  /* split-K needs the partial tiles' workspace; without one: one pass */
+  const int split = (p->batch > 1 || !work)
+                        ? 1 : gemmx_tune_pick_split(es, p->ta, p->tb, p->m, p->n, p->k, cfg);
+  e = es == 8 ? gemmx_native_f64(p, cfg, split, work, s)
+              : gemmx_native_f32(p, cfg, split, work, s);
   return e == hipSuccess ? GEMMX_OK : hip_fail(e, "native kernel launch");
 }
 
-/* Device scratch of the emulated forms (0 for the native path). */
-static size_t emu_ws(const call_t *x, int mode, int slices) {
+/* Bytes of split-K partial tiles a native call needs (0: one pass). */
+static size_t native_ws(int es, int ta, int tb, int m, int n, int k) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  const int cfg = gemmx_tune_pick(es, ta, tb, m, n, k);
+  const int split = gemmx_tune_pick_split(es, ta, tb, m, n, k, cfg);
+  return gemmx_native_workspace(es, cfg, m, n, k, split);
+}
+
+/* Device scratch of a call: the emulated forms' workspace, or the native
+ * path's split-K partial tiles. */
+static size_t call_ws(const call_t *x, int mode, int slices) {
   if (x->alpha == 0.0 || x->k == 0 || x->m == 0 || x->n == 0) return 0;
+  if (mode == GEMMX_MODE_NATIVE || (x->es == 4 && mode != GEMMX_MODE_BF16X3) ||
+      (x->es == 8 && mode == GEMMX_MODE_BF16X3))
+    return native_ws(x->es, !notrans(x->ta), !notrans(x->tb), x->m, x->n, x->k);
   if (x->es == 8 && mode == GEMMX_MODE_OZAKI)
     return gemmx_ozaki_workspace(x->m, x->n, x->k, slices);
   if (x->es == 8 && mode == GEMMX_MODE_OZAKI2)
@@ -406,7 +424,7 @@ static void fortran_call(call_t *x, int mode, int slices) {
   const size_t bytes_a = host_a ? round_up(lda2 * ca * es, 256) : 0;
   const size_t bytes_b = host_b ? round_up(ldb2 * cb * es, 256) : 0;
   const size_t bytes_c = host_c ? round_up(ldc2 * x->n * es, 256) : 0;
-  const size_t bytes_w = round_up(emu_ws(x, mode, slices), 256);
+  const size_t bytes_w = round_up(call_ws(x, mode, slices), 256);
   const size_t total = bytes_a + bytes_b + bytes_c + bytes_w;
 
   gemmx_problem p = {.ta = !notrans(x->ta), .tb = !notrans(x->tb), .m = x->m,
@@ -787,6 +805,24 @@ EXPORT int sgemm_x_set_tile_config(int cfg) {
   if (gemmx_tune_force(4, cfg) != 0) return set_error(GEMMX_ERR_ARG, "tile config %d", cfg);
   return GEMMX_OK;
 }
+/* Split-K factor of the native kernels: split >= 1 forces it for every
+ * call (1 = one pass), -1 returns to the tuning table / heuristic. */
+EXPORT int dgemm_x_set_split_config(int split) {
+  pthread_once(&g_once, init_once);
+  if (gemmx_tune_force_split(8, split) != 0)
+    return set_error(GEMMX_ERR_ARG, "split config %d", split);
+  return GEMMX_OK;
+}
+EXPORT int sgemm_x_set_split_config(int split) {
+  pthread_once(&g_once, init_once);
+  if (gemmx_tune_force_split(4, split) != 0)
+    return set_error(GEMMX_ERR_ARG, "split config %d", split);
+  return GEMMX_OK;
+}
+/* "major.minor" of the ROCm release the library was built with (the key a
+ * tuning table must carry). */
+EXPORT const char *dgemm_x_build_rocm(void) { return gemmx_tune_rocm(); }
+EXPORT const char *sgemm_x_build_rocm(void) { return gemmx_tune_rocm(); }
 /* Product kernel of the emulated forms: 1 direct-to-LDS (default), 0
  * register-staged, -1 back to GEMMX_SLICE_CFG / the default. */
 EXPORT int dgemm_x_set_product_config(int cfg) {
@@ -834,6 +870,24 @@ EXPORT int dgemm_x_tune_lookup(char transa, char transb, int m, int n, int k) {
 EXPORT int sgemm_x_tune_lookup(char transa, char transb, int m, int n, int k) {
   return tune_lookup(4, transa, transb, m, n, k);
 }
+/* The number of K ranges a native call with these arguments runs (1 = one
+ * pass), and the bytes of partial tiles they need.  Needs no device. */
+static int split_lookup(int es, char transa, char transb, int m, int n, int k, size_t *bytes) {
+  const int cfg = tune_lookup(es, transa, transb, m, n, k);
+  if (cfg < 0) return -1;
+  const int ta = !notrans(transa), tb = !notrans(transb);
+  const int split = gemmx_tune_pick_split(es, ta, tb, m, n, k, cfg);
+  if (bytes) *bytes = gemmx_native_workspace(es, cfg, m, n, k, split);
+  return gemmx_native_splits(cfg, k, split);
+}
+EXPORT int dgemm_x_tune_lookup_split(char transa, char transb, int m, int n, int k,
+                                     size_t *bytes) {
+  return split_lookup(8, transa, transb, m, n, k, bytes);
+}
+EXPORT int sgemm_x_tune_lookup_split(char transa, char transb, int m, int n, int k,
+                                     size_t *bytes) {
+  return split_lookup(4, transa, transb, m, n, k, bytes);
+}
 
 EXPORT int dgemm_x_last_error(const char **msg) {
   if (msg) *msg = t_msg;
@@ -842,6 +896,44 @@ EXPORT int dgemm_x_last_error(const char **msg) {
 EXPORT int sgemm_x_last_error(const char **msg) {
   if (msg) *msg = t_msg;
   return t_err;
+}
+
+/* The arithmetic launch_device runs for an element size and mode. */
+static int launch_mode(int es, int mode) {
+  if (es == 8 && (mode == GEMMX_MODE_OZAKI || mode == GEMMX_MODE_OZAKI2)) return mode;
+  if (es == 4 && mode == GEMMX_MODE_BF16X3) return mode;
+  return GEMMX_MODE_NATIVE;
+}
+
+/* `bytes` of device memory ordered on stream s, from a pool of the device
+ * that keeps up to 256 MB between calls; NULL when the pool is unavailable
+ * or the allocation fails (the caller then runs without it). */
+static void *stream_alloc(int dev, size_t bytes, hipStream_t s) {
+  devstate_t *d = &g_dev[dev];
+  pthread_mutex_lock(&d->lock);
+  if (!d->pool_tried) {
+    d->pool_tried = 1;
+    hipMemPoolProps props;
+    memset(&props, 0, sizeof props);
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    if (hipMemPoolCreate(&d->pool, &props) == hipSuccess) {
+      uint64_t keep = (uint64_t)256 << 20;
+      (void)hipMemPoolSetAttribute(d->pool, hipMemPoolAttrReleaseThreshold, &keep);
+    } else {
+      d->pool = NULL;
+      (void)hipGetLastError();
+    }
+  }
+  hipMemPool_t pool = d->pool;
+  pthread_mutex_unlock(&d->lock);
+  void *w = NULL;
+  if (!pool || hipMallocFromPoolAsync(&w, bytes, pool, s) != hipSuccess) {
+    (void)hipGetLastError();
+    return NULL;
+  }
+  return w;
 }
 
 static int dev_call(call_t *x, int mode, int slices, void *work, size_t work_bytes,
@@ -856,11 +948,19 @@ static int dev_call(call_t *x, int mode, int slices, void *work, size_t work_byt
                      .ldc = x->ldc, .alpha = x->alpha, .beta = x->beta,
                      .a = x->a, .b = x->b, .c = x->c};
   hipStream_t s = (hipStream_t)stream;
-  const size_t need = emu_ws(x, mode, slices);
+  const size_t need = call_ws(x, mode, slices);
   if (need == 0 || work) {
     if (need > 0 && work_bytes < need)
       return set_error(GEMMX_ERR_ARG, "workspace of %zu bytes < %zu needed", work_bytes, need);
     return launch_device(&p, x->es, mode, slices, work, s);
+  }
+  if (launch_mode(x->es, mode) == GEMMX_MODE_NATIVE) {
+    /* split-K partial tiles: stream-ordered memory of the device's pool, so
+     * the call stays asynchronous; without it the product runs in one pass */
+    void *w = stream_alloc(dev, need, s);
+    const int rc = launch_device(&p, x->es, mode, slices, w, s);
+    if (w) (void)hipFreeAsync(w, s);
+    return rc;
   }
   /* No caller workspace: borrow the device scratch for the duration. */
   devstate_t *d = &g_dev[dev];

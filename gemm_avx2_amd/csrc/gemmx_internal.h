@@ -107,9 +107,21 @@ int gemmx_stage_d2h(int dev, void *dst, size_t dld, const void *src, size_t sld,
 size_t gemmx_stage_chunk_bytes(void);
 
 /* Native tile kernels (gemm_native.hip).  alpha != 0 and k > 0 here; the
- * host layer handles the alpha == 0 / k == 0 / empty cases. */
-hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, hipStream_t s);
-hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, hipStream_t s);
+ * host layer handles the alpha == 0 / k == 0 / empty cases.  split > 1 (a
+ * single problem, configurations with split-K kernels) divides K over that
+ * many grid rows writing partial tiles to `work` (gemmx_native_workspace
+ * bytes), then reduces them into C; split 1 or work NULL is one pass. */
+hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, int split, void *work,
+                            hipStream_t s);
+hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, int split, void *work,
+                            hipStream_t s);
+/* BM x BN tile of a configuration; returns 1 when it has split-K kernels,
+ * 0 when not, -1 for an unknown configuration. */
+int gemmx_native_tile(int cfg, int *bm, int *bn);
+/* Splits actually run for a requested split (K ranges of whole BK slices),
+ * and the bytes of partial tiles they need (0 for one pass). */
+int gemmx_native_splits(int cfg, int k, int split);
+size_t gemmx_native_workspace(int es, int cfg, int m, int n, int k, int split);
 /* C[0:m,0:n] *= beta (beta != 1), the reference's C_MULT_BETA
  * (src/gemm_driver.c:106-116), for `batch` matrices sc elements apart. */
 hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta, int batch,

@@ -29,7 +29,8 @@
  * The epilogue writes C = fma(alpha, acc, beta*C) for rows < M, columns < N
  * only (the reference's masked stores, src/kernel/gemm_kernel_unroll6.S:
  * 145-155); alpha moves from the reference's packed B (src/gemm_copy.c:65)
- * into this epilogue.
+ * into this epilogue.  PARTIAL (split-K): the raw accumulators of one K
+ * range are stored instead, and a reduction kernel applies alpha and beta.
  */
 #ifndef GEMMX_TILE_H
 #define GEMMX_TILE_H
@@ -54,7 +55,7 @@ constexpr int tile_lds_max() {
 }
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KVMAX,
-          bool FAST, bool CLAMP = false, bool NARROW = false>
+          bool FAST, bool CLAMP = false, bool NARROW = false, bool PARTIAL = false>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           T *__restrict__ C, int M, int N, int K,
                                           int64_t lda, int64_t ldb, int64_t ldc, T alpha,
@@ -219,6 +220,20 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     }
   }
 
+  /* PARTIAL (one K range of a split-K product): the raw accumulators go to
+   * the BM*BN block C in the fragment order of the registers -- element
+   * (((wave*FM + i)*FN + j)*4 + r)*64 + lane -- so every store is 64
+   * consecutive elements; splitk_partial_coords maps them back. */
+  if (PARTIAL) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          C[(((wave * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+    return;
+  }
   /* Epilogue: rows m..ldc-1 and columns >= n are never touched, matching the
    * reference's masked stores (src/kernel/gemm_kernel_unroll6.S:145-155). */
 #pragma unroll
@@ -236,6 +251,18 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
         }
       }
     }
+}
+
+/* Row and column, inside its BM x BN tile, of element e of a partial tile
+ * stored by gemm_tile<..., PARTIAL> (the inverse of its fragment order). */
+template <typename T, int BM, int BN, int WM, int WN>
+__device__ __forceinline__ void splitk_partial_coords(int e, int *row, int *col) {
+  constexpr int WTM = BM / WM, WTN = BN / WN, FM = WTM / 16, FN = WTN / 16;
+  const int lane = e & 63, r = (e >> 6) & 3, f = e >> 8;
+  const int j = f % FN, i = (f / FN) % FM, wave = f / (FN * FM);
+  const int wm = wave % WM, wn = wave / WM;
+  *row = wm * WTM + i * 16 + Mfma16<T>::row(lane >> 4, r);
+  *col = wn * WTN + j * 16 + (lane & 15);
 }
 
 } // namespace gemmx

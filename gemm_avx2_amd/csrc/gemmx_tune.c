@@ -10,6 +10,7 @@
 #include "gemmx_tune.h"
 
 #include <dlfcn.h>
+#include <hip/hip_version.h>
 #include <libgen.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -20,12 +21,15 @@
 
 #define NB 12 /* buckets 4..15 */
 static signed char g_table[2][4][NB][NB][NB]; /* [f64/f32][trans][m][n][k] */
+/* split-K factor per bucket: 0 = not in the table (the heuristic decides) */
+static signed char g_split[2][4][NB][NB][NB];
 /* DGEMM arithmetic per bucket: 0 = no entry, else (mode + 1) << 8 | count */
 static short g_emu[4][NB][NB][NB];
 /* grouped-batch kernel configuration per element size: 0 = no entry, else cfg + 1 */
 static signed char g_batch[2];
 static int g_entries;
 static int g_force[2] = {-1, -1}; /* [f64/f32]: one configuration for every call */
+static int g_force_split[2] = {0, 0}; /* [f64/f32]: one split factor for every call (0: off) */
 static char g_path[1024];
 static char g_arch[64];           /* current device arch, "" = unknown */
 
@@ -53,25 +57,37 @@ static int trans_index(const char *t) {
   return -1;
 }
 
+/* The ROCm (HIP) release the library was compiled with, "major.minor": a
+ * table measured with kernels from another compiler is not used. */
+static const char *build_rocm(void) {
+  static char v[32];
+  if (!v[0]) snprintf(v, sizeof v, "%d.%d", HIP_VERSION_MAJOR, HIP_VERSION_MINOR);
+  return v;
+}
+
 static void clear_table(void) {
   memset(g_table, 0, sizeof g_table);
+  memset(g_split, 0, sizeof g_split);
   memset(g_emu, 0, sizeof g_emu);
   memset(g_batch, 0, sizeof g_batch);
   g_entries = 0;
   g_path[0] = 0;
 }
 
-/* Reads a table; on a format or arch mismatch nothing is kept. */
+/* Reads a table; on a format, arch or ROCm mismatch, or without a rocm
+ * directive, nothing is kept. */
 static int load_table(const char *path) {
   FILE *f = fopen(path, "r");
   if (!f) return -1;
   static signed char staged[2][4][NB][NB][NB];
+  static signed char staged_split[2][4][NB][NB][NB];
   static short staged_emu[4][NB][NB][NB];
   signed char staged_batch[2] = {0, 0};
   memset(staged, 0, sizeof staged);
+  memset(staged_split, 0, sizeof staged_split);
   memset(staged_emu, 0, sizeof staged_emu);
   char line[256];
-  int format = 0, entries = 0, ok = 1;
+  int format = 0, entries = 0, ok = 1, rocm = 0;
   while (ok && fgets(line, sizeof line, f)) {
     char *hash = strchr(line, '#');
     if (hash) *hash = 0;
@@ -86,9 +102,14 @@ static int load_table(const char *path) {
         if (g_arch[0] && strcmp(w1, g_arch) != 0) ok = 0;
         continue;
       }
+      if (strcmp(w0, "rocm") == 0) {
+        if (strcmp(w1, build_rocm()) != 0) ok = 0;
+        rocm = 1;
+        continue;
+      }
     }
     char dt[8], tr[8], sc[16];
-    int mb, nb, kb, cfg;
+    int mb, nb, kb, cfg, split = 0;
     if (sscanf(line, "batch %7s %d", dt, &cfg) == 2) {
       if (format != GEMMX_TUNE_FORMAT) {
         ok = 0;
@@ -119,7 +140,8 @@ static int load_table(const char *path) {
       entries++;
       continue;
     }
-    if (sscanf(line, "%7s %7s %d %d %d %d", dt, tr, &mb, &nb, &kb, &cfg) != 6) continue;
+    const int got = sscanf(line, "%7s %7s %d %d %d %d %d", dt, tr, &mb, &nb, &kb, &cfg, &split);
+    if (got != 6 && got != 7) continue;
     if (format != GEMMX_TUNE_FORMAT) {
       ok = 0; /* entries before (or without) a format directive */
       break;
@@ -128,12 +150,15 @@ static int load_table(const char *path) {
     int t = trans_index(tr);
     if (d < 0 || t < 0 || mb < 4 || mb > 15 || nb < 4 || nb > 15 || kb < 4 || kb > 15) continue;
     if (cfg < 0 || cfg >= GEMMX_NCFG_COUNT) continue;
+    if (got == 7 && (split < 1 || split > GEMMX_MAX_SPLIT)) continue;
     staged[d][t][mb - 4][nb - 4][kb - 4] = (signed char)(cfg + 1);
+    staged_split[d][t][mb - 4][nb - 4][kb - 4] = (signed char)(got == 7 ? split : 0);
     entries++;
   }
   fclose(f);
-  if (!ok || format != GEMMX_TUNE_FORMAT) return -1;
+  if (!ok || format != GEMMX_TUNE_FORMAT || !rocm) return -1;
   memcpy(g_table, staged, sizeof g_table);
+  memcpy(g_split, staged_split, sizeof g_split);
   memcpy(g_emu, staged_emu, sizeof g_emu);
   memcpy(g_batch, staged_batch, sizeof g_batch);
   g_entries = entries;
@@ -154,6 +179,11 @@ void gemmx_tune_init(const char *arch) {
   if (force && *force) {
     int c = atoi(force);
     if (c >= 0 && c < GEMMX_NCFG_COUNT) g_force[0] = g_force[1] = c;
+  }
+  const char *fsplit = getenv("GEMMX_NATIVE_SPLIT");
+  if (fsplit && *fsplit) {
+    int c = atoi(fsplit);
+    if (c >= 1 && c <= GEMMX_MAX_SPLIT) g_force_split[0] = g_force_split[1] = c;
   }
   const char *env = getenv("GEMMX_TUNE_FILE");
   if (env && *env) {
@@ -177,6 +207,14 @@ int gemmx_tune_force(int es, int cfg) {
   g_force[es == 8 ? 0 : 1] = cfg;
   return 0;
 }
+
+int gemmx_tune_force_split(int es, int split) {
+  if (split != -1 && (split < 1 || split > GEMMX_MAX_SPLIT)) return -1;
+  g_force_split[es == 8 ? 0 : 1] = split < 0 ? 0 : split;
+  return 0;
+}
+
+const char *gemmx_tune_rocm(void) { return build_rocm(); }
 
 int gemmx_tune_load(const char *path) {
   if (!path || !*path) {
@@ -212,3 +250,25 @@ int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {
 }
 
 int gemmx_tune_pick_batch(int es) { return g_batch[es == 8 ? 0 : 1] - 1; }
+
+int gemmx_tune_pick_split(int es, int ta, int tb, int m, int n, int k, int cfg) {
+  const int d = es == 8 ? 0 : 1, t = (ta ? 2 : 0) + (tb ? 1 : 0);
+  int bm, bn;
+  if (gemmx_native_tile(cfg, &bm, &bn) != 1) return 1; /* no split-K kernels */
+  if (g_force_split[d] >= 1) return g_force_split[d];
+  const int v = g_split[d][t][gemmx_tune_bucket(m) - 4][gemmx_tune_bucket(n) - 4]
+                       [gemmx_tune_bucket(k) - 4];
+  if (v > 0 && g_force[d] < 0) return v;
+  /* Heuristic: about 1024 workgroups (four per CU of the 256 on MI355X) when
+   * the tiles alone fill fewer than two per CU, K ranges of at least 256, and
+   * at most GEMMX_SPLIT_SCRATCH bytes of partial tiles. */
+  const long tiles = (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn);
+  if (tiles >= 512) return 1;
+  long sp = 1024 / tiles;
+  if (sp > k / 256) sp = k / 256;
+  if (sp > GEMMX_MAX_SPLIT) sp = GEMMX_MAX_SPLIT;
+  while (sp > 1 && (size_t)sp * (size_t)tiles * (size_t)bm * (size_t)bn * (size_t)es >
+                       (size_t)GEMMX_SPLIT_SCRATCH)
+    sp--;
+  return sp < 2 ? 1 : (int)sp;
+}

@@ -9,12 +9,17 @@
  *
  * Table file: GEMMX_TUNE_FILE, else gemm_mi355x_tune.txt beside the loaded
  * library.  '#' starts a comment.  Directives:
- *     format 1            table format version (required, first directive)
+ *     format 2            table format version (required, first directive)
  *     arch <gfxNNN>       the table is ignored on a device of another arch
+ *     rocm <major.minor>  required: the table is ignored by a library built
+ *                         with another ROCm (HIP) release, whose compiler
+ *                         makes other code of the same kernels
  * and one entry per line:
- *     <dtype:f64|f32> <trans:NN|NT|TN|TT> <mbucket> <nbucket> <kbucket> <cfg>
- * where a bucket is floor(log2(dim)) clamped to 4..15 and cfg is a
- * configuration id, or, for the arithmetic of DGEMM in the automatic mode,
+ *     <dtype:f64|f32> <trans:NN|NT|TN|TT> <mbucket> <nbucket> <kbucket> <cfg> [split]
+ * where a bucket is round(log2(dim)) clamped to 4..15, cfg is a
+ * configuration id and split (1..GEMMX_MAX_SPLIT, optional) the number of K
+ * ranges of a split-K launch (without it a heuristic decides), or, for the
+ * arithmetic of DGEMM in the automatic mode,
  *     emu <trans> <mbucket> <nbucket> <kbucket> <native|ozaki|ozaki2> <count>
  * (count: slices 3..8 for ozaki, moduli 8..22 for ozaki2, 0 for native), or,
  * for the grouped-batch kernel of one element type,
@@ -29,12 +34,24 @@
 extern "C" {
 #endif
 
-#define GEMMX_TUNE_FORMAT 1
+#define GEMMX_TUNE_FORMAT 2
+#define GEMMX_MAX_SPLIT 64
+/* The split heuristic keeps the partial tiles of one call within this many
+ * bytes. */
+#define GEMMX_SPLIT_SCRATCH (64u << 20)
 
 /* `arch` is the current device's architecture name ("gfx950"), or NULL when
  * there is no device (then a table's arch directive is not checked). */
 void gemmx_tune_init(const char *arch);
 int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k);
+/* Split-K factor for a native call with tile configuration cfg (1 = one
+ * pass): a forced factor, else the table's, else a heuristic. */
+int gemmx_tune_pick_split(int es, int ta, int tb, int m, int n, int k, int cfg);
+/* Forces one split factor (1..GEMMX_MAX_SPLIT) for every native call of the
+ * element size, -1 returns to the table; 0 on success, -1 otherwise. */
+int gemmx_tune_force_split(int es, int split);
+/* "major.minor" of the ROCm release the library was built with. */
+const char *gemmx_tune_rocm(void);
 /* Arithmetic of a DGEMM in the automatic mode: GEMMX_MODE_NATIVE, _OZAKI or
  * _OZAKI2, with the slice or modulus count in *count. */
 int gemmx_tune_pick_emulated(int ta, int tb, int m, int n, int k, int *count);

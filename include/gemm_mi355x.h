@@ -159,14 +159,40 @@ int sgemm_x_set_mode(int mode);
  * configuration for every call, -1 returns to the tuning table.
  * *_x_load_tuning reads a table written by the autotuner (path; NULL clears
  * it) and returns the number of entries read, or -1 (previous table kept)
- * when the file is unreadable or its format version or arch does not match;
- * the table is shared by both element types.  GEMMX_TUNE_FILE (else
- * gemm_mi355x_tune.txt beside the library) and GEMMX_NATIVE_CFG are read at
- * first use.  *_x_tune_lookup returns the configuration a call with these
- * arguments would use (no device needed), or -1 for negative sizes. */
+ * when the file is unreadable, its format version, arch or ROCm release
+ * does not match, or it names no ROCm release; the table is shared by both
+ * element types.  GEMMX_TUNE_FILE (else gemm_mi355x_tune.txt beside the
+ * library), GEMMX_NATIVE_CFG and GEMMX_NATIVE_SPLIT are read at first use.
+ * *_x_tune_lookup returns the configuration a call with these arguments
+ * would use (no device needed), or -1 for negative sizes. */
 #define GEMMX_TILE_CONFIGS 7
 int dgemm_x_set_tile_config(int cfg);
 int sgemm_x_set_tile_config(int cfg);
+/* Split-K of the native kernels, in place of the reference's N-split of
+ * every call over all its threads (src/gemm_driver.c:128-151): a product
+ * with few C tiles divides K into `split` ranges computed by separate
+ * workgroups into partial tiles, which a second kernel sums in range order
+ * (the result does not depend on scheduling) and combines with alpha and
+ * beta*C.  split >= 1 (1..64) forces a factor for every later call of the
+ * element type, -1 returns to the tuning table's factor or, without one,
+ * a heuristic (about four workgroups per CU when the tiles give fewer than
+ * two, K ranges of at least 256, partial tiles within 64 MiB).  Only tile
+ * configurations 0, 1 and 5 have split-K kernels; a strided batch is never
+ * split.  *_x_tune_lookup_split returns the number of K ranges a call with
+ * these arguments runs (1 = one pass) and, in *bytes (may be NULL), the
+ * bytes of partial tiles: the Fortran-ABI entries take them from the
+ * library's device scratch, the device-pointer entries from a stream-ordered
+ * pool of the current device (hipMallocFromPoolAsync on the call's stream,
+ * released to the pool after the reduction; when that memory is not
+ * available the call runs in one pass).  No device needed for the lookup. */
+int dgemm_x_set_split_config(int split);
+int sgemm_x_set_split_config(int split);
+int dgemm_x_tune_lookup_split(char transa, char transb, int m, int n, int k, size_t *bytes);
+int sgemm_x_tune_lookup_split(char transa, char transb, int m, int n, int k, size_t *bytes);
+/* "major.minor" of the ROCm release the library was built with: the value
+ * of the `rocm` directive a tuning table must carry to be used. */
+const char *dgemm_x_build_rocm(void);
+const char *sgemm_x_build_rocm(void);
 /* Product kernel of the emulated forms (Ozaki-I, Ozaki-II, BF16x3), for every
  * later call: 1 = direct-to-LDS loads with 128-byte K stages (the default),
  * 0 = register-staged; -1 returns to GEMMX_SLICE_CFG (read at first use) or

@@ -104,6 +104,16 @@ def lib():
         ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
         ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.oracle_dot2_pairs.restype = None
+    L.oracle_dot2_pairs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    for fn in ("oracle_ozaki_pairs", "oracle_ozaki2_pairs"):
+        f = getattr(L, fn)
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                      ctypes.c_int, ctypes.c_void_p]
     for fn, fl in (("oracle_zgemm", ctypes.c_double), ("oracle_cgemm", ctypes.c_float)):
         f = getattr(L, fn)
         f.restype = None
@@ -206,6 +216,50 @@ def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, sli
 def ozaki2_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, moduli):
     lib().oracle_ozaki2_dgemm(transa.encode()[:1], transb.encode()[:1], m, n, k, alpha,
                               _ptr(a), lda, _ptr(b), ldb, beta, _ptr(c), ldc, moduli)
+
+
+def _pairs(fn, rows, cols, ri, cj, alpha, c1, count_arg):
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    cols = np.ascontiguousarray(cols, dtype=np.float64)
+    k = rows.shape[1]
+    assert cols.shape[1] == k
+    ri = np.ascontiguousarray(ri, dtype=np.int32)
+    cj = np.ascontiguousarray(cj, dtype=np.int32)
+    assert ri.size == cj.size and ri.max(initial=0) < rows.shape[0] and cj.max(initial=0) < cols.shape[0]
+    c1 = np.ascontiguousarray(np.broadcast_to(np.asarray(c1, dtype=np.float64), ri.shape))
+    out = np.empty(ri.size, dtype=np.float64)
+    rc = fn(_ptr(rows), _ptr(cols), k, ri.size, _ptr(ri), _ptr(cj), alpha, _ptr(c1), count_arg,
+            _ptr(out))
+    if rc != 0:
+        raise ValueError("unsupported arguments for the sampled restatement")
+    return out
+
+
+def dot2_pairs(rows, cols, ri, cj, alpha, c1):
+    """Dot2 entries alpha*(row ri[t] . col cj[t]) + c1[t] (as sampled(), for
+    explicit operands); returns (values, sum|a||b|)."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    cols = np.ascontiguousarray(cols, dtype=np.float64)
+    ri = np.ascontiguousarray(ri, dtype=np.int32)
+    cj = np.ascontiguousarray(cj, dtype=np.int32)
+    assert rows.shape[1] == cols.shape[1] and ri.size == cj.size
+    c1 = np.ascontiguousarray(np.broadcast_to(np.asarray(c1, dtype=np.float64), ri.shape))
+    out = np.empty(ri.size, dtype=np.float64)
+    ab = np.empty(ri.size, dtype=np.float64)
+    lib().oracle_dot2_pairs(_ptr(rows), _ptr(cols), rows.shape[1], ri.size, _ptr(ri), _ptr(cj),
+                            alpha, _ptr(c1), _ptr(out), _ptr(ab))
+    return out, ab
+
+
+def ozaki_pairs(rows, cols, ri, cj, alpha, c1, slices):
+    """Ozaki-I entries alpha*(row ri[t] . col cj[t]) + c1[t]; rows (nr, k) are
+    whole rows of op(A), cols (nc, k) whole columns of op(B)."""
+    return _pairs(lib().oracle_ozaki_pairs, rows, cols, ri, cj, alpha, c1, slices)
+
+
+def ozaki2_pairs(rows, cols, ri, cj, alpha, c1, moduli):
+    """Ozaki-II entries, as ozaki_pairs."""
+    return _pairs(lib().oracle_ozaki2_pairs, rows, cols, ri, cj, alpha, c1, moduli)
 
 
 def complex_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):

@@ -292,6 +292,28 @@ void oracle_sampled_f32(char transa, char transb, int m, int n, int k,
   }
 }
 
+/* The same Dot2 entries for explicit operands: rows (nr x k) are whole rows
+ * of op(A), cols (nc x k) whole columns of op(B), entry t pairs rows[ri[t]]
+ * with cols[cj[t]] and starts from c1[t] (C already multiplied by beta);
+ * alpha is folded into the B entries as in oracle_sampled_f64. */
+void oracle_dot2_pairs(const double *rows, const double *cols, int k, int count,
+                       const int *ri, const int *cj, double alpha, const double *c1,
+                       double *out, double *absprod) {
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < count; t++) {
+    const double *x = rows + (size_t)ri[t] * k, *y = cols + (size_t)cj[t] * k;
+    dd_t s = {c1[t], 0.0};
+    double ab = 0.0;
+    if (alpha != 0.0)
+      for (int p = 0; p < k; p++) {
+        dot2_step(&s, x[p], y[p] * alpha);
+        ab += fabs(x[p] * y[p]);
+      }
+    out[t] = dd_round(s);
+    if (absprod) absprod[t] = ab;
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Ozaki-I INT8 slicing.  No reference counterpart: this restates the       */
 /* emulated DGEMM kernel of this library.                                   */
@@ -975,4 +997,79 @@ void oracle_ozaki2_dgemm(char transa, char transb, int m, int n, int k,
   free(pb);
   free(ia);
   free(ib);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Sampled entries of the two emulated forms (one K chunk), for products   */
+/* too large to restate whole.  Each entry repeats, for its one row and    */
+/* column, exactly the steps of oracle_ozaki_dgemm / oracle_ozaki2_dgemm.  */
+/* ---------------------------------------------------------------------- */
+int oracle_ozaki_pairs(const double *rows, const double *cols, int k, int count,
+                       const int *ri, const int *cj, double alpha, const double *c1,
+                       int slices, double *out) {
+  if (slices < 1 || slices > 16 || k < 1 || k > oracle_ozaki_kchunk(slices)) return -1;
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
+  for (int t = 0; t < count; t++) {
+    const double *x = rows + (size_t)ri[t] * k, *y = cols + (size_t)cj[t] * k;
+    const int ex = oracle_ozaki_exponent(x, k, 1), ey = oracle_ozaki_exponent(y, k, 1);
+    int8_t *dx = (int8_t *)malloc((size_t)slices * k), *dy = (int8_t *)malloc((size_t)slices * k);
+    if (!dx || !dy) {
+      bad = 1;
+      free(dx);
+      free(dy);
+      continue;
+    }
+    int8_t dig[16];
+    for (int p = 0; p < k; p++) {
+      oracle_ozaki_digits(x[p], ex, slices, dig);
+      for (int u = 0; u < slices; u++) dx[(size_t)u * k + p] = dig[u];
+      oracle_ozaki_digits(y[p], ey, slices, dig);
+      for (int u = 0; u < slices; u++) dy[(size_t)u * k + p] = dig[u];
+    }
+    double acc = 0.0;
+    for (int w = slices + 1; w >= 2; w--) {
+      int32_t sw = 0;
+      for (int ta = 1; ta <= slices; ta++) {
+        const int ub = w - ta;
+        if (ub < 1 || ub > slices) continue;
+        int32_t part = 0;
+        for (int p = 0; p < k; p++)
+          part += (int32_t)dx[(size_t)(ta - 1) * k + p] * (int32_t)dy[(size_t)(ub - 1) * k + p];
+        sw += part;
+      }
+      acc = fma(acc, 0x1p-7, (double)sw);
+    }
+    const double scaled = (ex == OZAKI_NONFINITE || ey == OZAKI_NONFINITE)
+                              ? NAN : ldexp(acc, ex + ey - 14);
+    out[t] = fma(alpha, scaled, c1[t]);
+    free(dx);
+    free(dy);
+  }
+  return bad ? -1 : 0;
+}
+
+int oracle_ozaki2_pairs(const double *rows, const double *cols, int k, int count,
+                        const int *ri, const int *cj, double alpha, const double *c1,
+                        int nmod, double *out) {
+  if (nmod < OZAKI2_MIN_MODULI || nmod > OZAKI2_MAX_MODULI || k < 1 || k > OZAKI2_KCHUNK)
+    return -1;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int t = 0; t < count; t++) {
+    const double *x = rows + (size_t)ri[t] * k, *y = cols + (size_t)cj[t] * k;
+    const int ex = oracle_ozaki_exponent(x, k, 1), ey = oracle_ozaki_exponent(y, k, 1);
+    const int px = oracle_ozaki2_vector_bits(x, k, 1, nmod);
+    const int py = oracle_ozaki2_vector_bits(y, k, 1, nmod);
+    double scaled;
+    if (ex == OZAKI_NONFINITE || ey == OZAKI_NONFINITE) {
+      scaled = NAN;
+    } else {
+      uint64_t acc[4] = {0, 0, 0, 0};
+      for (int p = 0; p < k; p++)
+        acc_add_product(acc, ozaki2_scaled(x[p], ex, px), ozaki2_scaled(y[p], ey, py));
+      scaled = oracle_ozaki2_round(acc, ex - px + ey - py);
+    }
+    out[t] = fma(alpha, scaled, c1[t]);
+  }
+  return 0;
 }

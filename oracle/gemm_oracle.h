@@ -89,6 +89,14 @@ void oracle_sampled_f32(char transa, char transb, int m, int n, int k,
                         double param, int count, const int *is, const int *js,
                         double *out, double *absprod);
 
+/* The same Dot2 entries for explicit operands (whole rows of op(A) and
+ * columns of op(B), k contiguous entries each; entry t pairs rows[ri[t]]
+ * with cols[cj[t]] and starts from c1[t] = beta-scaled C), for inputs that
+ * are not drawn on the host. */
+void oracle_dot2_pairs(const double *rows, const double *cols, int k, int count,
+                       const int *ri, const int *cj, double alpha, const double *c1,
+                       double *out, double *absprod);
+
 /* ---- Ozaki-I INT8 slicing (emulated DGEMM restatement) ---------------- */
 /* Exponent of a row/column: the e with max|x| < 2^e, max|x| >= 2^(e-1); 0 for
  * an all-zero vector; OZAKI_NONFINITE when any entry is Inf/NaN. */
@@ -165,6 +173,22 @@ void oracle_ozaki2_dgemm(char transa, char transb, int m, int n, int k,
                          double alpha, const double *a, int lda,
                          const double *b, int ldb, double beta, double *c,
                          int ldc, int nmod);
+
+/* ---- sampled entries of the emulated forms ---------------------------- */
+/* alpha * (op(A) op(B))[i,j] + c1 at `count` entries of a product too large
+ * to restate whole.  rows holds nr full rows of op(A) and cols nc full
+ * columns of op(B), each k contiguous entries (whole vectors: the exponents
+ * and bit counts depend on every entry); entry t pairs rows[ri[t]] with
+ * cols[cj[t]] and adds c1[t] (C already multiplied by beta).  The arithmetic
+ * is that of oracle_ozaki_dgemm / oracle_ozaki2_dgemm for one K chunk, so
+ * every out[t] equals the whole-matrix restatement's entry bit for bit.
+ * Returns 0, or -1 when k exceeds one K chunk or the count is out of range. */
+int oracle_ozaki_pairs(const double *rows, const double *cols, int k, int count,
+                       const int *ri, const int *cj, double alpha, const double *c1,
+                       int slices, double *out);
+int oracle_ozaki2_pairs(const double *rows, const double *cols, int k, int count,
+                        const int *ri, const int *cj, double alpha, const double *c1,
+                        int nmod, double *out);
 
 /* ---- split-BF16x3 (emulated SGEMM restatement) ------------------------ */
 uint16_t oracle_bf16_rne(float x);

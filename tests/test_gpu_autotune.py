@@ -1,7 +1,8 @@
 """The autotuner on the GPU: it writes a table the libraries load, the choice
 for each entry is the one it measured, every entry's configuration gives
 results within the native tolerance against the oracle (oracle_dgemm /
-oracle_sgemm), and a resumed run measures nothing again."""
+oracle_sgemm), its split-K factor is the one the libraries run, and a
+resumed run measures nothing again."""
 import numpy as np
 import pytest
 
@@ -19,8 +20,8 @@ def test_autotune_table_is_used_and_correct(cuda, tmp_path, capsys):
     shapes = "256,256,256;640,200,300"
     assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--reps", "2",
                     "--batch", "off"]) == 0
-    entries, notes, arch = at.read_table(out)
-    assert len(entries) == 8 and arch.startswith("gfx")
+    entries, notes, arch, rocm = at.read_table(out)
+    assert len(entries) == 8 and arch.startswith("gfx") and rocm == gx.build_rocm()
     capsys.readouterr()
     # resumed: nothing left to measure, same entries
     assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--resume",
@@ -31,11 +32,15 @@ def test_autotune_table_is_used_and_correct(cuda, tmp_path, capsys):
     try:
         for d in ("d", "s"):
             assert gx.load_tuning(out, d) == 8
-        for (dt, tr, mb, nb, kb), cfg in sorted(entries.items()):
+        for (dt, tr, mb, nb, kb), (cfg, split) in sorted(entries.items()):
             d = "d" if dt == "f64" else "s"
             npdt = np.float64 if d == "d" else np.float32
             m, n, k = (int(v) for v in notes[(dt, tr, mb, nb, kb)].split()[0].split("x"))
             assert gx.tune_lookup(d, tr[0], tr[1], m, n, k) == cfg
+            # the measured split: K ranges of whole 16-deep slices
+            kc = -(-(-(-k // split)) // 16) * 16
+            want = -(-k // kc) if cfg in at.SPLIT_CONFIGS and split > 1 else 1
+            assert gx.tune_lookup_split(d, tr[0], tr[1], m, n, k)[0] == want
             ra, ca = stored_shape(tr[0], m, k)
             rb, cb = stored_shape(tr[1], k, n)
             A = np.asfortranarray(rng.uniform(-1, 1, (ra, ca)).astype(npdt))
@@ -65,7 +70,7 @@ def test_autotune_emulated_choice_drives_auto_mode(cuda, tmp_path):
     out = str(tmp_path / "tune.txt")
     assert at.main(["--out", out, "--shapes", "1024,1024,1024", "--trans", "NN",
                     "--dtypes", "d", "--reps", "2"]) == 0
-    entries, notes, _ = at.read_table(out)
+    entries, notes, _, _ = at.read_table(out)
     scheme, cnt = entries[("emu", "NN", 10, 10, 10)]
     assert scheme in ("native", "ozaki2")
     rng = np.random.default_rng(12)

@@ -514,6 +514,58 @@ def test_ozaki2_semantics_error_and_chunks():
     assert C[0, 0] == (s0 + (-0.5)) + s1
 
 
+@pytest.mark.parametrize("ta,tb,dist,param", [("N", "N", 1, 4.0), ("T", "N", 0, 0.0),
+                                               ("N", "T", 1, 1.0), ("T", "T", 1, 0.0)])
+def test_dot2_pairs_equal_the_generator_sampled_entries(ta, tb, dist, param):
+    """Dot2 on explicit rows/columns gives the generator-based sampled
+    entries bit for bit when fed the same generator inputs."""
+    m, n, k, seed = 37, 29, 515, 8
+    alpha, beta = 1.5, -0.5
+    ra, ca = (m, k) if ta == "N" else (k, m)
+    rb, cb = (k, n) if tb == "N" else (n, k)
+    A = oracle.fill(ra, ca, ra, seed, 0, np.float64, dist, param)
+    B = oracle.fill(rb, cb, rb, seed, 1, np.float64, dist, param)
+    C = oracle.fill(m, n, m, seed, 2, np.float64, dist, param)
+    rng = np.random.default_rng(3)
+    is_, js = rng.integers(0, m, 200), rng.integers(0, n, 200)
+    want, ab_w = oracle.sampled(np.float64, ta, tb, m, n, k, alpha, beta, seed, is_, js,
+                                dist, param)
+    rows, cols = op(A, ta, m, k), op(B, tb, k, n).T
+    got, ab_g = oracle.dot2_pairs(rows, cols, is_, js, alpha, beta * C[is_, js])
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(ab_g, ab_w)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_emulated_pairs_equal_the_whole_matrix_restatements(ta, tb):
+    """The sampled-entry restatements (used on products too large to restate
+    whole) give the whole-matrix restatements' entries bit for bit: Ozaki-I
+    at every slice count, Ozaki-II at several modulus counts, on heavy-tailed
+    entries, with alpha/beta, repeated pairs and a non-finite row."""
+    rng = np.random.default_rng(41)
+    m, n, k = 23, 19, 700
+    ra, ca = (m, k) if ta == "N" else (k, m)
+    rb, cb = (k, n) if tb == "N" else (n, k)
+    A = np.asfortranarray(rng.uniform(-1, 1, (ra, ca)) * np.exp(4 * rng.standard_normal((ra, ca))))
+    B = np.asfortranarray(rng.uniform(-1, 1, (rb, cb)) * np.exp(2 * rng.standard_normal((rb, cb))))
+    opA = op(A, ta, m, k)
+    opA[5, 17] = np.inf  # row 5 of op(A) is non-finite
+    C0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    rows, cols = np.ascontiguousarray(op(A, ta, m, k)), np.ascontiguousarray(op(B, tb, k, n).T)
+    ri = np.concatenate([rng.integers(0, m, 60), [5, 5, 0]])
+    cj = np.concatenate([rng.integers(0, n, 60), [0, 3, 0]])
+    for fn, whole, counts in ((oracle.ozaki_pairs, oracle.ozaki_gemm, range(3, 9)),
+                              (oracle.ozaki2_pairs, oracle.ozaki2_gemm, (8, 13, 16, 18, 22))):
+        for cnt in counts:
+            R = C0.copy(order="F")
+            whole(ta, tb, m, n, k, 1.25, A, ra, B, rb, -0.5, R, m, cnt)
+            got = fn(rows, cols, ri, cj, 1.25, -0.5 * C0[ri, cj], cnt)
+            np.testing.assert_array_equal(got, R[ri, cj])
+    assert np.all(np.isnan(got[ri == 5]))
+    with pytest.raises(ValueError):  # beyond one K chunk
+        oracle.ozaki2_pairs(np.ones((1, 65537)), np.ones((1, 65537)), [0], [0], 1.0, 0.0, 16)
+
+
 # --------------------------------------------------------------- BF16x3
 def test_bf16_rounding_matches_torch():
     import torch

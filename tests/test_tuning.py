@@ -44,8 +44,8 @@ def test_write_read_roundtrip_is_deterministic(tmp_path):
     # insertion order must not matter
     at.write_table(p2, dict(reversed(list(e.items()))), notes, "gfx950", ["header"])
     assert open(p1).read() == open(p2).read()
-    got, got_notes, arch = at.read_table(p1)
-    assert got == e and arch == "gfx950" and got_notes == notes
+    got, got_notes, arch, rocm = at.read_table(p1)
+    assert got == e and arch == "gfx950" and got_notes == notes and rocm == gx.build_rocm()
 
 
 @pytest.mark.parametrize("dtype", ["d", "s"])
@@ -106,13 +106,14 @@ def test_reader_rejects_other_formats_and_keeps_the_previous_table(tmp_path, dty
         at.read_table(str(bad_format))
     # invalid lines are skipped, a later entry replaces an earlier one
     mixed = tmp_path / "mixed.txt"
-    mixed.write_text("format 1\n# comment\nf64 NN 13 13 13 2\nf64 NN 13 13 13 5  # later\n"
+    mixed.write_text(f"format 2\nrocm {gx.build_rocm()}\n# comment\n"
+                     "f64 NN 13 13 13 2\nf64 NN 13 13 13 5  # later\n"
                      "f32 NN 13 13 13 2\nf32 NN 13 13 13 5\n"
                      "f64 NN 3 13 13 1\nf64 NN 13 13 13 99\nf16 NN 13 13 13 1\n"
                      "f32 NN 3 13 13 1\nf32 NN 13 13 13 99\n")
     assert gx.load_tuning(str(mixed), dtype) == 4
     assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == 5
-    py, _, _ = at.read_table(str(mixed))
+    py, _, _, _ = at.read_table(str(mixed))
     assert py[("f64", "NN", 13, 13, 13)] == 5
     assert gx.load_tuning(None, dtype) == 0
     assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == 0
@@ -151,7 +152,7 @@ def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
     e[("emu", "TN", 12, 12, 12)] = ("native", 0)
     e[("emu", "NT", 14, 14, 10)] = ("ozaki", 6)
     at.write_table(path, e, {}, None)
-    got, _, _ = at.read_table(path)
+    got, _, _, _ = at.read_table(path)
     assert got == e
     assert gx.load_tuning(path, "d") == len(e)
     assert gx.tune_lookup_emulated("N", "N", 8192, 9000, 11000) == (gx.MODE_OZAKI2, 18)
@@ -162,7 +163,8 @@ def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
     # the native tile entries are unaffected
     assert gx.tune_lookup("d", "N", "N", 8192, 8192, 8192) == 6
     bad = tmp_path / "bad.txt"
-    bad.write_text("format 1\nemu NN 13 13 13 ozaki2 40\nemu NN 13 13 13 ozaki 2\n"
+    bad.write_text(f"format 2\nrocm {gx.build_rocm()}\nemu NN 13 13 13 ozaki2 40\n"
+                   "emu NN 13 13 13 ozaki 2\n"
                    "emu NN 13 13 13 fp16 4\nemu NN 13 13 13 ozaki2 20\n")
     assert gx.load_tuning(str(bad), "d") == 1
     assert gx.tune_lookup_emulated("N", "N", 8192, 8192, 8192) == (gx.MODE_OZAKI2, 20)
@@ -183,7 +185,7 @@ def test_batch_entry_drives_the_batch_configuration(tmp_path, dtype, restore_tab
     e[("batch", "f32" if dtype == "d" else "f64")] = 3
     path = str(tmp_path / "t.txt")
     at.write_table(path, e, {("batch", tag): "mix"}, None)
-    got, notes, _ = at.read_table(path)
+    got, notes, _, _ = at.read_table(path)
     assert got == e and notes[("batch", tag)] == "mix"
     try:
         assert gx.load_tuning(path, dtype) == len(e)
@@ -206,3 +208,77 @@ def test_batch_entry_drives_the_batch_configuration(tmp_path, dtype, restore_tab
         assert gx.batch_config(dtype) == 0
     finally:
         gx.set_batch_config(dtype, -1)
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_rocm_release_is_a_checked_key(tmp_path, dtype, restore_tables):
+    """A table names the ROCm release whose compiler built the kernels it was
+    measured with: the libraries use it only when that is their own release,
+    and refuse a table without the key (the previous table stays)."""
+    rocm = gx.build_rocm()
+    major, minor = (int(v) for v in rocm.split("."))
+    good = str(tmp_path / "good.txt")
+    at.write_table(good, _entries(), {}, "gfx950")
+    assert f"\nrocm {rocm}\n" in open(good).read()
+    assert gx.load_tuning(good, dtype) == 5
+    want = 6 if dtype == "d" else 2
+    body = "f64 NN 13 13 13 1\nf32 NN 13 13 13 1\n"
+    for i, head in enumerate((f"format 2\nrocm {major}.{minor + 1}\n",
+                              f"format 2\nrocm {major + 1}.{minor}\n",
+                              "format 2\n", f"format 2\nrocm {major}\n")):
+        bad = tmp_path / f"bad{i}.txt"
+        bad.write_text(head + body)
+        assert gx.load_tuning(str(bad), dtype) == -1, head
+        assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == want
+    ok = tmp_path / "ok.txt"
+    ok.write_text(f"format 2\nrocm {rocm}\n" + body)
+    assert gx.load_tuning(str(ok), dtype) == 2
+    assert gx.tune_lookup(dtype, "N", "N", 8192, 8192, 8192) == 1
+    # the shipped table carries the release the libraries are built with
+    shipped = os.path.join(os.path.dirname(at.__file__), "tuning", "gemm_mi355x_tune_gfx950.txt")
+    assert at.read_table(shipped)[3] == rocm
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_split_column_forced_split_and_heuristic(tmp_path, dtype, restore_tables):
+    """The optional split column of a tile entry sets the number of K ranges
+    of a split-K launch; without it a heuristic fills about four workgroups
+    per CU when the tiles give fewer than two, with K ranges of at least 256;
+    a forced split overrides both; configurations without split-K kernels,
+    and shapes with enough tiles, run in one pass."""
+    es = 8 if dtype == "d" else 4
+    tag = "f64" if dtype == "d" else "f32"
+    e = {(tag, "NN", 9, 9, 15): (1, 16), (tag, "NN", 10, 10, 10): (1, 1),
+         (tag, "TN", 10, 10, 10): 1, (tag, "NN", 12, 12, 12): (2, 8)}
+    path = str(tmp_path / "t.txt")
+    at.write_table(path, e, {}, None)
+    assert at.read_table(path)[0] == e
+    assert gx.load_tuning(path, dtype) == len(e)
+    ls = lambda *a: gx.tune_lookup_split(dtype, *a)
+    # table: 16 ranges of 4096 at 512 x 512 x 65536 on 64 x 64 tiles
+    assert ls("N", "N", 512, 512, 65536) == (16, 16 * 64 * 64 * 64 * es)
+    assert ls("N", "N", 1024, 1024, 1024) == (1, 0)  # split 1 in the table
+    # no split in the entry: heuristic, 256 tiles of 64x64 -> 4 ranges of 256
+    assert ls("T", "N", 1024, 1024, 1024) == (4, 4 * 256 * 64 * 64 * es)
+    # configuration 2 has no split-K kernels
+    assert ls("N", "N", 4096, 4096, 4096) == (1, 0)
+    # outside the table: 8192^2 has enough tiles; 100^2 x 5000: 4 tiles of
+    # 64 x 64 -> min(256, 5000 // 256) = 19 ranges of ceil(5000/19) -> 272
+    assert ls("N", "N", 8192, 8192, 8192) == (1, 0)
+    assert ls("N", "N", 100, 100, 5000)[0] == -(-5000 // 272)
+    assert ls("N", "N", 100, 100, 300) == (1, 0)  # K too short to split
+    gx.set_split_config(dtype, 4)
+    try:
+        assert ls("N", "N", 512, 512, 65536)[0] == 4
+        assert ls("N", "N", 8192, 8192, 8192)[0] == 4 if gx.tune_lookup(
+            dtype, "N", "N", 8192, 8192, 8192) in at.SPLIT_CONFIGS else True
+        assert ls("N", "N", 4096, 4096, 4096) == (1, 0)  # configuration 2
+        # 100 x 100 x 50: ranges of whole 16-deep slices, so 4 -> 4 ranges of 16
+        assert ls("N", "N", 100, 100, 50)[0] == 4
+        with pytest.raises(gx.GemmError):
+            gx.set_split_config(dtype, 0)
+        with pytest.raises(gx.GemmError):
+            gx.set_split_config(dtype, at.MAX_SPLIT + 1)
+    finally:
+        gx.set_split_config(dtype, -1)
+    assert ls("N", "N", 512, 512, 65536)[0] == 16
