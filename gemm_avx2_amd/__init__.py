@@ -79,11 +79,12 @@ def _declare(lib, dtype):
     f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_i64_p, _c_i64_p, _c_i64_p, flp,
                   ctypes.c_void_p, _c_i64_p, ctypes.c_void_p, _c_i64_p, flp, ctypes.c_void_p,
                   _c_i64_p]
-    f = getattr(lib, f"{p}gemm_x_dev")
-    f.restype = ctypes.c_int
-    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, fl,
-                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, fl,
-                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    for name in (f"{p}gemm_x_dev",):
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                      fl, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, fl,
+                      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     f = getattr(lib, f"{p}gemm_x_strided_batched_dev")
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, fl,

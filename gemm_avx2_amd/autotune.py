@@ -18,9 +18,9 @@ inputs (|difference| within 64 k u (|alpha| k + |beta|), the bound for
 U[-1,1) inputs).  For DGEMM shapes with min(m, n, k) >= 1024 it also picks
 the arithmetic of dgemm_'s automatic mode: Ozaki-II with the smallest
 modulus count whose error is no larger than native FP64's on phi in
-{0, 1, 4} inputs (relative Frobenius and largest componentwise error over
-2048 sampled entries, two seeds), when that is faster than native (an `emu`
-entry), and the grouped-batch
+{0, 1, 4} inputs (relative Frobenius error, and half of native's largest
+componentwise error, over 2048 sampled entries on two seeds), when that is
+faster than native (an `emu` entry), and the grouped-batch
 kernel configuration per element type on a mixed batch (a `batch` entry).
 With --resume, keys already
 in the output table are kept and not measured again.  The table is written in
@@ -341,6 +341,11 @@ PHIS = (0.0, 1.0, 4.0)
 MODULI = tuple(range(14, 23))
 EMU_SEEDS = (41, 43)
 EMU_SAMPLES = 2048
+# The largest componentwise error over a few thousand sampled entries varies
+# between inputs of one distribution by up to a factor of about 1.5 (measured
+# on 8192^3, phi = 4), so an emulated form must stay a factor 2 below native's
+# on that measure to match it on other inputs too.
+EMU_COMP_MARGIN = 0.5
 
 
 def _two_prod(a, b):
@@ -392,9 +397,10 @@ def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_S
     """DGEMM arithmetic for one shape: native FP64, or Ozaki-II with the
     smallest modulus count whose error at `samples` random entries (alpha 1,
     beta 0, against Dot2 values of the same inputs) is no larger than
-    native's -- in relative Frobenius norm and in the largest error relative
-    to sum |a||b| -- for entries (u - 0.5) exp(phi g), phi in {0, 1, 4}, on
-    every seed, if it is faster.  Returns (scheme, count, note)."""
+    native's in relative Frobenius norm and at most EMU_COMP_MARGIN times
+    native's largest error relative to sum |a||b|, for entries
+    (u - 0.5) exp(phi g), phi in {0, 1, 4}, on every seed, if it is faster.
+    Returns (scheme, count, note)."""
     import numpy as np
     ra, ca = (m, k) if trans[0] == "N" else (k, m)
     rb, cb = (k, n) if trans[1] == "N" else (n, k)
@@ -450,7 +456,8 @@ def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_S
             en = err(native)
             for nm in sorted(qual):
                 e = err(lambda nm=nm: emulated(nm))
-                if e["rel_fro"] > en["rel_fro"] or e["max_comp"] > en["max_comp"]:
+                if (e["rel_fro"] > en["rel_fro"] or
+                        e["max_comp"] > EMU_COMP_MARGIN * en["max_comp"]):
                     qual.discard(nm)
     t_nat = timed(native)
     if not qual:

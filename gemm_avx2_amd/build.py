@@ -120,6 +120,40 @@ def build(jobs: int = 8, verbose: bool = False, lib_dir: str = LIB, build_dir: s
     return outputs
 
 
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+             "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(out_dir: str, build_dir: str = BUILD, verbose: bool = False) -> dict:
+    """The host C layer compiled with AddressSanitizer and UBSan (gcc, host
+    code only) and linked with the gfx950 objects of the last build() into
+    out_dir/{DGEMM,SGEMM}.so, for the CPU-side sanitizer tests.  The HIP
+    objects are used as built (GPU code is never sanitized here)."""
+    hip_objs = [os.path.join(build_dir, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+    missing = [o for o in hip_objs if not os.path.exists(o)]
+    if missing:
+        raise RuntimeError(f"build() first: {missing[0]} is missing")
+    os.makedirs(out_dir, exist_ok=True)
+    objs = []
+    for src in C_SOURCES:
+        o = os.path.join(out_dir, os.path.splitext(src)[0] + ".san.o")
+        _run(["gcc", *CC_FLAGS, *SAN_FLAGS, "-c", os.path.join(CSRC, src), "-o", o], verbose)
+        objs.append(o)
+    outputs = {}
+    for lib, syms in EXPORTS.items():
+        vs = os.path.join(out_dir, lib + ".map")
+        with open(vs, "w") as f:
+            f.write("{\n  global:\n" + "".join(f"    {s};\n" for s in syms) + "  local: *;\n};\n")
+        out = os.path.join(out_dir, lib)
+        _run(["g++", "-shared", "-fPIC", *SAN_FLAGS, "-o", out, *hip_objs, *objs,
+              f"-Wl,--version-script={vs}", f"-L{ROCM}/lib", "-lamdhip64", "-lpthread", "-ldl"],
+             verbose)
+        outputs[lib] = out
+    if os.path.exists(SHIPPED_TABLE):
+        shutil.copyfile(SHIPPED_TABLE, os.path.join(out_dir, "gemm_mi355x_tune.txt"))
+    return outputs
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("-j", "--jobs", type=int, default=8)

@@ -5,12 +5,14 @@
  * src/gemm_driver.c:118); every problem keeps dgemm_'s semantics, including
  * the beta != 1 rule and the alpha == 0 gate.
  *
- * The grid is as large as the device holds at once; each workgroup takes the
- * next tile with an atomic ticket until the tickets run out, so every wave
- * reaches the exit.  The ticket for the next tile is requested when a tile
- * starts, so its latency hides behind the tile's work.  Tiles are numbered
- * heaviest problem first (the host sorts), which with dynamic tickets
- * approximates longest-processing-time scheduling.  A tile is computed by
+ * The grid is as large as the device holds at once.  The host deals whole
+ * problems, heaviest first, to eight queues, one per XCD (longest processing
+ * time first: each problem to the queue with the least work), so the tiles
+ * of a problem share the operand panels in one XCD's L2; a workgroup takes
+ * the next tile of its XCD's queue with an atomic ticket, and when that queue
+ * is empty, of the other queues in turn, so every wave reaches the exit
+ * after at most eight empty queues.  The ticket for the next tile is
+ * requested when a tile starts, so its latency hides behind the tile's work.  A tile is computed by
  * the same gemm_tile as the single-GEMM kernels: the branch-free loop with
  * an edge tile's loads clamped back into the operand, in 16-byte chunks
  * where both operands allow them and one element per load otherwise,
@@ -33,7 +35,11 @@ template <> struct BatchCfg<1> { static constexpr int BM = 64, BN = 64, BK = 32,
 template <> struct BatchCfg<2> { static constexpr int BM = 128, BN = 64, BK = 16, WM = 2, WN = 2; };
 template <> struct BatchCfg<3> { static constexpr int BM = 32, BN = 32, BK = 32, WM = 2, WN = 2; };
 
-__global__ void batch_reset_kernel(unsigned *counter) { *counter = 0u; }
+/* Zeroes the ticket counters of the XCD queues (a kernel, not a memset
+ * node, so a captured graph replays it reliably). */
+__global__ void batch_reset_kernel(unsigned *sched) {
+  if (threadIdx.x < GEMMX_BATCH_QUEUES) sched[threadIdx.x * GEMMX_BATCH_CTR_STRIDE] = 0u;
+}
 
 /* Operand pointers read from the descriptors are generic; building them in
  * the global address space lets every load through them be a global (not
@@ -46,22 +52,44 @@ __device__ __forceinline__ T *as_global(T *p) {
 template <typename T, int CFG>
 __global__ void __launch_bounds__(BT_THREADS, 2)
 gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
-                  const int *__restrict__ tile_prob, int total,
-                  unsigned *__restrict__ counter) {
+                  const int *__restrict__ tile_prob, unsigned *__restrict__ sched,
+                  gemmx_batch_queues Q) {
   typedef BatchCfg<CFG> G;
   static_assert(64 * G::WM * G::WN == BT_THREADS, "batch block size");
   __shared__ __align__(16) T lds[tile_lds_max<T, G::BM, G::BN, G::BK>()];
   __shared__ int ticket;
-  if (threadIdx.x == 0) ticket = (int)atomicAdd(counter, 1u);
-  __syncthreads();
-  int t = __builtin_amdgcn_readfirstlane(ticket);
-  while (t < total) {
-    /* request the next tile now; the value is needed only after this one */
+  /* sched: the queues' ticket counters, GEMMX_BATCH_CTR_STRIDE words apart
+   * (each on a line of its own); Q: the queues' first tiles, passed by value
+   * so that no tile waits on a load from a line the atomics keep busy.
+   * Workgroups are dispatched round-robin over the XCDs, so blockIdx.x % 8 is
+   * this workgroup's XCD: it takes the tiles of its XCD's queue (whole
+   * problems, so their operand panels stay in that XCD's L2), then those of
+   * the other queues. */
+  const int x = (int)(blockIdx.x % GEMMX_BATCH_QUEUES);
+  auto fetch = [&](int queue) {
+    __syncthreads(); /* every wave has read `ticket` and left the LDS */
+    if (threadIdx.x == 0)
+      ticket = (int)atomicAdd(sched + queue * GEMMX_BATCH_CTR_STRIDE, 1u);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(ticket);
+  };
+  int y = x, round = 0;
+  int t = fetch(y);
+  for (;;) {
+    /* an exhausted queue: move to the next one; after all of them, exit
+     * (uniform: every wave of the workgroup leaves together) */
+    while (t >= Q.qbeg[y + 1] - Q.qbeg[y]) {
+      if (++round == GEMMX_BATCH_QUEUES) return;
+      y = (x + round) % GEMMX_BATCH_QUEUES;
+      t = fetch(y);
+    }
+    /* request the next tile of this queue now; needed only after this one */
     unsigned next = 0;
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
-    const int p = __builtin_amdgcn_readfirstlane(tile_prob[t]);
+    if (threadIdx.x == 0) next = atomicAdd(sched + y * GEMMX_BATCH_CTR_STRIDE, 1u);
+    const int tt = Q.qbeg[y] + t;
+    const int p = __builtin_amdgcn_readfirstlane(tile_prob[tt]);
     const gemmx_batch_desc &q = desc[p];
-    const int local = t - q.tile_begin;
+    const int local = tt - q.tile_begin;
     const int tm = local % q.tiles_m, tn = local / q.tiles_m;
     const int m0 = tm * G::BM, n0 = tn * G::BN;
     const T alpha = (T)q.alpha, beta = (T)q.beta;
@@ -117,10 +145,10 @@ int batch_grid() {
 }
 
 template <typename T, int CFG>
-void launch(const gemmx_batch_desc *d, const int *tp, int total, unsigned *ctr, int grid,
-            hipStream_t s) {
+void launch(const gemmx_batch_desc *d, const int *tp, unsigned *sched,
+            const gemmx_batch_queues &q, int grid, hipStream_t s) {
   hipLaunchKernelGGL((gemm_batch_kernel<T, CFG>), dim3(grid), dim3(BT_THREADS), 0, s, d, tp,
-                     total, ctr);
+                     sched, q);
 }
 
 } // namespace
@@ -146,15 +174,15 @@ extern "C" int gemmx_batch_grid(int es, int cfg) {
 }
 
 extern "C" hipError_t gemmx_batch_launch(int es, int cfg, const gemmx_batch_desc *d_desc,
-                                         const int *d_tile_prob, int total_tiles,
-                                         unsigned *d_counter, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, d_counter);
+                                         const int *d_tile_prob, unsigned *d_sched,
+                                         const gemmx_batch_queues *q, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_sched);
   const bool d = es == 8;
   switch (cfg) {
-  case 0: (d ? launch<double, 0> : launch<float, 0>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
-  case 1: (d ? launch<double, 1> : launch<float, 1>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
-  case 2: (d ? launch<double, 2> : launch<float, 2>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
-  case 3: (d ? launch<double, 3> : launch<float, 3>)(d_desc, d_tile_prob, total_tiles, d_counter, grid, s); break;
+  case 0: (d ? launch<double, 0> : launch<float, 0>)(d_desc, d_tile_prob, d_sched, *q, grid, s); break;
+  case 1: (d ? launch<double, 1> : launch<float, 1>)(d_desc, d_tile_prob, d_sched, *q, grid, s); break;
+  case 2: (d ? launch<double, 2> : launch<float, 2>)(d_desc, d_tile_prob, d_sched, *q, grid, s); break;
+  case 3: (d ? launch<double, 3> : launch<float, 3>)(d_desc, d_tile_prob, d_sched, *q, grid, s); break;
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -136,7 +136,7 @@ extern "C" size_t gemmx_native_workspace(int es, int cfg, int m, int n, int k, i
   const int ns = gemmx_native_splits(cfg, k, split);
   if (ns <= 1 || m <= 0 || n <= 0) return 0;
   const NativeCfg &c = kNativeCfg[cfg];
-  return (size_t)ns * (size_t)((m + c.bm - 1) / c.bm) * (size_t)((n + c.bn - 1) / c.bn) *
+  return (size_t)ns * (((size_t)m + c.bm - 1) / c.bm) * (((size_t)n + c.bn - 1) / c.bn) *
          (size_t)c.bm * (size_t)c.bn * (size_t)es;
 }
 extern "C" hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n,

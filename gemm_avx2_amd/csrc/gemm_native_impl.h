@@ -131,7 +131,8 @@ hipError_t launch_trans(const gemmx_problem *p, int split, void *work, hipStream
   const int vecB = ((uintptr_t)p->b % 16 == 0) && (p->ldb % VE == 0) && (sb % VE == 0);
   const T alpha = (T)p->alpha, beta = (T)p->beta;
   const dim3 block(64 * WM * WN);
-  const int tiles_m = (p->m + BM - 1) / BM, tiles_n = (p->n + BN - 1) / BN;
+  const int tiles_m = (int)(((int64_t)p->m + BM - 1) / BM);
+  const int tiles_n = (int)(((int64_t)p->n + BN - 1) / BN);
   const int full_m = p->m / BM, full_n = p->n / BN;
   const int lead = tiles_m * tiles_n - full_m * full_n; /* edge tiles */
   if constexpr (SPLITOK) if (split > 1 && batch == 1 && work) {

@@ -416,7 +416,7 @@ hipError_t launch_dl2(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  const int tiles_m = (m + 255) / 256, tiles_n = (n + 255) / 256;
+  const int tiles_m = (int)(((int64_t)m + 255) / 256), tiles_n = (int)(((int64_t)n + 255) / 256);
   hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n * G.ngroups), dim3(512), lds, st, as, bs, m,
                      n, kpad, kb0, kb1, G, tiles_m, tiles_n, S, C, ldc, alpha, beta);
   return hipGetLastError();
@@ -431,7 +431,8 @@ hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
   hipError_t e = hipFuncSetAttribute((const void *)kern,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
+  const int tiles_m = (int)(((int64_t)m + BM - 1) / BM);
+  const int tiles_n = (int)(((int64_t)n + BN - 1) / BN);
   hipLaunchKernelGGL(kern, dim3(tiles_m * tiles_n * G.ngroups), dim3(64 * WM * WN), lds,
                      st, as, bs, m, n, kpad, kb0, kb1, G, tiles_m, tiles_n, S, C, ldc,
                      alpha, beta);

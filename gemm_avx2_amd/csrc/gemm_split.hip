@@ -696,7 +696,7 @@ hipError_t launch_ozaki_split(const double *x, int64_t vld, int nvec, int len, i
 template <int S>
 hipError_t launch_split_strided(const double *x, int64_t eld, int nvec, int len,
                                 int64_t kpad, const int *ex, int8_t *out, hipStream_t st) {
-  hipLaunchKernelGGL((ozaki_split_strided_kernel<S>), dim3((nvec + 31) / 32, (unsigned)(kpad / 128)),
+  hipLaunchKernelGGL((ozaki_split_strided_kernel<S>), dim3((unsigned)(((int64_t)nvec + 31) / 32), (unsigned)(kpad / 128)),
                      dim3(256), 0, st, x, eld, nvec, len, kpad, ex, out);
   return hipGetLastError();
 }
@@ -736,17 +736,17 @@ hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
                          int k, int64_t kpad, int s, int *ex, unsigned long long *mbits,
                          int8_t *slices, hipStream_t st) {
   if (contiguous) {
-    hipLaunchKernelGGL(exp_contig_kernel, dim3((nvec + 3) / 4), dim3(256), 0, st, x,
+    hipLaunchKernelGGL(exp_contig_kernel, dim3((unsigned)(((int64_t)nvec + 3) / 4)), dim3(256), 0, st, x,
                        ld, nvec, k, ex);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return ozaki_split(s, x, ld, nvec, k, kpad, ex, slices, st);
   }
-  hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, mbits,
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)(((int64_t)nvec + 255) / 256)), dim3(256), 0, st, mbits,
                      nvec);
-  hipLaunchKernelGGL(absmax_strided_kernel, dim3((nvec + 63) / 64, (k + 255) / 256),
+  hipLaunchKernelGGL(absmax_strided_kernel, dim3((unsigned)(((int64_t)nvec + 63) / 64), (unsigned)(((int64_t)k + 255) / 256)),
                      dim3(256), 0, st, x, ld, nvec, k, mbits);
-  hipLaunchKernelGGL(exp_from_bits_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
+  hipLaunchKernelGGL(exp_from_bits_kernel, dim3((unsigned)(((int64_t)nvec + 255) / 256)), dim3(256), 0, st,
                      mbits, nvec, ex);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -772,18 +772,18 @@ hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec
                        dim3(256), 0, st, x, ld, nvec, k, kpad, ex, nmod, res);
     return hipGetLastError();
   }
-  const dim3 red((nvec + 63) / 64, (k + 255) / 256);
-  hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, mbits,
+  const dim3 red((unsigned)(((int64_t)nvec + 63) / 64), (unsigned)(((int64_t)k + 255) / 256));
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)(((int64_t)nvec + 255) / 256)), dim3(256), 0, st, mbits,
                      nvec);
-  hipLaunchKernelGGL(zero_u64_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st, sums, nvec);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)(((int64_t)nvec + 255) / 256)), dim3(256), 0, st, sums, nvec);
   hipLaunchKernelGGL(absmax_strided_kernel, red, dim3(256), 0, st, x, ld, nvec, k, mbits);
-  hipLaunchKernelGGL(exp_from_bits_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
+  hipLaunchKernelGGL(exp_from_bits_kernel, dim3((unsigned)(((int64_t)nvec + 255) / 256)), dim3(256), 0, st,
                      mbits, nvec, ex);
   hipLaunchKernelGGL(oz2_sums_strided_kernel, red, dim3(256), 0, st, x, ld, nvec, k, ex, sums);
-  hipLaunchKernelGGL(oz2_shift_from_sums_kernel, dim3((nvec + 255) / 256), dim3(256), 0, st,
+  hipLaunchKernelGGL(oz2_shift_from_sums_kernel, dim3((unsigned)(((int64_t)nvec + 255) / 256)), dim3(256), 0, st,
                      sums, nvec, L, ex);
   hipLaunchKernelGGL(ozaki2_split_strided_kernel<NW>,
-                     dim3((nvec + 31) / 32, (unsigned)(kpad / 128)), dim3(1024 / NW), 0, st, x,
+                     dim3((unsigned)(((int64_t)nvec + 31) / 32), (unsigned)(kpad / 128)), dim3(1024 / NW), 0, st, x,
                      ld, nvec, k, kpad, ex, nmod, res);
   return hipGetLastError();
 }
@@ -795,7 +795,7 @@ hipError_t bf16_prepare(const float *x, int64_t ld, bool contiguous, int nvec,
   if (contiguous)
     hipLaunchKernelGGL(bf16_split_kernel, grid, dim3(256), 0, st, x, ld, nvec, k, kpad, out);
   else /* kpad is a multiple of 64 */
-    hipLaunchKernelGGL(bf16_split_strided_kernel, dim3((nvec + 31) / 32, (unsigned)(kpad / 64)),
+    hipLaunchKernelGGL(bf16_split_strided_kernel, dim3((unsigned)(((int64_t)nvec + 31) / 32), (unsigned)(kpad / 64)),
                        dim3(256), 0, st, x, ld, nvec, k, kpad, out);
   return hipGetLastError();
 }

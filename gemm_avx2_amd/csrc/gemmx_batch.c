@@ -5,9 +5,10 @@
  *
  * A plan is built once from host arrays that describe the problems (device
  * pointers for the matrices): the arguments are checked, problems with work
- * are sorted heaviest first, their tiles numbered, the descriptors and the
- * tile->problem map copied to the device, and the launch (ticket reset +
- * persistent grouped kernel) captured into a hipGraph.  Running the plan
+ * are sorted heaviest first and dealt to one tile queue per XCD, their tiles
+ * numbered queue by queue, the descriptors and the tile->problem map copied
+ * to the device, and the launch (ticket reset + persistent grouped kernel,
+ * the queue boundaries as a kernel argument) captured into a hipGraph.  Running the plan
  * replays the graph on the caller's stream.
  */
 #define _GNU_SOURCE
@@ -25,7 +26,8 @@ struct gemmx_batch {
   double flops;
   gemmx_batch_desc *d_desc;
   int *d_tile_prob;
-  unsigned *d_counter;
+  unsigned *d_sched; /* the queues' ticket counters */
+  gemmx_batch_queues queues;
   hipStream_t capture;
   hipGraph_t graph;
   hipGraphExec_t exec;
@@ -49,7 +51,7 @@ static void destroy(struct gemmx_batch *p) {
   if (p->capture) (void)hipStreamDestroy(p->capture);
   if (p->d_desc) (void)hipFree(p->d_desc);
   if (p->d_tile_prob) (void)hipFree(p->d_tile_prob);
-  if (p->d_counter) (void)hipFree(p->d_counter);
+  if (p->d_sched) (void)hipFree(p->d_sched);
   free(p);
 }
 
@@ -148,41 +150,70 @@ static int create(int es, void **out, int count, const char *transa, const char 
                ((((uintptr_t)a[i] % 16) == 0 && lda[i] % ve == 0) ? GEMMX_BF_VEC_A : 0) |
                ((((uintptr_t)b[i] % 16) == 0 && ldb[i] % ve == 0) ? GEMMX_BF_VEC_B : 0) |
                (product ? 0 : GEMMX_BF_SCALE);
-    d->tiles_m = (m[i] + tbm - 1) / tbm;
-    d->tiles_n = (n[i] + tbn - 1) / tbn;
+    d->tiles_m = (int)(((long)m[i] + tbm - 1) / tbm);
+    d->tiles_n = (int)(((long)n[i] + tbn - 1) / tbn);
     ord[used].work = product ? (double)k[i] * m[i] * n[i] : 0.0;
     ord[used].idx = used;
     if (product) p->flops += 2.0 * m[i] * n[i] * k[i];
     used++;
   }
-  /* Heaviest first: tiles of long-K problems are handed out first. */
+  /* Heaviest first, each problem dealt whole to the XCD queue with the least
+   * work so far (longest processing time first); a queue holds its problems'
+   * tiles in that order, so long-K tiles are handed out first. */
   qsort(ord, used, sizeof *ord, by_work_desc);
   gemmx_batch_desc *sorted = (gemmx_batch_desc *)calloc(used > 0 ? used : 1, sizeof *sorted);
-  if (!sorted) {
+  int *queue_of = (int *)calloc(used > 0 ? used : 1, sizeof *queue_of);
+  if (!sorted || !queue_of) {
+    free(sorted);
+    free(queue_of);
     rc = gemmx_report(GEMMX_ERR_ALLOC, "batch: host allocation failed");
     goto fail;
   }
+  double qwork[GEMMX_BATCH_QUEUES] = {0};
+  long qtiles[GEMMX_BATCH_QUEUES] = {0};
   long total = 0;
   for (int i = 0; i < used; i++) {
     sorted[i] = h[ord[i].idx];
-    sorted[i].tile_begin = (int)total;
-    total += (long)sorted[i].tiles_m * sorted[i].tiles_n;
+    const long tiles = (long)sorted[i].tiles_m * sorted[i].tiles_n;
+    int best = 0;
+    for (int x = 1; x < GEMMX_BATCH_QUEUES; x++)
+      if (qwork[x] < qwork[best]) best = x;
+    /* a tile costs its K slices plus a fixed part (prologue, epilogue) */
+    qwork[best] += (double)tiles * ((sorted[i].flags & GEMMX_BF_SCALE) ? 16.0
+                                                                        : sorted[i].k + 64.0);
+    qtiles[best] += tiles;
+    queue_of[i] = best;
+    total += tiles;
   }
   free(h);
   h = sorted;
   if (total > 0x7fffffffL) {
+    free(queue_of);
     rc = gemmx_report(GEMMX_ERR_UNSUPPORTED, "batch: %ld tiles", total);
     goto fail;
   }
+  int *const qbeg = p->queues.qbeg;
+  qbeg[0] = 0;
+  for (int x = 0; x < GEMMX_BATCH_QUEUES; x++) qbeg[x + 1] = qbeg[x] + (int)qtiles[x];
   p->count = used;
   p->total_tiles = (int)total;
   tp = (int *)malloc(sizeof(int) * (total > 0 ? total : 1));
   if (!tp) {
+    free(queue_of);
     rc = gemmx_report(GEMMX_ERR_ALLOC, "batch: host allocation failed");
     goto fail;
   }
-  for (int i = 0; i < used; i++)
-    for (int t = 0; t < h[i].tiles_m * h[i].tiles_n; t++) tp[h[i].tile_begin + t] = i;
+  {
+    int fill[GEMMX_BATCH_QUEUES];
+    memcpy(fill, qbeg, sizeof fill);
+    for (int i = 0; i < used; i++) {
+      const int tiles = h[i].tiles_m * h[i].tiles_n;
+      h[i].tile_begin = fill[queue_of[i]];
+      for (int t = 0; t < tiles; t++) tp[h[i].tile_begin + t] = i;
+      fill[queue_of[i]] += tiles;
+    }
+  }
+  free(queue_of);
 
   /* Arguments are checked before the device is looked up. */
   if (gemmx_current_device(&p->device) != GEMMX_OK) {
@@ -192,7 +223,8 @@ static int create(int es, void **out, int count, const char *transa, const char 
   hipError_t e;
   if ((e = hipMalloc((void **)&p->d_desc, sizeof *h * (used > 0 ? used : 1))) != hipSuccess ||
       (e = hipMalloc((void **)&p->d_tile_prob, sizeof(int) * (total > 0 ? total : 1))) != hipSuccess ||
-      (e = hipMalloc((void **)&p->d_counter, sizeof(unsigned))) != hipSuccess) {
+      (e = hipMalloc((void **)&p->d_sched,
+                     sizeof(unsigned) * GEMMX_BATCH_QUEUES * GEMMX_BATCH_CTR_STRIDE)) != hipSuccess) {
     (void)hipGetLastError();
     rc = gemmx_report(GEMMX_ERR_ALLOC, "batch: device allocation failed: %s", hipGetErrorString(e));
     goto fail;
@@ -207,7 +239,10 @@ static int create(int es, void **out, int count, const char *transa, const char 
     rc = gemmx_report(GEMMX_ERR_LAUNCH, "batch: occupancy query failed");
     goto fail;
   }
-  if (p->grid > p->total_tiles) p->grid = p->total_tiles > 0 ? p->total_tiles : 1;
+  /* fewer workgroups than tiles are never needed; a multiple of the queue
+   * count keeps every XCD's workgroups on its own queue */
+  if (p->grid > p->total_tiles)
+    p->grid = (p->total_tiles + GEMMX_BATCH_QUEUES - 1) / GEMMX_BATCH_QUEUES * GEMMX_BATCH_QUEUES;
   /* Capture ticket reset + persistent kernel into a graph (no memset nodes:
    * the ticket is cleared by a kernel). */
   if ((e = hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking)) != hipSuccess ||
@@ -216,8 +251,8 @@ static int create(int es, void **out, int count, const char *transa, const char 
     goto fail;
   }
   const hipError_t le = p->total_tiles > 0
-                           ? gemmx_batch_launch(es, cfg, p->d_desc, p->d_tile_prob, p->total_tiles,
-                                                p->d_counter, p->grid, p->capture)
+                           ? gemmx_batch_launch(es, cfg, p->d_desc, p->d_tile_prob, p->d_sched,
+                                                &p->queues, p->grid, p->capture)
                            : hipSuccess;
   e = hipStreamEndCapture(p->capture, &p->graph);
   if (le != hipSuccess) e = le;

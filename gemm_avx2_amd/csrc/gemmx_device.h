@@ -149,6 +149,28 @@ struct TileCopy {
     }
   }
 
+  /* store() for the last slice of the branch-free loop, which is loaded
+   * shifted back so that it ends at K (every load in bounds): its first
+   * `skip` K positions were summed by the previous slice and are written as
+   * zeros, in both operands (so no 0 * Inf).  K runs along the lines
+   * (KLINES: line index = K) or along each line.  The selects sit at the LDS
+   * writes, where the loaded values are consumed anyway, so the loads stay
+   * in flight over the slice's MFMAs. */
+  template <int LS, bool KLINES>
+  __device__ __forceinline__ void store_skip(T *s, int tid, int skip) const {
+#pragma unroll
+    for (int c = 0; c < PER; ++c) {
+      const int ch = tid + c * NT, line = ch / CPL, off = (ch % CPL) * VE;
+      vec_t v = r[c];
+#pragma unroll
+      for (int e = 0; e < VE; ++e) {
+        const bool z = KLINES ? line < skip : off + e < skip;
+        v[e] = z ? T(0) : v[e];
+      }
+      *reinterpret_cast<vec_t *>(s + line * LS + off) = v;
+    }
+  }
+
   template <int LS>
   __device__ __forceinline__ void store(T *s, int tid) const {
 #pragma unroll

@@ -122,6 +122,7 @@ int gemmx_native_tile(int cfg, int *bm, int *bn);
  * and the bytes of partial tiles they need (0 for one pass). */
 int gemmx_native_splits(int cfg, int k, int split);
 size_t gemmx_native_workspace(int es, int cfg, int m, int n, int k, int split);
+
 /* C[0:m,0:n] *= beta (beta != 1), the reference's C_MULT_BETA
  * (src/gemm_driver.c:106-116), for `batch` matrices sc elements apart. */
 hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta, int batch,
@@ -156,10 +157,19 @@ typedef struct gemmx_batch_desc {
 #define GEMMX_BATCH_CONFIGS 4
 int gemmx_batch_tile(int cfg, int *bm, int *bn);
 int gemmx_batch_grid(int es, int cfg);
-/* Zeroes the tile ticket, then runs the persistent grouped kernel. */
+/* Tile queues of a grouped launch, one per XCD of MI355X: queue x is tiles
+ * [qbeg[x], qbeg[x+1]) of d_tile_prob.  d_sched holds the queues' ticket
+ * counters, GEMMX_BATCH_CTR_STRIDE words apart (GEMMX_BATCH_QUEUES *
+ * GEMMX_BATCH_CTR_STRIDE words).  The launch zeroes the counters, then runs
+ * the persistent grouped kernel. */
+#define GEMMX_BATCH_QUEUES 8
+#define GEMMX_BATCH_CTR_STRIDE 64
+typedef struct gemmx_batch_queues {
+  int qbeg[GEMMX_BATCH_QUEUES + 1];
+} gemmx_batch_queues;
 hipError_t gemmx_batch_launch(int es, int cfg, const gemmx_batch_desc *d_desc,
-                              const int *d_tile_prob, int total_tiles,
-                              unsigned *d_counter, int grid, hipStream_t s);
+                              const int *d_tile_prob, unsigned *d_sched,
+                              const gemmx_batch_queues *q, int grid, hipStream_t s);
 
 /* Emulated DGEMM, Ozaki-I INT8 slices (gemm_slices.hip).  `work` is device
  * scratch of gemmx_ozaki_workspace() bytes. */

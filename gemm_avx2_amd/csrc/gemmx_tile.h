@@ -18,10 +18,12 @@
  * are read with one 16-byte LDS read.
  *
  * FAST (interior tile, 16-byte loadable operands): a branch-free loop over
- * the full K slices with an explicit issue order -- the next slice's global
+ * the K slices with an explicit issue order -- the next slice's global
  * loads, each step's fragment reads spread over the previous step's MFMAs,
- * the next slice's LDS writes spread over the last step -- and one peeled
- * predicated slice for the K remainder.  FAST with CLAMP: the same loop for
+ * the next slice's LDS writes spread over the last step.  A K remainder is
+ * the loop's last slice, loaded as the BK positions ending at K with the
+ * positions the previous slice already summed zeroed (no peeled iteration:
+ * its extra registers made the 128x128 FP64 tile spill).  FAST with CLAMP: the same loop for
  * an edge tile, its loads past M or N clamped back into the operand
  * (TileCopy::load_clamp).  FAST with NARROW: the same loop for operands that
  * do not allow 16-byte loads, one load per element (TileCopy::load_narrow).
@@ -113,6 +115,12 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     ca.template store<A_LS>(sA(buf), tid);
     cb.template store<B_LS>(sB(buf), tid);
   };
+  /* the stored slice starts `skip` positions before the first K position
+   * not yet summed: those positions are written as zeros */
+  auto store_skip = [&](int buf, int skip) {
+    ca.template store_skip<A_LS, !TA>(sA(buf), tid, skip);
+    cb.template store_skip<B_LS, TB>(sB(buf), tid, skip);
+  };
 
   acc_t acc[FM][FN];
 #pragma unroll
@@ -165,49 +173,48 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
   };
 
   const int nfull = K / BK, nk = (K + BK - 1) / BK;
+  /* Issue order of one branch-free iteration: the next slice's global
+   * loads first, each MFMA step's fragment reads spread over the previous
+   * step's MFMAs, the next slice's LDS writes spread over the last step. */
+  auto issue_order = [&]() {
+    constexpr int NLD = TileCopy<T, A_NL, A_LEN, NT>::PER + TileCopy<T, B_NL, B_LEN, NT>::PER;
+    constexpr int NGL = NLD * (NARROW ? VE : 1); /* global loads per slice */
+    constexpr int NR = (TA ? FM : FM * KV) + (TB ? FN * KV : FN); /* LDS reads per KV steps */
+    constexpr int NMF = KV * FM * FN;                             /* MFMAs per KV steps */
+    __builtin_amdgcn_sched_group_barrier(0x020, NGL, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
+#pragma unroll
+    for (int s = KV; s < KS; s += KV) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF / NR, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NR) * NR, 0);
+    }
+#pragma unroll
+    for (int w = 0; w < NLD; ++w) {
+      __builtin_amdgcn_sched_group_barrier(0x008, NMF / NLD, 0);
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NLD) * NLD, 0);
+  };
   if (FAST && nfull >= 1) {
     load_fast(0);
     store(0);
     __syncthreads();
-    for (int kt = 0; kt + 1 < nfull; ++kt) {
-      load_fast((kt + 1) * BK);
+    for (int kt = 0; kt + 1 < nk; ++kt) {
+      /* slice kt + 1; the last one (a K remainder) is loaded as the BK
+       * positions ending at K, its first `skip` positions zeroed */
+      const int k0 = (kt + 1) * BK;
+      const int ks = k0 + BK <= K ? k0 : K - BK;
+      load_fast(ks);
       compute(kt & 1);
-      store((kt + 1) & 1);
-      {
-        constexpr int NLD = TileCopy<T, A_NL, A_LEN, NT>::PER + TileCopy<T, B_NL, B_LEN, NT>::PER;
-        constexpr int NGL = NLD * (NARROW ? VE : 1); /* global loads per slice */
-        constexpr int NR = (TA ? FM : FM * KV) + (TB ? FN * KV : FN); /* LDS reads per KV steps */
-        constexpr int NMF = KV * FM * FN;                             /* MFMAs per KV steps */
-        __builtin_amdgcn_sched_group_barrier(0x020, NGL, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
-#pragma unroll
-        for (int s = KV; s < KS; s += KV) {
-#pragma unroll
-          for (int r = 0; r < NR; ++r) {
-            __builtin_amdgcn_sched_group_barrier(0x008, NMF / NR, 0);
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          }
-          __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NR) * NR, 0);
-        }
-#pragma unroll
-        for (int w = 0; w < NLD; ++w) {
-          __builtin_amdgcn_sched_group_barrier(0x008, NMF / NLD, 0);
-          __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-        }
-        __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NLD) * NLD, 0);
-      }
+      store_skip((kt + 1) & 1, k0 - ks);
+      issue_order();
       __syncthreads();
     }
-    const int last = nfull - 1;
-    if (nk > nfull) {
-      load_safe(nfull * BK); /* the K remainder, zero-filled past K */
-      compute(last & 1);
-      store(nfull & 1);
-      __syncthreads();
-      compute(nfull & 1);
-    } else {
-      compute(last & 1);
-    }
+    compute((nk - 1) & 1);
   } else {
     load_safe(0);
     store(0);

@@ -245,7 +245,7 @@ int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {
                        [gemmx_tune_bucket(k) - 4];
   if (c > 0) return c - 1;
   /* Heuristic: keep at least ~2 workgroups per CU of the 256 on MI355X. */
-  const long tiles128 = (long)((m + 127) / 128) * ((n + 127) / 128);
+  const long tiles128 = (((long)m + 127) / 128) * (((long)n + 127) / 128);
   return tiles128 >= 512 ? GEMMX_NCFG_128x128 : GEMMX_NCFG_64x64;
 }
 
@@ -262,8 +262,8 @@ int gemmx_tune_pick_split(int es, int ta, int tb, int m, int n, int k, int cfg) 
   /* Heuristic: about 1024 workgroups (four per CU of the 256 on MI355X) when
    * the tiles alone fill fewer than two per CU, K ranges of at least 256, and
    * at most GEMMX_SPLIT_SCRATCH bytes of partial tiles. */
-  const long tiles = (long)((m + bm - 1) / bm) * ((n + bn - 1) / bn);
-  if (tiles >= 512) return 1;
+  const long tiles = (((long)m + bm - 1) / bm) * (((long)n + bn - 1) / bn);
+  if (tiles < 1 || tiles >= 512) return 1;
   long sp = 1024 / tiles;
   if (sp > k / 256) sp = k / 256;
   if (sp > GEMMX_MAX_SPLIT) sp = GEMMX_MAX_SPLIT;

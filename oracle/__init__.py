@@ -44,7 +44,7 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
-    path = os.path.join(HERE, "liboracle.so")
+    path = os.environ.get("GEMMX_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
     if not os.path.exists(path):
         build()
     L = ctypes.CDLL(path)

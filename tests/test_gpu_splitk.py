@@ -129,14 +129,23 @@ def test_split_semantics_and_repeatability(cuda, dt):
 
 
 @pytest.mark.parametrize("ta,tb", TRANS)
-@pytest.mark.parametrize("m,n,k", [(512, 512, 65536), (1024, 1024, 1024)])
-def test_library_split_at_full_size(cuda, ta, tb, m, n, k):
-    """FP64 with the split the library picks for these few-tile shapes, 2048
-    sampled entries against Dot2 values of the same inputs."""
+@pytest.mark.parametrize("m,n,k,force", [(512, 512, 65536, -1), (1024, 1024, 1024, 4)])
+def test_library_split_at_full_size(cuda, ta, tb, m, n, k, force):
+    """FP64 split-K at full size, 2048 sampled entries against Dot2 values of
+    the same inputs: 512 x 512 x 65536 with the split the library picks, and
+    1024^3 (which the tuning table runs in one pass) with 4 ranges forced."""
     import torch
+    gx.set_split_config("d", force)
     splits, _ = gx.tune_lookup_split("d", ta, tb, m, n, k)
     print(f"case dgemm {ta}{tb} {m}x{n}x{k} splits {splits}", flush=True)
-    assert splits > 1
+    try:
+        assert splits > 1
+        _full_size_case(torch, ta, tb, m, n, k)
+    finally:
+        gx.set_split_config("d", -1)
+
+
+def _full_size_case(torch, ta, tb, m, n, k):
     ra, ca = stored_shape(ta, m, k)
     rb, cb = stored_shape(tb, k, n)
     A = torch.empty(ra * ca, dtype=torch.float64, device="cuda")
