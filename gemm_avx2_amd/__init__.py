@@ -25,7 +25,7 @@ import os
 __all__ = [
     "LIB_DIR", "GemmError", "load", "dgemm", "sgemm", "dgemm64", "sgemm64", "zgemm", "cgemm",
     "zgemm3m", "cgemm3m", "dtrmm", "dtrsm", "strmm", "strsm", "dgemm_dev",
-    "sgemm_dev", "dgemm_strided_batched_dev", "sgemm_strided_batched_dev",
+    "sgemm_dev", "dgemm_dev_noalloc", "sgemm_dev_noalloc", "dgemm_strided_batched_dev", "sgemm_strided_batched_dev",
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "set_split_config",
@@ -79,7 +79,7 @@ def _declare(lib, dtype):
     f.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _c_i64_p, _c_i64_p, _c_i64_p, flp,
                   ctypes.c_void_p, _c_i64_p, ctypes.c_void_p, _c_i64_p, flp, ctypes.c_void_p,
                   _c_i64_p]
-    for name in (f"{p}gemm_x_dev",):
+    for name in (f"{p}gemm_x_dev", f"{p}gemm_x_dev_noalloc"):
         f = getattr(lib, name)
         f.restype = ctypes.c_int
         f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -376,6 +376,26 @@ def sgemm_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stre
     lib = load("SGEMM.so")
     rc = lib.sgemm_x_dev(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda, _addr(b),
                          ldb, beta, _addr(c), ldc, _stream(stream))
+    if rc:
+        _check("s", rc)
+
+
+def dgemm_dev_noalloc(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                      stream=None):
+    """dgemm_dev without any device allocation (split-K partials in the
+    library's static device scratch; overlapping calls on one GPU race)."""
+    lib = load("DGEMM.so")
+    rc = lib.dgemm_x_dev_noalloc(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda,
+                                 _addr(b), ldb, beta, _addr(c), ldc, _stream(stream))
+    if rc:
+        _check("d", rc)
+
+
+def sgemm_dev_noalloc(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                      stream=None):
+    lib = load("SGEMM.so")
+    rc = lib.sgemm_x_dev_noalloc(_ch(transa), _ch(transb), m, n, k, alpha, _addr(a), lda,
+                                 _addr(b), ldb, beta, _addr(c), ldc, _stream(stream))
     if rc:
         _check("s", rc)
 

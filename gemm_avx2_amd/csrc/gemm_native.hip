@@ -51,6 +51,8 @@
  * loads first, each MFMA step's fragment reads spread over the previous
  * step's MFMAs, and the next slice's LDS writes spread over the last step.
  */
+#include <pthread.h>
+
 #include "gemm_native_impl.h"
 
 using namespace gemmx;
@@ -105,6 +107,35 @@ hipError_t launch_scale(T *c, int64_t ldc, int m, int n, T beta, int batch, int6
 }
 
 } // namespace
+
+/* Device memory of the allocation-free entry points (dgemm_x_dev_noalloc,
+ * sgemm_x_dev_noalloc): the split-K partial tiles of their calls.  One array
+ * per GPU the library runs on, held until the process ends; every such call
+ * on a GPU uses the same array, so calls that overlap on one GPU (on
+ * different streams, or from several host threads) race.  Nothing in it has
+ * to start at zero: the partials are written before they are read. */
+static __device__ __align__(256) unsigned char scratch[GEMMX_NOALLOC_SCRATCH];
+
+/* The address of `scratch` on each device: the array's name is a host-side
+ * stand-in, so its device address is taken once per device, with that device
+ * current, and only that address is handed to kernels. */
+static void *g_scratch_ptr[GEMMX_NOALLOC_MAX_DEV];
+static pthread_mutex_t g_scratch_lock = PTHREAD_MUTEX_INITIALIZER;
+
+extern "C" hipError_t gemmx_noalloc_scratch(int dev, void **ptr) {
+  if (dev < 0 || dev >= GEMMX_NOALLOC_MAX_DEV) return hipErrorInvalidDevice;
+  pthread_mutex_lock(&g_scratch_lock);
+  hipError_t e = hipSuccess;
+  if (!g_scratch_ptr[dev]) {
+    void *p = NULL;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) g_scratch_ptr[dev] = p;
+    else if (e == hipSuccess) e = hipErrorInvalidSymbol;
+  }
+  *ptr = g_scratch_ptr[dev];
+  pthread_mutex_unlock(&g_scratch_lock);
+  return e;
+}
 
 extern "C" hipError_t gemmx_native_f64(const gemmx_problem *p, int cfg, int split, void *work,
                                        hipStream_t s) {

@@ -990,6 +990,49 @@ EXPORT int sgemm_x_dev(char transa, char transb, int m, int n, int k,
   return dev_call(&x, GEMMX_MODE_NATIVE, 0, NULL, 0, stream);
 }
 
+/* The allocation-free form of dgemm_x_dev / sgemm_x_dev: the same kernels
+ * and split choice, the split-K partial tiles in the library's static device
+ * scratch instead of the stream-ordered pool; no device allocation or free
+ * of any kind.  A call whose partial tiles exceed the scratch is refused. */
+static int noalloc_call(call_t *x, void *stream) {
+  clear_error();
+  if (validate(x) != GEMMX_OK) return t_err;
+  if (x->m == 0 || x->n == 0) return GEMMX_OK;
+  int dev;
+  if (current_device(&dev) != GEMMX_OK) return t_err;
+  gemmx_problem p = {.ta = !notrans(x->ta), .tb = !notrans(x->tb), .m = x->m,
+                     .n = x->n, .k = x->k, .lda = x->lda, .ldb = x->ldb,
+                     .ldc = x->ldc, .alpha = x->alpha, .beta = x->beta,
+                     .a = x->a, .b = x->b, .c = x->c};
+  const size_t need = call_ws(x, GEMMX_MODE_NATIVE, 0);
+  void *w = NULL;
+  if (need > 0) {
+    if (need > (size_t)GEMMX_NOALLOC_SCRATCH)
+      return set_error(GEMMX_ERR_UNSUPPORTED,
+                       "%cgemm_x_dev_noalloc: the split-K partial tiles of this call need %zu "
+                       "bytes, more than the %u bytes of static scratch (use %cgemm_x_dev)",
+                       x->es == 8 ? 'd' : 's', need, (unsigned)GEMMX_NOALLOC_SCRATCH,
+                       x->es == 8 ? 'd' : 's');
+    const hipError_t e = gemmx_noalloc_scratch(dev, &w);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  }
+  return launch_device(&p, x->es, GEMMX_MODE_NATIVE, 0, w, (hipStream_t)stream);
+}
+
+EXPORT int dgemm_x_dev_noalloc(char transa, char transb, int m, int n, int k, double alpha,
+                               const double *a, int lda, const double *b, int ldb,
+                               double beta, double *c, int ldc, void *stream) {
+  call_t x = {8, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return noalloc_call(&x, stream);
+}
+
+EXPORT int sgemm_x_dev_noalloc(char transa, char transb, int m, int n, int k, float alpha,
+                               const float *a, int lda, const float *b, int ldb, float beta,
+                               float *c, int ldc, void *stream) {
+  call_t x = {4, transa, transb, m, n, k, lda, ldb, ldc, alpha, beta, a, b, c};
+  return noalloc_call(&x, stream);
+}
+
 /* Strided batch on device pointers, native arithmetic: `batch` problems of
  * one shape in one set of launches (grid row per problem). */
 static int strided_call(call_t *x, int64_t sa, int64_t sb, int64_t sc, int batch,

@@ -122,7 +122,13 @@ int gemmx_native_tile(int cfg, int *bm, int *bn);
  * and the bytes of partial tiles they need (0 for one pass). */
 int gemmx_native_splits(int cfg, int k, int split);
 size_t gemmx_native_workspace(int es, int cfg, int m, int n, int k, int split);
-
+/* Static device scratch of the allocation-free entry points (a file-scope
+ * array in gemm_native.hip, GEMMX_NOALLOC_SCRATCH bytes per device): its
+ * address on device `dev`, which must be current; taken on the first call
+ * per device. */
+#define GEMMX_NOALLOC_SCRATCH (64u << 20)
+#define GEMMX_NOALLOC_MAX_DEV 64
+hipError_t gemmx_noalloc_scratch(int dev, void **ptr);
 /* C[0:m,0:n] *= beta (beta != 1), the reference's C_MULT_BETA
  * (src/gemm_driver.c:106-116), for `batch` matrices sc elements apart. */
 hipError_t gemmx_scale_f64(double *c, int64_t ldc, int m, int n, double beta, int batch,

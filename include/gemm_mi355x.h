@@ -226,6 +226,24 @@ int sgemm_x_dev(char transa, char transb, int m, int n, int k, float alpha,
                 const float *a, int lda, const float *b, int ldb, float beta,
                 float *c, int ldc, void *stream);
 
+/* Allocation-free form of dgemm_x_dev / sgemm_x_dev.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels, split-K choice and results as the _x_dev form, but the
+ * split-K partial tiles go to a static device array of the library
+ * (64 MiB per GPU, held until the process ends) instead of the stream-ordered
+ * pool; the call makes no device allocation or free.  A call whose partial
+ * tiles would exceed 64 MiB (only a tuning-table or forced split can ask for
+ * that) returns GEMMX_ERR_UNSUPPORTED and leaves C untouched.  All such calls
+ * on one GPU share the array: calls that overlap on a GPU (on different
+ * streams or from several host threads) race. */
+int dgemm_x_dev_noalloc(char transa, char transb, int m, int n, int k, double alpha,
+                        const double *a, int lda, const double *b, int ldb, double beta,
+                        double *c, int ldc, void *stream);
+int sgemm_x_dev_noalloc(char transa, char transb, int m, int n, int k, float alpha,
+                        const float *a, int lda, const float *b, int ldb, float beta,
+                        float *c, int ldc, void *stream);
+
 /* Strided batch on device pointers (no counterpart in the reference; the
  * BLAS-style batched interface): `batch` problems of one shape, problem i
  * reading A + i*stride_a and B + i*stride_b and updating C + i*stride_c
