@@ -60,7 +60,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--profile-only",
-                    choices=["native", "ozaki", "ozaki2", "sgemm", "bf16x3", "batch", "copy"],
+                    choices=["native", "ozaki", "ozaki2", "sgemm", "bf16x3", "batch", "batch32",
+                             "splitk", "copy"],
                     help="run just one form (for rocprofv3)")
     ap.add_argument("--slices", type=int, default=8,
                     help="slice count (ozaki) or modulus count (ozaki2) for --profile-only")
@@ -217,9 +218,13 @@ def cpu_baseline_child():
     if info["ghz"]:
         # two 256-bit FMA pipes per core: 16 FP64 flops per cycle with AVX2
         peak = threads * info["ghz"] * 16 / 1e3
-        out.update({"nominal_ghz": info["ghz"], "ghz_source": info["ghz_source"],
-                    "nominal_avx2_peak_tflops": round(peak, 4),
-                    "frac_of_nominal_peak": round(out["value"] / peak, 3)})
+        out.update({"clock_ghz": info["ghz"], "ghz_source": info["ghz_source"],
+                    "avx2_peak_tflops_at_that_clock": round(peak, 4),
+                    "frac_of_that_peak": round(out["value"] / peak, 3)})
+        if not info["ghz_source"].startswith("cpufreq"):
+            out["clock_note"] = ("no cpufreq on this host: the clock is the one /proc/cpuinfo "
+                                 "reports, which boost clocks exceed, so the fraction can "
+                                 "pass 1")
     print(json.dumps(out), flush=True)
 
 
@@ -558,14 +563,13 @@ def run_bf16x3(torch, gx, stream):
 BATCH_COUNT, BATCH_LO, BATCH_HI = 4096, 16, 1024
 
 
-def batch_workload(torch, gx, dtype, stream, seed=11):
-    """4096 independent GEMMs, m, n, k log-uniform integers in [16, 1024],
-    transa and transb uniform over {N, T}, alpha ~ U[0.5, 2], beta ~ U[-1, 1],
-    ld = stored rows, operands packed back to back in three device buffers
-    filled from the generator.  Returns the plan, the single-call argument
-    lists of the same problems and the buffers."""
+def batch_shapes(seed=11):
+    """The problems of the grouped-batch workload: 4096 GEMMs, m, n, k
+    log-uniform integers in [16, 1024], transa and transb uniform over {N, T},
+    alpha ~ U[0.5, 2], beta ~ U[-1, 1], each with its operands' element
+    offsets in three buffers packed back to back (16-byte aligned starts).
+    Returns the problems and the three buffer lengths."""
     rng = np.random.default_rng(seed)
-    tdt = torch.float64 if dtype == "d" else torch.float32
     shapes = []
     offs = [0, 0, 0]
     for _ in range(BATCH_COUNT):
@@ -574,9 +578,17 @@ def batch_workload(torch, gx, dtype, stream, seed=11):
         ta, tb = "NT"[rng.integers(2)], "NT"[rng.integers(2)]
         alpha, beta = float(rng.uniform(0.5, 2.0)), float(rng.uniform(-1.0, 1.0))
         shapes.append((ta, tb, m, n, k, alpha, beta, tuple(offs)))
-        # 16-byte aligned starts for every operand
         for j, size in enumerate((m * k, k * n, m * n)):
             offs[j] += (size + 1) // 2 * 2
+    return shapes, offs
+
+
+def batch_workload(torch, gx, dtype, stream, seed=11):
+    """The grouped-batch workload (batch_shapes) in three device buffers
+    filled from the generator, ld = stored rows.  Returns the plan, the
+    single-call argument lists of the same problems and the buffers."""
+    tdt = torch.float64 if dtype == "d" else torch.float32
+    shapes, offs = batch_shapes(seed)
     bufs = [torch.empty(max(o, 1), dtype=tdt, device="cuda") for o in offs]
     for j, b in enumerate(bufs):
         gx.fill(b, b.numel(), 1, b.numel(), seed, j, stream=stream)
@@ -745,12 +757,26 @@ def profile_only(torch, gx, stream, args):
 
         def fn():
             dst.copy_(src)
-    elif args.profile_only == "batch":
-        plan, _, bufs = batch_workload(torch, gx, "d", stream)
+    elif args.profile_only in ("batch", "batch32"):
+        plan, _, bufs = batch_workload(torch, gx, "d" if args.profile_only == "batch" else "s",
+                                       stream)
         keep = bufs
 
         def fn():
             plan.run(stream)
+    elif args.profile_only == "splitk":
+        # FP64 512 x 512 x 65536 NN, alpha 1.5, beta -0.5: few C tiles, split K
+        m = n = 512
+        k = 65536
+        A = torch.empty(m * k, dtype=torch.float64, device="cuda")
+        B = torch.empty(k * n, dtype=torch.float64, device="cuda")
+        C = torch.empty(m * n, dtype=torch.float64, device="cuda")
+        for t, i, r, c in ((A, 0, m, k), (B, 1, k, n), (C, 2, m, n)):
+            gx.fill(t, r, c, r, SEED, i, stream=stream)
+        keep = [A, B, C]
+
+        def fn():
+            gx.dgemm_dev("N", "N", m, n, k, 1.5, A, m, B, k, -0.5, C, m, stream=stream)
     else:
         LD = 8200
         sA, sB, sC = (torch.empty(LD * K, dtype=torch.float32, device="cuda") for _ in range(3))
