@@ -145,13 +145,19 @@ static int current_device(int *dev) {
 int gemmx_current_device(int *dev) { return current_device(dev); }
 void gemmx_init(void) { pthread_once(&g_once, init_once); }
 
+/* GEMMX_FAIL_ALLOC=1 makes the library's device allocations fail as if the
+ * device were out of memory (failure-path testing; read at every attempt). */
+int gemmx_alloc_failure_injected(void) {
+  const char *fail = getenv("GEMMX_FAIL_ALLOC");
+  return fail && *fail && *fail != '0';
+}
+
 /* Grows the device's scratch to `bytes`; caller holds g_dev[dev].lock and
  * has no work in flight on the old scratch. */
 static int get_scratch(int dev, size_t bytes, void **out) {
   devstate_t *d = &g_dev[dev];
   if (bytes > d->scratch_bytes) {
-    const char *fail = getenv("GEMMX_FAIL_ALLOC"); /* failure-path testing */
-    if (fail && *fail && *fail != '0')
+    if (gemmx_alloc_failure_injected())
       return set_error(GEMMX_ERR_ALLOC, "device scratch of %zu bytes: allocation failure injected", bytes);
     if (d->scratch) {
       hipError_t e = hipFree(d->scratch);
@@ -929,6 +935,7 @@ static void *stream_alloc(int dev, size_t bytes, hipStream_t s) {
   hipMemPool_t pool = d->pool;
   pthread_mutex_unlock(&d->lock);
   void *w = NULL;
+  if (gemmx_alloc_failure_injected()) return NULL;
   if (!pool || hipMallocFromPoolAsync(&w, bytes, pool, s) != hipSuccess) {
     (void)hipGetLastError();
     return NULL;

@@ -220,6 +220,10 @@ static int create(int es, void **out, int count, const char *transa, const char 
     rc = gemmx_last_code();
     goto fail;
   }
+  if (gemmx_alloc_failure_injected()) {
+    rc = gemmx_report(GEMMX_ERR_ALLOC, "batch: device allocation failure injected");
+    goto fail;
+  }
   hipError_t e;
   if ((e = hipMalloc((void **)&p->d_desc, sizeof *h * (used > 0 ? used : 1))) != hipSuccess ||
       (e = hipMalloc((void **)&p->d_tile_prob, sizeof(int) * (total > 0 ? total : 1))) != hipSuccess ||

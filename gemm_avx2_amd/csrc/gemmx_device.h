@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gemmx_index.h"
+
 namespace gemmx {
 
 typedef double d2_t __attribute__((ext_vector_type(2)));
@@ -113,19 +115,19 @@ struct TileCopy {
 
   /* A full-K slice of an edge tile with aligned operands: lines at or past
    * nl_valid re-read line nl_valid-1, and chunks at or past len_valid re-read
-   * the last chunk holding a valid element, so every chunk is still one
-   * 16-byte load.  Each load is the aligned 16-byte block around a valid
-   * element (an aligned block never crosses a page), and what lands in a
-   * position past nl_valid/len_valid feeds only rows or columns of C that the
-   * epilogue does not store. */
+   * the last chunk holding a valid element (clamp_chunk), so every chunk is
+   * still one 16-byte load.  Each load is the aligned 16-byte block around a
+   * valid element (an aligned block never crosses a page); in the operand's
+   * last line that block can reach up to VE-1 elements past the last valid
+   * one, beyond the operand's minimal extent ld*(lines-1)+len.  What lands in
+   * a position past nl_valid/len_valid feeds only rows or columns of C that
+   * the epilogue does not store. */
   __device__ __forceinline__ void load_clamp(const T *__restrict__ g, int64_t ld, int nl_valid,
                                              int len_valid, int tid) {
-    const int last_off = ((len_valid - 1) / VE) * VE;
 #pragma unroll
     for (int c = 0; c < PER; ++c) {
-      const int ch = tid + c * NT;
-      const int line = min(ch / CPL, nl_valid - 1), off = min((ch % CPL) * VE, last_off);
-      r[c] = *reinterpret_cast<const vec_t *>(g + line * ld + off);
+      const ChunkAt at = clamp_chunk<CPL, VE>(tid + c * NT, nl_valid, len_valid);
+      r[c] = *reinterpret_cast<const vec_t *>(g + at.line * ld + at.off);
     }
   }
 
@@ -139,12 +141,12 @@ struct TileCopy {
 #pragma unroll
     for (int c = 0; c < PER; ++c) {
       const int ch = tid + c * NT;
-      const int line = CLAMP ? min(ch / CPL, nl_valid - 1) : ch / CPL;
+      const int line = CLAMP ? clamp_index(ch / CPL, nl_valid) : ch / CPL;
       const int off = (ch % CPL) * VE;
       const T *p = g + line * ld;
       vec_t v;
 #pragma unroll
-      for (int e = 0; e < VE; ++e) v[e] = p[CLAMP ? min(off + e, len_valid - 1) : off + e];
+      for (int e = 0; e < VE; ++e) v[e] = p[CLAMP ? clamp_index(off + e, len_valid) : off + e];
       r[c] = v;
     }
   }

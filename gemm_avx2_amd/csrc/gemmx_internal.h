@@ -86,6 +86,9 @@ enum {
  * (gemmx_api.c).  gemmx_report records the per-thread last error and
  * returns `code`. */
 int gemmx_report(int code, const char *fmt, ...);
+/* Nonzero when GEMMX_FAIL_ALLOC asks the library's device allocations to
+ * fail (failure-path testing). */
+int gemmx_alloc_failure_injected(void);
 void gemmx_clear_error(void);
 int gemmx_last_code(void);
 int gemmx_current_device(int *dev);
