@@ -11,8 +11,10 @@
  * of a problem share the operand panels in one XCD's L2; a workgroup takes
  * the next tile of its XCD's queue with an atomic ticket, and when that queue
  * is empty, of the other queues in turn, so every wave reaches the exit
- * after at most eight empty queues.  The ticket for the next tile is
- * requested when a tile starts, so its latency hides behind the tile's work.  A tile is computed by
+ * after at most eight empty queues.  The ticket for the next tile is taken
+ * after a tile's stores: a returning atomic issued before the tile's loads
+ * would hold back the first wait on them (vector memory operations retire in
+ * order), and the workgroup's barriers with it.  A tile is computed by
  * the same gemm_tile as the single-GEMM kernels: the branch-free loop with
  * an edge tile's loads clamped back into the operand, in 16-byte chunks
  * where both operands allow them and one element per load otherwise,
@@ -83,9 +85,6 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
       y = (x + round) % GEMMX_BATCH_QUEUES;
       t = fetch(y);
     }
-    /* request the next tile of this queue now; needed only after this one */
-    unsigned next = 0;
-    if (threadIdx.x == 0) next = atomicAdd(sched + y * GEMMX_BATCH_CTR_STRIDE, 1u);
     const int tt = Q.qbeg[y] + t;
     const int p = __builtin_amdgcn_readfirstlane(tile_prob[tt]);
     const gemmx_batch_desc &q = desc[p];
@@ -124,11 +123,9 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
       }
 #undef GEMMX_TILE
     }
-    /* every wave is done with this tile's LDS and has read `ticket` */
-    __syncthreads();
-    if (threadIdx.x == 0) ticket = (int)next;
-    __syncthreads();
-    t = __builtin_amdgcn_readfirstlane(ticket);
+    /* the next tile of this queue (fetch's first barrier also ends this
+     * tile's use of the LDS) */
+    t = fetch(y);
   }
 }
 
