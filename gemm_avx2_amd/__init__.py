@@ -29,7 +29,8 @@ __all__ = [
     "dgemm_ozaki_dev", "dgemm_ozaki_workspace", "dgemm_ozaki2_dev", "dgemm_ozaki2_workspace",
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "set_split_config",
-    "tune_lookup_split", "build_rocm", "set_product_config", "load_tuning", "tune_lookup", "tune_lookup_emulated", "last_error", "fill",
+    "tune_lookup_split", "build_rocm", "set_product_config", "load_tuning", "tune_lookup", "tune_lookup_emulated",
+    "tune_lookup_emulated32", "last_error", "fill",
     "BatchPlan", "set_batch_config", "batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2", "MODE_AUTO",
 ]
 
@@ -622,6 +623,16 @@ def tune_lookup_emulated(transa, transb, m, n, k):
     c = ctypes.c_int()
     mode = int(f(_ch(transa), _ch(transb), m, n, k, ctypes.byref(c)))
     return mode, c.value
+
+
+def tune_lookup_emulated32(transa, transb, m, n, k) -> int:
+    """The mode (MODE_NATIVE or MODE_BF16X3) sgemm_ uses in MODE_AUTO for
+    these arguments."""
+    lib = load("SGEMM.so")
+    f = lib.sgemm_x_tune_lookup_emulated
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    return int(f(_ch(transa), _ch(transb), m, n, k))
 
 
 def set_dgemm_mode(mode: int, slices: int = 8):

@@ -15,13 +15,15 @@ For each (dtype, trans, shape) every configuration runs on device-resident
 U[-1,1) operands (alpha = 1.5, beta = -0.5); the fastest by median event time
 is written, after its result is checked against configuration 0's on the same
 inputs (|difference| within 64 k u (|alpha| k + |beta|), the bound for
-U[-1,1) inputs).  For DGEMM shapes with min(m, n, k) >= 1024 it also picks
-the arithmetic of dgemm_'s automatic mode: Ozaki-II with the smallest
-modulus count whose error is no larger than native FP64's on phi in
-{0, 1, 4} inputs (relative Frobenius error, and half of native's largest
-componentwise error, over 2048 sampled entries on two seeds), when that is
-faster than native (an `emu` entry), and the grouped-batch
-kernel configuration per element type on a mixed batch (a `batch` entry).
+U[-1,1) inputs).  For shapes with min(m, n, k) >= 1024 it also picks the
+arithmetic of the automatic modes: for dgemm_, Ozaki-II with the smallest
+modulus count or Ozaki-I with the smallest slice count whose error is no
+larger than native FP64's on phi in {0, 1, 4} inputs (relative Frobenius
+error, and half of native's largest componentwise error, over 2048 sampled
+entries on two seeds), whichever is fastest, when that beats native (an
+`emu` entry); for sgemm_, split-BF16x3 under the same rule against native
+FP32 (an `emu32` entry); and the grouped-batch kernel configuration per
+element type on a mixed batch (a `batch` entry).
 With --resume, keys already
 in the output table are kept and not measured again.  The table is written in
 sorted key order with the measured times as comments.
@@ -72,12 +74,18 @@ SPLIT_CONFIGS = (0, 1, 5)  # native tile configurations with split-K kernels
 _EMU = re.compile(r"^\s*emu\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(native|ozaki|ozaki2)"
                   r"\s+(\d+)\s*(#.*)?$")
 _EMU_RANGE = {"native": (0, 0), "ozaki": (3, 8), "ozaki2": (8, 22)}
+_EMU32 = re.compile(r"^\s*emu32\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(native|bf16x3)"
+                    r"\s*(#.*)?$")
 _BATCH = re.compile(r"^\s*batch\s+(f64|f32)\s+(\d+)\s*(#.*)?$")
 BATCH_CONFIGS = 4  # compiled grouped-batch kernel configurations (GEMMX_BATCH_CONFIGS)
 
 
 def emu_key(trans, m, n, k):
     return ("emu", trans, bucket(m), bucket(n), bucket(k))
+
+
+def emu32_key(trans, m, n, k):
+    return ("emu32", trans, bucket(m), bucket(n), bucket(k))
 
 
 def read_table(path):
@@ -104,6 +112,13 @@ def read_table(path):
                 if 0 <= int(m.group(2)) < BATCH_CONFIGS:
                     entries[key] = int(m.group(2))
                     notes[key] = (m.group(3) or "").lstrip("# ").rstrip()
+                continue
+            m = _EMU32.match(line)
+            if m:
+                key = ("emu32", m.group(1), int(m.group(2)), int(m.group(3)), int(m.group(4)))
+                if all(4 <= b <= 15 for b in key[2:]):
+                    entries[key] = m.group(5)
+                    notes[key] = (m.group(6) or "").lstrip("# ").rstrip()
                 continue
             m = _EMU.match(line)
             if m:
@@ -144,8 +159,9 @@ def write_table(path, entries, notes, arch, header=(), rocm=None):
     lines += [f"rocm {rocm}"]
     lines += ["# dtype trans mbucket nbucket kbucket cfg [split]",
               "# emu trans mbucket nbucket kbucket scheme count   (DGEMM arithmetic, auto mode)",
+              "# emu32 trans mbucket nbucket kbucket scheme   (SGEMM arithmetic, auto mode)",
               "# batch dtype cfg   (grouped-batch kernel)"]
-    order = {"f64": 0, "f32": 1, "emu": 2}
+    order = {"f64": 0, "f32": 1, "emu": 2, "emu32": 3}
     tiles = [k for k in entries if k[0] != "batch"]
     for key in sorted(tiles, key=lambda k: (order[k[0]], TRANS.index(k[1]), k[2], k[3], k[4])):
         note = notes.get(key, "")
@@ -392,46 +408,33 @@ def sample_errors(got, exact, ab):
             "max_comp": float(np.max(err / (ab + 1e-300)))}
 
 
-def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_SEEDS,
-                     samples=EMU_SAMPLES):
-    """DGEMM arithmetic for one shape: native FP64, or Ozaki-II with the
-    smallest modulus count whose error at `samples` random entries (alpha 1,
-    beta 0, against Dot2 values of the same inputs) is no larger than
-    native's in relative Frobenius norm and at most EMU_COMP_MARGIN times
-    native's largest error relative to sum |a||b|, for entries
-    (u - 0.5) exp(phi g), phi in {0, 1, 4}, on every seed, if it is faster.
-    Returns (scheme, count, note)."""
+SLICES = tuple(range(3, 9))
+
+
+def _timed(torch, fn, reps, stream):
+    fn()
+    ev = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        ev.append(e0.elapsed_time(e1))
+    return _median(ev)
+
+
+def _qualify(torch, gx, trans, m, n, k, A, B, C, native, forms, seeds, samples):
+    """The names in `forms` (name -> launch) whose error at `samples` random
+    entries (alpha 1, beta 0, against Dot2 values of the same inputs) is no
+    larger than native's in relative Frobenius norm and at most
+    EMU_COMP_MARGIN times native's largest error relative to sum |a||b|, for
+    entries (u - 0.5) exp(phi g), phi in PHIS, on every seed."""
     import numpy as np
     ra, ca = (m, k) if trans[0] == "N" else (k, m)
     rb, cb = (k, n) if trans[1] == "N" else (n, k)
-    A = torch.empty(ra * ca, dtype=torch.float64, device="cuda")
-    B = torch.empty(rb * cb, dtype=torch.float64, device="cuda")
-    C = torch.empty(m * n, dtype=torch.float64, device="cuda")
-    ws = torch.empty(gx.dgemm_ozaki2_workspace(m, n, k, max(moduli)), dtype=torch.uint8,
-                     device="cuda")
-    stream = torch.cuda.current_stream()
-
-    def native():
-        gx.dgemm_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, stream=stream)
-
-    def emulated(nm):
-        gx.dgemm_ozaki2_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, nm, ws,
-                            ws.numel(), stream=stream)
-
-    def timed(fn):
-        fn()
-        ev = []
-        for _ in range(reps):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            fn()
-            e1.record(stream)
-            e1.synchronize()
-            ev.append(e0.elapsed_time(e1))
-        return _median(ev)
-
-    qual = set(moduli)
+    qual = set(forms)
     for seed in seeds:
         rng = np.random.default_rng(seed)
         is_, js = rng.integers(0, m, samples), rng.integers(0, n, samples)
@@ -446,26 +449,96 @@ def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_S
             # op(A) rows: A(i, :) of an m x k (N) or A(:, i) of a k x m (T) array
             rows = A.view(k, m)[:, tui].T if trans[0] == "N" else A.view(m, k)[tui]
             cols = B.view(n, k)[tuj] if trans[1] == "N" else B.view(k, n)[:, tuj].T
-            exact, ab = dot2_pairs(rows.contiguous().cpu().numpy(),
-                                   cols.contiguous().cpu().numpy(), ri, cj)
+            # FP32 entries: their products are exact in FP64, Dot2 sums them
+            exact, ab = dot2_pairs(rows.contiguous().double().cpu().numpy(),
+                                   cols.contiguous().double().cpu().numpy(), ri, cj)
 
             def err(fn):
                 fn()
                 torch.cuda.synchronize()
-                return sample_errors(C[flat].cpu().numpy(), exact, ab)
+                return sample_errors(C[flat].double().cpu().numpy(), exact, ab)
             en = err(native)
-            for nm in sorted(qual):
-                e = err(lambda nm=nm: emulated(nm))
+            for name in sorted(qual):
+                e = err(forms[name])
                 if (e["rel_fro"] > en["rel_fro"] or
                         e["max_comp"] > EMU_COMP_MARGIN * en["max_comp"]):
-                    qual.discard(nm)
-    t_nat = timed(native)
-    if not qual:
-        return "native", 0, f"{m}x{n}x{k} native {t_nat:.3f} ms; no modulus count matches its error"
-    nm = min(qual)
-    t_emu = timed(lambda: emulated(nm))
-    note = f"{m}x{n}x{k} native {t_nat:.3f} ms, ozaki2 n{nm} {t_emu:.3f} ms"
-    return ("ozaki2", nm, note) if t_emu < t_nat else ("native", 0, note)
+                    qual.discard(name)
+    return qual
+
+
+def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_SEEDS,
+                     samples=EMU_SAMPLES, slices=SLICES):
+    """DGEMM arithmetic for one shape: native FP64, Ozaki-II with the
+    smallest qualifying modulus count or Ozaki-I with the smallest qualifying
+    slice count (_qualify: error no larger than native's), whichever of them
+    is fastest, when it is faster than native.  Returns (scheme, count,
+    note)."""
+    ra, ca = (m, k) if trans[0] == "N" else (k, m)
+    rb, cb = (k, n) if trans[1] == "N" else (n, k)
+    A = torch.empty(ra * ca, dtype=torch.float64, device="cuda")
+    B = torch.empty(rb * cb, dtype=torch.float64, device="cuda")
+    C = torch.empty(m * n, dtype=torch.float64, device="cuda")
+    wbytes = max(gx.dgemm_ozaki2_workspace(m, n, k, max(moduli)) if moduli else 0,
+                 gx.dgemm_ozaki_workspace(m, n, k, max(slices)) if slices else 0)
+    ws = torch.empty(wbytes, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def native():
+        gx.dgemm_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, stream=stream)
+
+    def ozaki2(nm):
+        gx.dgemm_ozaki2_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, nm, ws,
+                            ws.numel(), stream=stream)
+
+    def ozaki(s):
+        gx.dgemm_ozaki_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, s, ws,
+                           ws.numel(), stream=stream)
+
+    forms = {("ozaki2", nm): (lambda nm=nm: ozaki2(nm)) for nm in moduli}
+    forms.update({("ozaki", s): (lambda s=s: ozaki(s)) for s in slices})
+    qual = _qualify(torch, gx, trans, m, n, k, A, B, C, native, forms, seeds, samples)
+    t_nat = _timed(torch, native, reps, stream)
+    best, t_best = ("native", 0), t_nat
+    parts = [f"{m}x{n}x{k} native {t_nat:.3f} ms"]
+    for scheme, tag, what in (("ozaki2", "n", "modulus"), ("ozaki", "s", "slice")):
+        counts = sorted(c for sc, c in qual if sc == scheme)
+        if not counts:
+            if (moduli if scheme == "ozaki2" else slices):
+                parts.append(f"no {what} count matches its error")
+            continue
+        t = _timed(torch, forms[(scheme, counts[0])], reps, stream)
+        parts.append(f"{scheme} {tag}{counts[0]} {t:.3f} ms")
+        if t < t_best:
+            best, t_best = (scheme, counts[0]), t
+    return best[0], best[1], "; ".join(parts)
+
+
+def measure_emulated32(torch, gx, trans, m, n, k, reps, seeds=EMU_SEEDS, samples=EMU_SAMPLES):
+    """SGEMM arithmetic for one shape: split-BF16x3 when its error is no
+    larger than native FP32's (_qualify) and it is faster, else native.
+    Returns (scheme, note)."""
+    ra, ca = (m, k) if trans[0] == "N" else (k, m)
+    rb, cb = (k, n) if trans[1] == "N" else (n, k)
+    A = torch.empty(ra * ca, dtype=torch.float32, device="cuda")
+    B = torch.empty(rb * cb, dtype=torch.float32, device="cuda")
+    C = torch.empty(m * n, dtype=torch.float32, device="cuda")
+    ws = torch.empty(gx.sgemm_bf16x3_workspace(m, n, k), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def native():
+        gx.sgemm_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, stream=stream)
+
+    def bf16x3():
+        gx.sgemm_bf16x3_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, ws,
+                            ws.numel(), stream=stream)
+
+    qual = _qualify(torch, gx, trans, m, n, k, A, B, C, native, {"bf16x3": bf16x3}, seeds,
+                    samples)
+    t_nat = _timed(torch, native, reps, stream)
+    t_b = _timed(torch, bf16x3, reps, stream)
+    note = (f"{m}x{n}x{k} native {t_nat:.3f} ms, bf16x3 {t_b:.3f} ms"
+            + ("" if qual else "; bf16x3 does not match its error"))
+    return ("bf16x3" if qual and t_b < t_nat else "native"), note
 
 
 def main(argv=None):
@@ -478,8 +551,8 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--resume", action="store_true")
     ap.add_argument("--emulated", choices=["on", "off"], default="on",
-                    help="also choose the DGEMM arithmetic (native or Ozaki-II) per bucket for "
-                         "shapes with min(m, n, k) >= 1024")
+                    help="also choose the arithmetic per bucket for shapes with min(m, n, k) >= "
+                         "1024: DGEMM native, Ozaki-II or Ozaki-I; SGEMM native or BF16x3")
     ap.add_argument("--batch", choices=["on", "off"], default="on",
                     help="also choose the grouped-batch kernel configuration per element type "
                          "on a mix of %d problems with m, n, k log-uniform in [%d, %d]" % BATCH_MIX)
@@ -520,22 +593,27 @@ def main(argv=None):
         write_table(args.out, entries, notes, arch, header)  # after every shape: resumable
         print(f"[{i + 1}/{len(todo)}] {key[0]} {t} {m}x{n}x{k}: cfg {best[0]} split {best[1]} "
               f"({times[best]:.3f} ms; cfg0 {times[(0, 1)]:.3f} ms)", flush=True)
-    if args.emulated == "on" and "d" in args.dtypes.split(","):
+    for d in (args.dtypes.split(",") if args.emulated == "on" else ()):
+        keyf = emu_key if d == "d" else emu32_key
         etodo, seen = [], set()
         for t in args.trans.split(","):
             for (m, n, k) in shapes:
-                key = emu_key(t, m, n, k)
+                key = keyf(t, m, n, k)
                 if min(m, n, k) < 1024 or key in seen or (args.resume and key in entries):
                     continue
                 seen.add(key)
                 etodo.append((t, m, n, k))
         for i, (t, m, n, k) in enumerate(etodo):
-            scheme, cnt, note = measure_emulated(torch, gx, t, m, n, k, args.reps)
-            key = emu_key(t, m, n, k)
-            entries[key] = (scheme, cnt)
+            key = keyf(t, m, n, k)
+            if d == "d":
+                scheme, cnt, note = measure_emulated(torch, gx, t, m, n, k, args.reps)
+                entries[key] = (scheme, cnt)
+            else:
+                scheme, note = measure_emulated32(torch, gx, t, m, n, k, args.reps)
+                entries[key], cnt = scheme, ""
             notes[key] = note
             write_table(args.out, entries, notes, arch, header)
-            print(f"[emu {i + 1}/{len(etodo)}] {t} {m}x{n}x{k}: {scheme} {cnt} ({note})",
+            print(f"[{key[0]} {i + 1}/{len(etodo)}] {t} {m}x{n}x{k}: {scheme} {cnt} ({note})",
                   flush=True)
     if args.batch == "on":
         for d in args.dtypes.split(","):

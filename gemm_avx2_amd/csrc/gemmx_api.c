@@ -119,6 +119,7 @@ static void init_once(void) {
   }
   const char *sm = getenv("GEMMX_SGEMM_MODE");
   if (sm && strcmp(sm, "bf16x3") == 0) g_smode = GEMMX_MODE_BF16X3;
+  if (sm && strcmp(sm, "auto") == 0) g_smode = GEMMX_MODE_AUTO;
   g_init_err = hipGetDeviceCount(&g_ndev);
   if (g_init_err != hipSuccess) {
     g_ndev = 0;
@@ -410,7 +411,8 @@ static void fortran_call(call_t *x, int mode, int slices) {
   if (mode == GEMMX_MODE_AUTO) /* per-shape arithmetic from the tuning table */
     mode = x->es == 8 ? gemmx_tune_pick_emulated(!notrans(x->ta), !notrans(x->tb), x->m, x->n,
                                                  x->k, &slices)
-                      : GEMMX_MODE_NATIVE;
+                      : gemmx_tune_pick_emulated32(!notrans(x->ta), !notrans(x->tb), x->m,
+                                                   x->n, x->k);
   int dev;
   if (current_device(&dev) != GEMMX_OK) return;
 
@@ -792,7 +794,7 @@ EXPORT int dgemm_x_set_mode(int mode, int slices) {
 
 EXPORT int sgemm_x_set_mode(int mode) {
   pthread_once(&g_once, init_once);
-  if (mode == GEMMX_MODE_NATIVE || mode == GEMMX_MODE_BF16X3) {
+  if (mode == GEMMX_MODE_NATIVE || mode == GEMMX_MODE_BF16X3 || mode == GEMMX_MODE_AUTO) {
     g_smode = mode;
     return GEMMX_OK;
   }
@@ -860,6 +862,13 @@ EXPORT int dgemm_x_tune_lookup_emulated(char transa, char transb, int m, int n, 
   const int mode = gemmx_tune_pick_emulated(!notrans(transa), !notrans(transb), m, n, k, &c);
   if (count) *count = c;
   return mode;
+}
+
+/* The arithmetic sgemm_ in the automatic mode would use for these
+ * arguments (GEMMX_MODE_NATIVE or GEMMX_MODE_BF16X3).  Needs no device. */
+EXPORT int sgemm_x_tune_lookup_emulated(char transa, char transb, int m, int n, int k) {
+  pthread_once(&g_once, init_once);
+  return gemmx_tune_pick_emulated32(!notrans(transa), !notrans(transb), m, n, k);
 }
 
 /* The tile configuration a call with these arguments would use (forced

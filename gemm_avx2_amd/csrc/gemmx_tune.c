@@ -25,6 +25,8 @@ static signed char g_table[2][4][NB][NB][NB]; /* [f64/f32][trans][m][n][k] */
 static signed char g_split[2][4][NB][NB][NB];
 /* DGEMM arithmetic per bucket: 0 = no entry, else (mode + 1) << 8 | count */
 static short g_emu[4][NB][NB][NB];
+/* SGEMM arithmetic per bucket: 0 = no entry, else mode + 1 */
+static signed char g_emu32[4][NB][NB][NB];
 /* grouped-batch kernel configuration per element size: 0 = no entry, else cfg + 1 */
 static signed char g_batch[2];
 static int g_entries;
@@ -69,6 +71,7 @@ static void clear_table(void) {
   memset(g_table, 0, sizeof g_table);
   memset(g_split, 0, sizeof g_split);
   memset(g_emu, 0, sizeof g_emu);
+  memset(g_emu32, 0, sizeof g_emu32);
   memset(g_batch, 0, sizeof g_batch);
   g_entries = 0;
   g_path[0] = 0;
@@ -82,10 +85,12 @@ static int load_table(const char *path) {
   static signed char staged[2][4][NB][NB][NB];
   static signed char staged_split[2][4][NB][NB][NB];
   static short staged_emu[4][NB][NB][NB];
+  static signed char staged_emu32[4][NB][NB][NB];
   signed char staged_batch[2] = {0, 0};
   memset(staged, 0, sizeof staged);
   memset(staged_split, 0, sizeof staged_split);
   memset(staged_emu, 0, sizeof staged_emu);
+  memset(staged_emu32, 0, sizeof staged_emu32);
   char line[256];
   int format = 0, entries = 0, ok = 1, rocm = 0;
   while (ok && fgets(line, sizeof line, f)) {
@@ -118,6 +123,21 @@ static int load_table(const char *path) {
       const int d = strcmp(dt, "f64") == 0 ? 0 : strcmp(dt, "f32") == 0 ? 1 : -1;
       if (d < 0 || cfg < 0 || cfg >= GEMMX_BATCH_CONFIGS) continue;
       staged_batch[d] = (signed char)(cfg + 1);
+      entries++;
+      continue;
+    }
+    if (sscanf(line, "emu32 %7s %d %d %d %15s", tr, &mb, &nb, &kb, sc) == 5) {
+      if (format != GEMMX_TUNE_FORMAT) {
+        ok = 0;
+        break;
+      }
+      const int t = trans_index(tr);
+      const int mode = strcmp(sc, "native") == 0   ? GEMMX_MODE_NATIVE
+                       : strcmp(sc, "bf16x3") == 0 ? GEMMX_MODE_BF16X3
+                                                   : -1;
+      if (t < 0 || mode < 0 || mb < 4 || mb > 15 || nb < 4 || nb > 15 || kb < 4 || kb > 15)
+        continue;
+      staged_emu32[t][mb - 4][nb - 4][kb - 4] = (signed char)(mode + 1);
       entries++;
       continue;
     }
@@ -160,6 +180,7 @@ static int load_table(const char *path) {
   memcpy(g_table, staged, sizeof g_table);
   memcpy(g_split, staged_split, sizeof g_split);
   memcpy(g_emu, staged_emu, sizeof g_emu);
+  memcpy(g_emu32, staged_emu32, sizeof g_emu32);
   memcpy(g_batch, staged_batch, sizeof g_batch);
   g_entries = entries;
   snprintf(g_path, sizeof g_path, "%s", path);
@@ -236,6 +257,13 @@ int gemmx_tune_pick_emulated(int ta, int tb, int m, int n, int k, int *count) {
   }
   *count = v & 0xff;
   return (v >> 8) - 1;
+}
+
+int gemmx_tune_pick_emulated32(int ta, int tb, int m, int n, int k) {
+  const int t = (ta ? 2 : 0) + (tb ? 1 : 0);
+  const int v = g_emu32[t][gemmx_tune_bucket(m) - 4][gemmx_tune_bucket(n) - 4]
+                       [gemmx_tune_bucket(k) - 4];
+  return v ? v - 1 : GEMMX_MODE_NATIVE;
 }
 
 int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {

@@ -55,6 +55,9 @@ const char *gemmx_tune_rocm(void);
 /* Arithmetic of a DGEMM in the automatic mode: GEMMX_MODE_NATIVE, _OZAKI or
  * _OZAKI2, with the slice or modulus count in *count. */
 int gemmx_tune_pick_emulated(int ta, int tb, int m, int n, int k, int *count);
+/* Arithmetic of an SGEMM in the automatic mode: GEMMX_MODE_NATIVE or
+ * GEMMX_MODE_BF16X3 (the table's `emu32` entry of the shape's bucket). */
+int gemmx_tune_pick_emulated32(int ta, int tb, int m, int n, int k);
 int gemmx_tune_bucket(int dim);
 /* Grouped-batch kernel configuration of the table for element size es (8 or
  * 4), or -1 when it has none. */

@@ -143,13 +143,15 @@ void sgemm_64_(const char *transa, const char *transb, const int64_t *m,
  *                             bucket, written by the autotuner; native when
  *                             there is none)
  *   GEMMX_SGEMM_MODE = native | bf16x3
+ *                    | auto  (the tuning table's `emu32` entry of the
+ *                             shape's bucket; native when there is none)
  * For GEMMX_MODE_OZAKI2 the second argument of dgemm_x_set_mode is the
  * modulus count. */
 #define GEMMX_MODE_NATIVE 0
 #define GEMMX_MODE_OZAKI 1  /* FP64 by INT8 slices on INT8 MFMA */
 #define GEMMX_MODE_BF16X3 2 /* FP32 by split-BF16x3 on BF16 MFMA */
 #define GEMMX_MODE_OZAKI2 3 /* FP64 by INT8 residues (CRT) on INT8 MFMA */
-#define GEMMX_MODE_AUTO 4   /* dgemm_ only: per-shape choice from the tuning table */
+#define GEMMX_MODE_AUTO 4   /* per-shape choice from the tuning table */
 int dgemm_x_set_mode(int mode, int slices);
 int sgemm_x_set_mode(int mode);
 
@@ -209,6 +211,9 @@ int dgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);
 int dgemm_x_tune_lookup_emulated(char transa, char transb, int m, int n, int k,
                                  int *count);
 int sgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);
+/* The arithmetic sgemm_ uses in GEMMX_MODE_AUTO for these arguments
+ * (GEMMX_MODE_NATIVE or GEMMX_MODE_BF16X3); no device needed. */
+int sgemm_x_tune_lookup_emulated(char transa, char transb, int m, int n, int k);
 
 /* Last error of the calling thread: returns its code and, when msg is not
  * NULL, a static description.  Cleared by the next successful call. */

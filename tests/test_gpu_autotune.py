@@ -62,17 +62,21 @@ def test_autotune_table_is_used_and_correct(cuda, tmp_path, capsys):
 
 
 def test_autotune_emulated_choice_drives_auto_mode(cuda, tmp_path):
-    """The autotuner's DGEMM arithmetic choice for a 1024^3 bucket is what
-    dgemm_ uses in the automatic mode: bit-identical to the oracle's Ozaki-II
-    restatement when it chose Ozaki-II, within the native tolerance
-    otherwise."""
+    """The autotuner's arithmetic choices for a 1024^3 bucket (DGEMM: native,
+    Ozaki-II or Ozaki-I; SGEMM: native or BF16x3) are what the libraries read
+    back, and dgemm_ in the automatic mode follows its entry: bit-identical to
+    the oracle's Ozaki-II or Ozaki-I restatement when it chose one, within the
+    native tolerance otherwise."""
     import os
     out = str(tmp_path / "tune.txt")
     assert at.main(["--out", out, "--shapes", "1024,1024,1024", "--trans", "NN",
-                    "--dtypes", "d", "--reps", "2"]) == 0
+                    "--dtypes", "d,s", "--reps", "2"]) == 0
     entries, notes, _, _ = at.read_table(out)
     scheme, cnt = entries[("emu", "NN", 10, 10, 10)]
-    assert scheme in ("native", "ozaki2")
+    assert scheme in ("native", "ozaki2", "ozaki")
+    # the SGEMM choice is recorded too, and the library reads it back
+    s32 = entries[("emu32", "NN", 10, 10, 10)]
+    assert s32 in ("native", "bf16x3")
     rng = np.random.default_rng(12)
     m = n = k = 1024
     A = np.asfortranarray((rng.uniform(0, 1, (m, k)) - 0.5) * np.exp(rng.standard_normal((m, k))))
@@ -81,7 +85,10 @@ def test_autotune_emulated_choice_drives_auto_mode(cuda, tmp_path):
     try:
         assert gx.load_tuning(out, "d") == len(entries)
         assert gx.tune_lookup_emulated("N", "N", m, n, k) == (
-            gx.MODE_OZAKI2 if scheme == "ozaki2" else gx.MODE_NATIVE, cnt)
+            {"ozaki2": gx.MODE_OZAKI2, "ozaki": gx.MODE_OZAKI}.get(scheme, gx.MODE_NATIVE), cnt)
+        assert gx.load_tuning(out, "s") == len(entries)
+        assert gx.tune_lookup_emulated32("N", "N", m, n, k) == (
+            gx.MODE_BF16X3 if s32 == "bf16x3" else gx.MODE_NATIVE)
         gx.set_dgemm_mode(gx.MODE_AUTO, 0)
         G = C.copy(order="F")
         gx.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, G, m)
@@ -89,13 +96,17 @@ def test_autotune_emulated_choice_drives_auto_mode(cuda, tmp_path):
         if scheme == "ozaki2":
             oracle.ozaki2_gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m, cnt)
             np.testing.assert_array_equal(G, R)
+        elif scheme == "ozaki":
+            oracle.ozaki_gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m, cnt)
+            np.testing.assert_array_equal(G, R)
         else:
             oracle.gemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, R, m)
             bound = abs_bound(A, B, C, "N", "N", m, n, k, 1.0, 0.0)
             assert max_comp(G, R, bound) <= 64 * k * U64
     finally:
         gx.set_dgemm_mode(gx.MODE_NATIVE)
-        gx.load_tuning(at.DEFAULT_OUT if os.path.exists(at.DEFAULT_OUT) else None, "d")
+        for d in ("d", "s"):
+            gx.load_tuning(at.DEFAULT_OUT if os.path.exists(at.DEFAULT_OUT) else None, d)
 
 
 @pytest.mark.parametrize("d", ["d", "s"])
@@ -110,3 +121,57 @@ def test_autotune_batch_measurement(cuda, d):
     assert len(times) == at.BATCH_CONFIGS and all(ok) and 0 <= best < at.BATCH_CONFIGS
     assert times[best] == min(times)
     assert gx.batch_config(d) == before  # no forced value left behind
+
+
+def test_table_entries_drive_both_auto_modes(cuda, tmp_path):
+    """Table entries written by the autotuner choose the arithmetic of both
+    automatic modes: an `emu ... ozaki 6` entry makes dgemm_ run Ozaki-I with
+    six slices (bit-identical to dgemm_x_ozaki_dev on the same inputs), an
+    `emu32 ... bf16x3` entry makes sgemm_ run split-BF16x3 (bit-identical to
+    sgemm_x_bf16x3_dev), and `native` entries the native kernels (within the
+    native tolerance of the oracle)."""
+    import os
+    import torch
+    path = str(tmp_path / "t.txt")
+    at.write_table(path, {("emu", "NN", 10, 10, 10): ("ozaki", 6),
+                          ("emu32", "NN", 10, 10, 10): "bf16x3",
+                          ("emu", "TN", 10, 10, 10): ("native", 0),
+                          ("emu32", "TN", 10, 10, 10): "native"}, {}, None)
+    m = n = k = 1024
+    rng = np.random.default_rng(21)
+    try:
+        for d in ("d", "s"):  # each library holds its own table
+            assert gx.load_tuning(path, d) == 4
+        gx.set_dgemm_mode(gx.MODE_AUTO, 0)
+        gx.set_sgemm_mode(gx.MODE_AUTO)
+        for dt, fn in ((np.float64, gx.dgemm), (np.float32, gx.sgemm)):
+            for ta in ("N", "T"):
+                print(f"case {fn.__name__} auto {ta}N {m}^3", flush=True)
+                A = np.asfortranarray(rng.uniform(-1, 1, (m, k)).astype(dt))
+                B = np.asfortranarray(rng.uniform(-1, 1, (k, n)).astype(dt))
+                C = np.asfortranarray(rng.uniform(-1, 1, (m, n)).astype(dt))
+                G = C.copy(order="F")
+                fn(ta, "N", m, n, k, 1.5, A, m, B, k, -0.5, G, m)
+                if ta == "N":
+                    dA, dB, dC = (torch.from_numpy(X.ravel(order="F")).cuda() for X in (A, B, C))
+                    if dt == np.float64:
+                        gx.dgemm_ozaki_dev("N", "N", m, n, k, 1.5, dA, m, dB, k, -0.5, dC, m, 6)
+                    else:
+                        ws = torch.empty(gx.sgemm_bf16x3_workspace(m, n, k), dtype=torch.uint8,
+                                         device="cuda")
+                        gx.sgemm_bf16x3_dev("N", "N", m, n, k, 1.5, dA, m, dB, k, -0.5, dC, m,
+                                            ws, ws.numel())
+                    torch.cuda.synchronize()
+                    np.testing.assert_array_equal(G, dC.cpu().numpy().reshape((m, n), order="F"))
+                else:
+                    R = C.astype(np.float64, order="F")
+                    oracle.gemm("T", "N", m, n, k, 1.5, A.astype(np.float64, order="F"), m,
+                                B.astype(np.float64, order="F"), k, -0.5, R, m)
+                    bound = abs_bound(A, B, C, "T", "N", m, n, k, 1.5, -0.5)
+                    u = U64 if dt == np.float64 else U32
+                    assert max_comp(G, R, bound) <= 64 * k * u
+    finally:
+        gx.set_dgemm_mode(gx.MODE_NATIVE)
+        gx.set_sgemm_mode(gx.MODE_NATIVE)
+        for d in ("d", "s"):
+            gx.load_tuning(at.DEFAULT_OUT if os.path.exists(at.DEFAULT_OUT) else None, d)

@@ -171,6 +171,38 @@ def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
     assert at.read_table(str(bad))[0] == {("emu", "NN", 13, 13, 13): ("ozaki2", 20)}
 
 
+def test_emu32_entries_pick_the_sgemm_arithmetic(tmp_path, restore_tables):
+    """`emu32` lines choose sgemm_'s arithmetic in the automatic mode (native
+    or split-BF16x3) per shape bucket, independently of the DGEMM `emu`
+    lines; unknown schemes and buckets are skipped, and the Python reader
+    and writer agree with the library."""
+    path = str(tmp_path / "t.txt")
+    e = dict(_entries())
+    e[("emu32", "NN", 14, 14, 14)] = "bf16x3"
+    e[("emu32", "TN", 13, 13, 13)] = "native"
+    e[("emu", "NN", 14, 14, 14)] = ("ozaki2", 19)
+    at.write_table(path, e, {}, None)
+    got, _, _, _ = at.read_table(path)
+    assert got == e
+    assert gx.load_tuning(path, "s") == len(e)
+    assert gx.tune_lookup_emulated32("N", "N", 16384, 16384, 16384) == gx.MODE_BF16X3
+    assert gx.tune_lookup_emulated32("n", "n", 14000, 20000, 16384) == gx.MODE_BF16X3
+    assert gx.tune_lookup_emulated32("T", "N", 8192, 8192, 8192) == gx.MODE_NATIVE
+    assert gx.tune_lookup_emulated32("T", "T", 16384, 16384, 16384) == gx.MODE_NATIVE
+    assert gx.tune_lookup_emulated("N", "N", 16384, 16384, 16384) == (gx.MODE_OZAKI2, 19)
+    bad = tmp_path / "bad.txt"
+    bad.write_text(f"format 2\nrocm {gx.build_rocm()}\nemu32 NN 14 14 14 fp16\n"
+                   "emu32 NN 3 14 14 bf16x3\nemu32 XX 14 14 14 bf16x3\n"
+                   "emu32 TT 12 12 12 bf16x3\n")
+    assert gx.load_tuning(str(bad), "s") == 1
+    assert gx.tune_lookup_emulated32("T", "T", 4096, 4096, 4096) == gx.MODE_BF16X3
+    assert gx.tune_lookup_emulated32("N", "N", 16384, 16384, 16384) == gx.MODE_NATIVE
+    assert at.read_table(str(bad))[0] == {("emu32", "TT", 12, 12, 12): "bf16x3"}
+    # the automatic mode is accepted for sgemm_ too
+    assert gx.load("SGEMM.so").sgemm_x_set_mode(gx.MODE_AUTO) == gx.OK
+    assert gx.load("SGEMM.so").sgemm_x_set_mode(gx.MODE_NATIVE) == gx.OK
+
+
 @pytest.mark.parametrize("dtype", ["d", "s"])
 def test_batch_entry_drives_the_batch_configuration(tmp_path, dtype, restore_tables,
                                                     monkeypatch):
