@@ -69,13 +69,15 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
       tm = e - row_tiles;
       tn = tiles_n;
     }
-    gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, true, NARROW, SPLIT>(
+    gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, true, NARROW, SPLIT,
+              (BM == 64 ? 2 : 1)>(
         A, B, SPLIT ? part(tm, tn) : C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN,
         vecA, vecB, lds);
     return;
   }
   tile_coords(blockIdx.x - lead, gridDim.x - lead, tiles_m, tiles_n, &tm, &tn);
-  gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, false, NARROW, SPLIT>(
+  gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, false, NARROW, SPLIT,
+              (BM == 64 ? 2 : 1)>(
       A, B, SPLIT ? part(tm, tn) : C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN,
       vecA, vecB, lds);
 }

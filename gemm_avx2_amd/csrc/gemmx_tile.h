@@ -37,6 +37,8 @@
 #ifndef GEMMX_TILE_H
 #define GEMMX_TILE_H
 
+#include <type_traits>
+
 #include "gemmx_device.h"
 
 namespace gemmx {
@@ -57,7 +59,8 @@ constexpr int tile_lds_max() {
 }
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KVMAX,
-          bool FAST, bool CLAMP = false, bool NARROW = false, bool PARTIAL = false>
+          bool FAST, bool CLAMP = false, bool NARROW = false, bool PARTIAL = false,
+          int DEPTH = 1>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           T *__restrict__ C, int M, int N, int K,
                                           int64_t lda, int64_t ldb, int64_t ldc, T alpha,
@@ -95,32 +98,34 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     if (!TB) cb.load(bbase(k0), ldb, N - n0, K - k0, vecB, tid);
     else     cb.load(bbase(k0), ldb, K - k0, N - n0, vecB, tid);
   };
-  auto load_fast = [&](int k0) {
+  auto load_into = [&](auto &xa, auto &xb, int k0) {
     if (NARROW) {
-      if (!TA) ca.template load_narrow<CLAMP>(abase(k0), lda, A_NL, M - m0, tid);
-      else     ca.template load_narrow<CLAMP>(abase(k0), lda, M - m0, A_LEN, tid);
-      if (!TB) cb.template load_narrow<CLAMP>(bbase(k0), ldb, N - n0, B_LEN, tid);
-      else     cb.template load_narrow<CLAMP>(bbase(k0), ldb, B_NL, N - n0, tid);
+      if (!TA) xa.template load_narrow<CLAMP>(abase(k0), lda, A_NL, M - m0, tid);
+      else     xa.template load_narrow<CLAMP>(abase(k0), lda, M - m0, A_LEN, tid);
+      if (!TB) xb.template load_narrow<CLAMP>(bbase(k0), ldb, N - n0, B_LEN, tid);
+      else     xb.template load_narrow<CLAMP>(bbase(k0), ldb, B_NL, N - n0, tid);
     } else if (CLAMP) {
-      if (!TA) ca.load_clamp(abase(k0), lda, A_NL, M - m0, tid);
-      else     ca.load_clamp(abase(k0), lda, M - m0, A_LEN, tid);
-      if (!TB) cb.load_clamp(bbase(k0), ldb, N - n0, B_LEN, tid);
-      else     cb.load_clamp(bbase(k0), ldb, B_NL, N - n0, tid);
+      if (!TA) xa.load_clamp(abase(k0), lda, A_NL, M - m0, tid);
+      else     xa.load_clamp(abase(k0), lda, M - m0, A_LEN, tid);
+      if (!TB) xb.load_clamp(bbase(k0), ldb, N - n0, B_LEN, tid);
+      else     xb.load_clamp(bbase(k0), ldb, B_NL, N - n0, tid);
     } else {
-      ca.load_vec(abase(k0), lda, tid);
-      cb.load_vec(bbase(k0), ldb, tid);
+      xa.load_vec(abase(k0), lda, tid);
+      xb.load_vec(bbase(k0), ldb, tid);
     }
   };
+  auto load_fast = [&](int k0) { load_into(ca, cb, k0); };
   auto store = [&](int buf) {
     ca.template store<A_LS>(sA(buf), tid);
     cb.template store<B_LS>(sB(buf), tid);
   };
   /* the stored slice starts `skip` positions before the first K position
    * not yet summed: those positions are written as zeros */
-  auto store_skip = [&](int buf, int skip) {
-    ca.template store_skip<A_LS, !TA>(sA(buf), tid, skip);
-    cb.template store_skip<B_LS, TB>(sB(buf), tid, skip);
+  auto store_skip_from = [&](const auto &xa, const auto &xb, int buf, int skip) {
+    xa.template store_skip<A_LS, !TA>(sA(buf), tid, skip);
+    xb.template store_skip<B_LS, TB>(sB(buf), tid, skip);
   };
+  auto store_skip = [&](int buf, int skip) { store_skip_from(ca, cb, buf, skip); };
 
   acc_t acc[FM][FN];
 #pragma unroll
@@ -199,13 +204,81 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     }
     __builtin_amdgcn_sched_group_barrier(0x008, NMF - (NMF / NLD) * NLD, 0);
   };
-  if (FAST && nfull >= 1) {
+  if (FAST && DEPTH == 2 && nfull >= 1) {
+    /* Two slices in flight: slice s + 2's global loads are issued while
+     * slice s is multiplied, into the register stage that slice s vacated,
+     * so a load has two iterations to land.  The loads are unconditional
+     * (past the end they re-read the last slice): loads behind a branch make
+     * the compiler wait for every outstanding load where the paths join. */
+    TileCopy<T, A_NL, A_LEN, NT> ca1;
+    TileCopy<T, B_NL, B_LEN, NT> cb1;
+    auto slice_k = [&](int sl) {
+      const int k0 = sl * BK;
+      return k0 + BK <= K ? k0 : K - BK;
+    };
+    /* slice sl (LDS buffer P) multiplied, slice sl + 1 (register stage !P)
+     * written to LDS buffer !P, slice sl + 2 loaded into stage P; LAST: the
+     * stored slice may be the shifted last one */
+    auto step = [&](auto par, auto last, int sl) {
+      constexpr int P = decltype(par)::value;
+      constexpr bool LAST = decltype(last)::value;
+      auto &la = P ? ca1 : ca;
+      auto &lb = P ? cb1 : cb;
+      const auto &xa = P ? ca : ca1;
+      const auto &xb = P ? cb : cb1;
+      load_into(la, lb, slice_k(sl + 2 < nk ? sl + 2 : nk - 1));
+      compute(P);
+      if (LAST) {
+        store_skip_from(xa, xb, P ^ 1, (sl + 1) * BK - slice_k(sl + 1));
+      } else {
+        xa.template store<A_LS>(sA(P ^ 1), tid);
+        xb.template store<B_LS>(sB(P ^ 1), tid);
+      }
+      issue_order();
+      __syncthreads();
+    };
+    typedef std::integral_constant<int, 0> I0;
+    typedef std::integral_constant<int, 1> I1;
+    typedef std::integral_constant<bool, false> F;
+    typedef std::integral_constant<bool, true> L;
+    load_fast(0);
+    load_into(ca1, cb1, slice_k(nk > 1 ? 1 : 0));
+    store(0);
+    __syncthreads();
+    int sl = 0;
+    /* every slice stored in the loop is a full one: only the last can be
+     * shifted, and it is stored by one of the steps after the loop */
+    for (; sl + 3 < nk; sl += 2) {
+      step(I0(), F(), sl);
+      step(I1(), F(), sl + 1);
+    }
+    if (sl + 1 < nk) {
+      step(I0(), L(), sl);
+      ++sl;
+      if (sl + 1 < nk) {
+        step(I1(), L(), sl);
+        ++sl;
+      }
+    }
+    compute(sl & 1);
+  } else if (FAST && nfull >= 1) {
     load_fast(0);
     store(0);
     __syncthreads();
-    for (int kt = 0; kt + 1 < nk; ++kt) {
-      /* slice kt + 1; the last one (a K remainder) is loaded as the BK
-       * positions ending at K, its first `skip` positions zeroed */
+    int kt = 0;
+    /* the slices the loop stores (kt + 1 <= nk - 2) are full ones, stored
+     * plainly: a select on loaded values in the loop lets the compiler
+     * consume the loads early and wait for them */
+    for (; kt + 2 < nk; ++kt) {
+      load_fast((kt + 1) * BK);
+      compute(kt & 1);
+      store((kt + 1) & 1);
+      issue_order();
+      __syncthreads();
+    }
+    if (kt + 1 < nk) {
+      /* the last slice (a K remainder) is loaded as the BK positions
+       * ending at K, its first `skip` positions zeroed */
       const int k0 = (kt + 1) * BK;
       const int ks = k0 + BK <= K ? k0 : K - BK;
       load_fast(ks);
@@ -213,8 +286,9 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
       store_skip((kt + 1) & 1, k0 - ks);
       issue_order();
       __syncthreads();
+      ++kt;
     }
-    compute((nk - 1) & 1);
+    compute(kt & 1);
   } else {
     load_safe(0);
     store(0);
