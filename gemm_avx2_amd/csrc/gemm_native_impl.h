@@ -39,10 +39,26 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
                    int tiles_n, int vecA, int vecB, int lead, int64_t sa, int64_t sb,
                    int64_t sc, int kc) {
   __shared__ __align__(16) T lds[tile_lds_elems<T, TA, TB, BM, BN, BK>()];
+  const int tiles_m_all = tiles_m + ((M % BM) ? 1 : 0);
+  const int tiles_all = tiles_m_all * (tiles_n + ((N % BN) ? 1 : 0));
+  /* bx: the tile (edge tiles first), by: the grid row (K range or problem),
+   * nx: the tiles of a row */
+  int bx = blockIdx.x, by = blockIdx.y, nx = gridDim.x;
   if (SPLIT) {
-    /* grid row blockIdx.y takes K range [k0, k0 + kc) of every tile; the
-     * partial tiles go to C (the workspace), split by split, tile by tile */
-    const int k0 = blockIdx.y * kc;
+    /* one-dimensional grid of K ranges x tiles, renumbered so that
+     * consecutive numbers run on one XCD (workgroups are dispatched
+     * round-robin over the 8 XCDs); number n takes tile n % tiles_all of
+     * K range n / tiles_all, so the tiles of one K range share that range's
+     * operand panels in one XCD's L2.  The partial tiles go to C (the
+     * workspace), range by range, tile by tile. */
+    constexpr int XCDS = 8;
+    const int total = gridDim.x, xcd = blockIdx.x % XCDS, local = blockIdx.x / XCDS;
+    const int per = total / XCDS, rem = total % XCDS;
+    const int n = xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+    by = n / tiles_all;
+    bx = n - by * tiles_all;
+    nx = tiles_all;
+    const int k0 = by * kc;
     K = K - k0 < kc ? K - k0 : kc;
     A += TA ? (int64_t)k0 : (int64_t)k0 * lda;
     B += TB ? (int64_t)k0 * ldb : (int64_t)k0;
@@ -51,16 +67,14 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
     B += blockIdx.y * sb;
     C += blockIdx.y * sc;
   }
-  const int tiles_m_all = tiles_m + ((M % BM) ? 1 : 0);
-  const int tiles_all = tiles_m_all * (tiles_n + ((N % BN) ? 1 : 0));
   auto part = [&](int tm, int tn) {
-    return C + ((int64_t)blockIdx.y * tiles_all + tm + (int64_t)tn * tiles_m_all) * (BM * BN);
+    return C + ((int64_t)by * tiles_all + tm + (int64_t)tn * tiles_m_all) * (BM * BN);
   };
   int tm, tn;
-  if ((int)blockIdx.x < lead) {
+  if (bx < lead) {
     const int row_tiles = (M % BM) ? tiles_n + ((N % BN) ? 1 : 0) : 0;
     const int col_tiles = (N % BN) ? tiles_m : 0;
-    const int e = blockIdx.x;
+    const int e = bx;
     if (e >= row_tiles + col_tiles) return; /* lead == row_tiles + col_tiles */
     if (e < row_tiles) {
       tm = tiles_m;
@@ -75,7 +89,7 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
         vecA, vecB, lds);
     return;
   }
-  tile_coords(blockIdx.x - lead, gridDim.x - lead, tiles_m, tiles_n, &tm, &tn);
+  tile_coords(bx - lead, nx - lead, tiles_m, tiles_n, &tm, &tn);
   gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, false, NARROW, SPLIT,
               (BM == 64 ? 2 : 1)>(
       A, B, SPLIT ? part(tm, tn) : C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN,
@@ -141,7 +155,7 @@ hipError_t launch_trans(const gemmx_problem *p, int split, void *work, hipStream
     const int kc = split_depth(p->k, split, BK);
     const unsigned ns = (unsigned)((p->k + kc - 1) / kc);
     T *P = (T *)work;
-    const dim3 grid(lead + full_m * full_n, ns);
+    const dim3 grid((unsigned)((lead + full_m * full_n) * ns));
     if (vecA && vecB)
       hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false, true>),
                          grid, block, 0, s, (const T *)p->a, (const T *)p->b, P, p->m, p->n,
