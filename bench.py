@@ -552,8 +552,12 @@ def run_bf16x3(torch, gx, stream):
                 _, per = time_launches(torch, fn, 3, 1)
                 fn()
                 torch.cuda.synchronize()
-                r[form] = {"tflops": round(2.0 * m * n * k / median(per) / 1e9, 2),
+                tf = 2.0 * m * n * k / median(per) / 1e9
+                r[form] = {"tflops": round(tf, 2), "frac_of_fp32_peak": round(tf / PEAK["f32"], 3),
                            **smp.error(torch, C, 1.0, 0.0)}
+                if form == "bf16x3":
+                    # three BF16 products per FP32 product on the BF16 matrix cores
+                    r[form]["bf16_unit_frac"] = round(3.0 * tf / PEAK["bf16"], 3)
             res[name] = r
         out[f"{ta}{tb}_{m}x{n}x{k}"] = res
         del A, B, C, ws
@@ -861,6 +865,7 @@ def main():
     line = {
         "metric": "DGEMM/SGEMM TFLOP/s 8192^3: emulated vs native at matched FP64 error",
         "value": round(flops / best_ms / 1e9, 3),
+        "frac_of_fp64_peak": round(flops / best_ms / 1e9 / PEAK["f64"], 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
