@@ -12,9 +12,11 @@
  * the next tile of its XCD's queue with an atomic ticket, and when that queue
  * is empty, of the other queues in turn, so every wave reaches the exit
  * after at most eight empty queues.  The ticket for the next tile is taken
- * after a tile's stores: a returning atomic issued before the tile's loads
- * would hold back the first wait on them (vector memory operations retire in
- * order), and the workgroup's barriers with it.  A tile is computed by
+ * while a tile's first K slice is being loaded, right after those loads are
+ * issued: the atomic's round trip overlaps the tile, and since vector memory
+ * operations retire in order, an atomic issued before the loads would hold
+ * back the first wait on them (and the workgroup's barriers with it), while
+ * one issued after them does not.  A tile is computed by
  * the same gemm_tile as the single-GEMM kernels: the branch-free loop with
  * an edge tile's loads clamped back into the operand, in 16-byte chunks
  * where both operands allow them and one element per load otherwise,
@@ -96,7 +98,15 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
     const T *B = as_global((const T *)q.b);
     T *C = as_global((T *)q.c);
     const int f = q.flags;
+    /* the next tile's ticket, taken by thread 0 while this tile's first
+     * loads are in flight (issued after them, so waits on those loads do
+     * not wait for the atomic) */
+    int nxt = 0;
+    auto take = [&]() {
+      if (threadIdx.x == 0) nxt = (int)atomicAdd(sched + y * GEMMX_BATCH_CTR_STRIDE, 1u);
+    };
     if (f & GEMMX_BF_SCALE) {
+      take();
       /* alpha == 0 or k == 0: C = beta*C on this tile (beta != 1 here) */
       for (int e = threadIdx.x; e < G::BM * G::BN; e += BT_THREADS) {
         const int i = m0 + e % G::BM, j = n0 + e / G::BM;
@@ -109,8 +119,9 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
        * allow them, else one element per load */
       const int fast = va && vb;
 #define GEMMX_TILE(TA, TB, FAST)                                                        \
-  gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, true, true, !(FAST)>(      \
-      A, B, C, q.m, q.n, q.k, q.lda, q.ldb, q.ldc, alpha, beta, m0, n0, va, vb, lds)
+  gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, true, true, !(FAST), false, \
+            1>(                                                                         \
+      A, B, C, q.m, q.n, q.k, q.lda, q.ldb, q.ldc, alpha, beta, m0, n0, va, vb, lds, take)
       switch ((f & (GEMMX_BF_TA | GEMMX_BF_TB)) | (fast ? 4 : 0)) {
       case 0: GEMMX_TILE(false, false, false); break;
       case GEMMX_BF_TB: GEMMX_TILE(false, true, false); break;
@@ -123,9 +134,12 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
       }
 #undef GEMMX_TILE
     }
-    /* the next tile of this queue (fetch's first barrier also ends this
-     * tile's use of the LDS) */
-    t = fetch(y);
+    /* the next tile of this queue: every wave is done with this tile's LDS
+     * and with the previous ticket before thread 0 publishes the new one */
+    __syncthreads();
+    if (threadIdx.x == 0) ticket = nxt;
+    __syncthreads();
+    t = __builtin_amdgcn_readfirstlane(ticket);
   }
 }
 

@@ -75,7 +75,7 @@ struct SliceTileCopy {
 };
 
 /* Epilogue of one wave's FM x FN fragment grid (rows from r0, columns from
- * c0): BF16 -> C = fma(alpha, acc, beta*C); INT8 sums -> S slot gout[g];
+ * c0): BF16 -> C = fma(alpha, acc, beta*C) (store_c_tile); INT8 sums -> S slot gout[g];
  * INT8 residue products -> the sum modulo gout[g] as a byte in plane g. */
 template <int KIND, int FM, int FN, typename acc_t>
 __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, const SliceGroups &G,
@@ -121,6 +121,11 @@ __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, co
     }
     return;
   }
+  if constexpr (BF16) {
+    /* the BF16 MFMA's result rows are 4*(lane/16) + r, as FP32's */
+    store_c_tile<float, FM, FN>(acc, C, M, N, ldc, alpha, beta, r0, c0, li, kg);
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -128,20 +133,13 @@ __device__ __forceinline__ void store_tile(const acc_t (&acc)[FM][FN], int g, co
       const int col = c0 + j * 16 + li;
       const int row = r0 + i * 16 + kg * 4;
       if (col >= N) continue;
-      if constexpr (BF16) {
-        float *cp = C + row + col * ldc;
+      int *sp = (int *)S + (int64_t)G.gout[g] * M * N + row + (int64_t)col * M;
+      if (row + 3 < M && (M & 3) == 0) {
+        *reinterpret_cast<i4_t *>(sp) = *reinterpret_cast<const i4_t *>(&acc[i][j]);
+      } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (row + r < M) cp[r] = __builtin_fmaf(alpha, (float)acc[i][j][r], beta * cp[r]);
-      } else {
-        int *sp = (int *)S + (int64_t)G.gout[g] * M * N + row + (int64_t)col * M;
-        if (row + 3 < M && (M & 3) == 0) {
-          *reinterpret_cast<i4_t *>(sp) = *reinterpret_cast<const i4_t *>(&acc[i][j]);
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (row + r < M) sp[r] = (int)acc[i][j][r];
-        }
+          if (row + r < M) sp[r] = (int)acc[i][j][r];
       }
     }
 }

@@ -66,6 +66,55 @@ template <> struct Vec16<int> {
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+/* Epilogue of one wave's FM x FN grid of 16x16 MFMA fragments (row r of
+ * fragment (i, j) at C row r0 + 16 i + Mfma16<T>::row(kg, r), column
+ * c0 + 16 j + li): C = fma(alpha, acc, beta*C) for rows < M and columns < N
+ * only, the reference's masked stores (wjc404/GEMM_AVX2
+ * src/kernel/gemm_kernel_unroll6.S:145-155).  The C values of RC fragment
+ * rows (at most 16 values per lane) are loaded together before any of their
+ * stores, from addresses clamped into C, so the loads are unconditional and
+ * in flight at once: a masked load-multiply-store per element made the
+ * compiler wait for each load in turn, one memory round trip per element.
+ * A clamped load may read an element an earlier batch already stored; it
+ * only feeds a position that is not stored. */
+template <typename T, int FM, int FN, typename acc_t>
+__device__ __forceinline__ void store_c_tile(const acc_t (&acc)[FM][FN], T *__restrict__ C, int M,
+                                             int N, int64_t ldc, T alpha, T beta, int r0, int c0,
+                                             int li, int kg) {
+  constexpr int PER_ROW = FN * 4;
+  constexpr int RC0 = PER_ROW >= 16 ? 1 : 16 / PER_ROW;
+  constexpr int RC = RC0 < FM ? RC0 : FM;
+  static_assert(FM % RC == 0, "fragment rows per load batch");
+#pragma unroll
+  for (int i0 = 0; i0 < FM; i0 += RC) {
+    T cv[RC][FN][4];
+#pragma unroll
+    for (int ii = 0; ii < RC; ++ii)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = c0 + j * 16 + li;
+        const int64_t cc = col < N ? col : N - 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + (i0 + ii) * 16 + Mfma16<T>::row(kg, r);
+          cv[ii][j][r] = C[(row < M ? row : M - 1) + cc * ldc];
+        }
+      }
+#pragma unroll
+    for (int ii = 0; ii < RC; ++ii)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = c0 + j * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = r0 + (i0 + ii) * 16 + Mfma16<T>::row(kg, r);
+          if (col < N && row < M)
+            C[row + (int64_t)col * ldc] = fma_t(alpha, (T)acc[i0 + ii][j][r], beta * cv[ii][j][r]);
+        }
+      }
+  }
+}
+
 /* Copies an NL x LEN tile -- NL lines of LEN contiguous elements, line
  * stride `ld` in global memory -- into LDS lines of stride LS, in 16-byte
  * chunks.  Elements outside (nl_valid, len_valid) read as zero and are never

@@ -58,14 +58,20 @@ constexpr int tile_lds_max() {
   return a > b ? (a > c ? (a > d ? a : d) : (c > d ? c : d)) : (b > c ? (b > d ? b : d) : (c > d ? c : d));
 }
 
+/* Work the caller overlaps with the first slice's global loads (default:
+ * none); the grouped batch takes its next tile's ticket there. */
+struct NoHook {
+  __device__ __forceinline__ void operator()() const {}
+};
+
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KVMAX,
           bool FAST, bool CLAMP = false, bool NARROW = false, bool PARTIAL = false,
-          int DEPTH = 1>
+          int DEPTH = 1, typename Hook = NoHook>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           T *__restrict__ C, int M, int N, int K,
                                           int64_t lda, int64_t ldb, int64_t ldc, T alpha,
                                           T beta, int m0, int n0, int vecA, int vecB,
-                                          T *lds) {
+                                          T *lds, Hook after_first_loads = Hook()) {
   constexpr int NT = 64 * WM * WN;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 16, FN = WTN / 16;
@@ -243,6 +249,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     typedef std::integral_constant<bool, true> L;
     load_fast(0);
     load_into(ca1, cb1, slice_k(nk > 1 ? 1 : 0));
+    after_first_loads();
     store(0);
     __syncthreads();
     int sl = 0;
@@ -263,6 +270,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     compute(sl & 1);
   } else if (FAST && nfull >= 1) {
     load_fast(0);
+    after_first_loads();
     store(0);
     __syncthreads();
     int kt = 0;
@@ -291,6 +299,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     compute(kt & 1);
   } else {
     load_safe(0);
+    after_first_loads();
     store(0);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
@@ -317,21 +326,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
   }
   /* Epilogue: rows m..ldc-1 and columns >= n are never touched, matching the
    * reference's masked stores (src/kernel/gemm_kernel_unroll6.S:145-155). */
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int col = n0 + wn * WTN + j * 16 + li;
-      const int row0 = m0 + wm * WTM + i * 16;
-      if (col < N) {
-        T *cp = C + row0 + col * ldc;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int dr = Mfma16<T>::row(kg, r);
-          if (row0 + dr < M) cp[dr] = fma_t(alpha, acc[i][j][r], beta * cp[dr]);
-        }
-      }
-    }
+  store_c_tile<T, FM, FN>(acc, C, M, N, ldc, alpha, beta, m0 + wm * WTM, n0 + wn * WTN, li, kg);
 }
 
 /* Row and column, inside its BM x BN tile, of element e of a partial tile
