@@ -105,7 +105,8 @@ constexpr int OZ2_LIMBS = 6;    /* 32-bit limbs: M < 2^171 */
 constexpr int OZ2_LIMBS16 = 11; /* 16-bit limbs of M_j */
 
 struct Oz2Crt {
-  int nmod, L; /* L = floor(log2 M) */
+  int nmod, L;    /* L = floor(log2 M) */
+  Oz2Bound bound; /* the bit-count budget of the split (gemmx_slices.h) */
   unsigned W16[OZ2_MAX_MODULI][OZ2_LIMBS16]; /* W_j = (w_j M_j) mod M, 16-bit limbs */
   unsigned Mc16[OZ2_LIMBS16];                /* 2^(16 LW) - M, 16-bit limbs */
   double rMs;                                /* 2^(16 (LW - 4)) / M, rounded */
@@ -175,9 +176,10 @@ bool oz2_constants(int nmod, Oz2Crt *c) {
   double md = 0.0;
   for (int l = OZ2_LIMBS - 1; l >= 0; --l) md = md * 4294967296.0 + (double)M[l];
   c->rMs = ldexp(1.0 / md, 16 * (lw - 4));
-  /* the rounding of X'/M needs its distance from a half-integer, at least
-   * (M - 2^L) / (2M), well above the estimate's error */
-  if ((md - ldexp(1.0, c->L)) / (2.0 * md) < 0x1p-10) return false;
+  /* the split's bit counts keep |X| <= (1 - 2^-10) M / 2, so X'/M stays
+   * 2^-11 or more from a half-integer, far above the estimate's error
+   * (below 2^-18: the four top limb sums, M >= 2^(16 (LW - 1))) */
+  c->bound = oz2_bound(nmod);
   return true;
 }
 
@@ -361,7 +363,7 @@ extern "C" size_t gemmx_ozaki2_workspace(int m, int n, int k, int moduli) {
 extern "C" int gemmx_ozaki2_bits(int moduli, int k) {
   Oz2Crt c;
   if (k < 1 || !oz2_constants(moduli, &c)) return -1;
-  return ozaki2_bits_for_sum(c.L, (unsigned long long)k << 20); /* every t = 1024 */
+  return ozaki2_bits_for_sum(c.bound, (unsigned long long)k << 20); /* every t = 1024 */
 }
 
 extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *work,
@@ -377,10 +379,10 @@ extern "C" hipError_t gemmx_ozaki2_f64(const gemmx_problem *p, int nmod, void *w
   unsigned long long *sq = (unsigned long long *)(w + L.sq);
   if (ev) (void)hipEventRecord(ev[0], st);
   hipError_t e = ozaki2_prepare((const double *)p->a, p->lda, p->ta != 0, p->m, p->k, L.kpad,
-                                nmod, crt.L, ea, mb, sq, as, st);
+                                nmod, crt.bound, ea, mb, sq, as, st);
   if (e != hipSuccess) return e;
   e = ozaki2_prepare((const double *)p->b, p->ldb, p->tb == 0, p->n, p->k, L.kpad, nmod,
-                     crt.L, fb, mb, sq, bs, st);
+                     crt.bound, fb, mb, sq, bs, st);
   if (e != hipSuccess) return e;
   if (ev) (void)hipEventRecord(ev[1], st);
   /* group g: plane g of A times plane g of B, stored modulo m_g */

@@ -92,8 +92,8 @@ exp_from_bits_kernel(const unsigned long long *__restrict__ maxbits, int nvec,
 }
 
 
-__device__ __forceinline__ int oz2_shift(int L, int e, unsigned long long S) {
-  return e == OZ_NONFINITE ? OZ_NONFINITE : ozaki2_bits_for_sum(L, S) - e;
+__device__ __forceinline__ int oz2_shift(Oz2Bound bd, int e, unsigned long long S) {
+  return e == OZ_NONFINITE ? OZ_NONFINITE : ozaki2_bits_for_sum(bd, S) - e;
 }
 
 /* Ozaki-II shift D = P - e per vector with elements contiguous: a block of
@@ -117,7 +117,7 @@ __device__ __forceinline__ unsigned long long max_u64(unsigned long long a,
 }
 
 __global__ void __launch_bounds__(256)
-oz2_shift_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len, int L,
+oz2_shift_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len, Oz2Bound bd,
                         int *__restrict__ out) {
   constexpr int R = OZ2_SHIFT_REGS;
   __shared__ unsigned long long red[4];
@@ -161,7 +161,7 @@ oz2_shift_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int
   __syncthreads(); /* red[] reused */
   if (lane == 0) red[wave] = S;
   __syncthreads();
-  if (tid == 0) out[v] = oz2_shift(L, e, red[0] + red[1] + red[2] + red[3]);
+  if (tid == 0) out[v] = oz2_shift(bd, e, red[0] + red[1] + red[2] + red[3]);
 }
 
 /* Bound sums per vector with vectors adjacent (x[v + p*eld]), in the blocks
@@ -194,10 +194,10 @@ oz2_sums_strided_kernel(const double *__restrict__ x, int64_t eld, int nvec, int
 
 /* ex: exponent -> shift D = P - e, from the bound sums. */
 __global__ void __launch_bounds__(256)
-oz2_shift_from_sums_kernel(const unsigned long long *__restrict__ sums, int nvec, int L,
+oz2_shift_from_sums_kernel(const unsigned long long *__restrict__ sums, int nvec, Oz2Bound bd,
                            int *__restrict__ ex) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < nvec) ex[v] = oz2_shift(L, ex[v], sums[v]);
+  if (v < nvec) ex[v] = oz2_shift(bd, ex[v], sums[v]);
 }
 
 /* 4 NW consecutive K entries of one vector per thread -> 4 NW bytes per
@@ -444,7 +444,7 @@ constexpr int OZ2_FUSED_CHUNKS = 4;
 
 __global__ void __launch_bounds__(256)
 ozaki2_shift_split_contig_kernel(const double *__restrict__ x, int64_t vld, int nvec, int len,
-                                 int64_t kpad, int L, int nmod, int *__restrict__ ex,
+                                 int64_t kpad, Oz2Bound bd, int nmod, int *__restrict__ ex,
                                  int8_t *__restrict__ out) {
   constexpr int R = OZ2_FUSED_CHUNKS, EPT = 8;
   typedef int vec_t __attribute__((ext_vector_type(2)));
@@ -496,7 +496,7 @@ ozaki2_shift_split_contig_kernel(const double *__restrict__ x, int64_t vld, int 
   if (lane == 0) red[wave] = S;
   __syncthreads();
   if (tid == 0) {
-    const int d = oz2_shift(L, e, red[0] + red[1] + red[2] + red[3]);
+    const int d = oz2_shift(bd, e, red[0] + red[1] + red[2] + red[3]);
     ex[v] = d;
     dshared = d;
   }
@@ -754,19 +754,19 @@ hipError_t ozaki_prepare(const double *x, int64_t ld, bool contiguous, int nvec,
 }
 
 hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec, int k,
-                          int64_t kpad, int nmod, int L, int *ex, unsigned long long *mbits,
+                          int64_t kpad, int nmod, Oz2Bound bd, int *ex, unsigned long long *mbits,
                           unsigned long long *sums, int8_t *res, hipStream_t st) {
   /* 8 entries per thread (two words per plane): fewer registers, so 5 waves
    * per SIMD, measured faster than 16 for both forms */
   constexpr int NW = 2;
   if (contiguous && kpad <= 256 * 8 * OZ2_FUSED_CHUNKS) {
     hipLaunchKernelGGL(ozaki2_shift_split_contig_kernel, dim3(nvec), dim3(256), 0, st, x, ld,
-                       nvec, k, kpad, L, nmod, ex, res);
+                       nvec, k, kpad, bd, nmod, ex, res);
     return hipGetLastError();
   }
   if (contiguous) {
     hipLaunchKernelGGL(oz2_shift_contig_kernel, dim3(nvec), dim3(256), 0, st, x, ld, nvec, k,
-                       L, ex);
+                       bd, ex);
     const int64_t threads = (kpad / (4 * NW)) * nvec;
     hipLaunchKernelGGL(ozaki2_split_kernel<NW>, dim3((unsigned)((threads + 255) / 256)),
                        dim3(256), 0, st, x, ld, nvec, k, kpad, ex, nmod, res);
@@ -781,7 +781,7 @@ hipError_t ozaki2_prepare(const double *x, int64_t ld, bool contiguous, int nvec
                      mbits, nvec, ex);
   hipLaunchKernelGGL(oz2_sums_strided_kernel, red, dim3(256), 0, st, x, ld, nvec, k, ex, sums);
   hipLaunchKernelGGL(oz2_shift_from_sums_kernel, dim3((unsigned)(((int64_t)nvec + 255) / 256)), dim3(256), 0, st,
-                     sums, nvec, L, ex);
+                     sums, nvec, bd, ex);
   hipLaunchKernelGGL(ozaki2_split_strided_kernel<NW>,
                      dim3((unsigned)(((int64_t)nvec + 31) / 32), (unsigned)(kpad / 128)), dim3(1024 / NW), 0, st, x,
                      ld, nvec, k, kpad, ex, nmod, res);

@@ -765,40 +765,61 @@ int oracle_ozaki2_modulus(int j) {
   return (j >= 0 && j < count) ? list[j] : -1;
 }
 
-/* floor(log2(M)), M the product of the first nmod moduli, exactly */
-static int ozaki2_log2m(int nmod) {
-  uint32_t limb[8] = {0};
-  limb[0] = 1;
+/* The bit-count budget of the first nmod moduli: T = 2^19 Mr with
+ * Mr = M - floor(M / 2^10) - 1, M the product of the moduli, rounded down to
+ * its top 64 bits, T >= t64 2^e (e = bitlen(T) - 64).  Computed with 32-bit
+ * limbs, one bit at a time. */
+static void ozaki2_budget(int nmod, uint64_t *t64, int *e) {
+  uint64_t limb[8] = {1};
   for (int j = 0; j < nmod; j++) {
     uint64_t carry = 0;
     for (int t = 0; t < 8; t++) {
-      uint64_t v = (uint64_t)limb[t] * (uint64_t)oracle_ozaki2_modulus(j) + carry;
-      limb[t] = (uint32_t)v;
+      const uint64_t v = limb[t] * (uint64_t)oracle_ozaki2_modulus(j) + carry;
+      limb[t] = v & 0xffffffffu;
       carry = v >> 32;
     }
   }
-  for (int t = 7; t >= 0; t--)
-    if (limb[t]) return 32 * t + 31 - __builtin_clz(limb[t]);
-  return 0;
+  /* Mr = M - (M >> 10) - 1, limb by limb with borrows */
+  uint64_t q[8];
+  for (int t = 0; t < 8; t++) q[t] = ((limb[t] >> 10) | (t < 7 ? limb[t + 1] << 22 : 0)) & 0xffffffffu;
+  int64_t borrow = 1;
+  for (int t = 0; t < 8; t++) {
+    int64_t v = (int64_t)limb[t] - (int64_t)q[t] - borrow;
+    borrow = v < 0;
+    limb[t] = (uint64_t)(v + (borrow ? ((int64_t)1 << 32) : 0));
+  }
+  int bl = 0;
+  for (int bit = 255; bit >= 0; bit--)
+    if ((limb[bit / 32] >> (bit % 32)) & 1) {
+      bl = bit + 1;
+      break;
+    }
+  uint64_t t = 0;
+  for (int bit = bl - 1; bit >= bl - 64; bit--) t = (t << 1) | ((limb[bit / 32] >> (bit % 32)) & 1);
+  *t64 = t;
+  *e = bl - 64 + 19;
 }
 
-/* bit length of s - 1 (0 for s <= 1): the smallest b with s <= 2^b */
-static int ceil_log2_u64(uint64_t s) { return s <= 1 ? 0 : 64 - __builtin_clzll(s - 1); }
-
-/* P for a vector whose bound sum is S (see oracle_ozaki2_vector_bits):
- * the largest P with 2P + bitlen(S - 1) <= floor(log2 M) + 19, capped at
- * OZAKI2_MAX_BITS.  Then |a_p| <= t_p 2^(P-10) (P >= 10) gives
- * ||a|| <= 2^(P - 10) sqrt(S) <= 2^((L - 1) / 2) and, for two vectors,
- * |sum_p a_p b_p| <= ||a|| ||b|| <= 2^(L-1) < M/2. */
-static int ozaki2_bits_for_sum(int L, uint64_t S) {
-  const int P = (L + 19 - ceil_log2_u64(S)) / 2;
-  return P < OZAKI2_MAX_BITS ? P : OZAKI2_MAX_BITS;
+/* P for a vector whose bound sum is S (see oracle_ozaki2_vector_bits): the
+ * largest P <= OZAKI2_MAX_BITS with S 4^P <= t64 2^e.  Then |a_p| <=
+ * t_p 2^(P-10) (P >= 10) gives ||a||^2 <= 2^(2P-20) S <= Mr / 2 and, for two
+ * vectors, |sum_p a_p b_p| <= ||a|| ||b|| <= Mr / 2 < M / 2. */
+static int ozaki2_bits_for_sum(int nmod, uint64_t S) {
+  uint64_t t64;
+  int e;
+  ozaki2_budget(nmod, &t64, &e);
+  for (int P = OZAKI2_MAX_BITS; P > 0; P--) {
+    const int d = 2 * P - e;
+    if (d <= 0) return P; /* S < 2^52 <= t64 2^(e - 2P) */
+    if (d < 64 && ((u128)S << d) <= (u128)t64) return P;
+  }
+  return 0;
 }
 
 int oracle_ozaki2_bits(int nmod, int k) {
   if (nmod < OZAKI2_MIN_MODULI || nmod > OZAKI2_MAX_MODULI || k < 1) return -1;
   /* every t_p = 1024 */
-  return ozaki2_bits_for_sum(ozaki2_log2m(nmod), (uint64_t)k << 20);
+  return ozaki2_bits_for_sum(nmod, (uint64_t)k << 20);
 }
 
 int oracle_ozaki2_vector_bits(const double *x, int count, int stride, int nmod) {
@@ -811,7 +832,7 @@ int oracle_ozaki2_vector_bits(const double *x, int count, int stride, int nmod) 
       const uint64_t t = (uint64_t)ceil(ldexp(fabs(x[(int64_t)p * stride]), 10 - e));
       S += t * t;
     }
-  return ozaki2_bits_for_sum(ozaki2_log2m(nmod), S);
+  return ozaki2_bits_for_sum(nmod, S);
 }
 
 static i128 ozaki2_scaled(double x, int e, int P) {

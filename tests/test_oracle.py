@@ -358,29 +358,30 @@ def _oz2_scaled_vector(x, P):
 @pytest.mark.parametrize("k", [1, 37, 300, 1024, 1537])
 def test_ozaki2_vector_bits_keep_every_dot_product_inside_the_crt_range(k):
     """Safety, exactly in integers: every vector's scaled integers have
-    ||a||^2 <= 2^(L-1), L = floor(log2 M), so by Cauchy-Schwarz any dot
-    product of two vectors' integers is below 2^(L-1) < M/2 (M is no power of
-    two).  Near-maximality: for the vector with every entry at the top of
-    its binade the same integers at P + 1 bits break that norm bound, and no
-    vector gets fewer bits than one bit count for k full-size terms,
-    floor((L - 1 - bitlen(k)) / 2), would give."""
+    ||a||^2 <= Mr/2, Mr = M - floor(M/2^10) - 1, so by Cauchy-Schwarz any dot
+    product of two vectors' integers is at most Mr/2 < (1 - 2^-10) M/2: the
+    reconstruction's X/M stays at least 2^-11 from a half-integer.
+    Near-maximality: for the vector with every entry at the top of its
+    binade the same integers at P + 1 bits break that norm bound, and no
+    vector gets fewer bits than one bit count for k full-size terms with
+    L = floor(log2 M), floor((L - 1 - bitlen(k)) / 2), would give."""
     rng = np.random.default_rng(20 + k)
     for nmod in (8, 11, 14, 16, 18, 20, 22):
         M = math.prod(oracle.ozaki2_modulus(j) for j in range(nmod))
         L = M.bit_length() - 1
-        assert 2 ** (L - 1) < M // 2 or M % 2 == 1
+        Mr = M - M // 1024 - 1
         floor_rule = min((L - 1 - k.bit_length()) // 2, 77)
         for idx, x in enumerate(_oz2_vectors(k, rng)):
             P = oracle.ozaki2_vector_bits(x, nmod)
             assert 10 <= P <= 77, (nmod, idx, P)
             a = _oz2_scaled_vector(x, P)
-            assert sum(v * v for v in a) <= 2 ** (L - 1), (nmod, k, idx, P)
+            assert 2 * sum(v * v for v in a) <= Mr, (nmod, k, idx, P)
             assert P >= floor_rule, (nmod, k, idx, P, floor_rule)
             if idx == 3:  # every entry at the top of its binade
                 assert P == oracle.ozaki2_bits(nmod, k)
                 if P < 77:
                     a1 = _oz2_scaled_vector(x, P + 1)
-                    assert sum(v * v for v in a1) > 2 ** (L - 1), (nmod, k, P)
+                    assert 2 * sum(v * v for v in a1) > Mr, (nmod, k, P)
     assert oracle.ozaki2_bits(7, k) == -1 and oracle.ozaki2_bits(23, k) == -1
     assert oracle.ozaki2_vector_bits(np.ones(4), 7) == -1
 
