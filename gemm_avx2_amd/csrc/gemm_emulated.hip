@@ -104,10 +104,14 @@ SliceGroups ozaki_groups(int s) {
 constexpr int OZ2_LIMBS = 6;    /* 32-bit limbs: M < 2^171 */
 constexpr int OZ2_LIMBS16 = 11; /* 16-bit limbs of M_j */
 
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+
 struct Oz2Crt {
   int nmod, L;    /* L = floor(log2 M) */
   Oz2Bound bound; /* the bit-count budget of the split (gemmx_slices.h) */
-  unsigned W16[OZ2_MAX_MODULI][OZ2_LIMBS16]; /* W_j = (w_j M_j) mod M, 16-bit limbs */
+  /* W_j = (w_j M_j) mod M in 16-bit limbs, two moduli per word: limb l of
+   * W_2p in the low half of W2[p][l], of W_2p+1 (0 past nmod) in the high */
+  unsigned W2[OZ2_MAX_MODULI / 2][OZ2_LIMBS16];
   unsigned Mc16[OZ2_LIMBS16];                /* 2^(16 LW) - M, 16-bit limbs */
   double rMs;                                /* 2^(16 (LW - 4)) / M, rounded */
 };
@@ -162,7 +166,7 @@ bool oz2_constants(int nmod, Oz2Crt *c) {
     }
     if (carry || (wj[OZ2_LIMBS - 1] >> 16)) return false; /* W_j < 2^176 */
     for (int l = 0; l < OZ2_LIMBS16; ++l)
-      c->W16[j][l] = (wj[l / 2] >> (16 * (l % 2))) & 0xffffu;
+      c->W2[j / 2][l] |= ((wj[l / 2] >> (16 * (l % 2))) & 0xffffu) << (16 * (j % 2));
   }
   const int lw = oz2_lw(c->L);
   /* 2^(16 lw) - M in 16-bit limbs: the complement of M's limbs plus one */
@@ -290,14 +294,21 @@ ozaki2_recon_kernel(const uint8_t *__restrict__ R, const Oz2Crt crt,
         c4[g] = *reinterpret_cast<const unsigned *>(R + (int64_t)g * total + t0);
       }
     }
+    /* two moduli per v_dot2_u32_u16: element q's residues of planes 2p
+     * and 2p + 1 as the two 16-bit halves of one word (one byte permute),
+     * times the paired limbs; each step adds below 2^25, so the limb sums
+     * stay below nmod 2^24 < 2^29 as with one multiply-add per modulus */
 #pragma unroll
-    for (int g = 0; g < OZ2_MAX_MODULI; ++g) {
-      if (g < crt.nmod) {
+    for (int p = 0; p < OZ2_MAX_MODULI / 2; ++p) {
+      if (2 * p < crt.nmod) {
 #pragma unroll
         for (int q = 0; q < E; ++q) {
-          const unsigned cg = (c4[g] >> (8 * q)) & 0xffu;
+          /* bytes: [c4[2p] byte q, 0, c4[2p+1] byte q, 0] */
+          const unsigned sel = (unsigned)q | 0x0c00u | ((4u + q) << 16) | 0x0c000000u;
+          const u16x2_t cp = __builtin_bit_cast(u16x2_t, __builtin_amdgcn_perm(c4[2 * p + 1], c4[2 * p], sel));
 #pragma unroll
-          for (int l = 0; l < LW; ++l) acc[q][l] += __umul24(cg, crt.W16[g][l]);
+          for (int l = 0; l < LW; ++l)
+            acc[q][l] = __builtin_amdgcn_udot2(cp, __builtin_bit_cast(u16x2_t, crt.W2[p][l]), acc[q][l], false);
         }
       }
     }
