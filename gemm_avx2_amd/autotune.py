@@ -38,7 +38,7 @@ import sys
 import time
 
 FORMAT = 2
-TILE_CONFIGS = 7  # compiled native tile configurations (GEMMX_TILE_CONFIGS)
+TILE_CONFIGS = 8  # compiled native tile configurations (GEMMX_TILE_CONFIGS)
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
                            "gemm_mi355x_tune.txt")
 TRANS = ("NN", "NT", "TN", "TT")

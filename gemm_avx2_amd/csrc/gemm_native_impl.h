@@ -200,7 +200,7 @@ struct NativeCfg {
 };
 constexpr NativeCfg kNativeCfg[GEMMX_NCFG_COUNT] = {
     {128, 128, 16, 1}, {64, 64, 16, 1}, {256, 128, 16, 0}, {128, 128, 32, 0},
-    {128, 128, 16, 0}, {128, 128, 16, 1}, {256, 128, 16, 0}};
+    {128, 128, 16, 0}, {128, 128, 16, 1}, {256, 128, 16, 0}, {64, 64, 32, 1}};
 
 /* One transpose pair: the configuration switch. */
 template <typename T, bool TA, bool TB>
@@ -218,6 +218,10 @@ hipError_t launch_native_trans(const gemmx_problem *p, int cfg, int split, void 
   case 4: return launch_trans<T, TA, TB, 128, 128, 16, 2, 2, 1, 4, false>(p, 1, NULL, s);
   case 5: return launch_trans<T, TA, TB, 128, 128, 16, 4, 2, 2, 1, true>(p, split, work, s);
   case 6: return launch_trans<T, TA, TB, 256, 128, 16, 4, 2, 1, 4, false>(p, 1, NULL, s);
+  /* 64x64 with two waves per SIMD (8 waves of 32x16) for products with
+   * about one tile per CU; FP32 takes 32-deep K slices so that each of the
+   * 512 threads copies a whole 16-byte chunk of each operand slice */
+  case 7: return launch_trans<T, TA, TB, 64, 64, std::is_same<T, float>::value ? 32 : 16, 2, 4, 2, 1, true>(p, split, work, s);
   case 0:
   default: return launch_trans<T, TA, TB, 128, 128, 16, 2, 2, 2, 1, true>(p, split, work, s);
   }
