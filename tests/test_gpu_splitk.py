@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import gemm_avx2_amd as gx
+from gemm_avx2_amd import autotune as at
 import oracle
 from gemm_util import (U32, U64, abs_bound, from_device, max_comp, rel_fro, stored_shape,
                        to_device)
@@ -60,7 +61,7 @@ def test_forced_splits_match_the_oracle(cuda, dt, ta, tb):
                         B.astype(R.dtype, order="F"), ldb, -0.5, R, ldc)
             bound = abs_bound(A, B, C, ta, tb, m, n, k, 1.5, -0.5)
             dA, dB = to_device(A, torch), to_device(B, torch)
-            for cfg in (0, 1, 5):
+            for cfg in at.SPLIT_CONFIGS:
                 gx.set_tile_config(d, cfg)
                 for split in (2, 3, 7, 16):
                     print(f"case {d} {ta}{tb} {m}x{n}x{k} cfg {cfg} split {split}", flush=True)

@@ -140,6 +140,15 @@ def test_configuration_counts_agree():
     nb = int(re.search(r"#define GEMMX_BATCH_CONFIGS (\d+)", hdr).group(1))
     assert gx.TILE_CONFIGS == at.TILE_CONFIGS == n
     assert gx.BATCH_CONFIGS == nb
+    # the autotuner's copy of each configuration's tile and split-K ability
+    # follows the kernels' own table (kNativeCfg: bm, bn, bk, split_ok)
+    impl = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                             "gemm_avx2_amd", "csrc", "gemm_native_impl.h")).read()
+    body = re.search(r"kNativeCfg\[GEMMX_NCFG_COUNT\] = \{(.*?)\};", impl, re.S).group(1)
+    cfgs = [tuple(int(v) for v in t) for t in re.findall(r"\{(\d+), (\d+), (\d+), (\d)\}", body)]
+    assert len(cfgs) == n
+    assert list(at.TILE_DIMS) == [(bm, bn) for bm, bn, _, _ in cfgs]
+    assert tuple(at.SPLIT_CONFIGS) == tuple(c for c, t in enumerate(cfgs) if t[3])
 
 
 def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
@@ -314,3 +323,12 @@ def test_split_column_forced_split_and_heuristic(tmp_path, dtype, restore_tables
     finally:
         gx.set_split_config(dtype, -1)
     assert ls("N", "N", 512, 512, 65536)[0] == 16
+
+
+def test_batch_pick_needs_a_clear_lead_over_the_default():
+    """The autotuner records a batch configuration other than 0 only when it
+    is at least 3% faster and its result agreed with configuration 0's."""
+    assert at.batch_pick([2.386, 2.380, 2.363, 2.868], [True] * 4) == 0
+    assert at.batch_pick([3.0, 2.8, 3.1, 3.2], [True] * 4) == 1
+    assert at.batch_pick([3.0, 2.8, 2.95, 3.2], [True, False, True, True]) == 0
+    assert at.batch_pick([3.0, 3.1, 2.5, 3.2], [True, True, True, False]) == 2
