@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--profile-only",
                     choices=["native", "ozaki", "ozaki2", "sgemm", "bf16x3", "batch", "batch32",
-                             "splitk", "copy"],
+                             "splitk", "native1024", "copy"],
                     help="run just one form (for rocprofv3)")
     ap.add_argument("--slices", type=int, default=8,
                     help="slice count (ozaki) or modulus count (ozaki2) for --profile-only")
@@ -768,6 +768,17 @@ def profile_only(torch, gx, stream, args):
 
         def fn():
             plan.run(stream)
+    elif args.profile_only == "native1024":
+        # FP64 1024^3 NN, alpha 1, beta 0 (one C tile per CU), the tuning
+        # table's kernel for it
+        n = 1024
+        A, B, C = (torch.empty(n * n, dtype=torch.float64, device="cuda") for _ in range(3))
+        for t, i in ((A, 0), (B, 1), (C, 2)):
+            gx.fill(t, n, n, n, 0, i, stream=stream)
+        keep = [A, B, C]
+
+        def fn():
+            gx.dgemm_dev("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n, stream=stream)
     elif args.profile_only == "splitk":
         # FP64 512 x 512 x 65536 NN, alpha 1.5, beta -0.5: few C tiles, split K
         m = n = 512

@@ -442,11 +442,15 @@ def _qualify(torch, gx, trans, m, n, k, A, B, C, native, forms, seeds, samples):
     entries (alpha 1, beta 0, against Dot2 values of the same inputs) is no
     larger than native's in relative Frobenius norm and at most
     EMU_COMP_MARGIN times native's largest error relative to sum |a||b|, for
-    entries (u - 0.5) exp(phi g), phi in PHIS, on every seed."""
+    entries (u - 0.5) exp(phi g), phi in PHIS, on every seed; also, per name,
+    the largest ratio of its error to native's on each measure over the cases
+    it was checked on (a name drops out at its first failing case).  A
+    non-finite error never qualifies; a non-finite native error is an error."""
     import numpy as np
     ra, ca = (m, k) if trans[0] == "N" else (k, m)
     rb, cb = (k, n) if trans[1] == "N" else (n, k)
     qual = set(forms)
+    ratio = {name: [0.0, 0.0] for name in forms}
     for seed in seeds:
         rng = np.random.default_rng(seed)
         is_, js = rng.integers(0, m, samples), rng.integers(0, n, samples)
@@ -466,16 +470,30 @@ def _qualify(torch, gx, trans, m, n, k, A, B, C, native, forms, seeds, samples):
                                    cols.contiguous().double().cpu().numpy(), ri, cj)
 
             def err(fn):
+                # beta = 0 still multiplies C (dgemm_'s rule), so C must hold
+                # no NaN left over from earlier allocations
+                C.zero_()
                 fn()
                 torch.cuda.synchronize()
                 return sample_errors(C[flat].double().cpu().numpy(), exact, ab)
             en = err(native)
+            if not (np.isfinite(en["rel_fro"]) and np.isfinite(en["max_comp"])):
+                raise RuntimeError(f"autotune: native {trans} {m}x{n}x{k} error is not finite "
+                                   f"(seed {seed}, phi {phi}): {en}")
             for name in sorted(qual):
                 e = err(forms[name])
-                if (e["rel_fro"] > en["rel_fro"] or
-                        e["max_comp"] > EMU_COMP_MARGIN * en["max_comp"]):
+                for i, what in enumerate(("rel_fro", "max_comp")):
+                    q = e[what] / max(en[what], 1e-300)
+                    ratio[name][i] = max(ratio[name][i], q if np.isfinite(q) else np.inf)
+                # written so that a NaN error fails
+                if not (e["rel_fro"] <= en["rel_fro"] and
+                        e["max_comp"] <= EMU_COMP_MARGIN * en["max_comp"]):
                     qual.discard(name)
-    return qual
+    return qual, ratio
+
+
+def _ratio_note(r):
+    return f"error x{r[0]:.2f} / x{r[1]:.2f} of native"
 
 
 def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_SEEDS,
@@ -508,7 +526,7 @@ def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_S
 
     forms = {("ozaki2", nm): (lambda nm=nm: ozaki2(nm)) for nm in moduli}
     forms.update({("ozaki", s): (lambda s=s: ozaki(s)) for s in slices})
-    qual = _qualify(torch, gx, trans, m, n, k, A, B, C, native, forms, seeds, samples)
+    qual, ratio = _qualify(torch, gx, trans, m, n, k, A, B, C, native, forms, seeds, samples)
     t_nat = _timed(torch, native, reps, stream)
     best, t_best = ("native", 0), t_nat
     parts = [f"{m}x{n}x{k} native {t_nat:.3f} ms"]
@@ -519,7 +537,8 @@ def measure_emulated(torch, gx, trans, m, n, k, reps, moduli=MODULI, seeds=EMU_S
                 parts.append(f"no {what} count matches its error")
             continue
         t = _timed(torch, forms[(scheme, counts[0])], reps, stream)
-        parts.append(f"{scheme} {tag}{counts[0]} {t:.3f} ms")
+        parts.append(f"{scheme} {tag}{counts[0]} {t:.3f} ms, "
+                     + _ratio_note(ratio[(scheme, counts[0])]))
         if t < t_best:
             best, t_best = (scheme, counts[0]), t
     return best[0], best[1], "; ".join(parts)
@@ -544,11 +563,12 @@ def measure_emulated32(torch, gx, trans, m, n, k, reps, seeds=EMU_SEEDS, samples
         gx.sgemm_bf16x3_dev(trans[0], trans[1], m, n, k, 1.0, A, ra, B, rb, 0.0, C, m, ws,
                             ws.numel(), stream=stream)
 
-    qual = _qualify(torch, gx, trans, m, n, k, A, B, C, native, {"bf16x3": bf16x3}, seeds,
-                    samples)
+    qual, ratio = _qualify(torch, gx, trans, m, n, k, A, B, C, native, {"bf16x3": bf16x3},
+                           seeds, samples)
     t_nat = _timed(torch, native, reps, stream)
     t_b = _timed(torch, bf16x3, reps, stream)
-    note = (f"{m}x{n}x{k} native {t_nat:.3f} ms, bf16x3 {t_b:.3f} ms"
+    note = (f"{m}x{n}x{k} native {t_nat:.3f} ms, bf16x3 {t_b:.3f} ms, "
+            + _ratio_note(ratio["bf16x3"])
             + ("" if qual else "; bf16x3 does not match its error"))
     return ("bf16x3" if qual and t_b < t_nat else "native"), note
 
