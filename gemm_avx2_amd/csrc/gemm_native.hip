@@ -157,14 +157,14 @@ extern "C" int gemmx_native_tile(int cfg, int *bm, int *bn) {
   *bn = kNativeCfg[cfg].bn;
   return kNativeCfg[cfg].split_ok;
 }
-extern "C" int gemmx_native_splits(int cfg, int k, int split) {
+extern "C" int gemmx_native_splits(int es, int cfg, int k, int split) {
   if (cfg < 0 || cfg >= GEMMX_NCFG_COUNT || !kNativeCfg[cfg].split_ok || split <= 1 || k < 1)
     return 1;
-  const int kc = split_depth(k, split, kNativeCfg[cfg].bk);
+  const int kc = split_depth(k, split, es == 8 ? kNativeCfg[cfg].bk64 : kNativeCfg[cfg].bk32);
   return (k + kc - 1) / kc;
 }
 extern "C" size_t gemmx_native_workspace(int es, int cfg, int m, int n, int k, int split) {
-  const int ns = gemmx_native_splits(cfg, k, split);
+  const int ns = gemmx_native_splits(es, cfg, k, split);
   if (ns <= 1 || m <= 0 || n <= 0) return 0;
   const NativeCfg &c = kNativeCfg[cfg];
   return (size_t)ns * (((size_t)m + c.bm - 1) / c.bm) * (((size_t)n + c.bn - 1) / c.bn) *

@@ -31,14 +31,14 @@ namespace {
  * give one XCD an extra tile and, when the tiles fill whole waves, a wave of
  * its own. */
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW,
-          int KVMAX, bool NARROW, bool SPLIT>
-__global__ void __launch_bounds__(64 * WM * WN, MINW)
+          int KVMAX, bool NARROW, bool SPLIT, int KG>
+__global__ void __launch_bounds__(64 * WM * WN * KG, MINW)
 gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
                    T *__restrict__ C, int M, int N, int K, int64_t lda,
                    int64_t ldb, int64_t ldc, T alpha, T beta, int tiles_m,
                    int tiles_n, int vecA, int vecB, int lead, int64_t sa, int64_t sb,
                    int64_t sc, int kc) {
-  __shared__ __align__(16) T lds[tile_lds_elems<T, TA, TB, BM, BN, BK>()];
+  __shared__ __align__(16) T lds[tile_lds_need<T, TA, TB, BM, BN, BK, KG>()];
   const int tiles_m_all = tiles_m + ((M % BM) ? 1 : 0);
   const int tiles_all = tiles_m_all * (tiles_n + ((N % BN) ? 1 : 0));
   /* bx: the tile (edge tiles first), by: the grid row (K range or problem),
@@ -84,14 +84,14 @@ gemm_native_kernel(const T *__restrict__ A, const T *__restrict__ B,
       tn = tiles_n;
     }
     gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, true, NARROW, SPLIT,
-              (BM == 64 ? 2 : 1)>(
+              (BM == 64 ? 2 : 1), KG>(
         A, B, SPLIT ? part(tm, tn) : C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN,
         vecA, vecB, lds);
     return;
   }
   tile_coords(bx - lead, nx - lead, tiles_m, tiles_n, &tm, &tn);
   gemm_tile<T, TA, TB, BM, BN, BK, WM, WN, KVMAX, true, false, NARROW, SPLIT,
-              (BM == 64 ? 2 : 1)>(
+              (BM == 64 ? 2 : 1), KG>(
       A, B, SPLIT ? part(tm, tn) : C, M, N, K, lda, ldb, ldc, alpha, beta, tm * BM, tn * BN,
       vecA, vecB, lds);
 }
@@ -137,7 +137,7 @@ inline int split_depth(int k, int split, int bk) {
 }
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int MINW, int KVMAX,
-          bool SPLITOK>
+          bool SPLITOK, int KG = 1>
 hipError_t launch_trans(const gemmx_problem *p, int split, void *work, hipStream_t s) {
   constexpr int VE = Vec16<T>::N;
   const int batch = p->batch > 1 ? p->batch : 1;
@@ -146,7 +146,7 @@ hipError_t launch_trans(const gemmx_problem *p, int split, void *work, hipStream
   const int vecA = ((uintptr_t)p->a % 16 == 0) && (p->lda % VE == 0) && (sa % VE == 0);
   const int vecB = ((uintptr_t)p->b % 16 == 0) && (p->ldb % VE == 0) && (sb % VE == 0);
   const T alpha = (T)p->alpha, beta = (T)p->beta;
-  const dim3 block(64 * WM * WN);
+  const dim3 block(64 * WM * WN * KG);
   const int tiles_m = (int)(((int64_t)p->m + BM - 1) / BM);
   const int tiles_n = (int)(((int64_t)p->n + BN - 1) / BN);
   const int full_m = p->m / BM, full_n = p->n / BN;
@@ -157,12 +157,12 @@ hipError_t launch_trans(const gemmx_problem *p, int split, void *work, hipStream
     T *P = (T *)work;
     const dim3 grid((unsigned)((lead + full_m * full_n) * ns));
     if (vecA && vecB)
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false, true>),
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false, true, KG>),
                          grid, block, 0, s, (const T *)p->a, (const T *)p->b, P, p->m, p->n,
                          p->k, p->lda, p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB,
                          lead, (int64_t)0, (int64_t)0, (int64_t)0, kc);
     else
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true, true>),
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true, true, KG>),
                          grid, block, 0, s, (const T *)p->a, (const T *)p->b, P, p->m, p->n,
                          p->k, p->lda, p->ldb, p->ldc, alpha, beta, full_m, full_n, vecA, vecB,
                          lead, (int64_t)0, (int64_t)0, (int64_t)0, kc);
@@ -181,11 +181,11 @@ hipError_t launch_trans(const gemmx_problem *p, int split, void *work, hipStream
     T *C = (T *)p->c + b0 * sc;
     const dim3 grid(lead + full_m * full_n, nb);
     if (vecA && vecB)
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false, false>),
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, false, false, KG>),
                          grid, block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb, p->ldc,
                          alpha, beta, full_m, full_n, vecA, vecB, lead, sa, sb, sc, 0);
     else
-      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true, false>),
+      hipLaunchKernelGGL((gemm_native_kernel<T, TA, TB, BM, BN, BK, WM, WN, MINW, KVMAX, true, false, KG>),
                          grid, block, 0, s, A, B, C, p->m, p->n, p->k, p->lda, p->ldb, p->ldc,
                          alpha, beta, full_m, full_n, vecA, vecB, lead, sa, sb, sc, 0);
     const hipError_t e = hipGetLastError();
@@ -194,13 +194,21 @@ hipError_t launch_trans(const gemmx_problem *p, int split, void *work, hipStream
   return hipSuccess;
 }
 
-/* Tile of each configuration (BM, BN, BK) and whether it has split-K kernels. */
+/* Tile of each configuration: BM, BN, the K slice depth of the FP64 and of
+ * the FP32 kernels, and whether it has split-K kernels.  The launchers take
+ * their slice depth from here, so the host's split arithmetic
+ * (gemmx_native_splits) and the kernels' agree. */
 struct NativeCfg {
-  int bm, bn, bk, split_ok;
+  int bm, bn, bk64, bk32, split_ok;
 };
 constexpr NativeCfg kNativeCfg[GEMMX_NCFG_COUNT] = {
-    {128, 128, 16, 1}, {64, 64, 16, 1}, {256, 128, 16, 0}, {128, 128, 32, 0},
-    {128, 128, 16, 0}, {128, 128, 16, 1}, {256, 128, 16, 0}, {64, 64, 32, 1}};
+    {128, 128, 16, 16, 1}, {64, 64, 16, 16, 1}, {256, 128, 16, 16, 0}, {128, 128, 32, 32, 0},
+    {128, 128, 16, 16, 0}, {128, 128, 16, 16, 1}, {256, 128, 16, 16, 0}, {64, 64, 16, 32, 1},
+    {64, 64, 32, 64, 1}};
+template <typename T>
+constexpr int cfg_bk(int cfg) {
+  return std::is_same<T, double>::value ? kNativeCfg[cfg].bk64 : kNativeCfg[cfg].bk32;
+}
 
 /* One transpose pair: the configuration switch. */
 template <typename T, bool TA, bool TB>
@@ -210,20 +218,24 @@ hipError_t launch_native_trans(const gemmx_problem *p, int cfg, int split, void 
   if (!kNativeCfg[cfg].split_ok) split = 1;
   /* BM, BN, BK, waves (M x N), min waves per SIMD, K steps per LDS read */
   switch (cfg) {
-  case 1: return launch_trans<T, TA, TB, 64, 64, 16, 2, 2, 2, 1, true>(p, split, work, s);
-  case 2: return launch_trans<T, TA, TB, 256, 128, 16, 4, 2, 2, 1, false>(p, 1, NULL, s);
+  case 1: return launch_trans<T, TA, TB, 64, 64, cfg_bk<T>(1), 2, 2, 2, 1, true>(p, split, work, s);
+  case 2: return launch_trans<T, TA, TB, 256, 128, cfg_bk<T>(2), 4, 2, 2, 1, false>(p, 1, NULL, s);
   /* FP32 128x128 with K-vector reads fits two workgroups per CU's register
    * file; FP64 (twice the accumulator registers) takes one */
-  case 3: return launch_trans<T, TA, TB, 128, 128, 32, 2, 2, std::is_same<T, float>::value ? 2 : 1, 4, false>(p, 1, NULL, s);
-  case 4: return launch_trans<T, TA, TB, 128, 128, 16, 2, 2, 1, 4, false>(p, 1, NULL, s);
-  case 5: return launch_trans<T, TA, TB, 128, 128, 16, 4, 2, 2, 1, true>(p, split, work, s);
-  case 6: return launch_trans<T, TA, TB, 256, 128, 16, 4, 2, 1, 4, false>(p, 1, NULL, s);
+  case 3: return launch_trans<T, TA, TB, 128, 128, cfg_bk<T>(3), 2, 2, std::is_same<T, float>::value ? 2 : 1, 4, false>(p, 1, NULL, s);
+  case 4: return launch_trans<T, TA, TB, 128, 128, cfg_bk<T>(4), 2, 2, 1, 4, false>(p, 1, NULL, s);
+  case 5: return launch_trans<T, TA, TB, 128, 128, cfg_bk<T>(5), 4, 2, 2, 1, true>(p, split, work, s);
+  case 6: return launch_trans<T, TA, TB, 256, 128, cfg_bk<T>(6), 4, 2, 1, 4, false>(p, 1, NULL, s);
   /* 64x64 with two waves per SIMD (8 waves of 32x16) for products with
    * about one tile per CU; FP32 takes 32-deep K slices so that each of the
    * 512 threads copies a whole 16-byte chunk of each operand slice */
-  case 7: return launch_trans<T, TA, TB, 64, 64, std::is_same<T, float>::value ? 32 : 16, 2, 4, 2, 1, true>(p, split, work, s);
+  case 7: return launch_trans<T, TA, TB, 64, 64, cfg_bk<T>(7), 2, 4, 2, 1, true>(p, split, work, s);
+  /* the same 64x64 tile with four waves per SIMD: two K groups of 8 waves
+   * of 32x16 share each (twice as deep) LDS slice, one multiplying its
+   * first half and one its second, and add their accumulators at the end */
+  case 8: return launch_trans<T, TA, TB, 64, 64, cfg_bk<T>(8), 2, 4, 4, 1, true, 2>(p, split, work, s);
   case 0:
-  default: return launch_trans<T, TA, TB, 128, 128, 16, 2, 2, 2, 1, true>(p, split, work, s);
+  default: return launch_trans<T, TA, TB, 128, 128, cfg_bk<T>(0), 2, 2, 2, 1, true>(p, split, work, s);
   }
 }
 

@@ -893,7 +893,7 @@ static int split_lookup(int es, char transa, char transb, int m, int n, int k, s
   const int ta = !notrans(transa), tb = !notrans(transb);
   const int split = gemmx_tune_pick_split(es, ta, tb, m, n, k, cfg);
   if (bytes) *bytes = gemmx_native_workspace(es, cfg, m, n, k, split);
-  return gemmx_native_splits(cfg, k, split);
+  return gemmx_native_splits(es, cfg, k, split);
 }
 EXPORT int dgemm_x_tune_lookup_split(char transa, char transb, int m, int n, int k,
                                      size_t *bytes) {

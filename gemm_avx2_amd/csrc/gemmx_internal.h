@@ -79,8 +79,10 @@ enum {
   GEMMX_NCFG_256x128 = 2, /* 256x128x16, 8 waves of 64x64, >= 2 waves/SIMD */
   /* 3: 128x128x32 1 wave/SIMD; 4: 128x128x16 1 wave/SIMD; 5: 128x128x16 with
    * 8 waves of 32x64; 6: 256x128x16 1 wave/SIMD; 7: 64x64 with 8 waves of
-   * 32x16, K slices of 16 (FP64) or 32 (FP32) (see gemm_native.hip) */
-  GEMMX_NCFG_COUNT = 8
+   * 32x16, K slices of 16 (FP64) or 32 (FP32); 8: 64x64 with two K groups of
+   * 8 waves of 32x16, K slices of 32 (FP64) or 64 (FP32) (see
+   * gemm_native_impl.h) */
+  GEMMX_NCFG_COUNT = 9
 };
 
 /* Host-side error state and device lookup shared by the host sources
@@ -122,9 +124,10 @@ hipError_t gemmx_native_f32(const gemmx_problem *p, int cfg, int split, void *wo
 /* BM x BN tile of a configuration; returns 1 when it has split-K kernels,
  * 0 when not, -1 for an unknown configuration. */
 int gemmx_native_tile(int cfg, int *bm, int *bn);
-/* Splits actually run for a requested split (K ranges of whole BK slices),
- * and the bytes of partial tiles they need (0 for one pass). */
-int gemmx_native_splits(int cfg, int k, int split);
+/* Splits actually run for a requested split (K ranges of whole BK slices of
+ * the element size's kernels), and the bytes of partial tiles they need (0
+ * for one pass). */
+int gemmx_native_splits(int es, int cfg, int k, int split);
 size_t gemmx_native_workspace(int es, int cfg, int m, int n, int k, int split);
 /* Static device scratch of the allocation-free entry points (a file-scope
  * array in gemm_native.hip, GEMMX_NOALLOC_SCRATCH bytes per device): its

@@ -49,6 +49,13 @@ constexpr int tile_lds_elems() {
   return 2 * ((TA ? BM : BK) * ((TA ? BK : BM) + Vec16<T>::N) +
               (TB ? BK : BN) * ((TB ? BN : BK) + Vec16<T>::N));
 }
+/* LDS elements of a tile with KG K groups: the staging buffers, or the
+ * accumulators groups 1..KG-1 hand to group 0 if that is more. */
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int KG>
+constexpr int tile_lds_need() {
+  constexpr int st = tile_lds_elems<T, TA, TB, BM, BN, BK>();
+  return st > (KG - 1) * BM * BN ? st : (KG - 1) * BM * BN;
+}
 template <typename T, int BM, int BN, int BK>
 constexpr int tile_lds_max() {
   constexpr int a = tile_lds_elems<T, false, false, BM, BN, BK>();
@@ -66,22 +73,24 @@ struct NoHook {
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KVMAX,
           bool FAST, bool CLAMP = false, bool NARROW = false, bool PARTIAL = false,
-          int DEPTH = 1, typename Hook = NoHook>
+          int DEPTH = 1, int KG = 1, typename Hook = NoHook>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           T *__restrict__ C, int M, int N, int K,
                                           int64_t lda, int64_t ldb, int64_t ldc, T alpha,
                                           T beta, int m0, int n0, int vecA, int vecB,
                                           T *lds, Hook after_first_loads = Hook()) {
-  constexpr int NT = 64 * WM * WN;
+  constexpr int NT = 64 * WM * WN * KG;
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int FM = WTM / 16, FN = WTN / 16;
   constexpr int KS = BK / 4;            /* MFMA steps per slice */
+  constexpr int KSG = KS / KG;          /* MFMA steps per slice of one K group */
   constexpr int VE = Vec16<T>::N;       /* elements per 16 bytes */
   /* MFMA steps read together where LDS is K-contiguous (more registers, fewer reads) */
-  constexpr int KV = (VE < KS ? VE : KS) < KVMAX ? (VE < KS ? VE : KS) : KVMAX;
+  constexpr int KV = (VE < KSG ? VE : KSG) < KVMAX ? (VE < KSG ? VE : KSG) : KVMAX;
   constexpr int A_NL = TA ? BM : BK, A_LEN = TA ? BK : BM, A_LS = A_LEN + VE;
   constexpr int B_NL = TB ? BK : BN, B_LEN = TB ? BN : BK, B_LS = B_LEN + VE;
-  static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0 && KS % KV == 0, "tile shape");
+  static_assert(WTM % 16 == 0 && WTN % 16 == 0 && BK % 4 == 0 && KS % KG == 0 && KSG % KV == 0,
+                "tile shape");
   typedef typename Mfma16<T>::acc_t acc_t;
   typedef T kvec_t __attribute__((ext_vector_type(KV)));
 
@@ -90,8 +99,12 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
   auto sA = [&](int st) { return lds + st * (A_NL * A_LS); };
   auto sB = [&](int st) { return lds + 2 * A_NL * A_LS + st * (B_NL * B_LS); };
 
+  /* KG > 1: KG groups of WM x WN waves share the tile and its LDS slices,
+   * group q multiplying MFMA steps [q KSG, (q+1) KSG) of every slice; their
+   * accumulators are summed in group order after the loop */
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WM, wn = wave / WM;
+  const int wq = wave / (WM * WN), wt = wave % (WM * WN);
+  const int wm = wt % WM, wn = wt / WM;
   const int li = lane & 15, kg = lane >> 4;
 
   TileCopy<T, A_NL, A_LEN, NT> ca;
@@ -146,8 +159,8 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     const T *as = sA(buf);
     const T *bs = sB(buf);
 #pragma unroll
-    for (int s0 = 0; s0 < KS; s0 += KV) {
-      const int kx = kg * KS + s0;
+    for (int s0 = 0; s0 < KSG; s0 += KV) {
+      const int kx = kg * KS + wq * KSG + s0;
       T af[KV][FM], bf[KV][FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
@@ -195,7 +208,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     __builtin_amdgcn_sched_group_barrier(0x020, NGL, 0);
     __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
 #pragma unroll
-    for (int s = KV; s < KS; s += KV) {
+    for (int s = KV; s < KSG; s += KV) {
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         __builtin_amdgcn_sched_group_barrier(0x008, NMF / NR, 0);
@@ -310,6 +323,35 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     }
   }
 
+  /* K groups: groups 1..KG-1 leave their accumulators in LDS (the staging
+   * buffers, free once every wave is past its last slice; the caller sizes
+   * `lds` for (KG-1)*BM*BN elements too) in fragment order, and group 0
+   * adds them in group order and goes on alone. */
+  if constexpr (KG > 1) {
+    __syncthreads();
+    if (wq > 0) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            lds[((((wq - 1) * WM * WN + wt) * FM + i) * FN + j) * 256 + r * 64 + lane] =
+                acc[i][j][r];
+    }
+    __syncthreads();
+    if (wq > 0) return;
+#pragma unroll
+    for (int q = 1; q < KG; ++q)
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            acc[i][j][r] += lds[((((q - 1) * WM * WN + wt) * FM + i) * FN + j) * 256 + r * 64 + lane];
+  }
+
   /* PARTIAL (one K range of a split-K product): the raw accumulators go to
    * the BM*BN block C in the fragment order of the registers -- element
    * (((wave*FM + i)*FN + j)*4 + r)*64 + lane -- so every store is 64
@@ -321,7 +363,7 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
       for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          C[(((wave * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+          C[(((wt * FM + i) * FN + j) * 4 + r) * 64 + lane] = acc[i][j][r];
     return;
   }
   /* Epilogue: rows m..ldc-1 and columns >= n are never touched, matching the

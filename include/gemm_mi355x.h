@@ -167,7 +167,7 @@ int sgemm_x_set_mode(int mode);
  * library), GEMMX_NATIVE_CFG and GEMMX_NATIVE_SPLIT are read at first use.
  * *_x_tune_lookup returns the configuration a call with these arguments
  * would use (no device needed), or -1 for negative sizes. */
-#define GEMMX_TILE_CONFIGS 8
+#define GEMMX_TILE_CONFIGS 9
 int dgemm_x_set_tile_config(int cfg);
 int sgemm_x_set_tile_config(int cfg);
 /* Split-K of the native kernels, in place of the reference's N-split of

@@ -38,7 +38,8 @@ def test_autotune_table_is_used_and_correct(cuda, tmp_path, capsys):
             m, n, k = (int(v) for v in notes[(dt, tr, mb, nb, kb)].split()[0].split("x"))
             assert gx.tune_lookup(d, tr[0], tr[1], m, n, k) == cfg
             # the measured split: K ranges of whole 16-deep slices
-            kc = -(-(-(-k // split)) // 16) * 16
+            bk = at.TILE_BK[d][cfg]
+            kc = -(-(-(-k // split)) // bk) * bk
             want = -(-k // kc) if cfg in at.SPLIT_CONFIGS and split > 1 else 1
             assert gx.tune_lookup_split(d, tr[0], tr[1], m, n, k)[0] == want
             ra, ca = stored_shape(tr[0], m, k)

@@ -90,6 +90,35 @@ def test_forced_configuration_overrides_the_table(tmp_path, dtype, restore_table
 
 
 @pytest.mark.parametrize("dtype", ["d", "s"])
+def test_split_ranges_follow_the_kernels_slice_depth(dtype):
+    """The K ranges and partial-tile bytes the host reserves for a forced
+    configuration and split are those the kernels run: ranges of
+    ceil(ceil(k / split) / BK) BK-deep slices with the BK of that element
+    type's kernels (configuration 7 takes 16-deep FP64 and 32-deep FP32
+    slices; sizing the FP64 workspace with 32 gave 3 ranges where the kernel
+    runs 4 at k = 160, split 4).  Needs no device."""
+    es = 8 if dtype == "d" else 4
+    try:
+        for cfg in at.SPLIT_CONFIGS:
+            gx.set_tile_config(dtype, cfg)
+            bk = at.TILE_BK[dtype][cfg]
+            bm, bn = at.TILE_DIMS[cfg]
+            for split in (2, 3, 4, 8, 16):
+                gx.set_split_config(dtype, split)
+                for k in (16, 40, 100, 144, 160, 176, 192, 600, 640, 1040, 4111):
+                    kc = -(-(-(-k // split)) // bk) * bk
+                    ns = -(-k // kc)
+                    m, n = 200, 130
+                    tiles = -(-m // bm) * -(-n // bn)
+                    got = gx.tune_lookup_split(dtype, "N", "N", m, n, k)
+                    want = (ns, ns * tiles * bm * bn * es if ns > 1 else 0)
+                    assert got == want, (cfg, split, k, got, want)
+    finally:
+        gx.set_tile_config(dtype, -1)
+        gx.set_split_config(dtype, -1)
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
 def test_reader_rejects_other_formats_and_keeps_the_previous_table(tmp_path, dtype,
                                                                     restore_tables):
     good = str(tmp_path / "good.txt")
@@ -140,15 +169,19 @@ def test_configuration_counts_agree():
     nb = int(re.search(r"#define GEMMX_BATCH_CONFIGS (\d+)", hdr).group(1))
     assert gx.TILE_CONFIGS == at.TILE_CONFIGS == n
     assert gx.BATCH_CONFIGS == nb
-    # the autotuner's copy of each configuration's tile and split-K ability
-    # follows the kernels' own table (kNativeCfg: bm, bn, bk, split_ok)
+    # the autotuner's copy of each configuration's tile, slice depths and
+    # split-K ability follows the kernels' own table (kNativeCfg: bm, bn,
+    # bk64, bk32, split_ok)
     impl = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                              "gemm_avx2_amd", "csrc", "gemm_native_impl.h")).read()
     body = re.search(r"kNativeCfg\[GEMMX_NCFG_COUNT\] = \{(.*?)\};", impl, re.S).group(1)
-    cfgs = [tuple(int(v) for v in t) for t in re.findall(r"\{(\d+), (\d+), (\d+), (\d)\}", body)]
+    cfgs = [tuple(int(v) for v in t)
+            for t in re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d)\}", body)]
     assert len(cfgs) == n
-    assert list(at.TILE_DIMS) == [(bm, bn) for bm, bn, _, _ in cfgs]
-    assert tuple(at.SPLIT_CONFIGS) == tuple(c for c, t in enumerate(cfgs) if t[3])
+    assert list(at.TILE_DIMS) == [(bm, bn) for bm, bn, _, _, _ in cfgs]
+    assert list(at.TILE_BK["d"]) == [t[2] for t in cfgs]
+    assert list(at.TILE_BK["s"]) == [t[3] for t in cfgs]
+    assert tuple(at.SPLIT_CONFIGS) == tuple(c for c, t in enumerate(cfgs) if t[4])
 
 
 def test_emulated_entries_pick_the_dgemm_arithmetic(tmp_path, restore_tables):
