@@ -179,7 +179,7 @@ int sgemm_x_set_tile_config(int cfg);
  * element type, -1 returns to the tuning table's factor or, without one,
  * a heuristic (about four workgroups per CU when the tiles give fewer than
  * two, K ranges of at least 256, partial tiles within 64 MiB).  Only tile
- * configurations 0, 1, 5 and 7 have split-K kernels; a strided batch is never
+ * configurations 0, 1, 5, 7 and 8 have split-K kernels; a strided batch is never
  * split.  *_x_tune_lookup_split returns the number of K ranges a call with
  * these arguments runs (1 = one pass) and, in *bytes (may be NULL), the
  * bytes of partial tiles: the Fortran-ABI entries take them from the
