@@ -825,6 +825,37 @@ def profiled_kernel_ms(summary):
     return None
 
 
+def run_vendor(torch, stream):
+    """The vendor BLAS through torch (torch.matmul / torch.bmm on hipBLASLt
+    or rocBLAS) at shapes this library also reports: native DGEMM/SGEMM
+    8192^3 and 1024^3, and uniform batches of 1024 x 256^3, U[0,1) entries,
+    timed like the other lines (HIP events on the current stream, median).
+    A reference point for the native kernels, not part of `value`."""
+    out = {}
+    for name, tdt in (("f64", torch.float64), ("f32", torch.float32)):
+        for n in (8192, 1024):
+            a = torch.rand(n, n, dtype=tdt, device="cuda")
+            b = torch.rand(n, n, dtype=tdt, device="cuda")
+            c = torch.empty(n, n, dtype=tdt, device="cuda")
+            _, per = time_launches(torch, lambda: torch.matmul(a, b, out=c), 5 if n > 4096 else 20,
+                                   2)
+            ms = median(per)
+            out[f"{name}_matmul_{n}"] = {"ms": round(ms, 4),
+                                         "tflops": round(2.0 * n ** 3 / ms / 1e9, 2)}
+            del a, b, c
+        bs, n = 1024, 256
+        a = torch.rand(bs, n, n, dtype=tdt, device="cuda")
+        b = torch.rand(bs, n, n, dtype=tdt, device="cuda")
+        c = torch.empty(bs, n, n, dtype=tdt, device="cuda")
+        _, per = time_launches(torch, lambda: torch.bmm(a, b, out=c), 10, 2)
+        ms = median(per)
+        out[f"{name}_bmm_{bs}x{n}"] = {"ms": round(ms, 4),
+                                        "tflops": round(2.0 * bs * n ** 3 / ms / 1e9, 2)}
+        del a, b, c
+    out["note"] = "torch.matmul / torch.bmm (vendor BLAS), HIP events, median"
+    return out
+
+
 # ---------------------------------------------------------------- main
 def main():
     args = parse()
@@ -867,7 +898,8 @@ def main():
                                                               max(3, args.steps // 2), 1)),
                         ("level3_siblings", lambda: run_complex(torch, gx, stream,
                                                                 max(3, args.steps // 2), 1)),
-                        ("dgemm_host_pointers", lambda: run_host_pointer(torch, gx))):
+                        ("dgemm_host_pointers", lambda: run_host_pointer(torch, gx)),
+                        ("vendor_blas", lambda: run_vendor(torch, stream))):
             result[key] = fn()
             torch.cuda.empty_cache()
             print(f"bench: {key} done", file=sys.stderr, flush=True)
