@@ -60,7 +60,7 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
                   gemmx_batch_queues Q) {
   typedef BatchCfg<CFG> G;
   static_assert(64 * G::WM * G::WN == BT_THREADS, "batch block size");
-  __shared__ __align__(16) T lds[tile_lds_max<T, G::BM, G::BN, G::BK>()];
+  __shared__ __align__(16) T lds[tile_lds_max<T, G::BM, G::BN, G::BK, G::WM, G::WN>()];
   __shared__ int ticket;
   /* sched: the queues' ticket counters, GEMMX_BATCH_CTR_STRIDE words apart
    * (each on a line of its own); Q: the queues' first tiles, passed by value
@@ -118,9 +118,11 @@ gemm_batch_kernel(const gemmx_batch_desc *__restrict__ desc,
        * or n clamped back into the operand: 16-byte loads where both operands
        * allow them, else one element per load */
       const int fast = va && vb;
+/* C through LDS in the epilogue for FP32 only: the FP64 kernel's extra
+ * registers would cost it a workgroup per CU */
 #define GEMMX_TILE(TA, TB, FAST)                                                        \
   gemm_tile<T, TA, TB, G::BM, G::BN, G::BK, G::WM, G::WN, 1, true, true, !(FAST), false, \
-            1>(                                                                         \
+            1, 1, std::is_same<T, float>::value>(                                       \
       A, B, C, q.m, q.n, q.k, q.lda, q.ldb, q.ldc, alpha, beta, m0, n0, va, vb, lds, take)
       switch ((f & (GEMMX_BF_TA | GEMMX_BF_TB)) | (fast ? 4 : 0)) {
       case 0: GEMMX_TILE(false, false, false); break;

@@ -115,6 +115,53 @@ __device__ __forceinline__ void store_c_tile(const acc_t (&acc)[FM][FN], T *__re
   }
 }
 
+/* The same epilogue with the C accesses coalesced through a per-wave LDS
+ * region of FN*16 columns of EPI_CS elements (epi_region_elems): fragment
+ * row i (16 rows of the wave's tile) is written there column by column,
+ * then read back so that lane t holds row t % 16 of column t / 16 (and the
+ * columns 4, 8, ... further on): every C load and store of a wave covers
+ * whole 16-row column segments, 128 contiguous bytes (FP64) or 64 (FP32) per
+ * 16 lanes, where the MFMA result layout puts 4 rows of 16 different columns
+ * in one instruction.  The region is private to the wave, and a wave's LDS
+ * operations complete in order, so no barrier is needed between its writes
+ * and its reads; the caller makes sure no other wave still reads that LDS.
+ * Addresses are clamped into C as in store_c_tile. */
+constexpr int EPI_CS = 17;
+template <int FN>
+constexpr int epi_region_elems() {
+  return FN * 16 * EPI_CS;
+}
+template <typename T, int FM, int FN, typename acc_t>
+__device__ __forceinline__ void store_c_tile_lds(const acc_t (&acc)[FM][FN], T *__restrict__ C,
+                                                 int M, int N, int64_t ldc, T alpha, T beta,
+                                                 int r0, int c0, int li, int kg, T *reg,
+                                                 int lane) {
+  constexpr int PER = FN * 16 * 16 / 64; /* region elements per lane */
+  const int er = lane & 15, ec = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        reg[(j * 16 + li) * EPI_CS + Mfma16<T>::row(kg, r)] = (T)acc[i][j][r];
+    const int row = r0 + i * 16 + er;
+    const int64_t rr = row < M ? row : M - 1;
+    T v[PER], cv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      v[q] = reg[(ec + 4 * q) * EPI_CS + er];
+      const int col = c0 + ec + 4 * q;
+      cv[q] = C[rr + (int64_t)(col < N ? col : N - 1) * ldc];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int col = c0 + ec + 4 * q;
+      if (col < N && row < M) C[row + (int64_t)col * ldc] = fma_t(alpha, v[q], beta * cv[q]);
+    }
+  }
+}
+
 /* Copies an NL x LEN tile -- NL lines of LEN contiguous elements, line
  * stride `ld` in global memory -- into LDS lines of stride LS, in 16-byte
  * chunks.  Elements outside (nl_valid, len_valid) read as zero and are never

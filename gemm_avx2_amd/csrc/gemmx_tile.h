@@ -49,19 +49,26 @@ constexpr int tile_lds_elems() {
   return 2 * ((TA ? BM : BK) * ((TA ? BK : BM) + Vec16<T>::N) +
               (TB ? BK : BN) * ((TB ? BN : BK) + Vec16<T>::N));
 }
-/* LDS elements of a tile with KG K groups: the staging buffers, or the
- * accumulators groups 1..KG-1 hand to group 0 if that is more. */
-template <typename T, bool TA, bool TB, int BM, int BN, int BK, int KG>
+/* LDS elements of a tile with KG K groups and WM x WN waves per group: the
+ * staging buffers, or, if that is more, the accumulators groups 1..KG-1 hand
+ * to group 0 followed by the epilogue regions of group 0's waves
+ * (store_c_tile_lds). */
+template <int BM, int BN, int WM, int WN, int KG>
+constexpr int tile_lds_epi() {
+  return (KG - 1) * BM * BN + WM * WN * epi_region_elems<BN / WN / 16>();
+}
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KG>
 constexpr int tile_lds_need() {
   constexpr int st = tile_lds_elems<T, TA, TB, BM, BN, BK>();
-  return st > (KG - 1) * BM * BN ? st : (KG - 1) * BM * BN;
+  constexpr int ep = tile_lds_epi<BM, BN, WM, WN, KG>();
+  return st > ep ? st : ep;
 }
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, int WM, int WN>
 constexpr int tile_lds_max() {
-  constexpr int a = tile_lds_elems<T, false, false, BM, BN, BK>();
-  constexpr int b = tile_lds_elems<T, false, true, BM, BN, BK>();
-  constexpr int c = tile_lds_elems<T, true, false, BM, BN, BK>();
-  constexpr int d = tile_lds_elems<T, true, true, BM, BN, BK>();
+  constexpr int a = tile_lds_need<T, false, false, BM, BN, BK, WM, WN, 1>();
+  constexpr int b = tile_lds_need<T, false, true, BM, BN, BK, WM, WN, 1>();
+  constexpr int c = tile_lds_need<T, true, false, BM, BN, BK, WM, WN, 1>();
+  constexpr int d = tile_lds_need<T, true, true, BM, BN, BK, WM, WN, 1>();
   return a > b ? (a > c ? (a > d ? a : d) : (c > d ? c : d)) : (b > c ? (b > d ? b : d) : (c > d ? c : d));
 }
 
@@ -73,7 +80,7 @@ struct NoHook {
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, int KVMAX,
           bool FAST, bool CLAMP = false, bool NARROW = false, bool PARTIAL = false,
-          int DEPTH = 1, int KG = 1, typename Hook = NoHook>
+          int DEPTH = 1, int KG = 1, bool EPI_LDS = true, typename Hook = NoHook>
 __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__restrict__ B,
                                           T *__restrict__ C, int M, int N, int K,
                                           int64_t lda, int64_t ldb, int64_t ldc, T alpha,
@@ -367,8 +374,20 @@ __device__ __forceinline__ void gemm_tile(const T *__restrict__ A, const T *__re
     return;
   }
   /* Epilogue: rows m..ldc-1 and columns >= n are never touched, matching the
-   * reference's masked stores (src/kernel/gemm_kernel_unroll6.S:145-155). */
-  store_c_tile<T, FM, FN>(acc, C, M, N, ldc, alpha, beta, m0 + wm * WTM, n0 + wn * WTN, li, kg);
+   * reference's masked stores (src/kernel/gemm_kernel_unroll6.S:145-155).
+   * EPI_LDS: C is read and written through the wave's LDS region
+   * (store_c_tile_lds), after the last slice's reads of every wave (one K
+   * group: a barrier; with K groups the regions lie past the handed-over
+   * accumulators, and group 0's waves are past their own reads). */
+  if constexpr (EPI_LDS) {
+    if constexpr (KG == 1) __syncthreads();
+    T *reg = lds + (KG - 1) * BM * BN + wt * epi_region_elems<FN>();
+    store_c_tile_lds<T, FM, FN>(acc, C, M, N, ldc, alpha, beta, m0 + wm * WTM, n0 + wn * WTN,
+                                li, kg, reg, lane);
+  } else {
+    store_c_tile<T, FM, FN>(acc, C, M, N, ldc, alpha, beta, m0 + wm * WTM, n0 + wn * WTN, li,
+                            kg);
+  }
 }
 
 /* Row and column, inside its BM x BN tile, of element e of a partial tile
