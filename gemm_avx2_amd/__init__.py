@@ -30,7 +30,7 @@ __all__ = [
     "dgemm_ozaki2_bits", "set_phase_timing", "phase_times", "sgemm_bf16x3_dev", "sgemm_bf16x3_workspace",
     "set_dgemm_mode", "set_sgemm_mode", "set_tile_config", "set_split_config",
     "tune_lookup_split", "build_rocm", "set_product_config", "load_tuning", "tune_lookup", "tune_lookup_emulated",
-    "tune_lookup_emulated32", "last_error", "fill",
+    "tune_lookup_emulated32", "tune_lookup_strided", "last_error", "fill",
     "BatchPlan", "set_batch_config", "batch_config", "MODE_NATIVE", "MODE_OZAKI", "MODE_BF16X3", "MODE_OZAKI2", "MODE_AUTO",
 ]
 
@@ -611,6 +611,15 @@ def tune_lookup(dtype, transa, transb, m, n, k) -> int:
     f.restype = ctypes.c_int
     f.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     return int(f(_ch(transa), _ch(transb), m, n, k))
+
+
+def tune_lookup_strided(dtype, transa, transb, m, n, k, batch) -> int:
+    """Tile configuration a strided batch of `batch` such problems uses."""
+    lib = load("DGEMM.so" if dtype == "d" else "SGEMM.so")
+    f = getattr(lib, f"{dtype}gemm_x_tune_lookup_strided")
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char, ctypes.c_char] + [ctypes.c_int] * 4
+    return int(f(_ch(transa), _ch(transb), m, n, k, batch))
 
 
 def tune_lookup_emulated(transa, transb, m, n, k):

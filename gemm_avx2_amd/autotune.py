@@ -10,6 +10,7 @@ shape and retuning needs no rebuild.
 
     python -m gemm_avx2_amd.autotune [--out PATH] [--dtypes d,s]
         [--trans NN,NT,TN,TT] [--set quick|full] [--reps N] [--resume]
+        [--emulated on|off] [--strided on|off] [--batch on|off]
 
 For each (dtype, trans, shape) every configuration runs on device-resident
 U[-1,1) operands (alpha = 1.5, beta = -0.5); the fastest by median event time
@@ -22,8 +23,11 @@ larger than native FP64's on phi in {0, 1, 4} inputs (relative Frobenius
 error, and half of native's largest componentwise error, over 2048 sampled
 entries on two seeds), whichever is fastest, when that beats native (an
 `emu` entry); for sgemm_, split-BF16x3 under the same rule against native
-FP32 (an `emu32` entry); and the grouped-batch kernel configuration per
-element type on a mixed batch (a `batch` entry).
+FP32 (an `emu32` entry); the tile configuration of strided batches of
+m = n = k problems (64 to 1024, about 2^34 flops per batch) per element type
+and transpose (a `strided` entry, used for batches of at least 256 tiles of
+128x128); and the grouped-batch kernel configuration per element type on a
+mixed batch (a `batch` entry).
 With --resume, keys already
 in the output table are kept and not measured again.  The table is written in
 sorted key order with the measured times as comments.
@@ -77,6 +81,8 @@ _EMU_RANGE = {"native": (0, 0), "ozaki": (3, 8), "ozaki2": (8, 22)}
 _EMU32 = re.compile(r"^\s*emu32\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(native|bf16x3)"
                     r"\s*(#.*)?$")
 _BATCH = re.compile(r"^\s*batch\s+(f64|f32)\s+(\d+)\s*(#.*)?$")
+_STRIDED = re.compile(r"^\s*strided\s+(f64|f32)\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)"
+                      r"\s+(\d+)\s*(#.*)?$")
 BATCH_CONFIGS = 4  # compiled grouped-batch kernel configurations (GEMMX_BATCH_CONFIGS)
 
 
@@ -112,6 +118,14 @@ def read_table(path):
                 if 0 <= int(m.group(2)) < BATCH_CONFIGS:
                     entries[key] = int(m.group(2))
                     notes[key] = (m.group(3) or "").lstrip("# ").rstrip()
+                continue
+            m = _STRIDED.match(line)
+            if m:
+                key = ("strided", m.group(1), m.group(2), int(m.group(3)), int(m.group(4)),
+                       int(m.group(5)))
+                if all(4 <= b <= 15 for b in key[3:]) and 0 <= int(m.group(6)) < TILE_CONFIGS:
+                    entries[key] = int(m.group(6))
+                    notes[key] = (m.group(7) or "").lstrip("# ").rstrip()
                 continue
             m = _EMU32.match(line)
             if m:
@@ -160,15 +174,21 @@ def write_table(path, entries, notes, arch, header=(), rocm=None):
     lines += ["# dtype trans mbucket nbucket kbucket cfg [split]",
               "# emu trans mbucket nbucket kbucket scheme count   (DGEMM arithmetic, auto mode)",
               "# emu32 trans mbucket nbucket kbucket scheme   (SGEMM arithmetic, auto mode)",
+              "# strided dtype trans mbucket nbucket kbucket cfg   (strided batches that fill the GPU)",
               "# batch dtype cfg   (grouped-batch kernel)"]
     order = {"f64": 0, "f32": 1, "emu": 2, "emu32": 3}
-    tiles = [k for k in entries if k[0] != "batch"]
+    tiles = [k for k in entries if k[0] not in ("batch", "strided")]
     for key in sorted(tiles, key=lambda k: (order[k[0]], TRANS.index(k[1]), k[2], k[3], k[4])):
         note = notes.get(key, "")
         val = entries[key]
         if isinstance(val, tuple):
             val = f"{val[0]} {val[1]}"
         lines.append(f"{key[0]} {key[1]} {key[2]} {key[3]} {key[4]} {val}"
+                     + (f"  # {note}" if note else ""))
+    strided = [k for k in entries if k[0] == "strided"]
+    for key in sorted(strided, key=lambda k: (k[1] != "f64", TRANS.index(k[2]), k[3:])):
+        note = notes.get(key, "")
+        lines.append("strided " + " ".join(str(v) for v in key[1:]) + f" {entries[key]}"
                      + (f"  # {note}" if note else ""))
     for key in sorted(k for k in entries if k[0] == "batch"):
         note = notes.get(key, "")
@@ -286,6 +306,59 @@ def _times_note(m, n, k, times, ok):
     return f"{m}x{n}x{k} ms: " + " ".join(
         (f"c{c}" if sp == 1 else f"c{c}s{sp}") + f"={tm:.3f}" + ("" if ok[(c, sp)] else "!")
         for (c, sp), tm in sorted(times.items()))
+
+
+# ------------------------------------------------------------------ strided batch
+STRIDED_SIZES = (64, 128, 256, 512, 1024)  # m = n = k of the measured strided batches
+STRIDED_FLOP = 2.0 ** 34  # the count is chosen for about this many flops (at least 64 problems)
+
+
+def strided_count(m, n, k):
+    return max(64, int(STRIDED_FLOP // (2 * m * n * k)))
+
+
+def measure_strided(torch, gx, dtype, trans, m, n, k, reps, count=None):
+    """Median event time (ms) of a strided batch of `count` problems (operands
+    back to back) per tile configuration; which results agree with
+    configuration 0's; and the fastest agreeing configuration."""
+    count = count or strided_count(m, n, k)
+    tdt = torch.float64 if dtype == "d" else torch.float32
+    u = 2.0 ** -53 if dtype == "d" else 2.0 ** -24
+    ra, ca = (m, k) if trans[0] == "N" else (k, m)
+    rb, cb = (k, n) if trans[1] == "N" else (n, k)
+    A = torch.empty(count * ra * ca, dtype=tdt, device="cuda")
+    B = torch.empty(count * rb * cb, dtype=tdt, device="cuda")
+    C0 = torch.empty(count * m * n, dtype=tdt, device="cuda")
+    gx.fill(A, ra, count * ca, ra, 97, 0)
+    gx.fill(B, rb, count * cb, rb, 97, 1)
+    gx.fill(C0, m, count * n, m, 97, 2)
+    C = torch.empty_like(C0)
+    sb = gx.dgemm_strided_batched_dev if dtype == "d" else gx.sgemm_strided_batched_dev
+    stream = torch.cuda.current_stream()
+
+    def call():
+        sb(trans[0], trans[1], m, n, k, ALPHA, A, ra, ra * ca, B, rb, rb * cb, BETA, C, m,
+           m * n, count, stream=stream)
+
+    times, ok, ref = {}, {}, None
+    bound = 64 * k * u * (abs(ALPHA) * k + abs(BETA))
+    try:
+        for cfg in range(gx.TILE_CONFIGS):
+            gx.set_tile_config(dtype, cfg)
+            C.copy_(C0)
+            call()
+            torch.cuda.synchronize()
+            if ref is None:
+                ref = C.clone()
+                ok[cfg] = True
+            else:
+                ok[cfg] = bool((C - ref).abs().max().item() <= 2 * bound)
+            times[cfg] = _timed(torch, call, reps, stream)
+    finally:
+        gx.set_tile_config(dtype, -1)
+    del A, B, C0, C, ref
+    best = min((t, c) for c, t in times.items() if ok[c])[1]
+    return times, ok, best, count
 
 
 # ------------------------------------------------------------------ grouped batch
@@ -591,6 +664,10 @@ def main(argv=None):
     ap.add_argument("--batch", choices=["on", "off"], default="on",
                     help="also choose the grouped-batch kernel configuration per element type "
                          "on a mix of %d problems with m, n, k log-uniform in [%d, %d]" % BATCH_MIX)
+    ap.add_argument("--strided", choices=["on", "off"], default="on",
+                    help="also choose the tile configuration of strided batches of m = n = k "
+                         "in %s per dtype and transpose, each batch about 2^34 flops"
+                         % (STRIDED_SIZES,))
     args = ap.parse_args(argv)
 
     import torch
@@ -650,6 +727,19 @@ def main(argv=None):
             write_table(args.out, entries, notes, arch, header)
             print(f"[{key[0]} {i + 1}/{len(etodo)}] {t} {m}x{n}x{k}: {scheme} {cnt} ({note})",
                   flush=True)
+    for d in (args.dtypes.split(",") if args.strided == "on" else ()):
+        for t in args.trans.split(","):
+            for s in STRIDED_SIZES:
+                key = ("strided", "f64" if d == "d" else "f32", t, bucket(s), bucket(s), bucket(s))
+                if args.resume and key in entries:
+                    continue
+                times, ok, best, count = measure_strided(torch, gx, d, t, s, s, s, args.reps)
+                entries[key] = best
+                notes[key] = f"{count} x {s}^3 ms: " + " ".join(
+                    f"c{c}={tm:.3f}" + ("" if ok[c] else "!") for c, tm in sorted(times.items()))
+                write_table(args.out, entries, notes, arch, header)
+                print(f"[strided] {key[1]} {t} {count}x{s}^3: cfg {best} ({times[best]:.3f} ms; "
+                      f"cfg0 {times[0]:.3f} ms)", flush=True)
     if args.batch == "on":
         for d in args.dtypes.split(","):
             key = ("batch", "f64" if d == "d" else "f32")

@@ -281,7 +281,9 @@ static int launch_device(const gemmx_problem *p, int es, int mode, int slices,
     e = gemmx_bf16x3_f32(p, work, s);
     return e == hipSuccess ? GEMMX_OK : hip_fail(e, "bf16x3 kernel launch");
   }
-  const int cfg = gemmx_tune_pick(es, p->ta, p->tb, p->m, p->n, p->k);
+  const int cfg = p->batch > 1
+                      ? gemmx_tune_pick_strided(es, p->ta, p->tb, p->m, p->n, p->k, p->batch)
+                      : gemmx_tune_pick(es, p->ta, p->tb, p->m, p->n, p->k);
   /* split-K needs the partial tiles' workspace; without one: one pass */
   const int split = (p->batch > 1 || !work)
                         ? 1 : gemmx_tune_pick_split(es, p->ta, p->tb, p->m, p->n, p->k, cfg);
@@ -884,6 +886,21 @@ EXPORT int dgemm_x_tune_lookup(char transa, char transb, int m, int n, int k) {
 }
 EXPORT int sgemm_x_tune_lookup(char transa, char transb, int m, int n, int k) {
   return tune_lookup(4, transa, transb, m, n, k);
+}
+/* The tile configuration of a strided batch of `batch` such problems. */
+static int strided_lookup(int es, char transa, char transb, int m, int n, int k, int batch) {
+  pthread_once(&g_once, init_once);
+  if (m < 0 || n < 0 || k < 0 || batch < 0)
+    return set_error(GEMMX_ERR_ARG, "negative dimension or batch count"), -1;
+  return gemmx_tune_pick_strided(es, !notrans(transa), !notrans(transb), m, n, k, batch);
+}
+EXPORT int dgemm_x_tune_lookup_strided(char transa, char transb, int m, int n, int k,
+                                       int batch) {
+  return strided_lookup(8, transa, transb, m, n, k, batch);
+}
+EXPORT int sgemm_x_tune_lookup_strided(char transa, char transb, int m, int n, int k,
+                                       int batch) {
+  return strided_lookup(4, transa, transb, m, n, k, batch);
 }
 /* The number of K ranges a native call with these arguments runs (1 = one
  * pass), and the bytes of partial tiles they need.  Needs no device. */

@@ -27,6 +27,8 @@ static signed char g_split[2][4][NB][NB][NB];
 static short g_emu[4][NB][NB][NB];
 /* SGEMM arithmetic per bucket: 0 = no entry, else mode + 1 */
 static signed char g_emu32[4][NB][NB][NB];
+/* strided-batch tile configuration per bucket: 0 = no entry, else cfg + 1 */
+static signed char g_strided[2][4][NB][NB][NB];
 /* grouped-batch kernel configuration per element size: 0 = no entry, else cfg + 1 */
 static signed char g_batch[2];
 static int g_entries;
@@ -72,6 +74,7 @@ static void clear_table(void) {
   memset(g_split, 0, sizeof g_split);
   memset(g_emu, 0, sizeof g_emu);
   memset(g_emu32, 0, sizeof g_emu32);
+  memset(g_strided, 0, sizeof g_strided);
   memset(g_batch, 0, sizeof g_batch);
   g_entries = 0;
   g_path[0] = 0;
@@ -86,11 +89,13 @@ static int load_table(const char *path) {
   static signed char staged_split[2][4][NB][NB][NB];
   static short staged_emu[4][NB][NB][NB];
   static signed char staged_emu32[4][NB][NB][NB];
+  static signed char staged_strided[2][4][NB][NB][NB];
   signed char staged_batch[2] = {0, 0};
   memset(staged, 0, sizeof staged);
   memset(staged_split, 0, sizeof staged_split);
   memset(staged_emu, 0, sizeof staged_emu);
   memset(staged_emu32, 0, sizeof staged_emu32);
+  memset(staged_strided, 0, sizeof staged_strided);
   char line[256];
   int format = 0, entries = 0, ok = 1, rocm = 0;
   while (ok && fgets(line, sizeof line, f)) {
@@ -123,6 +128,20 @@ static int load_table(const char *path) {
       const int d = strcmp(dt, "f64") == 0 ? 0 : strcmp(dt, "f32") == 0 ? 1 : -1;
       if (d < 0 || cfg < 0 || cfg >= GEMMX_BATCH_CONFIGS) continue;
       staged_batch[d] = (signed char)(cfg + 1);
+      entries++;
+      continue;
+    }
+    if (sscanf(line, "strided %7s %7s %d %d %d %d", dt, tr, &mb, &nb, &kb, &cfg) == 6) {
+      if (format != GEMMX_TUNE_FORMAT) {
+        ok = 0;
+        break;
+      }
+      const int d = strcmp(dt, "f64") == 0 ? 0 : strcmp(dt, "f32") == 0 ? 1 : -1;
+      const int t = trans_index(tr);
+      if (d < 0 || t < 0 || mb < 4 || mb > 15 || nb < 4 || nb > 15 || kb < 4 || kb > 15 ||
+          cfg < 0 || cfg >= GEMMX_NCFG_COUNT)
+        continue;
+      staged_strided[d][t][mb - 4][nb - 4][kb - 4] = (signed char)(cfg + 1);
       entries++;
       continue;
     }
@@ -181,6 +200,7 @@ static int load_table(const char *path) {
   memcpy(g_split, staged_split, sizeof g_split);
   memcpy(g_emu, staged_emu, sizeof g_emu);
   memcpy(g_emu32, staged_emu32, sizeof g_emu32);
+  memcpy(g_strided, staged_strided, sizeof g_strided);
   memcpy(g_batch, staged_batch, sizeof g_batch);
   g_entries = entries;
   snprintf(g_path, sizeof g_path, "%s", path);
@@ -275,6 +295,21 @@ int gemmx_tune_pick(int es, int ta, int tb, int m, int n, int k) {
   /* Heuristic: keep at least ~2 workgroups per CU of the 256 on MI355X. */
   const long tiles128 = (((long)m + 127) / 128) * (((long)n + 127) / 128);
   return tiles128 >= 512 ? GEMMX_NCFG_128x128 : GEMMX_NCFG_64x64;
+}
+
+int gemmx_tune_pick_strided(int es, int ta, int tb, int m, int n, int k, int batch) {
+  const int d = es == 8 ? 0 : 1, t = (ta ? 2 : 0) + (tb ? 1 : 0);
+  if (g_force[d] >= 0) return g_force[d];
+  /* A strided entry is measured on batches that fill the GPU many times
+   * over; a batch below one 128x128 tile per CU (256 CUs on MI355X) keeps
+   * the single problem's choice. */
+  const long tiles128 = (((long)m + 127) / 128) * (((long)n + 127) / 128);
+  if (batch > 1 && tiles128 * batch >= 256) {
+    const int c = g_strided[d][t][gemmx_tune_bucket(m) - 4][gemmx_tune_bucket(n) - 4]
+                           [gemmx_tune_bucket(k) - 4];
+    if (c > 0) return c - 1;
+  }
+  return gemmx_tune_pick(es, ta, tb, m, n, k);
 }
 
 int gemmx_tune_pick_batch(int es) { return g_batch[es == 8 ? 0 : 1] - 1; }

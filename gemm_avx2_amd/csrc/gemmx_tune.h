@@ -62,6 +62,10 @@ int gemmx_tune_bucket(int dim);
 /* Grouped-batch kernel configuration of the table for element size es (8 or
  * 4), or -1 when it has none. */
 int gemmx_tune_pick_batch(int es);
+/* Tile configuration of a strided batch of `batch` problems of one shape: a
+ * forced configuration, else the table's `strided` entry of the bucket when
+ * the batch holds at least 256 tiles of 128x128, else gemmx_tune_pick. */
+int gemmx_tune_pick_strided(int es, int ta, int tb, int m, int n, int k, int batch);
 /* Number of entries loaded from the table file (0 when none). */
 int gemmx_tune_entries(void);
 /* Forces one configuration for every call of the given element size (8 or

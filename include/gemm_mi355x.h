@@ -214,6 +214,15 @@ int sgemm_x_tune_lookup(char transa, char transb, int m, int n, int k);
 /* The arithmetic sgemm_ uses in GEMMX_MODE_AUTO for these arguments
  * (GEMMX_MODE_NATIVE or GEMMX_MODE_BF16X3); no device needed. */
 int sgemm_x_tune_lookup_emulated(char transa, char transb, int m, int n, int k);
+/* The tile configuration a strided batch of `batch` problems with these
+ * arguments uses (*_x_strided_batched_dev): a forced configuration, else
+ * the table's `strided` entry of the shape's bucket (measured on batches
+ * that fill the GPU many times over) when the batch holds at least 256
+ * tiles of 128x128, else what *_x_tune_lookup returns.  No device needed;
+ * -1 for negative arguments.  No counterpart in the reference (it has no
+ * batched entry point). */
+int dgemm_x_tune_lookup_strided(char transa, char transb, int m, int n, int k, int batch);
+int sgemm_x_tune_lookup_strided(char transa, char transb, int m, int n, int k, int batch);
 
 /* Last error of the calling thread: returns its code and, when msg is not
  * NULL, a static description.  Cleared by the next successful call. */

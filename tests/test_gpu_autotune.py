@@ -3,6 +3,8 @@ for each entry is the one it measured, every entry's configuration gives
 results within the native tolerance against the oracle (oracle_dgemm /
 oracle_sgemm), its split-K factor is the one the libraries run, and a
 resumed run measures nothing again."""
+import os
+
 import numpy as np
 import pytest
 
@@ -19,13 +21,13 @@ def test_autotune_table_is_used_and_correct(cuda, tmp_path, capsys):
     out = str(tmp_path / "tune.txt")
     shapes = "256,256,256;640,200,300"
     assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--reps", "2",
-                    "--batch", "off"]) == 0
+                    "--batch", "off", "--strided", "off"]) == 0
     entries, notes, arch, rocm = at.read_table(out)
     assert len(entries) == 8 and arch.startswith("gfx") and rocm == gx.build_rocm()
     capsys.readouterr()
     # resumed: nothing left to measure, same entries
     assert at.main(["--out", out, "--shapes", shapes, "--trans", "NN,TN", "--resume",
-                    "--batch", "off"]) == 0
+                    "--batch", "off", "--strided", "off"]) == 0
     assert "[1/" not in capsys.readouterr().out
     assert at.read_table(out)[0] == entries
     rng = np.random.default_rng(5)
@@ -71,7 +73,7 @@ def test_autotune_emulated_choice_drives_auto_mode(cuda, tmp_path):
     import os
     out = str(tmp_path / "tune.txt")
     assert at.main(["--out", out, "--shapes", "1024,1024,1024", "--trans", "NN",
-                    "--dtypes", "d,s", "--reps", "2"]) == 0
+                    "--dtypes", "d,s", "--reps", "2", "--strided", "off"]) == 0
     entries, notes, _, _ = at.read_table(out)
     scheme, cnt = entries[("emu", "NN", 10, 10, 10)]
     assert scheme in ("native", "ozaki2", "ozaki")
@@ -195,3 +197,23 @@ def test_table_entries_drive_both_auto_modes(cuda, tmp_path):
         gx.set_sgemm_mode(gx.MODE_NATIVE)
         for d in ("d", "s"):
             gx.load_tuning(at.DEFAULT_OUT if os.path.exists(at.DEFAULT_OUT) else None, d)
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
+def test_strided_measurement_picks_an_agreeing_configuration(cuda, dtype, tmp_path):
+    """The strided-batch measurement times every tile configuration on one
+    batch, checks each against configuration 0's result and records the
+    fastest agreeing one, which the library then uses for such batches."""
+    times, ok, best, count = at.measure_strided(__import__("torch"), gx, dtype, "TN", 128, 128,
+                                                128, 2, count=512)
+    assert count == 512 and sorted(times) == list(range(gx.TILE_CONFIGS))
+    assert all(ok.values()), ok
+    assert best == min(times, key=times.get)
+    out = str(tmp_path / "tune.txt")
+    key = ("strided", "f64" if dtype == "d" else "f32", "TN", 7, 7, 7)
+    at.write_table(out, {key: best}, {}, None)
+    try:
+        assert gx.load_tuning(out, dtype) == 1
+        assert gx.tune_lookup_strided(dtype, "T", "N", 128, 128, 128, 512) == best
+    finally:
+        gx.load_tuning(os.path.join(gx.LIB_DIR, "gemm_mi355x_tune.txt"), dtype)

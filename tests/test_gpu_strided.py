@@ -95,6 +95,35 @@ def test_strided_batch_shared_operand_and_single_problem(cuda, dt):
     np.testing.assert_array_equal(dC.cpu().numpy(), G)
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N")])
+def test_strided_batch_that_fills_the_gpu_takes_its_table_entry(cuda, dt, ta, tb):
+    """64 problems of 256^3 hold 256 tiles of 128x128, so the batch takes the
+    shipped table's `strided` entry of its bucket (when it has one) instead
+    of the single problem's choice; its result, and that of every tile
+    configuration forced on the same batch, matches the oracle on sampled
+    problems."""
+    import os
+    import torch
+    from gemm_avx2_amd import autotune as at
+    d = "d" if dt == np.float64 else "s"
+    m = n = k = 256
+    batch = 64
+    entries = at.read_table(os.path.join(gx.LIB_DIR, "gemm_mi355x_tune.txt"))[0]
+    key = ("strided", "f64" if d == "d" else "f32", ta + tb, 8, 8, 8)
+    want = entries.get(key, gx.tune_lookup(d, ta, tb, m, n, k))
+    assert gx.tune_lookup_strided(d, ta, tb, m, n, k, batch) == want
+    inputs, G = _run_batch(torch, dt, ta, tb, m, n, k, 1.5, -0.5, batch, 2, 5, seed=17)
+    _check_batch(dt, ta, tb, m, n, k, 1.5, -0.5, batch, inputs, G, problems=[0, 1, 31, 63])
+    for cfg in range(gx.TILE_CONFIGS):
+        gx.set_tile_config(d, cfg)
+        try:
+            inputs, G = _run_batch(torch, dt, ta, tb, m, n, k, 1.5, -0.5, batch, 2, 5, seed=17)
+        finally:
+            gx.set_tile_config(d, -1)
+        _check_batch(dt, ta, tb, m, n, k, 1.5, -0.5, batch, inputs, G, problems=[0, 63])
+
+
 def test_strided_batch_beyond_one_grid_row_launch(cuda):
     """More problems than one launch's grid rows (65535): every problem, the
     last of the first launch and the first of the second included, is

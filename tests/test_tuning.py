@@ -246,6 +246,54 @@ def test_emu32_entries_pick_the_sgemm_arithmetic(tmp_path, restore_tables):
 
 
 @pytest.mark.parametrize("dtype", ["d", "s"])
+def test_strided_entries_pick_the_strided_batch_configuration(tmp_path, dtype, restore_tables):
+    """`strided` lines choose the tile configuration of strided batches per
+    element type, transpose and shape bucket, but only for batches of at
+    least 256 tiles of 128x128; smaller batches, single calls and shapes
+    without a line keep the single-problem choice, and a forced
+    configuration overrides them.  The Python reader and writer agree with
+    the library, which skips malformed lines."""
+    path = str(tmp_path / "t.txt")
+    want = "f64" if dtype == "d" else "f32"
+    e = dict(_entries())
+    e[("f64", "NN", 8, 8, 8)] = 3
+    e[("f32", "NN", 8, 8, 8)] = 4
+    e[("strided", "f64", "NN", 8, 8, 8)] = 5
+    e[("strided", "f32", "NN", 8, 8, 8)] = 6
+    e[("strided", want, "TN", 10, 10, 10)] = 2
+    at.write_table(path, e, {}, None)
+    assert at.read_table(path)[0] == e
+    assert gx.load_tuning(path, dtype) == len(e)
+    single = 3 if dtype == "d" else 4
+    strided = 5 if dtype == "d" else 6
+    assert gx.tune_lookup(dtype, "N", "N", 256, 256, 256) == single
+    # 256^3: four 128x128 tiles a problem, so 64 problems reach 256 tiles
+    assert gx.tune_lookup_strided(dtype, "N", "N", 256, 256, 256, 64) == strided
+    assert gx.tune_lookup_strided(dtype, "N", "N", 300, 200, 256, 1024) == strided
+    assert gx.tune_lookup_strided(dtype, "N", "N", 256, 256, 256, 63) == single
+    assert gx.tune_lookup_strided(dtype, "N", "N", 256, 256, 256, 1) == single
+    assert gx.tune_lookup_strided(dtype, "T", "N", 1024, 1024, 1024, 4) == 2
+    assert gx.tune_lookup_strided(dtype, "T", "N", 1024, 1024, 1024, 3) == \
+        gx.tune_lookup(dtype, "T", "N", 1024, 1024, 1024)
+    # no line for NT: the single-problem choice
+    assert gx.tune_lookup_strided(dtype, "N", "T", 256, 256, 256, 4096) == \
+        gx.tune_lookup(dtype, "N", "T", 256, 256, 256)
+    assert gx.tune_lookup_strided(dtype, "N", "N", 256, 256, 256, -1) == -1
+    gx.set_tile_config(dtype, 7)
+    assert gx.tune_lookup_strided(dtype, "N", "N", 256, 256, 256, 4096) == 7
+    gx.set_tile_config(dtype, -1)
+    bad = tmp_path / "bad.txt"
+    bad.write_text(f"format 2\nrocm {gx.build_rocm()}\nstrided {want} NN 8 8 8 {gx.TILE_CONFIGS}\n"
+                   f"strided {want} NN 3 8 8 1\nstrided f16 NN 8 8 8 1\nstrided {want} XX 8 8 8 1\n"
+                   f"strided {want} NN 9 9 9 8\n")
+    assert gx.load_tuning(str(bad), dtype) == 1
+    assert gx.tune_lookup_strided(dtype, "N", "N", 512, 512, 512, 4096) == 8
+    assert gx.tune_lookup_strided(dtype, "N", "N", 256, 256, 256, 4096) == \
+        gx.tune_lookup(dtype, "N", "N", 256, 256, 256)
+    assert at.read_table(str(bad))[0] == {("strided", want, "NN", 9, 9, 9): 8}
+
+
+@pytest.mark.parametrize("dtype", ["d", "s"])
 def test_batch_entry_drives_the_batch_configuration(tmp_path, dtype, restore_tables,
                                                     monkeypatch):
     """A `batch <dtype> <cfg>` entry (written and read by the autotuner, read
