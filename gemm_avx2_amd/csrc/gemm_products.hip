@@ -448,8 +448,10 @@ hipError_t launch_cfg(const int8_t *as, const int8_t *bs, int m, int n, int64_t 
  * stages and direct-to-LDS with 4 waves of 128x128 (both spill), 256x128 and
  * 128x256 tiles (one wave per SIMD), two 256x128 blocks per CU, 32 bytes of
  * line padding, 64-byte stages with 8 waves and register staging, the
- * compiler's built-in GEMM interleaving, and a bank-conflict-free
- * fragment-ordered direct-to-LDS layout (32-byte global runs: slower).
+ * compiler's built-in GEMM interleaving, a bank-conflict-free
+ * fragment-ordered direct-to-LDS layout (32-byte global runs: slower), and
+ * raising the wave priority (s_setprio 1) over each k-step's MFMAs (5-8%
+ * slower: the waves in their MFMAs starve the other's loads and reads).
  * gemmx_set_product_config (else GEMMX_SLICE_CFG, read at first use)
  * selects one for every kind. */
 std::atomic<int> g_product_cfg{-2};
