@@ -543,7 +543,7 @@ def set_batch_config(dtype: str, cfg: int):
         _check(dtype, rc)
 
 
-TILE_CONFIGS = 9
+TILE_CONFIGS = 10
 
 
 def set_tile_config(dtype: str, cfg: int):

@@ -42,7 +42,7 @@ import sys
 import time
 
 FORMAT = 2
-TILE_CONFIGS = 9  # compiled native tile configurations (GEMMX_TILE_CONFIGS)
+TILE_CONFIGS = 10  # compiled native tile configurations (GEMMX_TILE_CONFIGS)
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
                            "gemm_mi355x_tune.txt")
 TRANS = ("NN", "NT", "TN", "TT")
@@ -74,7 +74,7 @@ def key_of(dtype: str, trans: str, m: int, n: int, k: int):
 _ENTRY = re.compile(r"^\s*(f64|f32)\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)"
                     r"(?:\s+(\d+))?\s*(#.*)?$")
 MAX_SPLIT = 64  # GEMMX_MAX_SPLIT
-SPLIT_CONFIGS = (0, 1, 5, 7, 8)  # native tile configurations with split-K kernels
+SPLIT_CONFIGS = (0, 1, 5, 7, 8, 9)  # native tile configurations with split-K kernels
 _EMU = re.compile(r"^\s*emu\s+(NN|NT|TN|TT)\s+(\d+)\s+(\d+)\s+(\d+)\s+(native|ozaki|ozaki2)"
                   r"\s+(\d+)\s*(#.*)?$")
 _EMU_RANGE = {"native": (0, 0), "ozaki": (3, 8), "ozaki2": (8, 22)}
@@ -221,10 +221,11 @@ def _median(xs):
 
 
 TILE_DIMS = ((128, 128), (64, 64), (256, 128), (128, 128), (128, 128), (128, 128), (256, 128),
-             (64, 64), (64, 64))
+             (64, 64), (64, 64), (64, 64))
 # K slice depth of each configuration's FP64 and FP32 kernels (split-K ranges
 # are whole slices)
-TILE_BK = {"d": (16, 16, 16, 32, 16, 16, 16, 16, 32), "s": (16, 16, 16, 32, 16, 16, 16, 32, 64)}
+TILE_BK = {"d": (16, 16, 16, 32, 16, 16, 16, 16, 32, 32),
+           "s": (16, 16, 16, 32, 16, 16, 16, 32, 64, 32)}
 
 
 def split_candidates(cfg, m, n, k, es):

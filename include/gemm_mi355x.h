@@ -167,7 +167,7 @@ int sgemm_x_set_mode(int mode);
  * library), GEMMX_NATIVE_CFG and GEMMX_NATIVE_SPLIT are read at first use.
  * *_x_tune_lookup returns the configuration a call with these arguments
  * would use (no device needed), or -1 for negative sizes. */
-#define GEMMX_TILE_CONFIGS 9
+#define GEMMX_TILE_CONFIGS 10
 int dgemm_x_set_tile_config(int cfg);
 int sgemm_x_set_tile_config(int cfg);
 /* Split-K of the native kernels, in place of the reference's N-split of
@@ -179,7 +179,7 @@ int sgemm_x_set_tile_config(int cfg);
  * element type, -1 returns to the tuning table's factor or, without one,
  * a heuristic (about four workgroups per CU when the tiles give fewer than
  * two, K ranges of at least 256, partial tiles within 64 MiB).  Only tile
- * configurations 0, 1, 5, 7 and 8 have split-K kernels; a strided batch is never
+ * configurations 0, 1, 5, 7, 8 and 9 have split-K kernels; a strided batch is never
  * split.  *_x_tune_lookup_split returns the number of K ranges a call with
  * these arguments runs (1 = one pass) and, in *bytes (may be NULL), the
  * bytes of partial tiles: the Fortran-ABI entries take them from the
