@@ -225,7 +225,7 @@ TILE_DIMS = ((128, 128), (64, 64), (256, 128), (128, 128), (128, 128), (128, 128
 # K slice depth of each configuration's FP64 and FP32 kernels (split-K ranges
 # are whole slices)
 TILE_BK = {"d": (16, 16, 16, 32, 16, 16, 16, 16, 32, 32),
-           "s": (16, 16, 16, 32, 16, 16, 16, 32, 64, 32)}
+           "s": (16, 16, 16, 32, 16, 32, 16, 32, 64, 32)}
 
 
 def split_candidates(cfg, m, n, k, es):

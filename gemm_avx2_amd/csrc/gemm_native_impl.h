@@ -208,7 +208,7 @@ struct NativeCfg {
 };
 constexpr NativeCfg kNativeCfg[GEMMX_NCFG_COUNT] = {
     {128, 128, 16, 16, 1}, {64, 64, 16, 16, 1}, {256, 128, 16, 16, 0}, {128, 128, 32, 32, 0},
-    {128, 128, 16, 16, 0}, {128, 128, 16, 16, 1}, {256, 128, 16, 16, 0}, {64, 64, 16, 32, 1},
+    {128, 128, 16, 16, 0}, {128, 128, 16, 32, 1}, {256, 128, 16, 16, 0}, {64, 64, 16, 32, 1},
     {64, 64, 32, 64, 1}, {64, 64, 32, 32, 1}};
 template <typename T>
 constexpr int cfg_bk(int cfg) {

@@ -77,8 +77,8 @@ enum {
   GEMMX_NCFG_128x128 = 0, /* 128x128x16, 4 waves of 64x64, >= 2 waves/SIMD */
   GEMMX_NCFG_64x64 = 1,   /* 64x64x16, 4 waves of 32x32 */
   GEMMX_NCFG_256x128 = 2, /* 256x128x16, 8 waves of 64x64, >= 2 waves/SIMD */
-  /* 3: 128x128x32 1 wave/SIMD; 4: 128x128x16 1 wave/SIMD; 5: 128x128x16 with
-   * 8 waves of 32x64; 6: 256x128x16 1 wave/SIMD; 7: 64x64 with 8 waves of
+  /* 3: 128x128x32 1 wave/SIMD; 4: 128x128x16 1 wave/SIMD; 5: 128x128 with
+   * 8 waves of 32x64, K slices of 16 (FP64) or 32 (FP32); 6: 256x128x16 1 wave/SIMD; 7: 64x64 with 8 waves of
    * 32x16, K slices of 16 (FP64) or 32 (FP32); 8: 64x64 with two K groups of
    * 8 waves of 32x16, K slices of 32 (FP64) or 64 (FP32); 9: 64x64x32, 4 waves
    * of 32x32 (see gemm_native_impl.h) */
